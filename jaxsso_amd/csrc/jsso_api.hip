@@ -1,0 +1,427 @@
+// C entry points of libjsso_hip.so and the host-side symbolic plan.
+//
+// Block pattern.  The reference emits, for every element, the full dense
+// meshgrid of its DOFs (element.py:141-149, 1097-1106) plus a (0,0) zero seed
+// (assemblemodel.py:202-204) and sums duplicates (assemblemodel.py:160-162), so
+// its scalar pattern is exactly the 6x6-block pattern of the node adjacency
+// graph (nodes sharing an element), which is what is built here, one sorted
+// block row per node.  Every node also gets its diagonal block (the solver
+// needs it; for a node outside all elements this adds zeros the reference
+// would not store, and such a node makes K singular in both).
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "jsso_internal.hpp"
+
+namespace jsso {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+template <typename T>
+static int upload(Plan* p, T** dst, const std::vector<T>& src) {
+    int rc = dalloc(p, dst, src.size());
+    if (rc) return rc;
+    if (!src.empty()) JSSO_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return JSSO_OK;
+}
+
+static int build_plan(Plan* p, int32_t n_sys, const int32_t* node_off, int32_t n_beam, const int32_t* cb,
+                      int32_t n_quad, const int32_t* cq, int32_t n_cons, const int32_t* cons) {
+    JSSO_REQUIRE(n_sys >= 1 && node_off, JSSO_EINVAL, "n_sys must be >= 1 with node offsets");
+    JSSO_REQUIRE(n_beam >= 0 && n_quad >= 0 && n_cons >= 0, JSSO_EINVAL, "negative element/constraint count");
+    JSSO_REQUIRE(node_off[0] == 0, JSSO_EINVAL, "node_off[0] must be 0");
+    for (int s = 0; s < n_sys; ++s)
+        JSSO_REQUIRE(node_off[s + 1] > node_off[s], JSSO_EINVAL, "structure %d has no nodes", s);
+    const int32_t n = node_off[n_sys];
+    p->n_sys = n_sys;
+    p->n_nodes = n;
+    p->n_beam = n_beam;
+    p->n_quad = n_quad;
+    p->n_cons = n_cons;
+    p->h_node_off.assign(node_off, node_off + n_sys + 1);
+    std::vector<int32_t> sys_of(n);
+    for (int s = 0; s < n_sys; ++s)
+        for (int i = node_off[s]; i < node_off[s + 1]; ++i) sys_of[i] = s;
+    auto check_el = [&](const int32_t* c, int nen, int e, const char* kind) -> int {
+        for (int a = 0; a < nen; ++a)
+            JSSO_REQUIRE(c[a] >= 0 && c[a] < n, JSSO_EINVAL, "%s %d: node %d out of range [0,%d)", kind, e, c[a], n);
+        for (int a = 1; a < nen; ++a)
+            JSSO_REQUIRE(sys_of[c[a]] == sys_of[c[0]], JSSO_EINVAL, "%s %d spans two structures", kind, e);
+        return JSSO_OK;
+    };
+    for (int e = 0; e < n_beam; ++e) {
+        int rc = check_el(cb + 2 * e, 2, e, "beam");
+        if (rc) return rc;
+    }
+    for (int e = 0; e < n_quad; ++e) {
+        int rc = check_el(cq + 4 * e, 4, e, "quad");
+        if (rc) return rc;
+    }
+    // constrained DOFs
+    std::vector<uint8_t> fixed(6 * (size_t)n, 0);
+    for (int k = 0; k < n_cons; ++k) {
+        const int64_t d = cons[k];
+        JSSO_REQUIRE(d >= 0 && d < 6 * (int64_t)n, JSSO_EINVAL, "constrained DOF %lld out of range", (long long)d);
+        JSSO_REQUIRE(!fixed[d], JSSO_EINVAL,
+                     "constrained DOF %lld listed twice (the reference's K_aug would be singular)", (long long)d);
+        fixed[d] = 1;
+    }
+    // node -> element incidence (element order: beams, then quads)
+    std::vector<int32_t> inc_ptr(n + 1, 0);
+    for (int e = 0; e < n_beam; ++e)
+        for (int a = 0; a < 2; ++a) inc_ptr[cb[2 * e + a] + 1]++;
+    for (int e = 0; e < n_quad; ++e)
+        for (int a = 0; a < 4; ++a) inc_ptr[cq[4 * e + a] + 1]++;
+    for (int i = 0; i < n; ++i) inc_ptr[i + 1] += inc_ptr[i];
+    std::vector<int32_t> inc(inc_ptr[n]), inc_pos(inc_ptr.begin(), inc_ptr.end() - 1);
+    std::vector<int8_t> inc_a(inc_ptr[n]);
+    for (int e = 0; e < n_beam; ++e)
+        for (int a = 0; a < 2; ++a) {
+            const int nd = cb[2 * e + a];
+            inc_a[inc_pos[nd]] = (int8_t)a;
+            inc[inc_pos[nd]++] = e;
+        }
+    for (int e = 0; e < n_quad; ++e)
+        for (int a = 0; a < 4; ++a) {
+            const int nd = cq[4 * e + a];
+            inc_a[inc_pos[nd]] = (int8_t)a;
+            inc[inc_pos[nd]++] = n_beam + e;
+        }
+    // block rows
+    std::vector<int32_t>& rowptr = p->h_rowptr;
+    std::vector<int32_t>& colind = p->h_colind;
+    rowptr.assign(n + 1, 0);
+    colind.clear();
+    std::vector<int32_t> row;
+    for (int i = 0; i < n; ++i) {
+        row.clear();
+        row.push_back(i);
+        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) {
+            const int el = inc[k];
+            if (el < n_beam) {
+                row.push_back(cb[2 * el]);
+                row.push_back(cb[2 * el + 1]);
+            } else {
+                const int e = el - n_beam;
+                for (int a = 0; a < 4; ++a) row.push_back(cq[4 * e + a]);
+            }
+        }
+        std::sort(row.begin(), row.end());
+        row.erase(std::unique(row.begin(), row.end()), row.end());
+        colind.insert(colind.end(), row.begin(), row.end());
+        rowptr[i + 1] = (int32_t)colind.size();
+    }
+    const int64_t nnzb = colind.size();
+    p->nnzb = nnzb;
+    auto find_block = [&](int i, int j) -> int64_t {
+        auto b = colind.begin() + rowptr[i], e = colind.begin() + rowptr[i + 1];
+        auto it = std::lower_bound(b, e, j);
+        return (it != e && *it == j) ? (int64_t)(it - colind.begin()) : -1;
+    };
+    std::vector<int32_t> browof(nnzb), diag(n);
+    for (int i = 0; i < n; ++i) {
+        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) browof[k] = i;
+        diag[i] = (int32_t)find_block(i, i);
+    }
+    // block -> element-block contributions, element order
+    p->n_contrib = 4 * (int64_t)n_beam + 16 * (int64_t)n_quad;
+    std::vector<int32_t> cptr(nnzb + 1, 0), cblk(p->n_contrib);
+    for (int e = 0; e < n_beam; ++e)
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) {
+                const int64_t k = find_block(cb[2 * e + a], cb[2 * e + b]);
+                cblk[(int64_t)e * 4 + a * 2 + b] = (int32_t)k;
+                cptr[k + 1]++;
+            }
+    for (int e = 0; e < n_quad; ++e)
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) {
+                const int64_t k = find_block(cq[4 * e + a], cq[4 * e + b]);
+                cblk[4 * (int64_t)n_beam + (int64_t)e * 16 + a * 4 + b] = (int32_t)k;
+                cptr[k + 1]++;
+            }
+    for (int64_t k = 0; k < nnzb; ++k) cptr[k + 1] += cptr[k];
+    std::vector<int32_t> csrc(p->n_contrib), cfill(cptr.begin(), cptr.end() - 1);
+    for (int64_t src = 0; src < p->n_contrib; ++src) csrc[cfill[cblk[src]]++] = (int32_t)src;
+    // node -> adjoint gradient buffer offsets
+    std::vector<int32_t> nptr(n + 1, 0), nsrc(inc_ptr[n]);
+    for (int i = 0; i < n; ++i) {
+        nptr[i + 1] = inc_ptr[i + 1];
+        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) {
+            const int el = inc[k];
+            const int a = inc_a[k];
+            nsrc[k] = el < n_beam ? el * BDIR + 3 * a : n_beam * BDIR + (el - n_beam) * QDIR + 3 * a;
+        }
+    }
+    // off-diagonal solver pattern
+    std::vector<int32_t> offrowptr(n + 1, 0), offcol, offsrc;
+    offcol.reserve(nnzb - n);
+    offsrc.reserve(nnzb - n);
+    for (int i = 0; i < n; ++i) {
+        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k)
+            if (colind[k] != i) {
+                offcol.push_back(colind[k]);
+                offsrc.push_back(k);
+            }
+        offrowptr[i + 1] = (int32_t)offcol.size();
+    }
+    p->nnzb_off = offcol.size();
+    // solver workgroups: contiguous chunks of ROWS_PER_WG block rows per structure
+    std::vector<int32_t> wg_sys, wg_row0, wg_row1, sys_wg_off(n_sys + 1, 0);
+    for (int s = 0; s < n_sys; ++s) {
+        for (int r0 = node_off[s]; r0 < node_off[s + 1]; r0 += ROWS_PER_WG) {
+            wg_sys.push_back(s);
+            wg_row0.push_back(r0);
+            wg_row1.push_back(std::min(r0 + ROWS_PER_WG, node_off[s + 1]));
+        }
+        sys_wg_off[s + 1] = (int32_t)wg_sys.size();
+    }
+    p->n_wg = (int32_t)wg_sys.size();
+    p->h_sys_wg_off = sys_wg_off;
+    p->h_browof.swap(browof);
+    p->h_diag.swap(diag);
+    p->h_offrowptr.swap(offrowptr);
+    p->h_offcol.swap(offcol);
+    p->h_offsrc.swap(offsrc);
+    p->h_cptr.swap(cptr);
+    p->h_csrc.swap(csrc);
+    p->h_nptr.swap(nptr);
+    p->h_nsrc.swap(nsrc);
+    p->h_cnct_beam.assign(cb, cb + 2 * (size_t)n_beam);
+    p->h_cnct_quad.assign(cq, cq + 4 * (size_t)n_quad);
+    p->h_wg_sys.swap(wg_sys);
+    p->h_wg_row0.swap(wg_row0);
+    p->h_wg_row1.swap(wg_row1);
+    p->h_fixed.swap(fixed);
+    return JSSO_OK;
+}
+
+// Upload the symbolic arrays and allocate the numeric buffers (first use).
+static int ensure_device(Plan* p) {
+    if (p->on_device) return JSSO_OK;
+    int rc = 0;
+    if ((rc = upload(p, &p->d_rowptr, p->h_rowptr)) || (rc = upload(p, &p->d_colind, p->h_colind)) ||
+        (rc = upload(p, &p->d_browof, p->h_browof)) || (rc = upload(p, &p->d_diag, p->h_diag)) ||
+        (rc = upload(p, &p->d_offrowptr, p->h_offrowptr)) || (rc = upload(p, &p->d_offcol, p->h_offcol)) ||
+        (rc = upload(p, &p->d_offsrc, p->h_offsrc)) || (rc = upload(p, &p->d_cptr, p->h_cptr)) ||
+        (rc = upload(p, &p->d_csrc, p->h_csrc)) || (rc = upload(p, &p->d_nptr, p->h_nptr)) ||
+        (rc = upload(p, &p->d_nsrc, p->h_nsrc)) || (rc = upload(p, &p->d_cnct_beam, p->h_cnct_beam)) ||
+        (rc = upload(p, &p->d_cnct_quad, p->h_cnct_quad)) || (rc = upload(p, &p->d_fixed, p->h_fixed)) ||
+        (rc = upload(p, &p->d_wg_sys, p->h_wg_sys)) || (rc = upload(p, &p->d_wg_row0, p->h_wg_row0)) ||
+        (rc = upload(p, &p->d_wg_row1, p->h_wg_row1)) || (rc = upload(p, &p->d_sys_wg_off, p->h_sys_wg_off)))
+        return rc;
+    const size_t n = p->n_nodes;
+    if ((rc = dalloc(p, &p->d_ke, (size_t)p->n_contrib * 36)) || (rc = dalloc(p, &p->d_K, (size_t)p->nnzb * 36)) ||
+        (rc = dalloc(p, &p->d_Kc, (size_t)p->nnzb * 36)) || (rc = dalloc(p, &p->d_Linv, n * 36)) ||
+        (rc = dalloc(p, &p->d_Minv, n * 36)) || (rc = dalloc(p, &p->d_Ks64, (size_t)p->nnzb_off * 36)) ||
+        (rc = dalloc(p, &p->d_Ks32, (size_t)p->nnzb_off * 36)) || (rc = dalloc(p, &p->d_status, 1)))
+        return rc;
+    JSSO_HIP(hipMemset(p->d_status, 0, sizeof(int32_t)));
+    p->on_device = true;
+    return JSSO_OK;
+}
+
+}  // namespace jsso
+
+using namespace jsso;
+
+struct jsso_plan {
+    Plan impl;
+};
+
+static Plan* P(jsso_plan* p) { return &p->impl; }
+static const Plan* P(const jsso_plan* p) { return &p->impl; }
+
+extern "C" {
+
+int jsso_version(void) { return 1; }
+
+int jsso_last_error(char* buf, size_t n) {
+    if (!buf || n == 0) return JSSO_EINVAL;
+    snprintf(buf, n, "%s", g_err);
+    return JSSO_OK;
+}
+
+int jsso_plan_create(int32_t n_sys, const int32_t* h_node_off, int32_t n_beam, const int32_t* h_cnct_beam,
+                     int32_t n_quad, const int32_t* h_cnct_quad, int32_t n_cons, const int32_t* h_cons_dofs,
+                     jsso_plan** out) {
+    if (!out) {
+        set_error("out is NULL");
+        return JSSO_EINVAL;
+    }
+    *out = nullptr;
+    if ((n_beam > 0 && !h_cnct_beam) || (n_quad > 0 && !h_cnct_quad) || (n_cons > 0 && !h_cons_dofs)) {
+        set_error("NULL connectivity or constraint array with a positive count");
+        return JSSO_EINVAL;
+    }
+    jsso_plan* p = new jsso_plan();
+    const int rc = build_plan(P(p), n_sys, h_node_off, n_beam, h_cnct_beam, n_quad, h_cnct_quad, n_cons,
+                              h_cons_dofs);
+    if (rc) {
+        Plan* q = P(p);
+        for (void* a : q->allocs) (void)hipFree(a);
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return JSSO_OK;
+}
+
+int jsso_plan_destroy(jsso_plan* plan) {
+    if (!plan) return JSSO_OK;
+    Plan* q = P(plan);
+    for (void* a : q->allocs) (void)hipFree(a);
+    for (auto& b : q->ws)
+        if (b.p) (void)hipFree(b.p);
+    delete plan;
+    return JSSO_OK;
+}
+
+int jsso_plan_sizes(const jsso_plan* plan, int64_t* h_sizes) {
+    if (!plan || !h_sizes) {
+        set_error("NULL argument");
+        return JSSO_EINVAL;
+    }
+    const Plan* p = P(plan);
+    h_sizes[JSSO_SZ_NODES] = p->n_nodes;
+    h_sizes[JSSO_SZ_NNZB] = p->nnzb;
+    h_sizes[JSSO_SZ_BEAMS] = p->n_beam;
+    h_sizes[JSSO_SZ_QUADS] = p->n_quad;
+    h_sizes[JSSO_SZ_SYSTEMS] = p->n_sys;
+    h_sizes[JSSO_SZ_CONTRIB] = p->n_contrib;
+    return JSSO_OK;
+}
+
+int jsso_plan_pattern(const jsso_plan* plan, int32_t* h_rowptr, int32_t* h_colind) {
+    if (!plan || !h_rowptr || !h_colind) {
+        set_error("NULL argument");
+        return JSSO_EINVAL;
+    }
+    const Plan* p = P(plan);
+    std::copy(p->h_rowptr.begin(), p->h_rowptr.end(), h_rowptr);
+    std::copy(p->h_colind.begin(), p->h_colind.end(), h_colind);
+    return JSSO_OK;
+}
+
+int jsso_element_ke_beam(int32_t n, const double* d_crds, const double* d_props, double* d_ke, int32_t* d_flags,
+                         void* stream) {
+    if (n < 0 || (n > 0 && (!d_crds || !d_props || !d_ke))) {
+        set_error("invalid arguments");
+        return JSSO_EINVAL;
+    }
+    return launch_ke_beam(n, d_crds, nullptr, nullptr, d_props, d_ke, d_flags, nullptr, (hipStream_t)stream);
+}
+
+int jsso_element_ke_quad(int32_t n, const double* d_crds, const double* d_props, double* d_ke, void* stream) {
+    if (n < 0 || (n > 0 && (!d_crds || !d_props || !d_ke))) {
+        set_error("invalid arguments");
+        return JSSO_EINVAL;
+    }
+    return launch_ke_quad(n, d_crds, nullptr, nullptr, d_props, d_ke, (hipStream_t)stream);
+}
+
+int jsso_assemble(jsso_plan* plan, const double* d_crds, const double* d_prop_beam, const double* d_prop_quad,
+                  double* d_K_out, void* stream) {
+    if (!plan || !d_crds) {
+        set_error("NULL plan or coordinates");
+        return JSSO_EINVAL;
+    }
+    Plan* p = P(plan);
+    if ((p->n_beam > 0 && !d_prop_beam) || (p->n_quad > 0 && !d_prop_quad)) {
+        set_error("missing element properties");
+        return JSSO_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    int rc0 = ensure_device(p);
+    if (rc0) return rc0;
+    JSSO_HIP(hipMemsetAsync(p->d_status, 0, sizeof(int32_t), s));
+    int rc = launch_ke_beam(p->n_beam, nullptr, p->d_cnct_beam, d_crds, d_prop_beam, p->d_ke, nullptr, p->d_status, s);
+    if (rc) return rc;
+    rc = launch_ke_quad(p->n_quad, nullptr, p->d_cnct_quad, d_crds, d_prop_quad, p->d_ke + (size_t)p->n_beam * 4 * 36, s);
+    if (rc) return rc;
+    rc = launch_assemble(p, s);
+    if (rc) return rc;
+    if (d_K_out && p->nnzb)
+        JSSO_HIP(hipMemcpyAsync(d_K_out, p->d_K, (size_t)p->nnzb * 36 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    p->assembled = true;
+    return JSSO_OK;
+}
+
+int jsso_solve(jsso_plan* plan, const double* d_f, double* d_u, int32_t nrhs, int32_t precision, double tol,
+               int32_t maxit, jsso_info* h_info, void* stream) {
+    if (!plan || !d_f || !d_u || nrhs < 1 || maxit < 1 || !(tol > 0.0)) {
+        set_error("invalid solve arguments");
+        return JSSO_EINVAL;
+    }
+    Plan* p = P(plan);
+    if (!p->assembled) {
+        set_error("jsso_solve before jsso_assemble");
+        return JSSO_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    switch (precision) {
+        case JSSO_PREC_FP64: return solve_batched(p, d_f, d_u, nrhs, false, tol, maxit, h_info, s);
+        case JSSO_PREC_MIXED: return solve_batched(p, d_f, d_u, nrhs, true, tol, maxit, h_info, s);
+        case JSSO_PREC_PLAIN: return solve_plain(p, d_f, d_u, nrhs, tol, maxit, h_info, s);
+        default: set_error("unknown precision %d", precision); return JSSO_EINVAL;
+    }
+}
+
+int jsso_adjoint_grad(jsso_plan* plan, const double* d_crds, const double* d_prop_beam, const double* d_prop_quad,
+                      const double* d_u, const double* d_lam, double lam_scale, int32_t nrhs, double* d_dcrds,
+                      double* d_dprop_beam, double* d_dprop_quad, void* stream) {
+    if (!plan || !d_crds || !d_u || nrhs < 1) {
+        set_error("invalid adjoint arguments");
+        return JSSO_EINVAL;
+    }
+    Plan* p = P(plan);
+    if ((p->n_beam > 0 && !d_prop_beam) || (p->n_quad > 0 && !d_prop_quad)) {
+        set_error("missing element properties");
+        return JSSO_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    // the adjoint reads the plan's connectivity and node adjacency on the device
+    int rc0 = ensure_device(p);
+    if (rc0) return rc0;
+    // gradient scratch: per element and coordinate direction, per rhs
+    const size_t need = ((size_t)p->n_beam * BDIR + (size_t)p->n_quad * QDIR) * nrhs * sizeof(double);
+    if (p->ws.size() < 12) p->ws.resize(12);
+    DBuf& g = p->ws[11];
+    if (g.bytes < need) {
+        if (g.p) JSSO_HIP(hipFree(g.p));
+        g.p = nullptr;
+        g.bytes = 0;
+        hipError_t e = hipMalloc(&g.p, need);
+        JSSO_REQUIRE(e == hipSuccess, JSSO_ENOMEM, "gradient scratch hipMalloc(%zu): %s", need, hipGetErrorString(e));
+        g.bytes = need;
+    }
+    int rc = launch_adjoint(p, d_crds, d_prop_beam, d_prop_quad, d_u, d_lam, lam_scale, nrhs, (double*)g.p,
+                            d_dprop_beam, d_dprop_quad, s);
+    if (rc) return rc;
+    return launch_gather_grad(p, (const double*)g.p, nrhs, d_dcrds, s);
+}
+
+int jsso_spmv(jsso_plan* plan, const double* d_x, double* d_y, void* stream) {
+    if (!plan || !d_x || !d_y) {
+        set_error("NULL argument");
+        return JSSO_EINVAL;
+    }
+    Plan* p = P(plan);
+    if (!p->assembled) {
+        set_error("jsso_spmv before jsso_assemble");
+        return JSSO_EINVAL;
+    }
+    return launch_spmv_full(p, p->d_K, d_x, d_y, (hipStream_t)stream);
+}
+
+}  // extern "C"

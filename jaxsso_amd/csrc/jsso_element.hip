@@ -1,0 +1,517 @@
+// Element stiffness kernels (beam-column, MITC-4) and the forward-mode adjoint
+// kernels.  Element math lives in element_math.hpp.
+#include "element_math.hpp"
+#include "jsso_internal.hpp"
+
+namespace jsso {
+
+// ---------------------------------------------------------------------------
+// Beam-column stiffness: one thread per (element, node pair a, b).
+// Ke_ab = blockdiag(D^-1, D^-1) K_loc,ab blockdiag(D, D)  (solve(T,K)@T,
+// element.py:130-139); D^-1 = D^T for the orthogonal general branch.
+// BLOCKS: output [e][a][b][6x6] (assembly), else row-major 12x12 per element.
+// ---------------------------------------------------------------------------
+template <bool BLOCKS>
+__global__ __launch_bounds__(256) void ke_beam_kernel(int32_t n, const double* __restrict__ crds_e,
+                                                      const int32_t* __restrict__ cnct,
+                                                      const double* __restrict__ crds_nodes,
+                                                      const double* __restrict__ props,
+                                                      double* __restrict__ ke, int32_t* flags,
+                                                      int32_t* status) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = tid >> 2;
+    if (e >= n) return;
+    const int a = (tid >> 1) & 1, b = tid & 1;
+    double c[6];
+    if (cnct) {
+        const int n0 = cnct[2 * e], n1 = cnct[2 * e + 1];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            c[k] = crds_nodes[3 * n0 + k];
+            c[3 + k] = crds_nodes[3 * n1 + k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) c[k] = crds_e[6 * e + k];
+    }
+    const double E = props[6 * e + 0], G = props[6 * e + 1], Iy = props[6 * e + 2],
+                 Iz = props[6 * e + 3], Jt = props[6 * e + 4], A = props[6 * e + 5];
+    double D[3][3], Di[3][3], L;
+    const bool special = beam_dircos<double>(c, D, L);
+    if (special) {
+        inv3(D, Di);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) Di[i][j] = D[j][i];
+    }
+    if (flags && a == 0 && b == 0) flags[e] = special ? 1 : 0;
+    if (status && special && a == 0 && b == 0) atomicOr(status, 2);
+    double out[6][6];
+#pragma unroll
+    for (int pq = 0; pq < 4; ++pq) {
+        const int p = pq >> 1, q = pq & 1;  // translation / rotation sub-blocks
+        double S[3][3], tmp[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                S[i][j] = beam_kloc<double>(6 * a + 3 * p + i, 6 * b + 3 * q + j, L, E, G, Iy, Iz, Jt, A);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                tmp[i][j] = S[i][0] * D[0][j] + S[i][1] * D[1][j] + S[i][2] * D[2][j];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                out[3 * p + i][3 * q + j] = Di[i][0] * tmp[0][j] + Di[i][1] * tmp[1][j] + Di[i][2] * tmp[2][j];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            if (BLOCKS)
+                ke[((int64_t)e * 4 + a * 2 + b) * 36 + i * 6 + j] = out[i][j];
+            else
+                ke[(int64_t)e * 144 + (6 * a + i) * 12 + 6 * b + j] = out[i][j];
+        }
+}
+
+// ---------------------------------------------------------------------------
+// MITC-4 stiffness: 16 lanes per element (one per node pair), 16 elements per
+// 256-thread workgroup.  Lanes 0-3 stage the per-Gauss-point tiles (det J, dH,
+// shear B) in LDS, lanes 0-7 the drilling candidates; then every lane forms
+// its 6x6 block of T^T (k_m + k_b) T (element.py:1073-1084).
+// ---------------------------------------------------------------------------
+struct QuadTile {
+    double R[9];
+    double t, E, nu, kx, ky;
+    double detJ[4];
+    double dH[4][8];
+    double Bg[4][24];
+    double diag[8];
+};
+
+template <bool BLOCKS>
+__global__ __launch_bounds__(256) void ke_quad_kernel(int32_t n, const double* __restrict__ crds_e,
+                                                      const int32_t* __restrict__ cnct,
+                                                      const double* __restrict__ crds_nodes,
+                                                      const double* __restrict__ props,
+                                                      double* __restrict__ ke) {
+    __shared__ QuadTile sh[16];
+    const int le = threadIdx.x >> 4, lane = threadIdx.x & 15;
+    const int e = blockIdx.x * 16 + le;
+    const bool valid = e < n;
+    QuadTile& T = sh[le];
+    if (valid && lane < 4) {
+        double c[12];
+        if (cnct) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int nd = cnct[4 * e + a];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) c[3 * a + k] = crds_nodes[3 * nd + k];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) c[k] = crds_e[12 * e + k];
+        }
+        V3<double> R[3];
+        double xy[8];
+        quad_frame<double>(c, R, xy);
+        QuadGP<double> G;
+        quad_gp<double>(xy, lane, G);
+        T.detJ[lane] = G.detJ;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            T.dH[lane][k] = G.dH[0][k];
+            T.dH[lane][4 + k] = G.dH[1][k];
+        }
+#pragma unroll
+        for (int k = 0; k < 12; ++k) quad_bgamma_col<double>(xy, G, k, T.Bg[lane][k], T.Bg[lane][12 + k]);
+        if (lane == 0) {
+            T.R[0] = R[0].x; T.R[1] = R[0].y; T.R[2] = R[0].z;
+            T.R[3] = R[1].x; T.R[4] = R[1].y; T.R[5] = R[1].z;
+            T.R[6] = R[2].x; T.R[7] = R[2].y; T.R[8] = R[2].z;
+            T.t = props[5 * e + 0];
+            T.E = props[5 * e + 1];
+            T.nu = props[5 * e + 2];
+            T.kx = props[5 * e + 3];
+            T.ky = props[5 * e + 4];
+        }
+    }
+    __syncthreads();
+    if (valid && lane < 8) {
+        double cb, cs;
+        quad_plate_consts<double>(T.t, T.E, T.nu, cb, cs);
+        const int d = (lane / 2) * 3 + 1 + (lane & 1);
+        const int a = d / 3, comp = d % 3;
+        double acc = 0.0;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            double k0 = 0, k1 = 0, k2 = 0;
+            if (comp == 1) { k1 = T.dH[g][4 + a]; k2 = T.dH[g][a]; }
+            else { k0 = -T.dH[g][a]; k2 = -T.dH[g][4 + a]; }
+            const double b0 = T.Bg[g][d], b1 = T.Bg[g][12 + d];
+            acc += (cb_form<double>(cb, T.nu, k0, k1, k2, k0, k1, k2) + cs * (b0 * b0 + b1 * b1)) * T.detJ[g];
+        }
+        T.diag[lane] = acc;
+    }
+    __syncthreads();
+    if (!valid) return;
+    const int a = lane >> 2, b = lane & 3;
+    double cb, cs, Cm[3][3];
+    quad_plate_consts<double>(T.t, T.E, T.nu, cb, cs);
+    quad_cm<double>(T.E, T.nu, T.kx, T.ky, Cm);
+    double mem[2][2] = {{0, 0}, {0, 0}}, ben[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const double dJ = T.detJ[g];
+        const double ha0 = T.dH[g][a], ha1 = T.dH[g][4 + a], hb0 = T.dH[g][b], hb1 = T.dH[g][4 + b];
+        // membrane columns: u -> (h0, 0, h1), v -> (0, h1, h0)
+        const double Ba[2][3] = {{ha0, 0.0, ha1}, {0.0, ha1, ha0}};
+        const double Bb[2][3] = {{hb0, 0.0, hb1}, {0.0, hb1, hb0}};
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                double s = 0.0;
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+                    s += Ba[i][r] * (Cm[r][0] * Bb[j][0] + Cm[r][1] * Bb[j][1] + Cm[r][2] * Bb[j][2]);
+                mem[i][j] += s * dJ;
+            }
+        // bending columns w, theta_x, theta_y
+        const double Ka[3][3] = {{0, 0, 0}, {0, ha1, ha0}, {-ha0, 0, -ha1}};
+        const double Kb[3][3] = {{0, 0, 0}, {0, hb1, hb0}, {-hb0, 0, -hb1}};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double kb_ = cb_form<double>(cb, T.nu, Ka[i][0], Ka[i][1], Ka[i][2], Kb[j][0], Kb[j][1], Kb[j][2]);
+                const double sh_ = cs * (T.Bg[g][3 * a + i] * T.Bg[g][3 * b + j] +
+                                         T.Bg[g][12 + 3 * a + i] * T.Bg[g][12 + 3 * b + j]);
+                ben[i][j] += (kb_ + sh_) * dJ;
+            }
+    }
+    // local 6x6 block, DOFs [u, v, w, rx, ry, rz]
+    double Lb[6][6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) Lb[i][j] = 0.0;
+    Lb[0][0] = T.t * mem[0][0]; Lb[0][1] = T.t * mem[0][1];
+    Lb[1][0] = T.t * mem[1][0]; Lb[1][1] = T.t * mem[1][1];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Lb[2 + i][2 + j] = ben[i][j];
+    if (a == b) {
+        double m = fabs(T.diag[0]);
+#pragma unroll
+        for (int i = 1; i < 8; ++i) m = fmin(m, fabs(T.diag[i]));
+        Lb[5][5] = m / 1000.0;
+    }
+    double R[3][3];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) R[i / 3][i % 3] = T.R[i];
+    double out[6][6];
+#pragma unroll
+    for (int pq = 0; pq < 4; ++pq) {
+        const int p = pq >> 1, q = pq & 1;
+        double tmp[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                tmp[i][j] = Lb[3 * p + i][3 * q + 0] * R[0][j] + Lb[3 * p + i][3 * q + 1] * R[1][j] +
+                            Lb[3 * p + i][3 * q + 2] * R[2][j];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                out[3 * p + i][3 * q + j] = R[0][i] * tmp[0][j] + R[1][i] * tmp[1][j] + R[2][i] * tmp[2][j];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            if (BLOCKS)
+                ke[((int64_t)e * 16 + lane) * 36 + i * 6 + j] = out[i][j];
+            else
+                ke[(int64_t)e * 576 + (6 * a + i) * 24 + 6 * b + j] = out[i][j];
+        }
+}
+
+int launch_ke_beam(int32_t n, const double* crds_e, const int32_t* cnct, const double* crds_nodes,
+                   const double* props, double* ke, int32_t* flags, int32_t* status, hipStream_t s) {
+    if (n <= 0) return JSSO_OK;
+    const int threads = 256;
+    const int64_t total = (int64_t)n * 4;
+    const int grid = (int)((total + threads - 1) / threads);
+    if (cnct)
+        hipLaunchKernelGGL(ke_beam_kernel<true>, dim3(grid), dim3(threads), 0, s, n, crds_e, cnct,
+                           crds_nodes, props, ke, flags, status);
+    else
+        hipLaunchKernelGGL(ke_beam_kernel<false>, dim3(grid), dim3(threads), 0, s, n, crds_e, cnct,
+                           crds_nodes, props, ke, flags, status);
+    JSSO_HIP(hipGetLastError());
+    return JSSO_OK;
+}
+
+int launch_ke_quad(int32_t n, const double* crds_e, const int32_t* cnct, const double* crds_nodes,
+                   const double* props, double* ke, hipStream_t s) {
+    if (n <= 0) return JSSO_OK;
+    const int grid = (n + 15) / 16;
+    if (cnct)
+        hipLaunchKernelGGL(ke_quad_kernel<true>, dim3(grid), dim3(256), 0, s, n, crds_e, cnct,
+                           crds_nodes, props, ke);
+    else
+        hipLaunchKernelGGL(ke_quad_kernel<false>, dim3(grid), dim3(256), 0, s, n, crds_e, cnct,
+                           crds_nodes, props, ke);
+    JSSO_HIP(hipGetLastError());
+    return JSSO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint: one thread per (element, input direction).  The element energy
+// phi = -lam_e^T K_e u_e is evaluated in dual numbers seeded on that input;
+// its tangent is dphi/dp.  Beam directions: 6 coordinates, then E,G,Iy,Iz,J,A.
+// Quad directions: 12 coordinates, then t,E,nu,kx_mod,ky_mod.
+// gbuf[(elem_dir) * nrhs + rhs] receives dphi/dp (coordinate directions);
+// property directions go straight to dprop (layout [e*nprop + k][rhs]).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double load_lam(const double* u, const double* lam, double scale, int64_t i) {
+    return lam ? lam[i] : scale * u[i];
+}
+
+__global__ __launch_bounds__(256) void adj_beam_kernel(int32_t n, const int32_t* __restrict__ cnct,
+                                                       const double* __restrict__ crds,
+                                                       const double* __restrict__ props,
+                                                       const double* __restrict__ u,
+                                                       const double* __restrict__ lam, double lam_scale,
+                                                       int nrhs, double* __restrict__ gbuf,
+                                                       double* __restrict__ dprop) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (int64_t)n * BDIR) return;
+    const int e = (int)(tid / BDIR), p = (int)(tid % BDIR);
+    const int n0 = cnct[2 * e], n1 = cnct[2 * e + 1];
+    Dual c[6], pr[6];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        c[k] = Dual(crds[3 * n0 + k]);
+        c[3 + k] = Dual(crds[3 * n1 + k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) pr[k] = Dual(props[6 * e + k]);
+    if (p < 6) c[p].d = 1.0; else pr[p - 6].d = 1.0;
+    Dual D[3][3], L;
+    const bool special = beam_dircos<Dual>(c, D, L);
+    double Dm[3][3], Dti[3][3];
+    if (special) {
+        double Dv[3][3], Di[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) Dv[i][j] = D[i][j].v;
+        inv3(Dv, Di);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) { Dm[i][j] = Dv[i][j]; Dti[i][j] = Di[j][i]; }
+    }
+    for (int rhs = 0; rhs < nrhs; ++rhs) {
+        double out;
+        if (special && p < 6) {
+            out = __builtin_nan("");
+        } else {
+            Dual ul[12], ll[12];
+            for (int a = 0; a < 2; ++a) {
+                const int nd = a ? n1 : n0;
+                for (int h = 0; h < 2; ++h) {
+                    double uv[3], lv[3];
+                    for (int k = 0; k < 3; ++k) {
+                        const int64_t gi = ((int64_t)6 * nd + 3 * h + k) * nrhs + rhs;
+                        uv[k] = u[gi];
+                        lv[k] = load_lam(u, lam, lam_scale, gi);
+                    }
+                    for (int i = 0; i < 3; ++i) {
+                        if (special) {
+                            ul[6 * a + 3 * h + i] = Dual(Dm[i][0] * uv[0] + Dm[i][1] * uv[1] + Dm[i][2] * uv[2]);
+                            ll[6 * a + 3 * h + i] = Dual(Dti[i][0] * lv[0] + Dti[i][1] * lv[1] + Dti[i][2] * lv[2]);
+                        } else {
+                            ul[6 * a + 3 * h + i] = D[i][0] * uv[0] + D[i][1] * uv[1] + D[i][2] * uv[2];
+                            ll[6 * a + 3 * h + i] = D[i][0] * lv[0] + D[i][1] * lv[1] + D[i][2] * lv[2];
+                        }
+                    }
+                }
+            }
+            Dual phi(0.0);
+            for (int I = 0; I < 12; ++I) {
+                Dual row(0.0);
+                for (int J = 0; J < 12; ++J) {
+                    const Dual k = beam_kloc<Dual>(I, J, L, pr[0], pr[1], pr[2], pr[3], pr[4], pr[5]);
+                    if (k.v != 0.0 || k.d != 0.0) row = row + k * ul[J];
+                }
+                phi = phi + ll[I] * row;
+            }
+            out = -phi.d;
+        }
+        if (p < 6)
+            gbuf[((int64_t)e * BDIR + p) * nrhs + rhs] = out;
+        else if (dprop)
+            dprop[((int64_t)e * 6 + (p - 6)) * nrhs + rhs] = out;
+    }
+}
+
+__global__ __launch_bounds__(128) void adj_quad_kernel(int32_t n, int32_t n_beam, const int32_t* __restrict__ cnct,
+                                                       const double* __restrict__ crds,
+                                                       const double* __restrict__ props,
+                                                       const double* __restrict__ u,
+                                                       const double* __restrict__ lam, double lam_scale,
+                                                       int nrhs, double* __restrict__ gbuf,
+                                                       double* __restrict__ dprop) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (int64_t)n * QDIR) return;
+    const int e = (int)(tid / QDIR), p = (int)(tid % QDIR);
+    int nd[4];
+    Dual c[12];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        nd[a] = cnct[4 * e + a];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) c[3 * a + k] = Dual(crds[3 * nd[a] + k]);
+    }
+    Dual t(props[5 * e + 0]), E(props[5 * e + 1]), nu(props[5 * e + 2]), kx(props[5 * e + 3]),
+        ky(props[5 * e + 4]);
+    if (p < 12) c[p].d = 1.0;
+    else if (p == 12) t.d = 1.0;
+    else if (p == 13) E.d = 1.0;
+    else if (p == 14) nu.d = 1.0;
+    else if (p == 15) kx.d = 1.0;
+    else ky.d = 1.0;
+    V3<Dual> R[3];
+    Dual xy[8];
+    quad_frame<Dual>(c, R, xy);
+    Dual cb, cs, Cm[3][3];
+    quad_plate_consts<Dual>(t, E, nu, cb, cs);
+    quad_cm<Dual>(E, nu, kx, ky, Cm);
+    Dual diag[8];
+    quad_rot_diag<Dual>(xy, cb, cs, nu, diag);
+    const Dual krz = quad_drill<Dual>(diag);
+    for (int rhs = 0; rhs < nrhs; ++rhs) {
+        Dual ul[24], ll[24];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                double uv[3], lv[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int64_t gi = ((int64_t)6 * nd[a] + 3 * h + k) * nrhs + rhs;
+                    uv[k] = u[gi];
+                    lv[k] = load_lam(u, lam, lam_scale, gi);
+                }
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    ul[6 * a + 3 * h + i] = R[i].x * uv[0] + R[i].y * uv[1] + R[i].z * uv[2];
+                    ll[6 * a + 3 * h + i] = R[i].x * lv[0] + R[i].y * lv[1] + R[i].z * lv[2];
+                }
+            }
+        }
+        Dual em(0.0), eb(0.0), es(0.0), ed(0.0);
+        for (int g = 0; g < 4; ++g) {
+            QuadGP<Dual> G;
+            quad_gp<Dual>(xy, g, G);
+            Dual mu[3] = {Dual(0.0), Dual(0.0), Dual(0.0)}, ml[3] = {Dual(0.0), Dual(0.0), Dual(0.0)};
+            Dual ku[3] = {Dual(0.0), Dual(0.0), Dual(0.0)}, kl[3] = {Dual(0.0), Dual(0.0), Dual(0.0)};
+            Dual gu[2] = {Dual(0.0), Dual(0.0)}, gl[2] = {Dual(0.0), Dual(0.0)};
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const Dual h0 = G.dH[0][a], h1 = G.dH[1][a];
+                mu[0] = mu[0] + h0 * ul[6 * a];
+                mu[1] = mu[1] + h1 * ul[6 * a + 1];
+                mu[2] = mu[2] + h1 * ul[6 * a] + h0 * ul[6 * a + 1];
+                ml[0] = ml[0] + h0 * ll[6 * a];
+                ml[1] = ml[1] + h1 * ll[6 * a + 1];
+                ml[2] = ml[2] + h1 * ll[6 * a] + h0 * ll[6 * a + 1];
+                ku[0] = ku[0] - h0 * ul[6 * a + 4];
+                ku[1] = ku[1] + h1 * ul[6 * a + 3];
+                ku[2] = ku[2] + h0 * ul[6 * a + 3] - h1 * ul[6 * a + 4];
+                kl[0] = kl[0] - h0 * ll[6 * a + 4];
+                kl[1] = kl[1] + h1 * ll[6 * a + 3];
+                kl[2] = kl[2] + h0 * ll[6 * a + 3] - h1 * ll[6 * a + 4];
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    Dual b0, b1;
+                    quad_bgamma_col<Dual>(xy, G, 3 * a + cc, b0, b1);
+                    gu[0] = gu[0] + b0 * ul[6 * a + 2 + cc];
+                    gu[1] = gu[1] + b1 * ul[6 * a + 2 + cc];
+                    gl[0] = gl[0] + b0 * ll[6 * a + 2 + cc];
+                    gl[1] = gl[1] + b1 * ll[6 * a + 2 + cc];
+                }
+            }
+            Dual mform(0.0);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) mform = mform + ml[r] * (Cm[r][0] * mu[0] + Cm[r][1] * mu[1] + Cm[r][2] * mu[2]);
+            em = em + mform * G.detJ;
+            eb = eb + cb_form<Dual>(cb, nu, kl[0], kl[1], kl[2], ku[0], ku[1], ku[2]) * G.detJ;
+            es = es + cs * (gl[0] * gu[0] + gl[1] * gu[1]) * G.detJ;
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a) ed = ed + ll[6 * a + 5] * ul[6 * a + 5];
+        const Dual phi = t * em + eb + es + krz * ed;
+        const double out = -phi.d;
+        if (p < 12)
+            gbuf[((int64_t)n_beam * BDIR + (int64_t)e * QDIR + p) * nrhs + rhs] = out;
+        else if (dprop)
+            dprop[((int64_t)e * 5 + (p - 12)) * nrhs + rhs] = out;
+    }
+}
+
+// node gradient: sum of the coordinate derivatives of the elements incident to
+// each node, in a fixed order (no atomics).
+__global__ void gather_grad_kernel(int32_t n_nodes, int nrhs, const int32_t* __restrict__ nptr,
+                                   const int32_t* __restrict__ nsrc, const double* __restrict__ gbuf,
+                                   double* __restrict__ dcrds) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (int64_t)n_nodes * 3 * nrhs) return;
+    const int rhs = (int)(tid % nrhs);
+    const int64_t nc = tid / nrhs;
+    const int node = (int)(nc / 3), comp = (int)(nc % 3);
+    double s = 0.0;
+    for (int k = nptr[node]; k < nptr[node + 1]; ++k) s += gbuf[((int64_t)nsrc[k] + comp) * nrhs + rhs];
+    dcrds[tid] = s;
+}
+
+int launch_adjoint(Plan* p, const double* crds, const double* prop_beam, const double* prop_quad,
+                        const double* u, const double* lam, double lam_scale, int nrhs, double* gbuf,
+                        double* dprop_beam, double* dprop_quad, hipStream_t s) {
+    JSSO_REQUIRE(p->on_device, JSSO_EINVAL, "plan not uploaded to the device");
+    if (p->n_beam > 0) {
+        const int64_t total = (int64_t)p->n_beam * BDIR;
+        hipLaunchKernelGGL(adj_beam_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                           p->n_beam, p->d_cnct_beam, crds, prop_beam, u, lam, lam_scale, nrhs, gbuf,
+                           dprop_beam);
+        JSSO_HIP(hipGetLastError());
+    }
+    if (p->n_quad > 0) {
+        const int64_t total = (int64_t)p->n_quad * QDIR;
+        hipLaunchKernelGGL(adj_quad_kernel, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s,
+                           p->n_quad, p->n_beam, p->d_cnct_quad, crds, prop_quad, u, lam, lam_scale,
+                           nrhs, gbuf, dprop_quad);
+        JSSO_HIP(hipGetLastError());
+    }
+    return JSSO_OK;
+}
+
+int launch_gather_grad(Plan* p, const double* gbuf, int nrhs, double* dcrds, hipStream_t s) {
+    JSSO_REQUIRE(p->on_device, JSSO_EINVAL, "plan not uploaded to the device");
+    if (!dcrds || p->n_nodes == 0) return JSSO_OK;
+    const int64_t total = (int64_t)p->n_nodes * 3 * nrhs;
+    hipLaunchKernelGGL(gather_grad_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       p->n_nodes, nrhs, p->d_nptr, p->d_nsrc, gbuf, dcrds);
+    JSSO_HIP(hipGetLastError());
+    return JSSO_OK;
+}
+
+}  // namespace jsso

@@ -1,0 +1,114 @@
+// Internal declarations of libjsso_hip.so: the plan object, error handling and
+// the kernel launchers shared between translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/jsso.h"
+
+namespace jsso {
+
+void set_error(const char* fmt, ...);
+
+#define JSSO_HIP(call)                                                                  \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ::jsso::set_error("%s:%d: %s: %s", __FILE__, __LINE__, #call,               \
+                              hipGetErrorString(e_));                                   \
+            return JSSO_EHIP;                                                           \
+        }                                                                               \
+    } while (0)
+
+#define JSSO_REQUIRE(cond, code, ...)     \
+    do {                                  \
+        if (!(cond)) {                    \
+            ::jsso::set_error(__VA_ARGS__); \
+            return (code);                \
+        }                                 \
+    } while (0)
+
+constexpr int BDIR = 12;   // beam adjoint directions: 6 coordinates + 6 properties
+constexpr int QDIR = 17;   // quad adjoint directions: 12 coordinates + 5 properties
+constexpr int ROWS_PER_WG = 64;   // block rows per solver workgroup (384 scalar rows)
+constexpr int SOLVER_WG = 6 * ROWS_PER_WG;
+
+// Device buffer owned by the plan.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Plan {
+    // sizes
+    int32_t n_sys = 0, n_nodes = 0, n_beam = 0, n_quad = 0, n_cons = 0;
+    int64_t nnzb = 0;      // blocks of K (block CSR incl. diagonal)
+    int64_t nnzb_off = 0;  // off-diagonal blocks (solver matrix)
+    int64_t n_contrib = 0; // element blocks = 4*n_beam + 16*n_quad
+    // host copies (symbolic analysis runs on the host; uploaded on first use)
+    std::vector<int32_t> h_node_off, h_rowptr, h_colind;
+    std::vector<int32_t> h_browof, h_diag, h_offrowptr, h_offcol, h_offsrc, h_cptr, h_csrc, h_nptr, h_nsrc;
+    std::vector<int32_t> h_cnct_beam, h_cnct_quad, h_wg_sys, h_wg_row0, h_wg_row1, h_sys_wg_off;
+    std::vector<uint8_t> h_fixed;
+    bool on_device = false;
+    // pattern (device)
+    int32_t *d_rowptr = nullptr, *d_colind = nullptr, *d_browof = nullptr, *d_diag = nullptr;
+    int32_t *d_offrowptr = nullptr, *d_offcol = nullptr, *d_offsrc = nullptr;
+    int32_t *d_cptr = nullptr, *d_csrc = nullptr;       // block -> element-block contributions
+    int32_t *d_nptr = nullptr, *d_nsrc = nullptr;       // node -> adjoint output offsets
+    int32_t *d_cnct_beam = nullptr, *d_cnct_quad = nullptr;
+    uint8_t* d_fixed = nullptr;                           // per DOF
+    int32_t *d_wg_sys = nullptr, *d_wg_row0 = nullptr, *d_wg_row1 = nullptr;
+    int32_t* d_sys_wg_off = nullptr;
+    int32_t n_wg = 0;
+    // numeric state (device)
+    double* d_ke = nullptr;       // element blocks, 36 doubles each
+    double* d_K = nullptr;        // K, 36 per block (unconstrained)
+    double* d_Linv = nullptr;     // inverse Cholesky factor of each constrained diagonal block
+    double* d_Ks64 = nullptr;     // scaled off-diagonal blocks, FP64
+    float* d_Ks32 = nullptr;      // scaled off-diagonal blocks, FP32
+    double* d_Minv = nullptr;     // block-Jacobi inverse (plain path)
+    double* d_Kc = nullptr;       // constrained K incl. diagonal (plain path)
+    int32_t* d_status = nullptr;  // device error flags
+    bool assembled = false;
+    // solver workspace, grown on demand
+    std::vector<DBuf> ws;
+    int ws_nrhs = 0;
+    std::vector<void*> allocs;    // everything to free
+};
+
+template <typename T>
+int dalloc(Plan* p, T** ptr, size_t count) {
+    *ptr = nullptr;
+    if (count == 0) return JSSO_OK;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+        return JSSO_ENOMEM;
+    }
+    p->allocs.push_back(q);
+    *ptr = static_cast<T*>(q);
+    return JSSO_OK;
+}
+
+// kernels launched from the API layer
+int launch_ke_beam(int32_t n, const double* crds_e, const int32_t* cnct, const double* crds_nodes,
+                   const double* props, double* ke, int32_t* flags, int32_t* status, hipStream_t s);
+int launch_ke_quad(int32_t n, const double* crds_e, const int32_t* cnct, const double* crds_nodes,
+                   const double* props, double* ke, hipStream_t s);
+int launch_assemble(Plan* p, hipStream_t s);
+int launch_adjoint(Plan* p, const double* crds, const double* prop_beam, const double* prop_quad,
+                   const double* u, const double* lam, double lam_scale, int nrhs, double* gbuf,
+                   double* dprop_beam, double* dprop_quad, hipStream_t s);
+int launch_gather_grad(Plan* p, const double* gbuf, int nrhs, double* dcrds, hipStream_t s);
+int launch_spmv_full(Plan* p, const double* K, const double* x, double* y, hipStream_t s);
+int solve_batched(Plan* p, const double* f, double* u, int nrhs, bool mixed, double tol,
+                  int maxit, jsso_info* info, hipStream_t s);
+int solve_plain(Plan* p, const double* f, double* u, int nrhs, double tol, int maxit,
+                jsso_info* info, hipStream_t s);
+
+}  // namespace jsso

@@ -1,0 +1,155 @@
+"""Device-side finite-element plan: one or many structures packed together.
+
+A :class:`FEPlan` owns a ``jsso_plan`` (libjsso_hip.so): the 6x6 block-CSR
+pattern of K, the element-to-block contribution map and the node adjacency.
+It is built once per topology (connectivity + supports) and then reused for
+any coordinates / section properties, which is how ``SSO_model`` evaluates
+many designs of one model.
+"""
+from __future__ import annotations
+
+import ctypes
+import warnings
+
+import numpy as np
+import torch
+
+from . import _native as nat
+
+DEFAULT_TOL = 1e-10
+DEFAULT_MAXIT = 200000
+
+
+class FEPlan:
+    """Symbolic plan for ``n_sys`` structures packed into one node numbering.
+
+    Parameters are host arrays: ``node_off`` (n_sys+1) node offsets,
+    ``cnct_beam`` (nb,2) and ``cnct_quad`` (nq,4) global node ids, ``known_id``
+    global constrained DOF ids.
+    """
+
+    def __init__(self, node_off, cnct_beam, cnct_quad, known_id):
+        L = nat.load_library()
+        self.node_off = np.ascontiguousarray(node_off, dtype=np.int32)
+        self.cnct_beam = np.ascontiguousarray(np.asarray(cnct_beam, dtype=np.int32).reshape(-1, 2))
+        self.cnct_quad = np.ascontiguousarray(np.asarray(cnct_quad, dtype=np.int32).reshape(-1, 4))
+        self.known_id = np.ascontiguousarray(np.asarray(known_id, dtype=np.int32).ravel())
+        self.n_sys = len(self.node_off) - 1
+        self.n_nodes = int(self.node_off[-1])
+        self.ndof = 6 * self.n_nodes
+        self.n_beam = len(self.cnct_beam)
+        self.n_quad = len(self.cnct_quad)
+        h = ctypes.c_void_p()
+        nat.check(L.jsso_plan_create(self.n_sys, nat.ptr(self.node_off), self.n_beam, nat.ptr(self.cnct_beam),
+                                     self.n_quad, nat.ptr(self.cnct_quad), len(self.known_id),
+                                     nat.ptr(self.known_id), ctypes.byref(h)))
+        self._h = h
+        sizes = np.zeros(6, np.int64)
+        nat.check(L.jsso_plan_sizes(self._h, nat.ptr(sizes)))
+        self.nnzb = int(sizes[1])
+        self.last_info = None
+        self._assembled_key = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and nat._lib is not None:
+            nat._lib.jsso_plan_destroy(h)
+            self._h = None
+
+    # -- pattern ---------------------------------------------------------
+    def pattern(self):
+        """Host copy of the block pattern (rowptr (n+1,), colind (nnzb,)) as int32."""
+        rowptr = np.zeros(self.n_nodes + 1, np.int32)
+        colind = np.zeros(self.nnzb, np.int32)
+        nat.check(nat.load_library().jsso_plan_pattern(self._h, nat.ptr(rowptr), nat.ptr(colind)))
+        return rowptr, colind
+
+    # -- numerics --------------------------------------------------------
+    def assemble(self, crds: torch.Tensor, prop_beam: torch.Tensor | None, prop_quad: torch.Tensor | None,
+                 export_K: bool = False):
+        """Assemble K (FP64) for node coordinates (n,3) and properties; optionally return K values (nnzb,6,6)."""
+        L = nat.lib()
+        crds = _dev64(crds)
+        pb = _dev64(prop_beam) if self.n_beam else None
+        pq = _dev64(prop_quad) if self.n_quad else None
+        assert crds.shape == (self.n_nodes, 3)
+        Kout = torch.empty((self.nnzb, 6, 6), dtype=torch.float64, device=crds.device) if export_K else None
+        nat.check(L.jsso_assemble(self._h, nat.ptr(crds), nat.ptr(pb), nat.ptr(pq), nat.ptr(Kout), nat.stream_ptr()))
+        self._keep = (crds, pb, pq)
+        return Kout
+
+    def solve(self, f: torch.Tensor, precision: str = "mixed", tol: float = DEFAULT_TOL,
+              maxit: int = DEFAULT_MAXIT, warn: bool = True) -> torch.Tensor:
+        """Solve K u = f (u = 0 at supports).  f: (ndof,) or (ndof, nrhs) FP64 on the device."""
+        L = nat.lib()
+        f = _dev64(f)
+        vec = f.dim() == 1
+        f2 = f.reshape(self.ndof, -1).contiguous()
+        nrhs = f2.shape[1]
+        u = torch.empty_like(f2)
+        info = nat.Info()
+        rc = L.jsso_solve(self._h, nat.ptr(f2), nat.ptr(u), nrhs, nat.PREC[precision], float(tol), int(maxit),
+                          ctypes.byref(info), nat.stream_ptr())
+        nat.check(rc, allow_not_converged=True)
+        self.last_info = info.as_dict()
+        if rc == nat.JSSO_NOT_CONVERGED and warn:
+            warnings.warn(f"CG did not converge: {self.last_info}", RuntimeWarning, stacklevel=2)
+        return u.reshape(f.shape) if vec else u
+
+    def adjoint(self, crds, prop_beam, prop_quad, u, lam=None, lam_scale: float = 0.5, want=("crds", "props")):
+        """g = -sum_e lam_e^T dK_e/dp u_e for coordinates and all element properties.
+
+        lam=None uses lam = lam_scale * u (compliance: lam_scale = 1/2).
+        Returns (dcrds (n,3[,nrhs]), dprop_beam (nb,6[,nrhs]), dprop_quad (nq,5[,nrhs])).
+        """
+        L = nat.lib()
+        crds = _dev64(crds)
+        pb = _dev64(prop_beam) if self.n_beam else None
+        pq = _dev64(prop_quad) if self.n_quad else None
+        u = _dev64(u)
+        vec = u.dim() == 1
+        u2 = u.reshape(self.ndof, -1).contiguous()
+        nrhs = u2.shape[1]
+        lam2 = None if lam is None else _dev64(lam).reshape(self.ndof, -1).contiguous()
+        dev = crds.device
+        dc = torch.empty((self.n_nodes, 3, nrhs), dtype=torch.float64, device=dev) if "crds" in want else None
+        db = torch.empty((self.n_beam, 6, nrhs), dtype=torch.float64, device=dev) if ("props" in want and self.n_beam) else None
+        dq = torch.empty((self.n_quad, 5, nrhs), dtype=torch.float64, device=dev) if ("props" in want and self.n_quad) else None
+        nat.check(L.jsso_adjoint_grad(self._h, nat.ptr(crds), nat.ptr(pb), nat.ptr(pq), nat.ptr(u2), nat.ptr(lam2),
+                                      float(lam_scale), nrhs, nat.ptr(dc), nat.ptr(db), nat.ptr(dq),
+                                      nat.stream_ptr()))
+        if vec:
+            dc = None if dc is None else dc[..., 0]
+            db = None if db is None else db[..., 0]
+            dq = None if dq is None else dq[..., 0]
+        return dc, db, dq
+
+    def spmv(self, x: torch.Tensor) -> torch.Tensor:
+        """y = K x with the assembled (unconstrained) K."""
+        L = nat.lib()
+        x = _dev64(x)
+        y = torch.empty_like(x)
+        nat.check(L.jsso_spmv(self._h, nat.ptr(x), nat.ptr(y), nat.stream_ptr()))
+        return y
+
+    def K_scalar_csr(self, Kvals: torch.Tensor):
+        """Expand block values (nnzb,6,6) to a host scipy CSR matrix."""
+        rowptr, colind = self.pattern()
+        return blocks_to_csr(rowptr, colind, Kvals.detach().cpu().numpy())
+
+
+def blocks_to_csr(rowptr, colind, vals):
+    """Scalar CSR (scipy.sparse.csr_matrix, explicit zeros kept) of a 6x6 block-CSR matrix."""
+    import scipy.sparse as sp
+    n = len(rowptr) - 1
+    B = sp.bsr_matrix((np.asarray(vals, dtype=np.float64).reshape(-1, 6, 6), np.asarray(colind),
+                       np.asarray(rowptr)), shape=(6 * n, 6 * n))
+    return B.tocsr()
+
+
+def _dev64(t):
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor):
+        t = torch.as_tensor(np.asarray(t, dtype=np.float64))
+    return t.to(device=nat.device(), dtype=torch.float64).contiguous()

@@ -1,0 +1,168 @@
+"""GPU parity of the HIP kernels against the oracle, through the C ABI.
+
+Element stiffness, assembly (bit-exact pattern, values to rounding), the
+three solver paths and the adjoint kernel, on seeded inputs small enough for
+the oracle to finish in seconds.
+"""
+import numpy as np
+import pytest
+import torch
+
+from jaxsso_amd import workloads as wl
+from jaxsso_amd import _native as nat
+from jaxsso_amd.plan import FEPlan, blocks_to_csr
+from oracle import elements as el
+from oracle import gradient as og
+from oracle import solve as osolve
+from oracle import system
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def random_quads(n, seed, distort=0.25):
+    """Quads: a square of random size, corners perturbed in-plane and out of plane, rotated."""
+    rng = np.random.default_rng(seed)
+    base = np.array([[0.5, 0.5, 0], [-0.5, 0.5, 0], [-0.5, -0.5, 0], [0.5, -0.5, 0.0]])
+    base = base[[2, 3, 0, 1]]  # [BL, BR, TR, TL]
+    out = np.zeros((n, 12))
+    for e in range(n):
+        h = rng.uniform(0.1, 3.0)
+        X = base * h + rng.uniform(-distort, distort, size=(4, 3)) * h * np.array([1, 1, 0.3])
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        out[e] = (X @ q.T + rng.normal(size=3) * 5).ravel()
+    props = np.stack([rng.uniform(0.05, 0.3, n), rng.uniform(1e9, 5e10, n), rng.uniform(0.0, 0.45, n),
+                      rng.uniform(0.5, 1.5, n), rng.uniform(0.5, 1.5, n)], 1)
+    return out, props
+
+
+def random_beams(n, seed):
+    rng = np.random.default_rng(seed)
+    a = rng.normal(size=(n, 3)) * 3
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    b = a + d * rng.uniform(0.2, 5.0, size=(n, 1))
+    props = np.stack([rng.uniform(1e9, 3e11, n), rng.uniform(1e9, 1e11, n), rng.uniform(1e-6, 1e-4, n),
+                      rng.uniform(1e-6, 1e-4, n), rng.uniform(1e-6, 1e-4, n), rng.uniform(1e-3, 1e-2, n)], 1)
+    return np.concatenate([a, b], 1), props
+
+
+def gpu_ke_quad(crds, props):
+    L = nat.lib()
+    c = torch.tensor(crds, dtype=torch.float64, device="cuda")
+    p = torch.tensor(props, dtype=torch.float64, device="cuda")
+    ke = torch.empty((len(crds), 24, 24), dtype=torch.float64, device="cuda")
+    nat.check(L.jsso_element_ke_quad(len(crds), nat.ptr(c), nat.ptr(p), nat.ptr(ke), nat.stream_ptr()))
+    return ke.cpu().numpy()
+
+
+def gpu_ke_beam(crds, props):
+    L = nat.lib()
+    c = torch.tensor(crds, dtype=torch.float64, device="cuda")
+    p = torch.tensor(props, dtype=torch.float64, device="cuda")
+    ke = torch.empty((len(crds), 12, 12), dtype=torch.float64, device="cuda")
+    fl = torch.empty(len(crds), dtype=torch.int32, device="cuda")
+    nat.check(L.jsso_element_ke_beam(len(crds), nat.ptr(c), nat.ptr(p), nat.ptr(ke), nat.ptr(fl), nat.stream_ptr()))
+    return ke.cpu().numpy(), fl.cpu().numpy()
+
+
+def test_quad_ke_matches_oracle():
+    crds, props = random_quads(500, 1)
+    st = wl.cfg1()
+    c1 = el.gather_crds(st.crds, st.cnct_quads[:300])
+    crds = np.concatenate([crds, c1])
+    props = np.concatenate([props, st.prop_quads[:300]])
+    K = gpu_ke_quad(crds, props)
+    R = el.quad_Ke(crds, props)
+    err = np.abs(K - R).max(axis=(1, 2)) / np.abs(R).max(axis=(1, 2))
+    assert err.max() < 1e-12, err.max()
+
+
+def test_beam_ke_matches_oracle_including_y_parallel_branch():
+    crds, props = random_beams(400, 2)
+    crds[:5, 3] = crds[:5, 0]       # parallel to global Y: x and z equal
+    crds[:5, 5] = crds[:5, 2]
+    K, fl = gpu_ke_beam(crds, props)
+    R = el.beam_Ke(crds, props)
+    _, yflag = el.beam_dircos(crds)
+    assert np.array_equal(fl.astype(bool), yflag) and yflag[:5].all()
+    err = np.abs(K - R).max(axis=(1, 2)) / np.abs(R).max(axis=(1, 2))
+    assert err.max() < 1e-12, err.max()
+
+
+@pytest.mark.parametrize("case", ["cfg0", "cfg1", "barrel", "arch"])
+def test_assembly_matches_oracle(case):
+    st = getattr(wl, case)() if case.startswith("cfg") else getattr(wl, "notebook_" + case)()
+    plan = FEPlan([0, st.n_node], st.cnct_beamcols, st.cnct_quads, st.known_id)
+    Kv = plan.assemble(torch.tensor(st.crds), torch.tensor(st.prop_beamcols), torch.tensor(st.prop_quads),
+                       export_K=True)
+    A = plan.K_scalar_csr(Kv)
+    R = system.assemble_K(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads)
+    assert np.array_equal(A.indptr, R.indptr) and np.array_equal(A.indices, R.indices)
+    assert np.abs(A.data - R.data).max() <= 1e-12 * np.abs(R.data).max()
+    # y = K x through the C ABI
+    x = np.random.default_rng(0).normal(size=st.ndof)
+    y = plan.spmv(torch.tensor(x)).cpu().numpy()
+    assert rel(y, R @ x) < 1e-13
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "plain"])
+@pytest.mark.parametrize("case", ["cfg0", "cfg1", "barrel", "arch"])
+def test_solve_matches_oracle(case, precision):
+    st = getattr(wl, case)() if case.startswith("cfg") else getattr(wl, "notebook_" + case)()
+    plan = FEPlan([0, st.n_node], st.cnct_beamcols, st.cnct_quads, st.known_id)
+    plan.assemble(torch.tensor(st.crds), torch.tensor(st.prop_beamcols), torch.tensor(st.prop_quads))
+    u = plan.solve(torch.tensor(st.nodal_loads), precision=precision, tol=1e-10).cpu().numpy()
+    K = system.assemble_K(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads)
+    ur = osolve.solve_reduced(K, st.known_id, st.nodal_loads)
+    info = plan.last_info
+    assert info["converged"] == info["n_systems"] == 1, info
+    assert rel(u, ur) < 1e-8, (rel(u, ur), info)
+    C, Cr = 0.5 * st.nodal_loads @ u, 0.5 * st.nodal_loads @ ur
+    assert abs(C - Cr) / abs(Cr) < 1e-10
+    assert np.all(u[st.known_id] == 0.0)
+
+
+@pytest.mark.parametrize("case", ["cfg0", "cfg1", "barrel"])
+def test_adjoint_kernel_matches_complex_step(case):
+    """Feed the oracle's u to the HIP adjoint: per-node and per-element gradients
+    match the oracle's complex-step element derivatives."""
+    st = getattr(wl, case)() if case.startswith("cfg") else getattr(wl, "notebook_" + case)()
+    plan = FEPlan([0, st.n_node], st.cnct_beamcols, st.cnct_quads, st.known_id)
+    K = system.assemble_K(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads)
+    u = osolve.solve_reduced(K, st.known_id, st.nodal_loads)
+    eg = og.element_gradients(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads,
+                              u, 0.5 * u)
+    gref = og.node_gradient(st.n_node, st.cnct_beamcols, st.cnct_quads, eg)
+    dc, db, dq = plan.adjoint(torch.tensor(st.crds), torch.tensor(st.prop_beamcols), torch.tensor(st.prop_quads),
+                              torch.tensor(u), lam=None, lam_scale=0.5)
+    scale = np.abs(gref).max()
+    assert np.abs(dc.cpu().numpy() - gref).max() <= 1e-10 * scale
+    if st.n_quad:
+        r = eg["quad_props"]
+        assert np.abs(dq.cpu().numpy() - r).max() <= 1e-10 * np.abs(r).max(axis=0).clip(1e-300).max()
+    if st.n_beamcol:
+        r = eg["beam_props"]
+        g = db.cpu().numpy()
+        for k in range(6):
+            assert np.abs(g[:, k] - r[:, k]).max() <= 1e-10 * np.abs(r[:, k]).max()
+
+
+def test_adjoint_general_lambda():
+    """A general adjoint vector (not u/2) through the same kernel."""
+    st = wl.notebook_barrel()
+    plan = FEPlan([0, st.n_node], st.cnct_beamcols, st.cnct_quads, st.known_id)
+    rng = np.random.default_rng(5)
+    u = rng.normal(size=st.ndof)
+    lam = rng.normal(size=st.ndof)
+    eg = og.element_gradients(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads, u, lam)
+    gref = og.node_gradient(st.n_node, st.cnct_beamcols, st.cnct_quads, eg)
+    dc, _, dq = plan.adjoint(torch.tensor(st.crds), None, torch.tensor(st.prop_quads), torch.tensor(u),
+                             lam=torch.tensor(lam))
+    assert np.abs(dc.cpu().numpy() - gref).max() <= 1e-9 * np.abs(gref).max()
+    assert np.abs(dq.cpu().numpy() - eg["quad_props"]).max() <= 1e-9 * np.abs(eg["quad_props"]).max()
