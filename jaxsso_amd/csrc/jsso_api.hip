@@ -424,4 +424,10 @@ int jsso_spmv(jsso_plan* plan, const double* d_x, double* d_y, void* stream) {
     return launch_spmv_full(p, p->d_K, d_x, d_y, (hipStream_t)stream);
 }
 
+int jsso_timing_enable(int32_t every) { return timing_enable(every); }
+
+int jsso_timing_read(double* h_avg_ms, int64_t* h_samples, int32_t reset) {
+    return timing_read(h_avg_ms, h_samples, reset);
+}
+
 }  // extern "C"

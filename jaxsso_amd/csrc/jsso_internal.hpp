@@ -108,6 +108,8 @@ int launch_gather_grad(Plan* p, const double* gbuf, int nrhs, double* dcrds, hip
 int launch_spmv_full(Plan* p, const double* K, const double* x, double* y, hipStream_t s);
 int solve_batched(Plan* p, const double* f, double* u, int nrhs, bool mixed, double tol,
                   int maxit, jsso_info* info, hipStream_t s);
+int timing_enable(int every);
+int timing_read(double* avg_ms, int64_t* samples, int reset);
 int solve_plain(Plan* p, const double* f, double* u, int nrhs, double tol, int maxit,
                 jsso_info* info, hipStream_t s);
 

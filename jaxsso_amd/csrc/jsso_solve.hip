@@ -26,6 +26,65 @@ namespace jsso {
 
 constexpr int NWAVES = SOLVER_WG / 64;
 
+// ---------------------------------------------------------------------------
+// sampled event timing of cg1 (jsso_timing_enable / jsso_timing_read)
+// ---------------------------------------------------------------------------
+struct KernelTiming {
+    int every = 0;
+    int64_t counter = 0, samples = 0;
+    double total_ms = 0.0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool, pending;
+};
+static KernelTiming g_timing;
+
+static void timing_begin(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev) {
+    ev = nullptr;
+    if (g_timing.every <= 0 || (g_timing.counter++ % g_timing.every) != 0) return;
+    std::pair<hipEvent_t, hipEvent_t> e;
+    if (!g_timing.pool.empty()) {
+        e = g_timing.pool.back();
+        g_timing.pool.pop_back();
+    } else if (hipEventCreate(&e.first) != hipSuccess || hipEventCreate(&e.second) != hipSuccess) {
+        return;
+    }
+    g_timing.pending.push_back(e);
+    ev = &g_timing.pending.back();
+    (void)hipEventRecord(ev->first, s);
+}
+
+static void timing_end(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>* ev) {
+    if (ev) (void)hipEventRecord(ev->second, s);
+}
+
+// after a stream synchronisation: fold the completed samples
+static void timing_collect() {
+    for (auto& e : g_timing.pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
+            g_timing.total_ms += ms;
+            g_timing.samples++;
+        }
+        g_timing.pool.push_back(e);
+    }
+    g_timing.pending.clear();
+}
+
+int timing_enable(int every) {
+    g_timing.every = every > 0 ? every : 0;
+    g_timing.counter = 0;
+    return JSSO_OK;
+}
+
+int timing_read(double* avg_ms, int64_t* samples, int reset) {
+    if (avg_ms) *avg_ms = g_timing.samples ? g_timing.total_ms / g_timing.samples : 0.0;
+    if (samples) *samples = g_timing.samples;
+    if (reset) {
+        g_timing.total_ms = 0.0;
+        g_timing.samples = 0;
+    }
+    return JSSO_OK;
+}
+
 struct SysState {
     double *bnorm2, *rr, *rr_prev, *alpha, *beta, *relres;
     int32_t *active, *counter;
@@ -428,8 +487,11 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
     while (!done && it < maxit) {
         const int steps = std::min(replace_every, maxit - it);
         for (int j = 0; j < steps; ++j) {
+            std::pair<hipEvent_t, hipEvent_t>* ev;
+            timing_begin(s, ev);
             hipLaunchKernelGGL((k_cg1<MT, VT, NR>), grid, block, 0, s, rows, w.st, pl->d_offrowptr,
                                pl->d_offcol, A, (const VT*)rlo, p, q);
+            timing_end(s, ev);
             hipLaunchKernelGGL((k_cg2<VT, NR>), grid, block, 0, s, rows, w.st, (const VT*)p, (const VT*)q,
                                xlo, rlo);
         }
@@ -442,6 +504,7 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
         ++repl;
         JSSO_HIP(hipMemcpyAsync(act.data(), w.st.active, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, s));
         JSSO_HIP(hipStreamSynchronize(s));
+        timing_collect();
         done = true;
         for (int k = 0; k < nsys; ++k)
             if (act[k]) { done = false; break; }

@@ -1,0 +1,64 @@
+"""Linear solvers -- the API of ``JaxSSO/solver.py``.
+
+The reference offers a dense LU (``jax_dense_solve``, solver.py:75-93), a GPU
+sparse direct solve (``jax_sparse_solve``, :102-168) and a host SuperLU through
+``pure_callback`` (``sci_sparse_solve``, :176-250), each on the Lagrange-
+augmented system.  Here every sparse solve is the GPU conjugate-gradient solver
+of ``libjsso_hip.so`` (block-Jacobi scaled; FP32 recurrences with FP64
+reliable updates by default), and the dense solve is an FP64 direct solve of
+the constrained system on the GPU.  All return ``u_aug = [u; mu]`` with the
+Lagrange multipliers ``mu = f_c - (K u)_c`` (support reactions), as the
+reference's augmented solve does.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+DEFAULT_PRECISION = "mixed"
+DEFAULT_TOL = 1e-10
+DEFAULT_MAXIT = 200000
+
+
+def _finish(K_aug, f_aug, u):
+    ndof = K_aug.plan.ndof
+    known = torch.as_tensor(K_aug.known_id, device=u.device)
+    Ku = K_aug.plan.spmv(u)
+    mu = f_aug[:ndof][known] - Ku[known]
+    return torch.cat([u, mu])
+
+
+def _cg(K_aug, f_aug, precision, tol, maxit):
+    ndof = K_aug.plan.ndof
+    f = torch.as_tensor(f_aug, dtype=torch.float64, device=K_aug.values.device)[:ndof].contiguous()
+    crds, pb, pq = K_aug._inputs
+    K_aug.plan.assemble(crds, pb, pq)   # the plan may be shared with another evaluation
+    u = K_aug.plan.solve(f, precision=precision, tol=tol, maxit=maxit)
+    return _finish(K_aug, torch.as_tensor(f_aug, dtype=torch.float64, device=u.device), u)
+
+
+def sci_sparse_solve(K_aug, f_aug, precision=None, tol=None, maxit=None):
+    """GPU CG solve of the augmented system (replaces host SuperLU, solver.py:176-218)."""
+    return _cg(K_aug, f_aug, precision or DEFAULT_PRECISION, tol or DEFAULT_TOL, maxit or DEFAULT_MAXIT)
+
+
+def jax_sparse_solve(K_aug, f_aug, precision=None, tol=None, maxit=None):
+    """GPU CG solve (replaces cuSolverSp spsolve, solver.py:102-125)."""
+    return _cg(K_aug, f_aug, precision or DEFAULT_PRECISION, tol or DEFAULT_TOL, maxit or DEFAULT_MAXIT)
+
+
+def jax_dense_solve(K_aug, f_aug, precision=None, tol=None, maxit=None):
+    """Dense FP64 direct solve of the constrained system on the GPU (solver.py:75-93).
+
+    Like the reference's dense path it forms a dense matrix and is meant for
+    small models (memory grows as ndof^2).
+    """
+    ndof = K_aug.plan.ndof
+    K = torch.as_tensor(K_aug.to_scipy()[:ndof, :ndof].toarray(), device=K_aug.values.device)
+    f = torch.as_tensor(f_aug, dtype=torch.float64, device=K.device)[:ndof]
+    free = np.ones(ndof, bool)
+    free[K_aug.known_id] = False
+    fi = torch.as_tensor(np.nonzero(free)[0], device=K.device)
+    u = torch.zeros(ndof, dtype=torch.float64, device=K.device)
+    u[fi] = torch.linalg.solve(K[fi][:, fi], f[fi])
+    return _finish(K_aug, f_aug, u)

@@ -133,10 +133,12 @@ def cfg0() -> Structure:
     return gridshell(10, cfg=0)
 
 
-def cfg1() -> Structure:
-    """40x40 MITC-4 shell on 10 m x 10 m, four fully fixed corners, -2 kN/node."""
+def cfg1(design: int = 0) -> Structure:
+    """40x40 MITC-4 shell on 10 m x 10 m, four fully fixed corners, -2 kN/node.
+
+    ``design`` > 0 draws another seeded thickness field of the same shell."""
     n, L = 40, 10.0
-    rng = _rng(1, 0)
+    rng = _rng(1, design)
     x, y, i, j = grid_nodes(n, L, L)
     z = 1.5 * np.sin(np.pi * x / L) * np.sin(np.pi * y / L)
     cnct = grid_quads(n)

@@ -1,0 +1,127 @@
+"""Host-side logic without a GPU: the C-ABI library loads and exports every
+symbol of include/jsso.h, the symbolic block pattern equals the reference's
+scalar pattern, plan validation, and the Model builder."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import jaxsso_amd.model as Model
+from jaxsso_amd import _native as nat
+from jaxsso_amd import workloads as wl
+from jaxsso_amd.plan import FEPlan, blocks_to_csr
+from oracle import system
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = nat.load_library()
+    header = open(os.path.join(ROOT, "include", "jsso.h")).read()
+    declared = set(re.findall(r"^\s*int\s+(jsso_\w+)\s*\(", header, flags=re.M))
+    assert declared == set(nat.EXPORTS)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.jsso_version() == 1
+
+
+def _pattern_equal(st, plan=None):
+    plan = plan or FEPlan([0, st.n_node], st.cnct_beamcols, st.cnct_quads, st.known_id)
+    rp, ci = plan.pattern()
+    A = blocks_to_csr(rp, ci, np.ones((len(ci), 6, 6)))
+    P = system.pattern_K(st.n_node, st.cnct_beamcols, st.cnct_quads)
+    return np.array_equal(A.indptr, P.indptr) and np.array_equal(A.indices, P.indices)
+
+
+@pytest.mark.parametrize("name", ["cfg0", "cfg1", "notebook_arch", "notebook_barrel"])
+def test_block_pattern_is_reference_pattern(name):
+    assert _pattern_equal(getattr(wl, name)())
+
+
+def test_block_pattern_mixed_elements_and_irregular_connectivity():
+    rng = np.random.default_rng(3)
+    st = wl.shell_pinned_edges(5, 5.0, 1.0)
+    perm = rng.permutation(st.n_quad)
+    st.cnct_quads = np.roll(st.cnct_quads[perm], 1, axis=1).astype(np.int32)   # element order, node order
+    st.prop_quads = st.prop_quads[perm]
+    st.cnct_beamcols = np.array([[0, 35], [7, 7 + 6], [20, 3]], np.int32)       # beams across the shell
+    st.prop_beamcols = np.tile([2e11, 8e10, 1e-6, 1e-6, 2e-6, 1e-3], (3, 1))
+    assert _pattern_equal(st)
+
+
+def test_batched_plan_is_block_diagonal():
+    a, b = wl.gridshell(3), wl.notebook_barrel()
+    n = a.n_node
+    plan = FEPlan([0, n, n + b.n_node], a.cnct_beamcols, b.cnct_quads + n,
+                  np.concatenate([a.known_id, b.known_id + 6 * n]))
+    rp, ci = plan.pattern()
+    pa = FEPlan([0, n], a.cnct_beamcols, np.zeros((0, 4)), a.known_id).pattern()
+    pb = FEPlan([0, b.n_node], np.zeros((0, 2)), b.cnct_quads, b.known_id).pattern()
+    assert np.array_equal(rp[:n + 1], pa[0])
+    assert np.array_equal(ci[:rp[n]], pa[1])
+    assert np.array_equal(rp[n:] - rp[n], pb[0])
+    assert np.array_equal(ci[rp[n]:] - n, pb[1])
+
+
+@pytest.mark.parametrize("kwargs,msg", [
+    (dict(known_id=[3, 3]), "listed twice"),
+    (dict(known_id=[10 ** 6]), "out of range"),
+    (dict(cnct_beam=[[0, 99]]), "out of range"),
+    (dict(node_off=[0, 2, 4], cnct_beam=[[1, 2]]), "spans two structures"),
+    (dict(node_off=[0, 0, 4]), "no nodes"),
+])
+def test_plan_validation(kwargs, msg):
+    args = dict(node_off=[0, 4], cnct_beam=[[0, 1], [1, 2], [2, 3]], cnct_quad=np.zeros((0, 4)), known_id=[0, 1, 2])
+    args.update(kwargs)
+    with pytest.raises(nat.JssoError, match=msg):
+        FEPlan(args["node_off"], args["cnct_beam"], args["cnct_quad"], args["known_id"])
+
+
+def test_model_builder_matches_workload_arrays():
+    st = wl.notebook_barrel()
+    m = Model.Model()
+    for i, (x, y, z) in enumerate(st.crds):
+        m.add_node(i, x, y, z)
+        if i not in set(st.design_nodes.tolist()):
+            m.add_support(i, [1, 1, 1, 0, 0, 0])
+        else:
+            m.add_nodal_load(i, [0, 0, st.nodal_loads[6 * i + 2], 0, 0, 0])
+    for e, c in enumerate(st.cnct_quads):
+        m.add_quad(e, *[int(v) for v in c], *st.prop_quads[e, :3])
+    m.add_truss(0, 0, 1, 1.0, 1.0)      # accepted and ignored, as the reference
+    m.model_ready()
+    assert np.array_equal(m.crds, st.crds)
+    assert np.array_equal(m.nodal_loads, st.nodal_loads)
+    assert np.array_equal(np.sort(m.known_id), np.sort(st.known_id))
+    assert np.array_equal(m.cnct_quads, st.cnct_quads)
+    assert np.array_equal(m.prop_quads, st.prop_quads)
+    assert m.ndof == st.ndof and m.n_quad == st.n_quad and m.n_beamcol == 0
+    assert len(m.unknown_id) + len(m.known_id) == m.ndof
+    # the plan is built from the topology alone, on the host
+    assert _pattern_equal(st, m.plan())
+
+
+def test_model_rejects_out_of_order_tags():
+    m = Model.Model()
+    m.add_node(1, 0.0, 0.0, 0.0)
+    m.add_node(0, 1.0, 0.0, 0.0)
+    with pytest.raises(ValueError, match="insertion order"):
+        m.model_ready()
+
+
+def test_compute_without_gpu_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU visible")
+    st = wl.cfg0()
+    plan = FEPlan([0, st.n_node], st.cnct_beamcols, st.cnct_quads, st.known_id)
+    with pytest.raises(nat.NativeUnavailable):
+        plan.assemble(st.crds, st.prop_beamcols, st.prop_quads)
+
+
+def test_quad_area_matches_reference_formula():
+    from jaxsso_amd.element import Quad
+    c = np.array([0, 0, 0, 2, 0, 0, 2, 3, 0, 0, 3, 0.0])
+    assert abs(Quad.A(c) - 6.0) < 1e-14
