@@ -161,7 +161,7 @@ class SSO_model:
 
     def _opts(self, which_solver, enforce_scipy_sparse):
         m = self.model
-        return ag.SolveOptions(precision=getattr(m, "precision", "mixed"), tol=getattr(m, "tol", 1e-10),
+        return ag.SolveOptions(precision=getattr(m, "precision", "fp64"), tol=getattr(m, "tol", 1e-10),
                                maxit=getattr(m, "maxit", 200000))
 
     # -- parameters -> results (SSO_model.py:224-270) ----------------------

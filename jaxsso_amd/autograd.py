@@ -17,7 +17,7 @@ from .plan import DEFAULT_MAXIT, DEFAULT_TOL, FEPlan
 
 
 class SolveOptions:
-    def __init__(self, precision="mixed", tol=DEFAULT_TOL, maxit=DEFAULT_MAXIT):
+    def __init__(self, precision="fp64", tol=DEFAULT_TOL, maxit=DEFAULT_MAXIT):
         self.precision = precision
         self.tol = tol
         self.maxit = maxit

@@ -58,7 +58,7 @@ class StructureBatch:
     def assemble(self):
         self.plan.assemble(self.crds, self.prop_beam, self.prop_quad)
 
-    def solve(self, precision="mixed", tol=DEFAULT_TOL, maxit=DEFAULT_MAXIT, assemble=True):
+    def solve(self, precision="fp64", tol=DEFAULT_TOL, maxit=DEFAULT_MAXIT, assemble=True):
         if assemble:
             self.assemble()
         f = self.f[:, 0] if self.n_cases == 1 else self.f
@@ -79,7 +79,7 @@ class StructureBatch:
         u = self.u if u is None else u
         return self.plan.adjoint(self.crds, self.prop_beam, self.prop_quad, u, lam=None, lam_scale=0.5)
 
-    def compliance_and_gradient(self, precision="mixed", tol=DEFAULT_TOL, maxit=DEFAULT_MAXIT):
+    def compliance_and_gradient(self, precision="fp64", tol=DEFAULT_TOL, maxit=DEFAULT_MAXIT):
         u = self.solve(precision=precision, tol=tol, maxit=maxit)
         return self.compliance(u), self.gradient(u)
 

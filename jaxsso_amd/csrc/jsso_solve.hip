@@ -18,6 +18,12 @@
 // last workgroup of a system (atomic arrival counter) adds them in a fixed
 // order and updates the system's scalars.
 //
+// Safeguards: the iterate with the smallest true residual is kept and is what
+// the solve returns; a system stops when its true residual has not improved
+// by 10% for max(2048, it/2) iterations (the attainable-accuracy floor of the
+// arithmetic), grows 1000-fold past its best, or is not finite; a
+// non-positive or non-finite curvature <p,q> restarts the search direction.
+//
 // Plain path (JSSO_PREC_PLAIN): textbook block-Jacobi PCG on the constrained
 // FP64 K, one structure and one load case per launch sequence.
 #include "jsso_internal.hpp"
@@ -25,6 +31,7 @@
 namespace jsso {
 
 constexpr int NWAVES = SOLVER_WG / 64;
+enum { SYS_RUNNING = 0, SYS_CONVERGED = 1, SYS_STALLED = 2 };
 
 // ---------------------------------------------------------------------------
 // sampled event timing of cg1 (jsso_timing_enable / jsso_timing_read)
@@ -86,8 +93,8 @@ int timing_read(double* avg_ms, int64_t* samples, int reset) {
 }
 
 struct SysState {
-    double *bnorm2, *rr, *rr_prev, *alpha, *beta, *relres;
-    int32_t *active, *counter;
+    double *bnorm2, *rr, *rr_prev, *alpha, *beta, *relres, *best;
+    int32_t *active, *counter, *status, *stall, *restart, *save;
     double* partial;
 };
 
@@ -166,7 +173,8 @@ __global__ __launch_bounds__(SOLVER_WG) void k_init(Rows rows, SysState st, cons
                                                     const double* __restrict__ Linv, const double* __restrict__ f,
                                                     double* __restrict__ b64, double* __restrict__ x64,
                                                     double* __restrict__ r64, VT* __restrict__ xlo,
-                                                    VT* __restrict__ rlo, VT* __restrict__ p, VT* __restrict__ q) {
+                                                    VT* __restrict__ rlo, VT* __restrict__ p, VT* __restrict__ q,
+                                                    double* __restrict__ xbest) {
     __shared__ double lds[NWAVES * 2 * NR];
     const int wg = blockIdx.x + rows.wg_base;
     const int sys = rows.wg_sys[wg];
@@ -190,6 +198,7 @@ __global__ __launch_bounds__(SOLVER_WG) void k_init(Rows rows, SysState st, cons
             const int64_t o = R * NR + h;
             b64[o] = s;
             x64[o] = 0.0;
+            xbest[o] = 0.0;
             r64[o] = s;
             xlo[o] = VT(0);
             rlo[o] = VT(s);
@@ -212,6 +221,11 @@ __global__ __launch_bounds__(SOLVER_WG) void k_init(Rows rows, SysState st, cons
             st.alpha[k] = 0.0;
             st.active[k] = tot[h] > 0.0 ? 1 : 0;
             st.relres[k] = tot[h] > 0.0 ? 1.0 : 0.0;
+            st.best[k] = st.relres[k];
+            st.status[k] = tot[h] > 0.0 ? SYS_RUNNING : SYS_CONVERGED;
+            st.stall[k] = 0;
+            st.restart[k] = 0;
+            st.save[k] = 0;
         }
     }
 }
@@ -266,10 +280,11 @@ __global__ __launch_bounds__(SOLVER_WG) void k_cg1(Rows rows, SysState st, const
         for (int h = 0; h < NR; ++h) {
             const int k = sys * NR + h;
             if (!st.active[k]) continue;
-            if (tot[h] > 0.0) {
+            if (tot[h] > 0.0 && isfinite(tot[h])) {
                 st.alpha[k] = st.rr[k] / tot[h];
             } else {
-                st.alpha[k] = 0.0;  // breakdown: the next replacement decides
+                st.alpha[k] = 0.0;  // breakdown: no step, restart the direction
+                st.restart[k] = 1;
             }
         }
     }
@@ -309,7 +324,8 @@ __global__ __launch_bounds__(SOLVER_WG) void k_cg2(Rows rows, SysState st, const
             if (!st.active[k]) continue;
             st.rr_prev[k] = st.rr[k];
             st.rr[k] = tot[h];
-            st.beta[k] = st.rr_prev[k] > 0.0 ? tot[h] / st.rr_prev[k] : 0.0;
+            st.beta[k] = (st.rr_prev[k] > 0.0 && !st.restart[k]) ? tot[h] / st.rr_prev[k] : 0.0;
+            st.restart[k] = 0;
         }
     }
 }
@@ -337,7 +353,7 @@ __global__ __launch_bounds__(SOLVER_WG) void k_replace(Rows rows, SysState st, c
                                                        const int32_t* __restrict__ col,
                                                        const double* __restrict__ A, const double* __restrict__ b64,
                                                        const double* __restrict__ x64, double* __restrict__ r64,
-                                                       VT* __restrict__ rlo, double tol) {
+                                                       VT* __restrict__ rlo, double tol, int it) {
     __shared__ double lds[NWAVES * 2 * NR];
     const int wg = blockIdx.x + rows.wg_base;
     const int sys = rows.wg_sys[wg];
@@ -382,9 +398,34 @@ __global__ __launch_bounds__(SOLVER_WG) void k_replace(Rows rows, SysState st, c
             st.relres[k] = rel;
             st.beta[k] = st.rr_prev[k] > 0.0 ? tot[NR + h] / st.rr_prev[k] : 0.0;
             st.rr[k] = tot[NR + h];
-            if (rel <= tol || tot[NR + h] == 0.0) st.active[k] = 0;
+            const double best = st.best[k];
+            const bool improved = rel < best;
+            st.save[k] = improved ? 1 : 0;
+            if (improved) st.best[k] = rel;
+            if (rel <= 0.9 * best) st.stall[k] = it;   // iteration of the last 10% gain
+            const int window = it / 2 > 2048 ? it / 2 : 2048;
+            if (rel <= tol || tot[NR + h] == 0.0) {
+                st.active[k] = 0;
+                st.status[k] = SYS_CONVERGED;
+            } else if (!isfinite(rel) || rel > 1e3 * st.best[k] || it - st.stall[k] > window) {
+                st.active[k] = 0;
+                st.status[k] = SYS_STALLED;
+            }
         }
     }
+}
+
+// keep the best iterate: xbest = x64 where the last replacement improved it
+template <int NR>
+__global__ __launch_bounds__(SOLVER_WG) void k_save(Rows rows, const int32_t* __restrict__ save,
+                                                    const double* __restrict__ x64, double* __restrict__ xbest) {
+    const int wg = blockIdx.x + rows.wg_base;
+    const int sys = rows.wg_sys[wg];
+    const int64_t R = 6 * (int64_t)rows.wg_row0[wg] + threadIdx.x;
+    if (R >= 6 * (int64_t)rows.wg_row1[wg]) return;
+#pragma unroll
+    for (int h = 0; h < NR; ++h)
+        if (save[sys * NR + h]) xbest[R * NR + h] = x64[R * NR + h];
 }
 
 // u = L^-T x
@@ -408,7 +449,7 @@ __global__ __launch_bounds__(256) void k_unscale(int32_t n_nodes, const double* 
 // workspace
 // ---------------------------------------------------------------------------
 struct Workspace {
-    double *b64, *x64, *r64;
+    double *b64, *x64, *r64, *xbest;
     void *xlo, *rlo, *p, *q;
     SysState st;
 };
@@ -417,10 +458,12 @@ static int get_workspace(Plan* pl, int nrhs, Workspace& w) {
     const size_t n = 6 * (size_t)pl->n_nodes * nrhs;
     const size_t nsys = (size_t)pl->n_sys * nrhs;
     const size_t need[] = {n * 8, n * 8, n * 8, n * 8, n * 8, n * 8, n * 8,
-                           (size_t)pl->n_wg * 4 * nrhs * 8, nsys * 8 * 6, nsys * 4, (size_t)pl->n_sys * 4};
+                           (size_t)pl->n_wg * 4 * nrhs * 8, nsys * 8 * 7, nsys * 4 * 5, (size_t)pl->n_sys * 4,
+                           0, n * 8};
     const int nb = sizeof(need) / sizeof(need[0]);
-    if (pl->ws.size() < 12) pl->ws.resize(12);  // slot 11: adjoint scratch
+    if (pl->ws.size() < 13) pl->ws.resize(13);  // slot 11: adjoint scratch
     for (int i = 0; i < nb; ++i) {
+        if (i == 11) continue;
         if (pl->ws[i].bytes < need[i]) {
             if (pl->ws[i].p) JSSO_HIP(hipFree(pl->ws[i].p));
             pl->ws[i].p = nullptr;
@@ -447,8 +490,15 @@ static int get_workspace(Plan* pl, int nrhs, Workspace& w) {
     w.st.alpha = sc + 3 * nsys;
     w.st.beta = sc + 4 * nsys;
     w.st.relres = sc + 5 * nsys;
-    w.st.active = (int32_t*)pl->ws[9].p;
+    w.st.best = sc + 6 * nsys;
+    int32_t* ic = (int32_t*)pl->ws[9].p;
+    w.st.active = ic;
+    w.st.status = ic + nsys;
+    w.st.stall = ic + 2 * nsys;
+    w.st.restart = ic + 3 * nsys;
+    w.st.save = ic + 4 * nsys;
     w.st.counter = (int32_t*)pl->ws[10].p;
+    w.xbest = (double*)pl->ws[12].p;
     return JSSO_OK;
 }
 
@@ -478,7 +528,7 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
     if constexpr (sizeof(MT) == 4) A = (const MT*)pl->d_Ks32; else A = (const MT*)pl->d_Ks64;
     VT *xlo = (VT*)w.xlo, *rlo = (VT*)w.rlo, *p = (VT*)w.p, *q = (VT*)w.q;
     hipLaunchKernelGGL((k_init<VT, NR>), grid, block, 0, s, rows, w.st, pl->d_fixed, pl->d_Linv, f, w.b64,
-                       w.x64, w.r64, xlo, rlo, p, q);
+                       w.x64, w.r64, xlo, rlo, p, q, w.xbest);
     JSSO_HIP(hipGetLastError());
     const int nsys = pl->n_sys * NR;
     std::vector<int32_t> act(nsys);
@@ -499,7 +549,9 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
         hipLaunchKernelGGL((k_fold<VT, NR>), grid, block, 0, s, rows, (const int32_t*)w.st.active, xlo, w.x64);
         hipLaunchKernelGGL((k_replace<VT, NR>), grid, block, 0, s, rows, w.st, pl->d_offrowptr, pl->d_offcol,
                            (const double*)pl->d_Ks64, (const double*)w.b64, (const double*)w.x64, w.r64, rlo,
-                           tol);
+                           tol, it);
+        hipLaunchKernelGGL((k_save<NR>), grid, block, 0, s, rows, (const int32_t*)w.st.save,
+                           (const double*)w.x64, w.xbest);
         JSSO_HIP(hipGetLastError());
         ++repl;
         JSSO_HIP(hipMemcpyAsync(act.data(), w.st.active, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -511,16 +563,16 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
     }
     const int64_t nr = 6 * (int64_t)pl->n_nodes;
     hipLaunchKernelGGL((k_unscale<NR>), dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, pl->n_nodes,
-                       pl->d_Linv, (const double*)w.x64, u);
+                       pl->d_Linv, (const double*)w.xbest, u);
     JSSO_HIP(hipGetLastError());
     std::vector<double> rel(nsys);
-    JSSO_HIP(hipMemcpyAsync(rel.data(), w.st.relres, nsys * sizeof(double), hipMemcpyDeviceToHost, s));
-    JSSO_HIP(hipMemcpyAsync(act.data(), w.st.active, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    JSSO_HIP(hipMemcpyAsync(rel.data(), w.st.best, nsys * sizeof(double), hipMemcpyDeviceToHost, s));
+    JSSO_HIP(hipMemcpyAsync(act.data(), w.st.status, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     JSSO_HIP(hipStreamSynchronize(s));
     int conv = 0;
     double mx = 0.0;
     for (int k = 0; k < nsys; ++k) {
-        conv += act[k] ? 0 : 1;
+        conv += act[k] == SYS_CONVERGED ? 1 : 0;
         mx = std::max(mx, rel[k]);
     }
     if (info) {

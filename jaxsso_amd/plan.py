@@ -78,7 +78,7 @@ class FEPlan:
         self._keep = (crds, pb, pq)
         return Kout
 
-    def solve(self, f: torch.Tensor, precision: str = "mixed", tol: float = DEFAULT_TOL,
+    def solve(self, f: torch.Tensor, precision: str = "fp64", tol: float = DEFAULT_TOL,
               maxit: int = DEFAULT_MAXIT, warn: bool = True) -> torch.Tensor:
         """Solve K u = f (u = 0 at supports).  f: (ndof,) or (ndof, nrhs) FP64 on the device."""
         L = nat.lib()

@@ -4,9 +4,10 @@ The reference offers a dense LU (``jax_dense_solve``, solver.py:75-93), a GPU
 sparse direct solve (``jax_sparse_solve``, :102-168) and a host SuperLU through
 ``pure_callback`` (``sci_sparse_solve``, :176-250), each on the Lagrange-
 augmented system.  Here every sparse solve is the GPU conjugate-gradient solver
-of ``libjsso_hip.so`` (block-Jacobi scaled; FP32 recurrences with FP64
-reliable updates by default), and the dense solve is an FP64 direct solve of
-the constrained system on the GPU.  All return ``u_aug = [u; mu]`` with the
+of ``libjsso_hip.so`` (block-Jacobi scaled; FP64 by default, or FP32
+recurrences with FP64 reliable updates when ``precision="mixed"`` -- faster on
+shells, but slender beam chains need a tighter tolerance in that mode), and the
+dense solve is an FP64 direct solve of the constrained system on the GPU.  All return ``u_aug = [u; mu]`` with the
 Lagrange multipliers ``mu = f_c - (K u)_c`` (support reactions), as the
 reference's augmented solve does.
 """
@@ -15,7 +16,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-DEFAULT_PRECISION = "mixed"
+DEFAULT_PRECISION = "fp64"
 DEFAULT_TOL = 1e-10
 DEFAULT_MAXIT = 200000
 

@@ -49,6 +49,9 @@ def test_arch_notebook_pins(precision):
     st = wl.notebook_arch()
     m = build_model(st)
     m.precision = precision
+    # the slender arch is the worst case for FP32 recurrences: the mixed path
+    # needs a 1e-11 residual for a 1e-8 gradient (FP64 reaches it at 1e-10)
+    m.tol = 1e-11 if precision == "mixed" else 1e-10
     m.solve(which_solver="sparse")
     u = m.u.cpu().numpy()
     C = float(m.strain_energy())
@@ -105,12 +108,13 @@ def test_cfg1_shape_and_thickness_gradient_vs_oracle_and_fd():
     gt = g[len(st.design_nodes):]
     assert rel(gz, ref["dcrds"][st.design_nodes, 2]) < 1e-8
     assert rel(gt, ref["dprop_quad"][:, 0]) < 1e-8
-    # central finite differences (h = 1e-6) through the GPU FP64 path
+    # central finite differences through the GPU FP64 path; C is good to ~1e-12
+    # relative at this tolerance, so the FD noise is ~1e-12 C / h
     m.precision = "fp64"
-    m.tol = 1e-13
+    m.tol = 1e-11
     p0 = sso.parameter_values.copy()
     for k in [5, 400, len(st.design_nodes) + 17]:
-        h = 1e-6
+        h = 1e-5
         pp, pm = p0.copy(), p0.copy()
         pp[k] += h
         pm[k] -= h
@@ -118,7 +122,22 @@ def test_cfg1_shape_and_thickness_gradient_vs_oracle_and_fd():
             Cp = float(sso.params_c(pp))
             Cm = float(sso.params_c(pm))
         fd = (Cp - Cm) / (2 * h)
-        assert abs(fd - g[k]) <= 1e-5 * abs(g[k]) + 1e-9 * abs(val), (k, fd, g[k])
+        assert abs(fd - g[k]) <= 1e-5 * abs(g[k]) + 1e-11 * abs(val) / h, (k, fd, g[k])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_unattainable_tolerance_stops_at_floor_with_best_iterate(precision):
+    st = wl.cfg1()
+    from jaxsso_amd.plan import FEPlan
+    plan = FEPlan([0, st.n_node], st.cnct_beamcols, st.cnct_quads, st.known_id)
+    plan.assemble(torch.tensor(st.crds), None, torch.tensor(st.prop_quads))
+    with pytest.warns(RuntimeWarning, match="did not converge"):
+        u = plan.solve(torch.tensor(st.nodal_loads), precision=precision, tol=1e-17, maxit=60000).cpu().numpy()
+    info = plan.last_info
+    assert info["converged"] == 0 and info["iterations"] < 60000, info    # stopped by stagnation
+    K = system.assemble_K(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads)
+    ur = osolve.solve_reduced(K, st.known_id, st.nodal_loads)
+    assert np.all(np.isfinite(u)) and rel(u, ur) < 1e-9, (rel(u, ur), info)
 
 
 def test_user_objective_uses_second_solve():
@@ -170,6 +189,7 @@ def test_batch_uneven_gridshells_four_cases():
     designs = wl.cfg3(n_designs=8)
     b = StructureBatch(designs)
     C, (dc, _, _) = b.compliance_and_gradient(precision="mixed", tol=1e-10)
+    assert b.plan.last_info["converged"] == 32
     C = C.cpu().numpy()
     dcs = b.split_nodes(dc.cpu().numpy())
     for i, st in enumerate(designs):
