@@ -188,6 +188,33 @@ static int build_plan(Plan* p, int32_t n_sys, const int32_t* node_off, int32_t n
     }
     p->n_wg = (int32_t)wg_sys.size();
     p->h_sys_wg_off = sys_wg_off;
+    // sliced ELL of the off-diagonal blocks: slice = solver workgroup (<= 64 block
+    // rows of one structure), slot j of lane l = j-th off-diagonal block of row
+    // row0 + l; padding slots point at the row itself with zero values
+    std::vector<int32_t> slice_voff(p->n_wg + 1, 0), ell_col;
+    std::vector<int64_t> ell_base(offcol.size(), -1);
+    for (int w = 0; w < p->n_wg; ++w) {
+        int nslot = 0;
+        for (int i = wg_row0[w]; i < wg_row1[w]; ++i) nslot = std::max(nslot, offrowptr[i + 1] - offrowptr[i]);
+        slice_voff[w + 1] = slice_voff[w] + nslot * ROWS_PER_WG;
+        const size_t base = ell_col.size();
+        ell_col.resize(base + (size_t)nslot * ROWS_PER_WG);
+        for (int j = 0; j < nslot; ++j)
+            for (int l = 0; l < ROWS_PER_WG; ++l) {
+                const int i = wg_row0[w] + l;
+                int32_t c = i < wg_row1[w] ? i : wg_row0[w];
+                if (i < wg_row1[w] && j < offrowptr[i + 1] - offrowptr[i]) {
+                    const int k = offrowptr[i] + j;
+                    c = offcol[k];
+                    ell_base[k] = 36 * (int64_t)slice_voff[w] + (int64_t)j * 36 * ROWS_PER_WG + l;
+                }
+                ell_col[base + (size_t)j * ROWS_PER_WG + l] = c;
+            }
+    }
+    p->ell_slots = slice_voff[p->n_wg];
+    p->h_slice_voff.swap(slice_voff);
+    p->h_ell_col.swap(ell_col);
+    p->h_ell_base.swap(ell_base);
     p->h_browof.swap(browof);
     p->h_diag.swap(diag);
     p->h_offrowptr.swap(offrowptr);
@@ -218,15 +245,23 @@ static int ensure_device(Plan* p) {
         (rc = upload(p, &p->d_nsrc, p->h_nsrc)) || (rc = upload(p, &p->d_cnct_beam, p->h_cnct_beam)) ||
         (rc = upload(p, &p->d_cnct_quad, p->h_cnct_quad)) || (rc = upload(p, &p->d_fixed, p->h_fixed)) ||
         (rc = upload(p, &p->d_wg_sys, p->h_wg_sys)) || (rc = upload(p, &p->d_wg_row0, p->h_wg_row0)) ||
-        (rc = upload(p, &p->d_wg_row1, p->h_wg_row1)) || (rc = upload(p, &p->d_sys_wg_off, p->h_sys_wg_off)))
+        (rc = upload(p, &p->d_wg_row1, p->h_wg_row1)) || (rc = upload(p, &p->d_sys_wg_off, p->h_sys_wg_off)) ||
+        (rc = upload(p, &p->d_slice_voff, p->h_slice_voff)) || (rc = upload(p, &p->d_ell_col, p->h_ell_col)) ||
+        (rc = upload(p, &p->d_ell_base, p->h_ell_base)))
         return rc;
     const size_t n = p->n_nodes;
     if ((rc = dalloc(p, &p->d_ke, (size_t)p->n_contrib * 36)) || (rc = dalloc(p, &p->d_K, (size_t)p->nnzb * 36)) ||
         (rc = dalloc(p, &p->d_Kc, (size_t)p->nnzb * 36)) || (rc = dalloc(p, &p->d_Linv, n * 36)) ||
         (rc = dalloc(p, &p->d_Minv, n * 36)) || (rc = dalloc(p, &p->d_Ks64, (size_t)p->nnzb_off * 36)) ||
-        (rc = dalloc(p, &p->d_Ks32, (size_t)p->nnzb_off * 36)) || (rc = dalloc(p, &p->d_status, 1)))
+        (rc = dalloc(p, &p->d_Ks32, (size_t)p->nnzb_off * 36)) || (rc = dalloc(p, &p->d_status, 1)) ||
+        (rc = dalloc(p, &p->d_Ke64, (size_t)p->ell_slots * 36)) || (rc = dalloc(p, &p->d_Ke32, (size_t)p->ell_slots * 36)))
         return rc;
     JSSO_HIP(hipMemset(p->d_status, 0, sizeof(int32_t)));
+    // padding slots of the ELL arrays stay zero
+    if (p->ell_slots) {
+        JSSO_HIP(hipMemset(p->d_Ke64, 0, (size_t)p->ell_slots * 36 * sizeof(double)));
+        JSSO_HIP(hipMemset(p->d_Ke32, 0, (size_t)p->ell_slots * 36 * sizeof(float)));
+    }
     p->on_device = true;
     return JSSO_OK;
 }

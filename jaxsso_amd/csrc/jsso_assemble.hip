@@ -110,7 +110,9 @@ __global__ __launch_bounds__(256) void scale_blocks_kernel(int64_t nnzb_off, con
                                                            const int32_t* __restrict__ colind,
                                                            const double* __restrict__ Kc,
                                                            const double* __restrict__ Linv,
-                                                           double* __restrict__ Ks64, float* __restrict__ Ks32) {
+                                                           const int64_t* __restrict__ ell_base,
+                                                           double* __restrict__ Ks64, float* __restrict__ Ks32,
+                                                           double* __restrict__ Ke64, float* __restrict__ Ke32) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nnzb_off) return;
     const int64_t src = offsrc[k];
@@ -137,6 +139,9 @@ __global__ __launch_bounds__(256) void scale_blocks_kernel(int64_t nnzb_off, con
             for (int k2 = 0; k2 <= r; ++k2) s += Li[r * 6 + k2] * T[k2][c];
             Ks64[k * 36 + r * 6 + c] = s;
             Ks32[k * 36 + r * 6 + c] = (float)s;
+            const int64_t e = ell_base[k] + (int64_t)(r * 6 + c) * ROWS_PER_WG;
+            Ke64[e] = s;
+            Ke32[e] = (float)s;
         }
 }
 
@@ -157,7 +162,7 @@ int launch_assemble(Plan* p, hipStream_t s) {
     if (p->nnzb_off > 0) {
         hipLaunchKernelGGL(scale_blocks_kernel, dim3((unsigned)((p->nnzb_off + 255) / 256)), dim3(256), 0, s,
                            p->nnzb_off, p->d_offsrc, p->d_browof, p->d_colind, p->d_Kc, p->d_Linv,
-                           p->d_Ks64, p->d_Ks32);
+                           p->d_ell_base, p->d_Ks64, p->d_Ks32, p->d_Ke64, p->d_Ke32);
         JSSO_HIP(hipGetLastError());
     }
     return JSSO_OK;

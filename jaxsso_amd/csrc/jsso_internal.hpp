@@ -33,8 +33,8 @@ void set_error(const char* fmt, ...);
 
 constexpr int BDIR = 12;   // beam adjoint directions: 6 coordinates + 6 properties
 constexpr int QDIR = 17;   // quad adjoint directions: 12 coordinates + 5 properties
-constexpr int ROWS_PER_WG = 64;   // block rows per solver workgroup (384 scalar rows)
-constexpr int SOLVER_WG = 6 * ROWS_PER_WG;
+constexpr int ROWS_PER_WG = 64;   // block rows per solver slice (one wavefront in the SpMV)
+constexpr int SOLVER_WG = 6 * ROWS_PER_WG;   // threads of the scalar-row solver kernels
 
 // Device buffer owned by the plan.
 struct DBuf {
@@ -52,6 +52,8 @@ struct Plan {
     std::vector<int32_t> h_node_off, h_rowptr, h_colind;
     std::vector<int32_t> h_browof, h_diag, h_offrowptr, h_offcol, h_offsrc, h_cptr, h_csrc, h_nptr, h_nsrc;
     std::vector<int32_t> h_cnct_beam, h_cnct_quad, h_wg_sys, h_wg_row0, h_wg_row1, h_sys_wg_off;
+    std::vector<int32_t> h_slice_voff, h_ell_col;   // sliced-ELL layout of the off-diagonal blocks
+    std::vector<int64_t> h_ell_base;
     std::vector<uint8_t> h_fixed;
     bool on_device = false;
     // pattern (device)
@@ -64,6 +66,12 @@ struct Plan {
     int32_t *d_wg_sys = nullptr, *d_wg_row0 = nullptr, *d_wg_row1 = nullptr;
     int32_t* d_sys_wg_off = nullptr;
     int32_t n_wg = 0;
+    // sliced ELL: slice = workgroup above; values [slot][36][64 lanes] from 36*slice_voff[sl]
+    int32_t *d_slice_voff = nullptr, *d_ell_col = nullptr;
+    int64_t* d_ell_base = nullptr;   // per off-diagonal block: index of its entry 0 in the ELL arrays
+    int64_t ell_slots = 0;           // sum over slices of (slots x 64)
+    double* d_Ke64 = nullptr;        // scaled off-diagonal blocks, sliced ELL, FP64
+    float* d_Ke32 = nullptr;         // same, FP32
     // numeric state (device)
     double* d_ke = nullptr;       // element blocks, 36 doubles each
     double* d_K = nullptr;        // K, 36 per block (unconstrained)

@@ -1,31 +1,39 @@
-// Conjugate-gradient solvers on the 6x6 block-CSR stiffness.
+// Conjugate-gradient solvers on the 6x6 block stiffness.
 //
 // Batched path (JSSO_PREC_FP64 / JSSO_PREC_MIXED).  Every structure x load
 // case of a plan is one CG system; one launch advances all of them.  The
 // system is the block-Jacobi-scaled one, A = L^-1 Kc L^-T with unit diagonal
-// blocks (split block-Jacobi preconditioning, identical iterates to PCG with
+// blocks (split block-Jacobi preconditioning: the same iterates as PCG with
 // M = blockdiag(Kc_ii)^-1), so a step needs no preconditioner pass and the
 // diagonal blocks are never read.  Two kernels per iteration:
-//     cg1:  s = A r; p = r + beta p; q = s + beta q; <p,q>   -> alpha
-//     cg2:  x += alpha p; r -= alpha q; <r,r>                 -> beta
-// (q = A p is carried by recurrence).  Recurrences run in VT (FP32 for MIXED);
-// every `replace_every` iterations the FP32 partial solution is folded into
-// the FP64 solution and the residual is replaced by the true FP64 residual
-// b - A64 x64 (reliable updates, Sleijpen & van der Vorst 1996), which is also
-// where convergence is decided.
+//     cg1:  s = A r; p = r + beta p; q = s + beta q; <p,q>
+//     cg2:  x += alpha p; r -= alpha q; <r,r>
+// (q = A p carried by recurrence).  Recurrences run in VT (FP32 for MIXED);
+// every `replace_every` iterations the partial solution is folded into the
+// FP64 solution and the residual replaced by the true FP64 residual b - A64 x64
+// (reliable updates, Sleijpen & van der Vorst 1996), where convergence is
+// decided.
 //
-// Reductions are deterministic: each workgroup writes its partial sums; the
-// last workgroup of a system (atomic arrival counter) adds them in a fixed
-// order and updates the system's scalars.
+// Layout.  A slice is <= 64 block rows of one structure.  The off-diagonal
+// blocks of a slice are stored as sliced ELL, values [slot][36 entries][64
+// lanes]: in the SpMV one lane owns one block row, so each of the 36 loads of
+// a slot is one contiguous 256 B (FP32) / 512 B (FP64) wavefront access.
+//
+// Reductions without atomics or fences.  Each kernel writes one partial sum
+// per slice; every workgroup of the next kernel adds the partials of its own
+// structure in a fixed order (so all of them obtain bitwise-identical CG
+// scalars) -- the kernel boundary is the only synchronisation.  <r,r> partials
+// alternate between two buffers so beta = rr_new / rr_old needs no stored state.
 //
 // Safeguards: the iterate with the smallest true residual is kept and is what
-// the solve returns; a system stops when its true residual has not improved
-// by 10% for max(2048, it/2) iterations (the attainable-accuracy floor of the
-// arithmetic), grows 1000-fold past its best, or is not finite; a
-// non-positive or non-finite curvature <p,q> restarts the search direction.
+// the solve returns; a system stops when its true residual has not improved by
+// 10% for max(2048, it/2) iterations (the attainable-accuracy floor), grows
+// 1000-fold past its best, or is not finite; a non-positive or non-finite
+// curvature <p,q> restarts the search direction.
 //
 // Plain path (JSSO_PREC_PLAIN): textbook block-Jacobi PCG on the constrained
-// FP64 K, one structure and one load case per launch sequence.
+// FP64 K in block CSR, thread per scalar row, one structure and one load case
+// per launch sequence -- the baseline the batched paths are measured against.
 #include "jsso_internal.hpp"
 
 namespace jsso {
@@ -63,7 +71,6 @@ static void timing_end(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>* ev) {
     if (ev) (void)hipEventRecord(ev->second, s);
 }
 
-// after a stream synchronisation: fold the completed samples
 static void timing_collect() {
     for (auto& e : g_timing.pending) {
         float ms = 0.f;
@@ -92,24 +99,41 @@ int timing_read(double* avg_ms, int64_t* samples, int reset) {
     return JSSO_OK;
 }
 
+// ---------------------------------------------------------------------------
+// slices and reductions
+// ---------------------------------------------------------------------------
+struct Slices {
+    const int32_t *sys, *row0, *row1, *sys_off;  // slice -> structure, rows; structure -> first slice
+    const int32_t *voff, *ell_col;                // sliced ELL
+    int base;                                     // first slice of the launch (plain path)
+};
+
+// per-(structure, rhs) state, written only by k_decide (and, for restart, cg2)
 struct SysState {
-    double *bnorm2, *rr, *rr_prev, *alpha, *beta, *relres, *best;
-    int32_t *active, *counter, *status, *stall, *restart, *save;
-    double* partial;
+    double *bnorm2, *relres, *best;
+    int32_t *active, *status, *stall, *save;
+    int32_t* restart;  // [2][nsys], written by cg2 (first slice of the structure)
 };
 
-struct Rows {
-    const int32_t *wg_sys, *wg_row0, *wg_row1, *sys_wg_off;
-    int wg_base;  // plain path: launches cover one system's workgroups
+// partial-sum buffers, one value per slice and rhs
+struct Partials {
+    double *pq, *rr0, *rr1, *b, *t;
+    __host__ __device__ double* rr(int it) const { return (it & 1) ? rr1 : rr0; }
 };
 
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
+// all-lanes sum over one wavefront (butterfly: identical result in every lane)
+__device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Sum over the whole workgroup, result in every thread.  `lds` holds >= NW*NV.
+template <int NV, int NW>
+__device__ __forceinline__ void group_sum(double (&v)[NV], double* lds) {
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off);
-    }
+    for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+    if (NW == 1) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     __syncthreads();
     if (lane == 0) {
@@ -117,72 +141,67 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* lds) {
         for (int i = 0; i < NV; ++i) lds[wave * NV + i] = v[i];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            double s = 0.0;
-            for (int w = 0; w < NWAVES; ++w) s += lds[w * NV + i];
-            v[i] = s;
-        }
+    for (int i = 0; i < NV; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) s += lds[w * NV + i];
+        v[i] = s;
     }
 }
 
-// Writes this workgroup's NV partials; returns true (in every thread) for the
-// last workgroup of system `sys`, in which `tot` then holds the system totals
-// (valid in thread 0).
-template <int NV>
-__device__ bool reduce_system(double (&v)[NV], double (&tot)[NV], double* lds, int wg, int sys,
-                              const Rows& rows, const SysState& st) {
-    __shared__ int am_last;
-    block_sum<NV>(v, lds);
+// Totals over the slices of `sys` of up to two partial buffers (NR values per
+// slice each), identical in every thread of every workgroup of the structure.
+template <int NR, int NW>
+__device__ __forceinline__ void system_totals(const Slices& sl, int sys, const double* a, const double* b,
+                                              double (&ta)[NR], double (&tb)[NR], double* lds) {
+    const int s0 = sl.sys_off[sys], s1 = sl.sys_off[sys + 1];
+    double v[2 * NR];
+#pragma unroll
+    for (int i = 0; i < 2 * NR; ++i) v[i] = 0.0;
+    for (int w = s0 + (int)threadIdx.x; w < s1; w += NW * 64) {
+#pragma unroll
+        for (int h = 0; h < NR; ++h) {
+            v[h] += a[(int64_t)w * NR + h];
+            if (b) v[NR + h] += b[(int64_t)w * NR + h];
+        }
+    }
+    group_sum<2 * NR, NW>(v, lds);
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+        ta[h] = v[h];
+        tb[h] = v[NR + h];
+    }
+}
+
+// write this slice's partial sums (thread 0)
+template <int NV, int NW>
+__device__ __forceinline__ void write_partials(double (&v)[NV], double* lds, double* dst, int slice) {
+    group_sum<NV, NW>(v, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) st.partial[(int64_t)wg * NV + i] = v[i];
-        __threadfence();
-        const int w0 = rows.sys_wg_off[sys], w1 = rows.sys_wg_off[sys + 1];
-        const int prev = atomicAdd(&st.counter[sys], 1);
-        am_last = (prev == w1 - w0 - 1);
+        for (int i = 0; i < NV; ++i) dst[(int64_t)slice * NV + i] = v[i];
     }
-    __syncthreads();
-    if (!am_last) return false;
-    __threadfence();
-    const int w0 = rows.sys_wg_off[sys], w1 = rows.sys_wg_off[sys + 1];
-    double s[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) s[i] = 0.0;
-    for (int w = w0 + (int)threadIdx.x; w < w1; w += blockDim.x) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) s[i] += __hip_atomic_load(&st.partial[(int64_t)w * NV + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    block_sum<NV>(s, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) tot[i] = s[i];
-        st.counter[sys] = 0;
-    }
-    return true;
 }
 
 // ---------------------------------------------------------------------------
 // batched kernels
 // ---------------------------------------------------------------------------
 
-// b = L^-1 f (constrained DOFs zeroed), x = 0, r = b.
+// b = L^-1 f (constrained DOFs zeroed), x = 0, r = b; partials <b,b>, <r,r>.
 template <typename VT, int NR>
-__global__ __launch_bounds__(SOLVER_WG) void k_init(Rows rows, SysState st, const uint8_t* __restrict__ fixed,
+__global__ __launch_bounds__(SOLVER_WG) void k_init(Slices sl, Partials P, const uint8_t* __restrict__ fixed,
                                                     const double* __restrict__ Linv, const double* __restrict__ f,
                                                     double* __restrict__ b64, double* __restrict__ x64,
-                                                    double* __restrict__ r64, VT* __restrict__ xlo,
-                                                    VT* __restrict__ rlo, VT* __restrict__ p, VT* __restrict__ q,
-                                                    double* __restrict__ xbest) {
-    __shared__ double lds[NWAVES * 2 * NR];
-    const int wg = blockIdx.x + rows.wg_base;
-    const int sys = rows.wg_sys[wg];
-    const int64_t R = 6 * (int64_t)rows.wg_row0[wg] + threadIdx.x;
-    const bool on = R < 6 * (int64_t)rows.wg_row1[wg];
-    double v[2 * NR];
+                                                    double* __restrict__ xbest, VT* __restrict__ xlo,
+                                                    VT* __restrict__ rlo, VT* __restrict__ p, VT* __restrict__ q) {
+    __shared__ double lds[NWAVES * NR];
+    const int w = blockIdx.x + sl.base;
+    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
+    const bool on = R < 6 * (int64_t)sl.row1[w];
+    double vb[NR], vr[NR];
 #pragma unroll
-    for (int i = 0; i < 2 * NR; ++i) v[i] = 0.0;
+    for (int h = 0; h < NR; ++h) vb[h] = vr[h] = 0.0;
     if (on) {
         const int64_t i = R / 6;
         const int r = (int)(R % 6);
@@ -199,145 +218,188 @@ __global__ __launch_bounds__(SOLVER_WG) void k_init(Rows rows, SysState st, cons
             b64[o] = s;
             x64[o] = 0.0;
             xbest[o] = 0.0;
-            r64[o] = s;
             xlo[o] = VT(0);
-            rlo[o] = VT(s);
+            const VT sl_ = VT(s);
+            rlo[o] = sl_;
             p[o] = VT(0);
             q[o] = VT(0);
-            v[h] = s * s;
-            const double sl = (double)VT(s);
-            v[NR + h] = sl * sl;
+            vb[h] = s * s;
+            vr[h] = (double)sl_ * (double)sl_;
         }
     }
-    double tot[2 * NR];
-    if (reduce_system<2 * NR>(v, tot, lds, wg, sys, rows, st) && threadIdx.x == 0) {
-#pragma unroll
-        for (int h = 0; h < NR; ++h) {
-            const int k = sys * NR + h;
-            st.bnorm2[k] = tot[h];
-            st.rr[k] = tot[NR + h];
-            st.rr_prev[k] = tot[NR + h];
-            st.beta[k] = 0.0;
-            st.alpha[k] = 0.0;
-            st.active[k] = tot[h] > 0.0 ? 1 : 0;
-            st.relres[k] = tot[h] > 0.0 ? 1.0 : 0.0;
-            st.best[k] = st.relres[k];
-            st.status[k] = tot[h] > 0.0 ? SYS_RUNNING : SYS_CONVERGED;
-            st.stall[k] = 0;
-            st.restart[k] = 0;
-            st.save[k] = 0;
-        }
+    write_partials<NR, NWAVES>(vb, lds, P.b, w);
+    write_partials<NR, NWAVES>(vr, lds, P.rr0, w);
+}
+
+// One thread per (structure, rhs): convergence bookkeeping from the partials.
+// mode 0 (after k_init): bnorm2, activity.  mode 1 (after k_replace): true
+// relative residual, best iterate, stagnation, status.
+template <int NR>
+__global__ void k_decide(Slices sl, Partials P, SysState st, int nsys_struct, int mode, double tol, int it) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nsys_struct * NR) return;
+    const int sys = k / NR, h = k % NR;
+    const int s0 = sl.sys_off[sys], s1 = sl.sys_off[sys + 1];
+    if (mode == 0) {
+        double bb = 0.0;
+        for (int w = s0; w < s1; ++w) bb += P.b[(int64_t)w * NR + h];
+        st.bnorm2[k] = bb;
+        st.relres[k] = bb > 0.0 ? 1.0 : 0.0;
+        st.best[k] = st.relres[k];
+        st.active[k] = bb > 0.0 ? 1 : 0;
+        st.status[k] = bb > 0.0 ? SYS_RUNNING : SYS_CONVERGED;
+        st.stall[k] = 0;
+        st.save[k] = 0;
+        st.restart[k] = 0;
+        st.restart[nsys_struct * NR + k] = 0;
+        return;
+    }
+    if (!st.active[k]) {
+        st.save[k] = 0;
+        return;
+    }
+    double tt = 0.0, rr = 0.0;
+    for (int w = s0; w < s1; ++w) {
+        tt += P.t[(int64_t)w * NR + h];
+        rr += P.rr(it)[(int64_t)w * NR + h];
+    }
+    const double rel = sqrt(tt / st.bnorm2[k]);
+    st.relres[k] = rel;
+    const double best = st.best[k];
+    const bool improved = rel < best;
+    st.save[k] = improved ? 1 : 0;
+    if (improved) st.best[k] = rel;
+    if (rel <= 0.9 * best) st.stall[k] = it;   // iteration of the last 10% gain
+    const int window = it / 2 > 2048 ? it / 2 : 2048;
+    if (rel <= tol || rr == 0.0) {
+        st.active[k] = 0;
+        st.status[k] = SYS_CONVERGED;
+    } else if (!isfinite(rel) || rel > 1e3 * st.best[k] || it - st.stall[k] > window) {
+        st.active[k] = 0;
+        st.status[k] = SYS_STALLED;
     }
 }
 
-// s = A r (A = I + off-diagonal scaled blocks); p = r + beta p; q = s + beta q; <p,q>.
+// cg1 of iteration `it`: s = A r (A = I + scaled off-diagonal blocks, sliced
+// ELL, one lane per block row); p = r + beta p; q = s + beta q; partial <p,q>.
 template <typename MT, typename VT, int NR>
-__global__ __launch_bounds__(SOLVER_WG) void k_cg1(Rows rows, SysState st, const int32_t* __restrict__ rowptr,
-                                                   const int32_t* __restrict__ col, const MT* __restrict__ A,
-                                                   const VT* __restrict__ rlo, VT* __restrict__ p,
-                                                   VT* __restrict__ q) {
-    __shared__ double lds[NWAVES * NR];
-    const int wg = blockIdx.x + rows.wg_base;
-    const int sys = rows.wg_sys[wg];
-    const int64_t R = 6 * (int64_t)rows.wg_row0[wg] + threadIdx.x;
-    const bool on = R < 6 * (int64_t)rows.wg_row1[wg];
+__global__ __launch_bounds__(64) void k_cg1(Slices sl, Partials P, SysState st, int nsys, int it,
+                                            const MT* __restrict__ A, const VT* __restrict__ rlo,
+                                            VT* __restrict__ p, VT* __restrict__ q) {
+    __shared__ double lds[2 * NR];
+    const int w = blockIdx.x + sl.base;
+    const int sys = sl.sys[w];
+    const int lane = threadIdx.x;
+    const int64_t i = (int64_t)sl.row0[w] + lane;
+    const bool on = i < sl.row1[w];
+    // beta from the two alternating <r,r> buffers
+    double rr_new[NR], rr_old[NR];
+    system_totals<NR, 1>(sl, sys, P.rr(it - 1), P.rr(it), rr_new, rr_old, lds);
+    VT beta[NR];
+    bool act[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+        const int k = sys * NR + h;
+        act[h] = st.active[k] != 0;
+        const bool restart = it == 1 || st.restart[((it - 1) & 1) * nsys + k] != 0;
+        beta[h] = (restart || !(rr_old[h] > 0.0)) ? VT(0) : VT(rr_new[h] / rr_old[h]);
+    }
+    const int64_t ii = on ? i : (int64_t)sl.row0[w];
+    const VT* ri = rlo + 6 * ii * NR;
+    VT acc[6][NR];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int h = 0; h < NR; ++h) acc[c][h] = ri[c * NR + h];
+    const int v0 = sl.voff[w], v1 = sl.voff[w + 1];
+    const MT* Aw = A + 36 * (int64_t)v0 + lane;
+    const int32_t* cw = sl.ell_col + v0 + lane;
+    const int nslot = (v1 - v0) / ROWS_PER_WG;
+    for (int j = 0; j < nslot; ++j) {
+        const int64_t col = cw[(int64_t)j * ROWS_PER_WG];
+        VT xj[6][NR];
+        const VT* xs = rlo + 6 * col * NR;
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int h = 0; h < NR; ++h) xj[c][h] = xs[c * NR + h];
+        const MT* a = Aw + (int64_t)j * 36 * ROWS_PER_WG;
+#pragma unroll
+        for (int e = 0; e < 36; ++e) {
+            const VT av = (VT)a[e * ROWS_PER_WG];
+#pragma unroll
+            for (int h = 0; h < NR; ++h) acc[e / 6][h] += av * xj[e % 6][h];
+        }
+    }
     double v[NR];
 #pragma unroll
     for (int h = 0; h < NR; ++h) v[h] = 0.0;
     if (on) {
-        const int64_t i = R / 6;
-        const int r = (int)(R % 6);
-        VT acc[NR];
 #pragma unroll
-        for (int h = 0; h < NR; ++h) acc[h] = rlo[R * NR + h];
-        const int k0 = rowptr[i], k1 = rowptr[i + 1];
-        for (int k = k0; k < k1; ++k) {
-            const MT* a = A + (int64_t)k * 36 + r * 6;
-            const VT* xj = rlo + (int64_t)6 * col[k] * NR;
+        for (int c = 0; c < 6; ++c)
 #pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                const VT av = (VT)a[c];
-#pragma unroll
-                for (int h = 0; h < NR; ++h) acc[h] += av * xj[c * NR + h];
+            for (int h = 0; h < NR; ++h) {
+                if (!act[h]) continue;
+                const int64_t o = (6 * i + c) * NR + h;
+                const VT pn = ri[c * NR + h] + beta[h] * p[o];
+                const VT qn = acc[c][h] + beta[h] * q[o];
+                p[o] = pn;
+                q[o] = qn;
+                v[h] += (double)pn * (double)qn;
             }
-        }
-#pragma unroll
-        for (int h = 0; h < NR; ++h) {
-            const int ks = sys * NR + h;
-            if (!st.active[ks]) continue;
-            const VT beta = (VT)st.beta[ks];
-            const int64_t o = R * NR + h;
-            const VT pn = rlo[o] + beta * p[o];
-            const VT qn = acc[h] + beta * q[o];
-            p[o] = pn;
-            q[o] = qn;
-            v[h] = (double)pn * (double)qn;
-        }
     }
-    double tot[NR];
-    if (reduce_system<NR>(v, tot, lds, wg, sys, rows, st) && threadIdx.x == 0) {
-#pragma unroll
-        for (int h = 0; h < NR; ++h) {
-            const int k = sys * NR + h;
-            if (!st.active[k]) continue;
-            if (tot[h] > 0.0 && isfinite(tot[h])) {
-                st.alpha[k] = st.rr[k] / tot[h];
-            } else {
-                st.alpha[k] = 0.0;  // breakdown: no step, restart the direction
-                st.restart[k] = 1;
-            }
-        }
-    }
+    write_partials<NR, 1>(v, lds, P.pq, w);
 }
 
-// x += alpha p; r -= alpha q; <r,r> -> beta.
+// cg2 of iteration `it`: alpha = rr / <p,q>; x += alpha p; r -= alpha q; partial <r,r>.
 template <typename VT, int NR>
-__global__ __launch_bounds__(SOLVER_WG) void k_cg2(Rows rows, SysState st, const VT* __restrict__ p,
-                                                   const VT* __restrict__ q, VT* __restrict__ xlo,
-                                                   VT* __restrict__ rlo) {
-    __shared__ double lds[NWAVES * NR];
-    const int wg = blockIdx.x + rows.wg_base;
-    const int sys = rows.wg_sys[wg];
-    const int64_t R = 6 * (int64_t)rows.wg_row0[wg] + threadIdx.x;
-    const bool on = R < 6 * (int64_t)rows.wg_row1[wg];
+__global__ __launch_bounds__(SOLVER_WG) void k_cg2(Slices sl, Partials P, SysState st, int nsys, int it,
+                                                   const VT* __restrict__ p, const VT* __restrict__ q,
+                                                   VT* __restrict__ xlo, VT* __restrict__ rlo) {
+    __shared__ double lds[NWAVES * 2 * NR];
+    const int w = blockIdx.x + sl.base;
+    const int sys = sl.sys[w];
+    double rr[NR], pq[NR];
+    system_totals<NR, NWAVES>(sl, sys, P.rr(it - 1), P.pq, rr, pq, lds);
+    VT alpha[NR];
+    bool act[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+        const int k = sys * NR + h;
+        act[h] = st.active[k] != 0;
+        const bool ok = pq[h] > 0.0 && isfinite(pq[h]);
+        alpha[h] = ok ? VT(rr[h] / pq[h]) : VT(0);
+        if (threadIdx.x == 0 && w == sl.sys_off[sys]) st.restart[(it & 1) * nsys + k] = ok ? 0 : 1;
+    }
+    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
+    const bool on = R < 6 * (int64_t)sl.row1[w];
     double v[NR];
 #pragma unroll
     for (int h = 0; h < NR; ++h) v[h] = 0.0;
     if (on) {
 #pragma unroll
         for (int h = 0; h < NR; ++h) {
-            const int ks = sys * NR + h;
-            if (!st.active[ks]) continue;
-            const VT alpha = (VT)st.alpha[ks];
             const int64_t o = R * NR + h;
-            xlo[o] += alpha * p[o];
-            const VT rn = rlo[o] - alpha * q[o];
-            rlo[o] = rn;
+            VT rn = rlo[o];
+            if (act[h]) {
+                xlo[o] += alpha[h] * p[o];
+                rn = rn - alpha[h] * q[o];
+                rlo[o] = rn;
+            }
             v[h] = (double)rn * (double)rn;
         }
     }
-    double tot[NR];
-    if (reduce_system<NR>(v, tot, lds, wg, sys, rows, st) && threadIdx.x == 0) {
-#pragma unroll
-        for (int h = 0; h < NR; ++h) {
-            const int k = sys * NR + h;
-            if (!st.active[k]) continue;
-            st.rr_prev[k] = st.rr[k];
-            st.rr[k] = tot[h];
-            st.beta[k] = (st.rr_prev[k] > 0.0 && !st.restart[k]) ? tot[h] / st.rr_prev[k] : 0.0;
-            st.restart[k] = 0;
-        }
-    }
+    write_partials<NR, NWAVES>(v, lds, P.rr(it), w);
 }
 
 // x64 += xlo; xlo = 0.
 template <typename VT, int NR>
-__global__ __launch_bounds__(SOLVER_WG) void k_fold(Rows rows, const int32_t* __restrict__ active,
+__global__ __launch_bounds__(SOLVER_WG) void k_fold(Slices sl, const int32_t* __restrict__ active,
                                                     VT* __restrict__ xlo, double* __restrict__ x64) {
-    const int wg = blockIdx.x + rows.wg_base;
-    const int sys = rows.wg_sys[wg];
-    const int64_t R = 6 * (int64_t)rows.wg_row0[wg] + threadIdx.x;
-    if (R >= 6 * (int64_t)rows.wg_row1[wg]) return;
+    const int w = blockIdx.x + sl.base;
+    const int sys = sl.sys[w];
+    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
+    if (R >= 6 * (int64_t)sl.row1[w]) return;
 #pragma unroll
     for (int h = 0; h < NR; ++h) {
         if (!active[sys * NR + h]) continue;
@@ -347,82 +409,79 @@ __global__ __launch_bounds__(SOLVER_WG) void k_fold(Rows rows, const int32_t* __
     }
 }
 
-// r64 = b - A64 x64; rlo = r64; <r64,r64>, <rlo,rlo>; convergence test.
+// After cg2 of iteration `it`: r64 = b - A64 x64, rlo = r64; partials <r64,r64>
+// (FP64, convergence) and <rlo,rlo> (replaces the recursive <r,r> of `it`).
 template <typename VT, int NR>
-__global__ __launch_bounds__(SOLVER_WG) void k_replace(Rows rows, SysState st, const int32_t* __restrict__ rowptr,
-                                                       const int32_t* __restrict__ col,
-                                                       const double* __restrict__ A, const double* __restrict__ b64,
-                                                       const double* __restrict__ x64, double* __restrict__ r64,
-                                                       VT* __restrict__ rlo, double tol, int it) {
-    __shared__ double lds[NWAVES * 2 * NR];
-    const int wg = blockIdx.x + rows.wg_base;
-    const int sys = rows.wg_sys[wg];
-    const int64_t R = 6 * (int64_t)rows.wg_row0[wg] + threadIdx.x;
-    const bool on = R < 6 * (int64_t)rows.wg_row1[wg];
-    double v[2 * NR];
+__global__ __launch_bounds__(64) void k_replace(Slices sl, Partials P, SysState st, int it,
+                                                const double* __restrict__ A, const double* __restrict__ b64,
+                                                const double* __restrict__ x64, double* __restrict__ r64,
+                                                VT* __restrict__ rlo) {
+    __shared__ double lds[2 * NR];
+    const int w = blockIdx.x + sl.base;
+    const int sys = sl.sys[w];
+    const int lane = threadIdx.x;
+    const int64_t i = (int64_t)sl.row0[w] + lane;
+    const bool on = i < sl.row1[w];
+    const int64_t ii = on ? i : (int64_t)sl.row0[w];
+    double acc[6][NR];
 #pragma unroll
-    for (int i = 0; i < 2 * NR; ++i) v[i] = 0.0;
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int h = 0; h < NR; ++h) acc[c][h] = x64[(6 * ii + c) * NR + h];
+    const int v0 = sl.voff[w], v1 = sl.voff[w + 1];
+    const double* Aw = A + 36 * (int64_t)v0 + lane;
+    const int32_t* cw = sl.ell_col + v0 + lane;
+    const int nslot = (v1 - v0) / ROWS_PER_WG;
+    for (int j = 0; j < nslot; ++j) {
+        const int64_t col = cw[(int64_t)j * ROWS_PER_WG];
+        const double* xs = x64 + 6 * col * NR;
+        double xj[6][NR];
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int h = 0; h < NR; ++h) xj[c][h] = xs[c * NR + h];
+        const double* a = Aw + (int64_t)j * 36 * ROWS_PER_WG;
+#pragma unroll
+        for (int e = 0; e < 36; ++e) {
+            const double av = a[e * ROWS_PER_WG];
+#pragma unroll
+            for (int h = 0; h < NR; ++h) acc[e / 6][h] += av * xj[e % 6][h];
+        }
+    }
+    double vt[NR], vr[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) vt[h] = vr[h] = 0.0;
     if (on) {
-        const int64_t i = R / 6;
-        const int r = (int)(R % 6);
-        double acc[NR];
 #pragma unroll
-        for (int h = 0; h < NR; ++h) acc[h] = x64[R * NR + h];
-        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-            const double* a = A + (int64_t)k * 36 + r * 6;
-            const double* xj = x64 + (int64_t)6 * col[k] * NR;
+        for (int c = 0; c < 6; ++c)
 #pragma unroll
-            for (int c = 0; c < 6; ++c)
-#pragma unroll
-                for (int h = 0; h < NR; ++h) acc[h] += a[c] * xj[c * NR + h];
-        }
-#pragma unroll
-        for (int h = 0; h < NR; ++h) {
-            if (!st.active[sys * NR + h]) continue;
-            const int64_t o = R * NR + h;
-            const double rv = b64[o] - acc[h];
-            r64[o] = rv;
-            const VT rl = VT(rv);
-            rlo[o] = rl;
-            v[h] = rv * rv;
-            v[NR + h] = (double)rl * (double)rl;
-        }
-    }
-    double tot[2 * NR];
-    if (reduce_system<2 * NR>(v, tot, lds, wg, sys, rows, st) && threadIdx.x == 0) {
-#pragma unroll
-        for (int h = 0; h < NR; ++h) {
-            const int k = sys * NR + h;
-            if (!st.active[k]) continue;
-            const double rel = sqrt(tot[h] / st.bnorm2[k]);
-            st.relres[k] = rel;
-            st.beta[k] = st.rr_prev[k] > 0.0 ? tot[NR + h] / st.rr_prev[k] : 0.0;
-            st.rr[k] = tot[NR + h];
-            const double best = st.best[k];
-            const bool improved = rel < best;
-            st.save[k] = improved ? 1 : 0;
-            if (improved) st.best[k] = rel;
-            if (rel <= 0.9 * best) st.stall[k] = it;   // iteration of the last 10% gain
-            const int window = it / 2 > 2048 ? it / 2 : 2048;
-            if (rel <= tol || tot[NR + h] == 0.0) {
-                st.active[k] = 0;
-                st.status[k] = SYS_CONVERGED;
-            } else if (!isfinite(rel) || rel > 1e3 * st.best[k] || it - st.stall[k] > window) {
-                st.active[k] = 0;
-                st.status[k] = SYS_STALLED;
+            for (int h = 0; h < NR; ++h) {
+                const int64_t o = (6 * i + c) * NR + h;
+                if (!st.active[sys * NR + h]) {
+                    const double rl = (double)rlo[o];
+                    vr[h] += rl * rl;
+                    continue;
+                }
+                const double rv = b64[o] - acc[c][h];
+                r64[o] = rv;
+                const VT rl = VT(rv);
+                rlo[o] = rl;
+                vt[h] += rv * rv;
+                vr[h] += (double)rl * (double)rl;
             }
-        }
     }
+    write_partials<NR, 1>(vt, lds, P.t, w);
+    write_partials<NR, 1>(vr, lds, P.rr(it), w);
 }
 
 // keep the best iterate: xbest = x64 where the last replacement improved it
 template <int NR>
-__global__ __launch_bounds__(SOLVER_WG) void k_save(Rows rows, const int32_t* __restrict__ save,
+__global__ __launch_bounds__(SOLVER_WG) void k_save(Slices sl, const int32_t* __restrict__ save,
                                                     const double* __restrict__ x64, double* __restrict__ xbest) {
-    const int wg = blockIdx.x + rows.wg_base;
-    const int sys = rows.wg_sys[wg];
-    const int64_t R = 6 * (int64_t)rows.wg_row0[wg] + threadIdx.x;
-    if (R >= 6 * (int64_t)rows.wg_row1[wg]) return;
+    const int w = blockIdx.x + sl.base;
+    const int sys = sl.sys[w];
+    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
+    if (R >= 6 * (int64_t)sl.row1[w]) return;
 #pragma unroll
     for (int h = 0; h < NR; ++h)
         if (save[sys * NR + h]) xbest[R * NR + h] = x64[R * NR + h];
@@ -452,53 +511,57 @@ struct Workspace {
     double *b64, *x64, *r64, *xbest;
     void *xlo, *rlo, *p, *q;
     SysState st;
+    Partials P;
 };
+
+enum { WS_B, WS_X, WS_R, WS_XLO, WS_RLO, WS_P, WS_Q, WS_PART, WS_SD, WS_SI, WS_UNUSED, WS_ADJ, WS_XBEST, WS_N };
 
 static int get_workspace(Plan* pl, int nrhs, Workspace& w) {
     const size_t n = 6 * (size_t)pl->n_nodes * nrhs;
     const size_t nsys = (size_t)pl->n_sys * nrhs;
-    const size_t need[] = {n * 8, n * 8, n * 8, n * 8, n * 8, n * 8, n * 8,
-                           (size_t)pl->n_wg * 4 * nrhs * 8, nsys * 8 * 7, nsys * 4 * 5, (size_t)pl->n_sys * 4,
-                           0, n * 8};
-    const int nb = sizeof(need) / sizeof(need[0]);
-    if (pl->ws.size() < 13) pl->ws.resize(13);  // slot 11: adjoint scratch
-    for (int i = 0; i < nb; ++i) {
-        if (i == 11) continue;
+    const size_t npart = (size_t)pl->n_wg * nrhs;
+    size_t need[WS_N] = {};
+    need[WS_B] = need[WS_X] = need[WS_R] = need[WS_XLO] = need[WS_RLO] = need[WS_P] = need[WS_Q] = n * 8;
+    need[WS_XBEST] = n * 8;
+    need[WS_PART] = npart * 8 * 5;
+    need[WS_SD] = nsys * 8 * 3;
+    need[WS_SI] = nsys * 4 * 6;
+    if (pl->ws.size() < WS_N) pl->ws.resize(WS_N);
+    for (int i = 0; i < WS_N; ++i) {
+        if (i == WS_ADJ || need[i] == 0) continue;
         if (pl->ws[i].bytes < need[i]) {
             if (pl->ws[i].p) JSSO_HIP(hipFree(pl->ws[i].p));
             pl->ws[i].p = nullptr;
             pl->ws[i].bytes = 0;
             hipError_t e = hipMalloc(&pl->ws[i].p, need[i]);
-            JSSO_REQUIRE(e == hipSuccess, JSSO_ENOMEM, "workspace hipMalloc(%zu): %s", need[i],
-                         hipGetErrorString(e));
+            JSSO_REQUIRE(e == hipSuccess, JSSO_ENOMEM, "workspace hipMalloc(%zu): %s", need[i], hipGetErrorString(e));
             pl->ws[i].bytes = need[i];
-            if (i == 10) JSSO_HIP(hipMemset(pl->ws[i].p, 0, need[i]));
         }
     }
-    w.b64 = (double*)pl->ws[0].p;
-    w.x64 = (double*)pl->ws[1].p;
-    w.r64 = (double*)pl->ws[2].p;
-    w.xlo = pl->ws[3].p;
-    w.rlo = pl->ws[4].p;
-    w.p = pl->ws[5].p;
-    w.q = pl->ws[6].p;
-    w.st.partial = (double*)pl->ws[7].p;
-    double* sc = (double*)pl->ws[8].p;
-    w.st.bnorm2 = sc;
-    w.st.rr = sc + nsys;
-    w.st.rr_prev = sc + 2 * nsys;
-    w.st.alpha = sc + 3 * nsys;
-    w.st.beta = sc + 4 * nsys;
-    w.st.relres = sc + 5 * nsys;
-    w.st.best = sc + 6 * nsys;
-    int32_t* ic = (int32_t*)pl->ws[9].p;
-    w.st.active = ic;
-    w.st.status = ic + nsys;
-    w.st.stall = ic + 2 * nsys;
-    w.st.restart = ic + 3 * nsys;
-    w.st.save = ic + 4 * nsys;
-    w.st.counter = (int32_t*)pl->ws[10].p;
-    w.xbest = (double*)pl->ws[12].p;
+    w.b64 = (double*)pl->ws[WS_B].p;
+    w.x64 = (double*)pl->ws[WS_X].p;
+    w.r64 = (double*)pl->ws[WS_R].p;
+    w.xlo = pl->ws[WS_XLO].p;
+    w.rlo = pl->ws[WS_RLO].p;
+    w.p = pl->ws[WS_P].p;
+    w.q = pl->ws[WS_Q].p;
+    w.xbest = (double*)pl->ws[WS_XBEST].p;
+    double* pp = (double*)pl->ws[WS_PART].p;
+    w.P.pq = pp;
+    w.P.rr0 = pp + npart;
+    w.P.rr1 = pp + 2 * npart;
+    w.P.b = pp + 3 * npart;
+    w.P.t = pp + 4 * npart;
+    double* sd = (double*)pl->ws[WS_SD].p;
+    w.st.bnorm2 = sd;
+    w.st.relres = sd + nsys;
+    w.st.best = sd + 2 * nsys;
+    int32_t* si = (int32_t*)pl->ws[WS_SI].p;
+    w.st.active = si;
+    w.st.status = si + nsys;
+    w.st.stall = si + 2 * nsys;
+    w.st.save = si + 3 * nsys;
+    w.st.restart = si + 4 * nsys;   // [2][nsys]
     return JSSO_OK;
 }
 
@@ -516,42 +579,48 @@ static int check_status(Plan* p, hipStream_t s) {
     return JSSO_OK;
 }
 
+static Slices make_slices(Plan* pl, int base) {
+    return Slices{pl->d_wg_sys, pl->d_wg_row0, pl->d_wg_row1, pl->d_sys_wg_off, pl->d_slice_voff, pl->d_ell_col, base};
+}
+
 template <typename MT, typename VT, int NR>
 static int run_batched(Plan* pl, const double* f, double* u, double tol, int maxit, int replace_every,
                        jsso_info* info, hipStream_t s) {
     Workspace w;
     int rc = get_workspace(pl, NR, w);
     if (rc) return rc;
-    Rows rows{pl->d_wg_sys, pl->d_wg_row0, pl->d_wg_row1, pl->d_sys_wg_off, 0};
-    const dim3 grid(pl->n_wg), block(SOLVER_WG);
-    const MT* A = nullptr;
-    if constexpr (sizeof(MT) == 4) A = (const MT*)pl->d_Ks32; else A = (const MT*)pl->d_Ks64;
-    VT *xlo = (VT*)w.xlo, *rlo = (VT*)w.rlo, *p = (VT*)w.p, *q = (VT*)w.q;
-    hipLaunchKernelGGL((k_init<VT, NR>), grid, block, 0, s, rows, w.st, pl->d_fixed, pl->d_Linv, f, w.b64,
-                       w.x64, w.r64, xlo, rlo, p, q, w.xbest);
-    JSSO_HIP(hipGetLastError());
+    const Slices sl = make_slices(pl, 0);
+    const dim3 gridS(pl->n_wg), blockS(SOLVER_WG), block64(64);
     const int nsys = pl->n_sys * NR;
+    const dim3 gridD((nsys + 127) / 128), blockD(128);
+    const MT* A = nullptr;
+    if constexpr (sizeof(MT) == 4) A = (const MT*)pl->d_Ke32; else A = (const MT*)pl->d_Ke64;
+    VT *xlo = (VT*)w.xlo, *rlo = (VT*)w.rlo, *p = (VT*)w.p, *q = (VT*)w.q;
+    hipLaunchKernelGGL((k_init<VT, NR>), gridS, blockS, 0, s, sl, w.P, pl->d_fixed, pl->d_Linv, f, w.b64, w.x64,
+                       w.xbest, xlo, rlo, p, q);
+    hipLaunchKernelGGL((k_decide<NR>), gridD, blockD, 0, s, sl, w.P, w.st, pl->n_sys, 0, tol, 0);
+    JSSO_HIP(hipGetLastError());
     std::vector<int32_t> act(nsys);
     int it = 0, repl = 0;
     bool done = false;
     while (!done && it < maxit) {
         const int steps = std::min(replace_every, maxit - it);
         for (int j = 0; j < steps; ++j) {
+            ++it;
             std::pair<hipEvent_t, hipEvent_t>* ev;
             timing_begin(s, ev);
-            hipLaunchKernelGGL((k_cg1<MT, VT, NR>), grid, block, 0, s, rows, w.st, pl->d_offrowptr,
-                               pl->d_offcol, A, (const VT*)rlo, p, q);
+            hipLaunchKernelGGL((k_cg1<MT, VT, NR>), gridS, block64, 0, s, sl, w.P, w.st, nsys, it, A,
+                               (const VT*)rlo, p, q);
             timing_end(s, ev);
-            hipLaunchKernelGGL((k_cg2<VT, NR>), grid, block, 0, s, rows, w.st, (const VT*)p, (const VT*)q,
-                               xlo, rlo);
+            hipLaunchKernelGGL((k_cg2<VT, NR>), gridS, blockS, 0, s, sl, w.P, w.st, nsys, it, (const VT*)p,
+                               (const VT*)q, xlo, rlo);
         }
-        it += steps;
-        hipLaunchKernelGGL((k_fold<VT, NR>), grid, block, 0, s, rows, (const int32_t*)w.st.active, xlo, w.x64);
-        hipLaunchKernelGGL((k_replace<VT, NR>), grid, block, 0, s, rows, w.st, pl->d_offrowptr, pl->d_offcol,
-                           (const double*)pl->d_Ks64, (const double*)w.b64, (const double*)w.x64, w.r64, rlo,
-                           tol, it);
-        hipLaunchKernelGGL((k_save<NR>), grid, block, 0, s, rows, (const int32_t*)w.st.save,
-                           (const double*)w.x64, w.xbest);
+        hipLaunchKernelGGL((k_fold<VT, NR>), gridS, blockS, 0, s, sl, (const int32_t*)w.st.active, xlo, w.x64);
+        hipLaunchKernelGGL((k_replace<VT, NR>), gridS, block64, 0, s, sl, w.P, w.st, it, (const double*)pl->d_Ke64,
+                           (const double*)w.b64, (const double*)w.x64, w.r64, rlo);
+        hipLaunchKernelGGL((k_decide<NR>), gridD, blockD, 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, it);
+        hipLaunchKernelGGL((k_save<NR>), gridS, blockS, 0, s, sl, (const int32_t*)w.st.save, (const double*)w.x64,
+                           w.xbest);
         JSSO_HIP(hipGetLastError());
         ++repl;
         JSSO_HIP(hipMemcpyAsync(act.data(), w.st.active, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -609,18 +678,19 @@ int solve_batched(Plan* p, const double* f, double* u, int nrhs, bool mixed, dou
 }
 
 // ---------------------------------------------------------------------------
-// plain path: block-Jacobi PCG on Kc (FP64), one system per launch sequence
+// plain path: block-Jacobi PCG on Kc (FP64, block CSR), one system per launch
+// sequence; reductions as above (per-slice partials summed by the consumers).
+// <r,z> partials alternate between P.rr0 / P.rr1; <r0,z0> lives in P.b.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(SOLVER_WG) void kp_init(Rows rows, SysState st, int sysk, int h, int nrhs,
+__global__ __launch_bounds__(SOLVER_WG) void kp_init(Slices sl, Partials P, int h, int nrhs,
                                                      const uint8_t* __restrict__ fixed,
                                                      const double* __restrict__ Minv, const double* __restrict__ f,
                                                      double* __restrict__ x, double* __restrict__ r,
                                                      double* __restrict__ z, double* __restrict__ pp) {
     __shared__ double lds[NWAVES];
-    const int wg = blockIdx.x + rows.wg_base;
-    const int sys = rows.wg_sys[wg];
-    const int64_t R = 6 * (int64_t)rows.wg_row0[wg] + threadIdx.x;
-    const bool on = R < 6 * (int64_t)rows.wg_row1[wg];
+    const int w = blockIdx.x + sl.base;
+    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
+    const bool on = R < 6 * (int64_t)sl.row1[w];
     double v[1] = {0.0};
     if (on) {
         const int64_t i = R / 6;
@@ -639,26 +709,31 @@ __global__ __launch_bounds__(SOLVER_WG) void kp_init(Rows rows, SysState st, int
         pp[R] = zs;
         v[0] = rv * zs;
     }
-    double tot[1];
-    if (reduce_system<1>(v, tot, lds, wg, sys, rows, st) && threadIdx.x == 0) {
-        st.bnorm2[sysk] = tot[0];
-        st.rr[sysk] = tot[0];
-        st.active[sysk] = tot[0] > 0.0 ? 1 : 0;
-        st.relres[sysk] = tot[0] > 0.0 ? 1.0 : 0.0;
-    }
+    double v2[1] = {v[0]};
+    write_partials<1, NWAVES>(v, lds, P.rr0, w);
+    write_partials<1, NWAVES>(v2, lds, P.b, w);
 }
 
-__global__ __launch_bounds__(SOLVER_WG) void kp_spmv(Rows rows, SysState st, int sysk,
+// one thread: anything to solve?
+__global__ void kp_start(Slices sl, Partials P, int32_t* active, double* relres) {
+    const int sys = sl.sys[sl.base];
+    double rz0 = 0.0;
+    for (int w = sl.sys_off[sys]; w < sl.sys_off[sys + 1]; ++w) rz0 += P.b[w];
+    active[0] = rz0 > 0.0 ? 1 : 0;
+    relres[0] = rz0 > 0.0 ? 1.0 : 0.0;
+}
+
+// q = Kc p; partial <p,q>.
+__global__ __launch_bounds__(SOLVER_WG) void kp_spmv(Slices sl, Partials P, const int32_t* __restrict__ active,
                                                      const int32_t* __restrict__ rowptr,
                                                      const int32_t* __restrict__ col, const double* __restrict__ K,
                                                      const double* __restrict__ pp, double* __restrict__ q) {
     __shared__ double lds[NWAVES];
-    const int wg = blockIdx.x + rows.wg_base;
-    const int sys = rows.wg_sys[wg];
-    const int64_t R = 6 * (int64_t)rows.wg_row0[wg] + threadIdx.x;
-    const bool on = R < 6 * (int64_t)rows.wg_row1[wg];
+    const int w = blockIdx.x + sl.base;
+    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
+    const bool on = R < 6 * (int64_t)sl.row1[w];
     double v[1] = {0.0};
-    if (on && st.active[sysk]) {
+    if (on && active[0]) {
         const int64_t i = R / 6;
         const int r = (int)(R % 6);
         double acc = 0.0;
@@ -671,63 +746,71 @@ __global__ __launch_bounds__(SOLVER_WG) void kp_spmv(Rows rows, SysState st, int
         q[R] = acc;
         v[0] = pp[R] * acc;
     }
-    double tot[1];
-    if (reduce_system<1>(v, tot, lds, wg, sys, rows, st) && threadIdx.x == 0) {
-        if (st.active[sysk]) st.alpha[sysk] = tot[0] > 0.0 ? st.rr[sysk] / tot[0] : 0.0;
-    }
+    write_partials<1, NWAVES>(v, lds, P.pq, w);
 }
 
-__global__ __launch_bounds__(SOLVER_WG) void kp_update(Rows rows, SysState st, int sysk,
-                                                       const double* __restrict__ Minv, const double* __restrict__ pp,
-                                                       const double* __restrict__ q, double* __restrict__ x,
-                                                       double* __restrict__ r, double* __restrict__ z, double tol) {
-    __shared__ double lds[NWAVES];
-    const int wg = blockIdx.x + rows.wg_base;
-    const int sys = rows.wg_sys[wg];
-    const int64_t R = 6 * (int64_t)rows.wg_row0[wg] + threadIdx.x;
-    const bool on = R < 6 * (int64_t)rows.wg_row1[wg];
+// alpha = rz / <p,q>; x += alpha p; r -= alpha q; z = M r; partial <r,z> into rr(it).
+__global__ __launch_bounds__(SOLVER_WG) void kp_update(Slices sl, Partials P, const int32_t* __restrict__ active,
+                                                       int it, const double* __restrict__ Minv,
+                                                       const double* __restrict__ pp, const double* __restrict__ q,
+                                                       double* __restrict__ x, double* __restrict__ r,
+                                                       double* __restrict__ z) {
+    __shared__ double lds[NWAVES * 2];
     __shared__ double rs[SOLVER_WG];
-    double v[1] = {0.0};
-    const bool act = st.active[sysk] != 0;
-    const double alpha = st.alpha[sysk];
+    const int w = blockIdx.x + sl.base;
+    double rz[1], pq[1];
+    system_totals<1, NWAVES>(sl, sl.sys[w], P.rr(it - 1), P.pq, rz, pq, lds);
+    const bool act = active[0] != 0;
+    const double alpha = (pq[0] > 0.0 && isfinite(pq[0])) ? rz[0] / pq[0] : 0.0;
+    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
+    const bool on = R < 6 * (int64_t)sl.row1[w];
     double rn = 0.0;
-    if (on && act) {
-        x[R] += alpha * pp[R];
-        rn = r[R] - alpha * q[R];
-        r[R] = rn;
+    if (on) {
+        rn = r[R];
+        if (act) {
+            x[R] += alpha * pp[R];
+            rn -= alpha * q[R];
+            r[R] = rn;
+        }
     }
     rs[threadIdx.x] = rn;
     __syncthreads();
-    if (on && act) {
+    double v[1] = {0.0};
+    if (on) {
         const int64_t i = R / 6;
         const int rr = (int)(R % 6);
         const int base = (int)threadIdx.x - rr;
         double zs = 0.0;
 #pragma unroll
         for (int c = 0; c < 6; ++c) zs += Minv[i * 36 + rr * 6 + c] * rs[base + c];
-        z[R] = zs;
+        if (act) z[R] = zs;
         v[0] = rn * zs;
     }
-    double tot[1];
-    if (reduce_system<1>(v, tot, lds, wg, sys, rows, st) && threadIdx.x == 0) {
-        if (act) {
-            const double rz_old = st.rr[sysk];
-            st.beta[sysk] = rz_old > 0.0 ? tot[0] / rz_old : 0.0;
-            st.rr[sysk] = tot[0];
-            const double rel = sqrt(fabs(tot[0]) / st.bnorm2[sysk]);
-            st.relres[sysk] = rel;
-            if (rel <= tol) st.active[sysk] = 0;
-        }
-    }
+    write_partials<1, NWAVES>(v, lds, P.rr(it), w);
 }
 
-__global__ __launch_bounds__(SOLVER_WG) void kp_dir(Rows rows, const SysState st, int sysk,
+// beta = rz_new / rz_old; p = z + beta p.  Convergence on sqrt(rz / rz0); the
+// first slice records it for the host and the following launches.
+__global__ __launch_bounds__(SOLVER_WG) void kp_dir(Slices sl, Partials P, int32_t* __restrict__ active,
+                                                    double* __restrict__ relres, int it, double tol,
                                                     const double* __restrict__ z, double* __restrict__ pp) {
-    const int wg = blockIdx.x + rows.wg_base;
-    const int64_t R = 6 * (int64_t)rows.wg_row0[wg] + threadIdx.x;
-    if (R >= 6 * (int64_t)rows.wg_row1[wg]) return;
-    if (!st.active[sysk]) return;
-    pp[R] = z[R] + st.beta[sysk] * pp[R];
+    __shared__ double lds[NWAVES * 2];
+    const int w = blockIdx.x + sl.base;
+    double rzn[1], rzo[1], rz0[1], unused[1];
+    system_totals<1, NWAVES>(sl, sl.sys[w], P.rr(it), P.rr(it - 1), rzn, rzo, lds);
+    system_totals<1, NWAVES>(sl, sl.sys[w], P.b, nullptr, rz0, unused, lds);
+    const bool act = active[0] != 0;
+    const double rel = sqrt(fabs(rzn[0]) / rz0[0]);
+    const bool conv = !(rel > tol);
+    __syncthreads();   // every thread has read active[0] before it may change
+    if (threadIdx.x == 0 && w == sl.base && act) {
+        relres[0] = rel;
+        if (conv) active[0] = 0;
+    }
+    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
+    if (!act || conv || R >= 6 * (int64_t)sl.row1[w]) return;
+    const double beta = rzo[0] > 0.0 ? rzn[0] / rzo[0] : 0.0;
+    pp[R] = z[R] + beta * pp[R];
 }
 
 __global__ void kp_store(int64_t r0, int64_t r1, int nrhs, int h, const double* __restrict__ x,
@@ -742,22 +825,21 @@ int solve_plain(Plan* pl, const double* f, double* u, int nrhs, double tol, int 
     int rc = check_status(pl, s);
     if (rc) return rc;
     Workspace w;
-    rc = get_workspace(pl, nrhs, w);
+    rc = get_workspace(pl, 1, w);
     if (rc) return rc;
     double *x = w.x64, *r = w.r64, *z = w.b64, *pp = (double*)w.p, *q = (double*)w.q;
+    int32_t* active = w.st.active;
+    double* relres = w.st.relres;
     const int nsys = pl->n_sys * nrhs;
     int itmax = 0, conv = 0;
     double mx = 0.0;
-    std::vector<int32_t> sys_wg(pl->h_sys_wg_off);
     for (int sidx = 0; sidx < pl->n_sys; ++sidx) {
-        const int w0 = sys_wg[sidx], w1 = sys_wg[sidx + 1];
-        if (w1 == w0) continue;
-        Rows rows{pl->d_wg_sys, pl->d_wg_row0, pl->d_wg_row1, pl->d_sys_wg_off, w0};
+        const int w0 = pl->h_sys_wg_off[sidx], w1 = pl->h_sys_wg_off[sidx + 1];
+        const Slices sl = make_slices(pl, w0);
         const dim3 grid(w1 - w0), block(SOLVER_WG);
         for (int h = 0; h < nrhs; ++h) {
-            const int sysk = sidx * nrhs + h;
-            hipLaunchKernelGGL(kp_init, grid, block, 0, s, rows, w.st, sysk, h, nrhs, pl->d_fixed, pl->d_Minv, f,
-                               x, r, z, pp);
+            hipLaunchKernelGGL(kp_init, grid, block, 0, s, sl, w.P, h, nrhs, pl->d_fixed, pl->d_Minv, f, x, r, z, pp);
+            hipLaunchKernelGGL(kp_start, dim3(1), dim3(1), 0, s, sl, w.P, active, relres);
             JSSO_HIP(hipGetLastError());
             int it = 0;
             int32_t act = 1;
@@ -765,19 +847,20 @@ int solve_plain(Plan* pl, const double* f, double* u, int nrhs, double tol, int 
             while (it < maxit) {
                 const int steps = std::min(32, maxit - it);
                 for (int j = 0; j < steps; ++j) {
-                    hipLaunchKernelGGL(kp_spmv, grid, block, 0, s, rows, w.st, sysk, pl->d_rowptr, pl->d_colind,
-                                       (const double*)pl->d_Kc, (const double*)pp, q);
-                    hipLaunchKernelGGL(kp_update, grid, block, 0, s, rows, w.st, sysk, (const double*)pl->d_Minv,
-                                       (const double*)pp, (const double*)q, x, r, z, tol);
-                    hipLaunchKernelGGL(kp_dir, grid, block, 0, s, rows, w.st, sysk, (const double*)z, pp);
+                    ++it;
+                    hipLaunchKernelGGL(kp_spmv, grid, block, 0, s, sl, w.P, (const int32_t*)active, pl->d_rowptr,
+                                       pl->d_colind, (const double*)pl->d_Kc, (const double*)pp, q);
+                    hipLaunchKernelGGL(kp_update, grid, block, 0, s, sl, w.P, (const int32_t*)active, it,
+                                       (const double*)pl->d_Minv, (const double*)pp, (const double*)q, x, r, z);
+                    hipLaunchKernelGGL(kp_dir, grid, block, 0, s, sl, w.P, active, relres, it, tol,
+                                       (const double*)z, pp);
                 }
-                it += steps;
                 JSSO_HIP(hipGetLastError());
-                JSSO_HIP(hipMemcpyAsync(&act, w.st.active + sysk, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+                JSSO_HIP(hipMemcpyAsync(&act, active, sizeof(int32_t), hipMemcpyDeviceToHost, s));
                 JSSO_HIP(hipStreamSynchronize(s));
                 if (!act) break;
             }
-            JSSO_HIP(hipMemcpy(&rel, w.st.relres + sysk, sizeof(double), hipMemcpyDeviceToHost));
+            JSSO_HIP(hipMemcpy(&rel, relres, sizeof(double), hipMemcpyDeviceToHost));
             const int64_t r0 = 6 * (int64_t)pl->h_node_off[sidx], r1 = 6 * (int64_t)pl->h_node_off[sidx + 1];
             hipLaunchKernelGGL(kp_store, dim3((unsigned)((r1 - r0 + 255) / 256)), dim3(256), 0, s, r0, r1, nrhs, h,
                                (const double*)x, u);
