@@ -34,6 +34,8 @@
 // Plain path (JSSO_PREC_PLAIN): textbook block-Jacobi PCG on the constrained
 // FP64 K in block CSR, thread per scalar row, one structure and one load case
 // per launch sequence -- the baseline the batched paths are measured against.
+#include <cstdlib>
+
 #include "jsso_internal.hpp"
 
 namespace jsso {
@@ -106,7 +108,21 @@ struct Slices {
     const int32_t *sys, *row0, *row1, *sys_off;  // slice -> structure, rows; structure -> first slice
     const int32_t *voff, *ell_col;                // sliced ELL
     int base;                                     // first slice of the launch (plain path)
+    int count;                                    // slices in the launch
+    int chunk;                                    // ceil(count / 8): slices per XCD
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs; map them so that each
+// XCD processes a contiguous run of slices, identically in every kernel: the
+// vectors a slice touches then stay in one XCD's L2 from kernel to kernel.
+// Returns -1 for the (at most 7) padding workgroups of the grid.
+__device__ __forceinline__ int slice_of_block(const Slices& sl) {
+    const int b = blockIdx.x;
+    const int l = (b & 7) * sl.chunk + (b >> 3);
+    return l < sl.count ? sl.base + l : -1;
+}
+
+static inline dim3 slice_grid(int count) { return dim3(8 * ((count + 7) / 8)); }
 
 // per-(structure, rhs) state, written only by k_decide (and, for restart, cg2)
 struct SysState {
@@ -120,6 +136,16 @@ struct Partials {
     double *pq, *rr0, *rr1, *b, *t;
     __host__ __device__ double* rr(int it) const { return (it & 1) ? rr1 : rr0; }
 };
+
+// true when every load case of structure `sys` has stopped: its slices have
+// nothing to do (and nothing will read their partial sums)
+template <int NR>
+__device__ __forceinline__ bool structure_done(const SysState& st, int sys) {
+    bool any = false;
+#pragma unroll
+    for (int h = 0; h < NR; ++h) any |= st.active[sys * NR + h] != 0;
+    return !any;
+}
 
 // all-lanes sum over one wavefront (butterfly: identical result in every lane)
 __device__ __forceinline__ double wave_sum(double v) {
@@ -196,7 +222,8 @@ __global__ __launch_bounds__(SOLVER_WG) void k_init(Slices sl, Partials P, const
                                                     double* __restrict__ xbest, VT* __restrict__ xlo,
                                                     VT* __restrict__ rlo, VT* __restrict__ p, VT* __restrict__ q) {
     __shared__ double lds[NWAVES * NR];
-    const int w = blockIdx.x + sl.base;
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
     const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
     const bool on = R < 6 * (int64_t)sl.row1[w];
     double vb[NR], vr[NR];
@@ -287,8 +314,10 @@ __global__ __launch_bounds__(64) void k_cg1(Slices sl, Partials P, SysState st, 
                                             const MT* __restrict__ A, const VT* __restrict__ rlo,
                                             VT* __restrict__ p, VT* __restrict__ q) {
     __shared__ double lds[2 * NR];
-    const int w = blockIdx.x + sl.base;
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
     const int sys = sl.sys[w];
+    if (structure_done<NR>(st, sys)) return;
     const int lane = threadIdx.x;
     const int64_t i = (int64_t)sl.row0[w] + lane;
     const bool on = i < sl.row1[w];
@@ -357,8 +386,10 @@ __global__ __launch_bounds__(SOLVER_WG) void k_cg2(Slices sl, Partials P, SysSta
                                                    const VT* __restrict__ p, const VT* __restrict__ q,
                                                    VT* __restrict__ xlo, VT* __restrict__ rlo) {
     __shared__ double lds[NWAVES * 2 * NR];
-    const int w = blockIdx.x + sl.base;
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
     const int sys = sl.sys[w];
+    if (structure_done<NR>(st, sys)) return;
     double rr[NR], pq[NR];
     system_totals<NR, NWAVES>(sl, sys, P.rr(it - 1), P.pq, rr, pq, lds);
     VT alpha[NR];
@@ -396,7 +427,8 @@ __global__ __launch_bounds__(SOLVER_WG) void k_cg2(Slices sl, Partials P, SysSta
 template <typename VT, int NR>
 __global__ __launch_bounds__(SOLVER_WG) void k_fold(Slices sl, const int32_t* __restrict__ active,
                                                     VT* __restrict__ xlo, double* __restrict__ x64) {
-    const int w = blockIdx.x + sl.base;
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
     const int sys = sl.sys[w];
     const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
     if (R >= 6 * (int64_t)sl.row1[w]) return;
@@ -417,8 +449,10 @@ __global__ __launch_bounds__(64) void k_replace(Slices sl, Partials P, SysState 
                                                 const double* __restrict__ x64, double* __restrict__ r64,
                                                 VT* __restrict__ rlo) {
     __shared__ double lds[2 * NR];
-    const int w = blockIdx.x + sl.base;
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
     const int sys = sl.sys[w];
+    if (structure_done<NR>(st, sys)) return;
     const int lane = threadIdx.x;
     const int64_t i = (int64_t)sl.row0[w] + lane;
     const bool on = i < sl.row1[w];
@@ -478,7 +512,8 @@ __global__ __launch_bounds__(64) void k_replace(Slices sl, Partials P, SysState 
 template <int NR>
 __global__ __launch_bounds__(SOLVER_WG) void k_save(Slices sl, const int32_t* __restrict__ save,
                                                     const double* __restrict__ x64, double* __restrict__ xbest) {
-    const int w = blockIdx.x + sl.base;
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
     const int sys = sl.sys[w];
     const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
     if (R >= 6 * (int64_t)sl.row1[w]) return;
@@ -579,8 +614,9 @@ static int check_status(Plan* p, hipStream_t s) {
     return JSSO_OK;
 }
 
-static Slices make_slices(Plan* pl, int base) {
-    return Slices{pl->d_wg_sys, pl->d_wg_row0, pl->d_wg_row1, pl->d_sys_wg_off, pl->d_slice_voff, pl->d_ell_col, base};
+static Slices make_slices(Plan* pl, int base, int count) {
+    return Slices{pl->d_wg_sys, pl->d_wg_row0, pl->d_wg_row1, pl->d_sys_wg_off, pl->d_slice_voff, pl->d_ell_col,
+                  base, count, (count + 7) / 8};
 }
 
 template <typename MT, typename VT, int NR>
@@ -589,8 +625,8 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
     Workspace w;
     int rc = get_workspace(pl, NR, w);
     if (rc) return rc;
-    const Slices sl = make_slices(pl, 0);
-    const dim3 gridS(pl->n_wg), blockS(SOLVER_WG), block64(64);
+    const Slices sl = make_slices(pl, 0, pl->n_wg);
+    const dim3 gridS = slice_grid(pl->n_wg), blockS(SOLVER_WG), block64(64);
     const int nsys = pl->n_sys * NR;
     const dim3 gridD((nsys + 127) / 128), blockD(128);
     const MT* A = nullptr;
@@ -669,12 +705,20 @@ static int dispatch_nr(Plan* pl, const double* f, double* u, int nrhs, double to
     }
 }
 
+// Iterations between FP64 residual replacements; JSSO_REPLACE_EVERY overrides
+// the defaults (64 mixed, 256 FP64) for tuning.
+static int replace_interval(bool mixed) {
+    const char* e = getenv("JSSO_REPLACE_EVERY");
+    const int v = e ? atoi(e) : 0;
+    return v > 0 ? v : (mixed ? 64 : 256);
+}
+
 int solve_batched(Plan* p, const double* f, double* u, int nrhs, bool mixed, double tol, int maxit,
                   jsso_info* info, hipStream_t s) {
     int rc = check_status(p, s);
     if (rc) return rc;
-    if (mixed) return dispatch_nr<float, float>(p, f, u, nrhs, tol, maxit, 64, info, s);
-    return dispatch_nr<double, double>(p, f, u, nrhs, tol, maxit, 256, info, s);
+    if (mixed) return dispatch_nr<float, float>(p, f, u, nrhs, tol, maxit, replace_interval(true), info, s);
+    return dispatch_nr<double, double>(p, f, u, nrhs, tol, maxit, replace_interval(false), info, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -688,7 +732,8 @@ __global__ __launch_bounds__(SOLVER_WG) void kp_init(Slices sl, Partials P, int 
                                                      double* __restrict__ x, double* __restrict__ r,
                                                      double* __restrict__ z, double* __restrict__ pp) {
     __shared__ double lds[NWAVES];
-    const int w = blockIdx.x + sl.base;
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
     const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
     const bool on = R < 6 * (int64_t)sl.row1[w];
     double v[1] = {0.0};
@@ -729,7 +774,8 @@ __global__ __launch_bounds__(SOLVER_WG) void kp_spmv(Slices sl, Partials P, cons
                                                      const int32_t* __restrict__ col, const double* __restrict__ K,
                                                      const double* __restrict__ pp, double* __restrict__ q) {
     __shared__ double lds[NWAVES];
-    const int w = blockIdx.x + sl.base;
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
     const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
     const bool on = R < 6 * (int64_t)sl.row1[w];
     double v[1] = {0.0};
@@ -757,7 +803,8 @@ __global__ __launch_bounds__(SOLVER_WG) void kp_update(Slices sl, Partials P, co
                                                        double* __restrict__ z) {
     __shared__ double lds[NWAVES * 2];
     __shared__ double rs[SOLVER_WG];
-    const int w = blockIdx.x + sl.base;
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
     double rz[1], pq[1];
     system_totals<1, NWAVES>(sl, sl.sys[w], P.rr(it - 1), P.pq, rz, pq, lds);
     const bool act = active[0] != 0;
@@ -795,7 +842,8 @@ __global__ __launch_bounds__(SOLVER_WG) void kp_dir(Slices sl, Partials P, int32
                                                     double* __restrict__ relres, int it, double tol,
                                                     const double* __restrict__ z, double* __restrict__ pp) {
     __shared__ double lds[NWAVES * 2];
-    const int w = blockIdx.x + sl.base;
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
     double rzn[1], rzo[1], rz0[1], unused[1];
     system_totals<1, NWAVES>(sl, sl.sys[w], P.rr(it), P.rr(it - 1), rzn, rzo, lds);
     system_totals<1, NWAVES>(sl, sl.sys[w], P.b, nullptr, rz0, unused, lds);
@@ -835,8 +883,8 @@ int solve_plain(Plan* pl, const double* f, double* u, int nrhs, double tol, int 
     double mx = 0.0;
     for (int sidx = 0; sidx < pl->n_sys; ++sidx) {
         const int w0 = pl->h_sys_wg_off[sidx], w1 = pl->h_sys_wg_off[sidx + 1];
-        const Slices sl = make_slices(pl, w0);
-        const dim3 grid(w1 - w0), block(SOLVER_WG);
+        const Slices sl = make_slices(pl, w0, w1 - w0);
+        const dim3 grid = slice_grid(w1 - w0), block(SOLVER_WG);
         for (int h = 0; h < nrhs; ++h) {
             hipLaunchKernelGGL(kp_init, grid, block, 0, s, sl, w.P, h, nrhs, pl->d_fixed, pl->d_Minv, f, x, r, z, pp);
             hipLaunchKernelGGL(kp_start, dim3(1), dim3(1), 0, s, sl, w.P, active, relres);
