@@ -83,15 +83,17 @@ def main():
     value = args.batch * args.steps / (t1 - t0)
     print(f"# timed: {ms_per_step:.1f} ms/step, {value:.1f} evals/s, CG iters {iters}, info {info}", flush=True)
 
-    # -- dominant kernel roofline: cg1 (scaled off-diagonal blocks x vector + p,q updates)
+    # -- dominant kernel roofline: the fused CG iteration k_cgf (scaled off-diagonal
+    # blocks + column indices once; r_{k-1}, q_{k-1}, p, x read and r_k, q_k, p, x
+    # written once per DOF and load case)
     p = batch.plan
     nnzb_off = p.nnzb - p.n_nodes
     s_m = 4 if args.precision == "mixed" else 8
     s_v = s_m
     ndof = p.ndof
-    alg_bytes = nnzb_off * (36 * s_m + 4) + 4 * (p.n_nodes + 1) + 5 * ndof * s_v
+    alg_bytes = nnzb_off * (36 * s_m + 4) + 4 * (p.n_nodes + 1) + 8 * ndof * s_v
     achieved = alg_bytes / (avg_ms[0] * 1e-3) / 1e9 if avg_ms[0] > 0 else None
-    roofline = {"kernel": f"k_cg1<{'float' if s_m == 4 else 'double'}>", "bound": "hbm",
+    roofline = {"kernel": f"k_cgf<{'float' if s_m == 4 else 'double'}>", "bound": "hbm",
                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
                 "avg_launch_us": avg_ms[0] * 1e3, "sampled_launches": int(nsamp[0]),

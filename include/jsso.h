@@ -134,8 +134,8 @@ int jsso_adjoint_grad(jsso_plan* plan, const double* d_crds, const double* d_pro
 int jsso_spmv(jsso_plan* plan, const double* d_x, double* d_y, void* stream);
 
 /* Measurement hook (no counterpart in the reference): sampled HIP-event timing
- * of the batched solver's dominant kernel, the CG step that applies the scaled
- * stiffness (cg1).  every > 0 brackets one cg1 launch in `every` with events on
+ * of the batched solver's dominant kernel, the fused CG iteration that applies
+ * the scaled stiffness.  every > 0 brackets one such launch in `every` with events on
  * the solver's stream; 0 disables.  jsso_timing_read returns the mean duration
  * (ms) and the number of sampled launches, optionally resetting the counters. */
 int jsso_timing_enable(int32_t every);

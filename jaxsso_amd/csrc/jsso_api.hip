@@ -430,8 +430,8 @@ int jsso_adjoint_grad(jsso_plan* plan, const double* d_crds, const double* d_pro
     if (rc0) return rc0;
     // gradient scratch: per element and coordinate direction, per rhs
     const size_t need = ((size_t)p->n_beam * BDIR + (size_t)p->n_quad * QDIR) * nrhs * sizeof(double);
-    if (p->ws.size() < 12) p->ws.resize(12);
-    DBuf& g = p->ws[11];
+    if (p->ws.size() <= WS_ADJ_SLOT) p->ws.resize(WS_ADJ_SLOT + 1);
+    DBuf& g = p->ws[WS_ADJ_SLOT];
     if (g.bytes < need) {
         if (g.p) JSSO_HIP(hipFree(g.p));
         g.p = nullptr;

@@ -33,6 +33,7 @@ void set_error(const char* fmt, ...);
 
 constexpr int BDIR = 12;   // beam adjoint directions: 6 coordinates + 6 properties
 constexpr int QDIR = 17;   // quad adjoint directions: 12 coordinates + 5 properties
+constexpr int WS_ADJ_SLOT = 12;  // Plan::ws slot of the adjoint's gradient scratch (solver slots are below it)
 constexpr int ROWS_PER_WG = 64;   // block rows per solver slice (one wavefront in the SpMV)
 constexpr int SOLVER_WG = 6 * ROWS_PER_WG;   // threads of the scalar-row solver kernels
 

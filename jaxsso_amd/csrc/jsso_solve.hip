@@ -5,9 +5,10 @@
 // system is the block-Jacobi-scaled one, A = L^-1 Kc L^-T with unit diagonal
 // blocks (split block-Jacobi preconditioning: the same iterates as PCG with
 // M = blockdiag(Kc_ii)^-1), so a step needs no preconditioner pass and the
-// diagonal blocks are never read.  Two kernels per iteration:
-//     cg1:  s = A r; p = r + beta p; q = s + beta q; <p,q>
-//     cg2:  x += alpha p; r -= alpha q; <r,r>
+// diagonal blocks are never read.  One kernel (k_cgf) per iteration k:
+//     alpha = rr/<p,q>;  r_k = r_{k-1} - alpha q_{k-1}  (own and gathered rows)
+//     x += alpha p;  beta = |r_k|^2 / rr  with |r_k|^2 from <r,r>, <r,q>, <q,q>
+//     s = A r_k;  p = r_k + beta p;  q = s + beta q;  partials <p,q> <r,q> <q,q> <r,r>
 // (q = A p carried by recurrence).  Recurrences run in VT (FP32 for MIXED);
 // every `replace_every` iterations the partial solution is folded into the
 // FP64 solution and the residual replaced by the true FP64 residual b - A64 x64
@@ -22,8 +23,8 @@
 // Reductions without atomics or fences.  Each kernel writes one partial sum
 // per slice; every workgroup of the next kernel adds the partials of its own
 // structure in a fixed order (so all of them obtain bitwise-identical CG
-// scalars) -- the kernel boundary is the only synchronisation.  <r,r> partials
-// alternate between two buffers so beta = rr_new / rr_old needs no stored state.
+// scalars) -- the kernel boundary is the only synchronisation.  Partials and
+// the r, q vectors alternate between two buffers by iteration parity.
 //
 // Safeguards: the iterate with the smallest true residual is kept and is what
 // the solve returns; a system stops when its true residual has not improved by
@@ -44,7 +45,7 @@ constexpr int NWAVES = SOLVER_WG / 64;
 enum { SYS_RUNNING = 0, SYS_CONVERGED = 1, SYS_STALLED = 2 };
 
 // ---------------------------------------------------------------------------
-// sampled event timing of cg1 (jsso_timing_enable / jsso_timing_read)
+// sampled event timing of k_cgf (jsso_timing_enable / jsso_timing_read)
 // ---------------------------------------------------------------------------
 struct KernelTiming {
     int every = 0;
@@ -124,16 +125,19 @@ __device__ __forceinline__ int slice_of_block(const Slices& sl) {
 
 static inline dim3 slice_grid(int count) { return dim3(8 * ((count + 7) / 8)); }
 
-// per-(structure, rhs) state, written only by k_decide (and, for restart, cg2)
+// per-(structure, rhs) state, written only by k_decide
 struct SysState {
     double *bnorm2, *relres, *best;
     int32_t *active, *status, *stall, *save;
-    int32_t* restart;  // [2][nsys], written by cg2 (first slice of the structure)
 };
 
-// partial-sum buffers, one value per slice and rhs
+// Partial-sum buffers, one value per slice and rhs.  The fused CG kernel of
+// iteration k writes pq/rq/qq/rr into slot k & 1 and reads slot (k - 1) & 1.
 struct Partials {
-    double *pq, *rr0, *rr1, *b, *t;
+    double *pq[2], *rq[2], *qq[2], *rrf[2];   // fused CG: <p,q>, <r,q>, <q,q>, <r,r>
+    double *b, *t, *rtrue;                     // <b,b>, true FP64 <r,r>, low-precision <r,r> after replacement
+    double *rr0, *rr1;                          // plain path: alternating <r,z>
+    double* pqp;                                // plain path: <p,q>
     __host__ __device__ double* rr(int it) const { return (it & 1) ? rr1 : rr0; }
 };
 
@@ -176,27 +180,38 @@ __device__ __forceinline__ void group_sum(double (&v)[NV], double* lds) {
     }
 }
 
-// Totals over the slices of `sys` of up to two partial buffers (NR values per
-// slice each), identical in every thread of every workgroup of the structure.
+// Totals over the slices of `sys` of K partial buffers (NR values per slice
+// each), identical in every thread of every workgroup of the structure.
+template <int K, int NR, int NW>
+__device__ __forceinline__ void system_totals_k(const Slices& sl, int sys, const double* const (&src)[K],
+                                                double (&tot)[K][NR], double* lds) {
+    const int s0 = sl.sys_off[sys], s1 = sl.sys_off[sys + 1];
+    double v[K * NR];
+#pragma unroll
+    for (int i = 0; i < K * NR; ++i) v[i] = 0.0;
+    for (int w = s0 + (int)threadIdx.x; w < s1; w += NW * 64) {
+#pragma unroll
+        for (int a = 0; a < K; ++a)
+#pragma unroll
+            for (int h = 0; h < NR; ++h) v[a * NR + h] += src[a][(int64_t)w * NR + h];
+    }
+    group_sum<K * NR, NW>(v, lds);
+#pragma unroll
+    for (int a = 0; a < K; ++a)
+#pragma unroll
+        for (int h = 0; h < NR; ++h) tot[a][h] = v[a * NR + h];
+}
+
 template <int NR, int NW>
 __device__ __forceinline__ void system_totals(const Slices& sl, int sys, const double* a, const double* b,
                                               double (&ta)[NR], double (&tb)[NR], double* lds) {
-    const int s0 = sl.sys_off[sys], s1 = sl.sys_off[sys + 1];
-    double v[2 * NR];
-#pragma unroll
-    for (int i = 0; i < 2 * NR; ++i) v[i] = 0.0;
-    for (int w = s0 + (int)threadIdx.x; w < s1; w += NW * 64) {
-#pragma unroll
-        for (int h = 0; h < NR; ++h) {
-            v[h] += a[(int64_t)w * NR + h];
-            if (b) v[NR + h] += b[(int64_t)w * NR + h];
-        }
-    }
-    group_sum<2 * NR, NW>(v, lds);
+    const double* const src[2] = {a, b ? b : a};
+    double tot[2][NR];
+    system_totals_k<2, NR, NW>(sl, sys, src, tot, lds);
 #pragma unroll
     for (int h = 0; h < NR; ++h) {
-        ta[h] = v[h];
-        tb[h] = v[NR + h];
+        ta[h] = tot[0][h];
+        tb[h] = tot[1][h];
     }
 }
 
@@ -214,21 +229,21 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double* lds, dou
 // batched kernels
 // ---------------------------------------------------------------------------
 
-// b = L^-1 f (constrained DOFs zeroed), x = 0, r = b; partials <b,b>, <r,r>.
+// b = L^-1 f (constrained DOFs zeroed), x = 0, r_0 = b (buffer 0); partials <b,b>.
 template <typename VT, int NR>
 __global__ __launch_bounds__(SOLVER_WG) void k_init(Slices sl, Partials P, const uint8_t* __restrict__ fixed,
                                                     const double* __restrict__ Linv, const double* __restrict__ f,
                                                     double* __restrict__ b64, double* __restrict__ x64,
                                                     double* __restrict__ xbest, VT* __restrict__ xlo,
-                                                    VT* __restrict__ rlo, VT* __restrict__ p, VT* __restrict__ q) {
+                                                    VT* __restrict__ r0, VT* __restrict__ p) {
     __shared__ double lds[NWAVES * NR];
     const int w = slice_of_block(sl);
     if (w < 0) return;
     const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
     const bool on = R < 6 * (int64_t)sl.row1[w];
-    double vb[NR], vr[NR];
+    double vb[NR];
 #pragma unroll
-    for (int h = 0; h < NR; ++h) vb[h] = vr[h] = 0.0;
+    for (int h = 0; h < NR; ++h) vb[h] = 0.0;
     if (on) {
         const int64_t i = R / 6;
         const int r = (int)(R % 6);
@@ -246,16 +261,12 @@ __global__ __launch_bounds__(SOLVER_WG) void k_init(Slices sl, Partials P, const
             x64[o] = 0.0;
             xbest[o] = 0.0;
             xlo[o] = VT(0);
-            const VT sl_ = VT(s);
-            rlo[o] = sl_;
+            r0[o] = VT(s);
             p[o] = VT(0);
-            q[o] = VT(0);
             vb[h] = s * s;
-            vr[h] = (double)sl_ * (double)sl_;
         }
     }
     write_partials<NR, NWAVES>(vb, lds, P.b, w);
-    write_partials<NR, NWAVES>(vr, lds, P.rr0, w);
 }
 
 // One thread per (structure, rhs): convergence bookkeeping from the partials.
@@ -277,8 +288,6 @@ __global__ void k_decide(Slices sl, Partials P, SysState st, int nsys_struct, in
         st.status[k] = bb > 0.0 ? SYS_RUNNING : SYS_CONVERGED;
         st.stall[k] = 0;
         st.save[k] = 0;
-        st.restart[k] = 0;
-        st.restart[nsys_struct * NR + k] = 0;
         return;
     }
     if (!st.active[k]) {
@@ -288,7 +297,7 @@ __global__ void k_decide(Slices sl, Partials P, SysState st, int nsys_struct, in
     double tt = 0.0, rr = 0.0;
     for (int w = s0; w < s1; ++w) {
         tt += P.t[(int64_t)w * NR + h];
-        rr += P.rr(it)[(int64_t)w * NR + h];
+        rr += P.rtrue[(int64_t)w * NR + h];
     }
     const double rel = sqrt(tt / st.bnorm2[k]);
     st.relres[k] = rel;
@@ -307,13 +316,25 @@ __global__ void k_decide(Slices sl, Partials P, SysState st, int nsys_struct, in
     }
 }
 
-// cg1 of iteration `it`: s = A r (A = I + scaled off-diagonal blocks, sliced
-// ELL, one lane per block row); p = r + beta p; q = s + beta q; partial <p,q>.
-template <typename MT, typename VT, int NR>
-__global__ __launch_bounds__(64) void k_cg1(Slices sl, Partials P, SysState st, int nsys, int it,
-                                            const MT* __restrict__ A, const VT* __restrict__ rlo,
-                                            VT* __restrict__ p, VT* __restrict__ q) {
-    __shared__ double lds[2 * NR];
+enum { CG_START = 0, CG_STEP = 1, CG_FRESH = 2 };
+
+// Fused CG iteration k (one lane per block row, sliced ELL).
+//   CG_STEP : alpha = rr_{k-1}/<p,q>_{k-1}; r_k = r_{k-1} - alpha q_{k-1} (own and
+//             gathered rows, on the fly); x += alpha p_{k-1};
+//             rr_k = rr_{k-1} - 2 alpha <r,q>_{k-1} + alpha^2 <q,q>_{k-1};
+//             beta = rr_k / rr_{k-1}
+//   CG_START: r_0 given, alpha = beta = 0
+//   CG_FRESH: r_k already in buffer k & 1 (a replacement), alpha = 0,
+//             beta = <r_k,r_k> (replacement partials) / rr_{k-1}
+// then s = A r_k; p_k = r_k + beta p_{k-1}; q_k = s + beta q_{k-1}; partials
+// <p,q>, <r,q>, <q,q>, <r,r> of iteration k.  A non-positive curvature or a
+// negative recurrence restarts the direction (alpha = beta = 0).
+template <typename MT, typename VT, int NR, int MODE>
+__global__ __launch_bounds__(64) void k_cgf(Slices sl, Partials P, SysState st, int k, const MT* __restrict__ A,
+                                            const VT* __restrict__ rprev, const VT* __restrict__ qprev,
+                                            VT* __restrict__ rcur, VT* __restrict__ qcur, VT* __restrict__ p,
+                                            VT* __restrict__ xlo) {
+    __shared__ double lds[1];
     const int w = slice_of_block(sl);
     if (w < 0) return;
     const int sys = sl.sys[w];
@@ -321,25 +342,50 @@ __global__ __launch_bounds__(64) void k_cg1(Slices sl, Partials P, SysState st, 
     const int lane = threadIdx.x;
     const int64_t i = (int64_t)sl.row0[w] + lane;
     const bool on = i < sl.row1[w];
-    // beta from the two alternating <r,r> buffers
-    double rr_new[NR], rr_old[NR];
-    system_totals<NR, 1>(sl, sys, P.rr(it - 1), P.rr(it), rr_new, rr_old, lds);
-    VT beta[NR];
+    const int pk = (k - 1) & 1, ck = k & 1;
+    VT alpha[NR], beta[NR];
     bool act[NR];
 #pragma unroll
     for (int h = 0; h < NR; ++h) {
-        const int k = sys * NR + h;
-        act[h] = st.active[k] != 0;
-        const bool restart = it == 1 || st.restart[((it - 1) & 1) * nsys + k] != 0;
-        beta[h] = (restart || !(rr_old[h] > 0.0)) ? VT(0) : VT(rr_new[h] / rr_old[h]);
+        alpha[h] = VT(0);
+        beta[h] = VT(0);
+        act[h] = st.active[sys * NR + h] != 0;
     }
+    if (MODE == CG_STEP) {
+        const double* const src[4] = {P.rrf[pk], P.pq[pk], P.rq[pk], P.qq[pk]};
+        double tot[4][NR];
+        system_totals_k<4, NR, 1>(sl, sys, src, tot, lds);
+#pragma unroll
+        for (int h = 0; h < NR; ++h) {
+            const double rr = tot[0][h], pq = tot[1][h], rq = tot[2][h], qq = tot[3][h];
+            if (pq > 0.0 && isfinite(pq) && rr > 0.0) {
+                const double a = rr / pq;
+                const double rrn = rr - 2.0 * a * rq + a * a * qq;
+                alpha[h] = VT(a);
+                beta[h] = rrn > 0.0 ? VT(rrn / rr) : VT(0);
+            }
+        }
+    } else if (MODE == CG_FRESH) {
+        const double* const src[2] = {P.rtrue, P.rrf[pk]};
+        double tot[2][NR];
+        system_totals_k<2, NR, 1>(sl, sys, src, tot, lds);
+#pragma unroll
+        for (int h = 0; h < NR; ++h) beta[h] = tot[1][h] > 0.0 ? VT(tot[0][h] / tot[1][h]) : VT(0);
+    }
+    const VT* rsrc = (MODE == CG_STEP) ? rprev : rcur;   // where r_{k-1} (STEP) or r_k lives
+    // own row: r_k, x update
     const int64_t ii = on ? i : (int64_t)sl.row0[w];
-    const VT* ri = rlo + 6 * ii * NR;
-    VT acc[6][NR];
+    VT rk[6][NR], acc[6][NR];
 #pragma unroll
     for (int c = 0; c < 6; ++c)
 #pragma unroll
-        for (int h = 0; h < NR; ++h) acc[c][h] = ri[c * NR + h];
+        for (int h = 0; h < NR; ++h) {
+            const int64_t o = (6 * ii + c) * NR + h;
+            VT v = rsrc[o];
+            if (MODE == CG_STEP) v = v - alpha[h] * qprev[o];
+            rk[c][h] = v;
+            acc[c][h] = v;
+        }
     const int v0 = sl.voff[w], v1 = sl.voff[w + 1];
     const MT* Aw = A + 36 * (int64_t)v0 + lane;
     const int32_t* cw = sl.ell_col + v0 + lane;
@@ -347,11 +393,15 @@ __global__ __launch_bounds__(64) void k_cg1(Slices sl, Partials P, SysState st, 
     for (int j = 0; j < nslot; ++j) {
         const int64_t col = cw[(int64_t)j * ROWS_PER_WG];
         VT xj[6][NR];
-        const VT* xs = rlo + 6 * col * NR;
 #pragma unroll
         for (int c = 0; c < 6; ++c)
 #pragma unroll
-            for (int h = 0; h < NR; ++h) xj[c][h] = xs[c * NR + h];
+            for (int h = 0; h < NR; ++h) {
+                const int64_t o = (6 * col + c) * NR + h;
+                VT v = rsrc[o];
+                if (MODE == CG_STEP) v = v - alpha[h] * qprev[o];
+                xj[c][h] = v;
+            }
         const MT* a = Aw + (int64_t)j * 36 * ROWS_PER_WG;
 #pragma unroll
         for (int e = 0; e < 36; ++e) {
@@ -360,9 +410,9 @@ __global__ __launch_bounds__(64) void k_cg1(Slices sl, Partials P, SysState st, 
             for (int h = 0; h < NR; ++h) acc[e / 6][h] += av * xj[e % 6][h];
         }
     }
-    double v[NR];
+    double vpq[NR], vrq[NR], vqq[NR], vrr[NR];
 #pragma unroll
-    for (int h = 0; h < NR; ++h) v[h] = 0.0;
+    for (int h = 0; h < NR; ++h) vpq[h] = vrq[h] = vqq[h] = vrr[h] = 0.0;
     if (on) {
 #pragma unroll
         for (int c = 0; c < 6; ++c)
@@ -370,57 +420,70 @@ __global__ __launch_bounds__(64) void k_cg1(Slices sl, Partials P, SysState st, 
             for (int h = 0; h < NR; ++h) {
                 if (!act[h]) continue;
                 const int64_t o = (6 * i + c) * NR + h;
-                const VT pn = ri[c * NR + h] + beta[h] * p[o];
-                const VT qn = acc[c][h] + beta[h] * q[o];
+                const VT pold = MODE == CG_START ? VT(0) : p[o];
+                const VT qold = MODE == CG_START ? VT(0) : qprev[o];
+                if (MODE == CG_STEP) {
+                    xlo[o] += alpha[h] * pold;
+                    rcur[o] = rk[c][h];
+                }
+                const VT pn = rk[c][h] + beta[h] * pold;
+                const VT qn = acc[c][h] + beta[h] * qold;
                 p[o] = pn;
-                q[o] = qn;
-                v[h] += (double)pn * (double)qn;
+                qcur[o] = qn;
+                const double r_ = (double)rk[c][h], q_ = (double)qn;
+                vpq[h] += (double)pn * q_;
+                vrq[h] += r_ * q_;
+                vqq[h] += q_ * q_;
+                vrr[h] += r_ * r_;
             }
     }
-    write_partials<NR, 1>(v, lds, P.pq, w);
+    double v[4 * NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+        v[h] = vpq[h];
+        v[NR + h] = vrq[h];
+        v[2 * NR + h] = vqq[h];
+        v[3 * NR + h] = vrr[h];
+    }
+#pragma unroll
+    for (int a = 0; a < 4 * NR; ++a) v[a] = wave_sum(v[a]);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int h = 0; h < NR; ++h) {
+            P.pq[ck][(int64_t)w * NR + h] = v[h];
+            P.rq[ck][(int64_t)w * NR + h] = v[NR + h];
+            P.qq[ck][(int64_t)w * NR + h] = v[2 * NR + h];
+            P.rrf[ck][(int64_t)w * NR + h] = v[3 * NR + h];
+        }
+    }
 }
 
-// cg2 of iteration `it`: alpha = rr / <p,q>; x += alpha p; r -= alpha q; partial <r,r>.
+// Update half of iteration k before a replacement: alpha from the partials of
+// k - 1; x += alpha p; r_k = r_{k-1} - alpha q_{k-1} into buffer k & 1.
 template <typename VT, int NR>
-__global__ __launch_bounds__(SOLVER_WG) void k_cg2(Slices sl, Partials P, SysState st, int nsys, int it,
-                                                   const VT* __restrict__ p, const VT* __restrict__ q,
-                                                   VT* __restrict__ xlo, VT* __restrict__ rlo) {
+__global__ __launch_bounds__(SOLVER_WG) void k_upd(Slices sl, Partials P, SysState st, int k,
+                                                   const VT* __restrict__ rprev, const VT* __restrict__ qprev,
+                                                   VT* __restrict__ rcur, const VT* __restrict__ p,
+                                                   VT* __restrict__ xlo) {
     __shared__ double lds[NWAVES * 2 * NR];
     const int w = slice_of_block(sl);
     if (w < 0) return;
     const int sys = sl.sys[w];
     if (structure_done<NR>(st, sys)) return;
+    const int pk = (k - 1) & 1;
     double rr[NR], pq[NR];
-    system_totals<NR, NWAVES>(sl, sys, P.rr(it - 1), P.pq, rr, pq, lds);
-    VT alpha[NR];
-    bool act[NR];
+    system_totals<NR, NWAVES>(sl, sys, P.rrf[pk], P.pq[pk], rr, pq, lds);
+    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
+    if (R >= 6 * (int64_t)sl.row1[w]) return;
 #pragma unroll
     for (int h = 0; h < NR; ++h) {
-        const int k = sys * NR + h;
-        act[h] = st.active[k] != 0;
-        const bool ok = pq[h] > 0.0 && isfinite(pq[h]);
-        alpha[h] = ok ? VT(rr[h] / pq[h]) : VT(0);
-        if (threadIdx.x == 0 && w == sl.sys_off[sys]) st.restart[(it & 1) * nsys + k] = ok ? 0 : 1;
+        if (!st.active[sys * NR + h]) continue;
+        const bool ok = pq[h] > 0.0 && isfinite(pq[h]) && rr[h] > 0.0;
+        const VT alpha = ok ? VT(rr[h] / pq[h]) : VT(0);
+        const int64_t o = R * NR + h;
+        xlo[o] += alpha * p[o];
+        rcur[o] = rprev[o] - alpha * qprev[o];
     }
-    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
-    const bool on = R < 6 * (int64_t)sl.row1[w];
-    double v[NR];
-#pragma unroll
-    for (int h = 0; h < NR; ++h) v[h] = 0.0;
-    if (on) {
-#pragma unroll
-        for (int h = 0; h < NR; ++h) {
-            const int64_t o = R * NR + h;
-            VT rn = rlo[o];
-            if (act[h]) {
-                xlo[o] += alpha[h] * p[o];
-                rn = rn - alpha[h] * q[o];
-                rlo[o] = rn;
-            }
-            v[h] = (double)rn * (double)rn;
-        }
-    }
-    write_partials<NR, NWAVES>(v, lds, P.rr(it), w);
 }
 
 // x64 += xlo; xlo = 0.
@@ -441,13 +504,12 @@ __global__ __launch_bounds__(SOLVER_WG) void k_fold(Slices sl, const int32_t* __
     }
 }
 
-// After cg2 of iteration `it`: r64 = b - A64 x64, rlo = r64; partials <r64,r64>
-// (FP64, convergence) and <rlo,rlo> (replaces the recursive <r,r> of `it`).
+// r64 = b - A64 x64; r_k = r64 (low precision, buffer k & 1); partials
+// <r64,r64> (FP64, convergence) and <r_k,r_k>.
 template <typename VT, int NR>
-__global__ __launch_bounds__(64) void k_replace(Slices sl, Partials P, SysState st, int it,
+__global__ __launch_bounds__(64) void k_replace(Slices sl, Partials P, SysState st,
                                                 const double* __restrict__ A, const double* __restrict__ b64,
-                                                const double* __restrict__ x64, double* __restrict__ r64,
-                                                VT* __restrict__ rlo) {
+                                                const double* __restrict__ x64, VT* __restrict__ rcur) {
     __shared__ double lds[2 * NR];
     const int w = slice_of_block(sl);
     if (w < 0) return;
@@ -490,22 +552,17 @@ __global__ __launch_bounds__(64) void k_replace(Slices sl, Partials P, SysState 
         for (int c = 0; c < 6; ++c)
 #pragma unroll
             for (int h = 0; h < NR; ++h) {
+                if (!st.active[sys * NR + h]) continue;
                 const int64_t o = (6 * i + c) * NR + h;
-                if (!st.active[sys * NR + h]) {
-                    const double rl = (double)rlo[o];
-                    vr[h] += rl * rl;
-                    continue;
-                }
                 const double rv = b64[o] - acc[c][h];
-                r64[o] = rv;
                 const VT rl = VT(rv);
-                rlo[o] = rl;
+                rcur[o] = rl;
                 vt[h] += rv * rv;
                 vr[h] += (double)rl * (double)rl;
             }
     }
     write_partials<NR, 1>(vt, lds, P.t, w);
-    write_partials<NR, 1>(vr, lds, P.rr(it), w);
+    write_partials<NR, 1>(vr, lds, P.rtrue, w);
 }
 
 // keep the best iterate: xbest = x64 where the last replacement improved it
@@ -543,24 +600,24 @@ __global__ __launch_bounds__(256) void k_unscale(int32_t n_nodes, const double* 
 // workspace
 // ---------------------------------------------------------------------------
 struct Workspace {
-    double *b64, *x64, *r64, *xbest;
-    void *xlo, *rlo, *p, *q;
+    double *b64, *x64, *xbest;
+    void *xlo, *r[2], *q[2], *p;
     SysState st;
     Partials P;
 };
 
-enum { WS_B, WS_X, WS_R, WS_XLO, WS_RLO, WS_P, WS_Q, WS_PART, WS_SD, WS_SI, WS_UNUSED, WS_ADJ, WS_XBEST, WS_N };
+enum { WS_B, WS_X, WS_XBEST, WS_XLO, WS_R0, WS_R1, WS_Q0, WS_Q1, WS_P, WS_PART, WS_SD, WS_SI, WS_ADJ, WS_N };
+static_assert(WS_ADJ == WS_ADJ_SLOT, "adjoint scratch slot");
 
 static int get_workspace(Plan* pl, int nrhs, Workspace& w) {
     const size_t n = 6 * (size_t)pl->n_nodes * nrhs;
     const size_t nsys = (size_t)pl->n_sys * nrhs;
     const size_t npart = (size_t)pl->n_wg * nrhs;
     size_t need[WS_N] = {};
-    need[WS_B] = need[WS_X] = need[WS_R] = need[WS_XLO] = need[WS_RLO] = need[WS_P] = need[WS_Q] = n * 8;
-    need[WS_XBEST] = n * 8;
-    need[WS_PART] = npart * 8 * 5;
+    for (int i = WS_B; i <= WS_P; ++i) need[i] = n * 8;
+    need[WS_PART] = npart * 8 * 14;
     need[WS_SD] = nsys * 8 * 3;
-    need[WS_SI] = nsys * 4 * 6;
+    need[WS_SI] = nsys * 4 * 4;
     if (pl->ws.size() < WS_N) pl->ws.resize(WS_N);
     for (int i = 0; i < WS_N; ++i) {
         if (i == WS_ADJ || need[i] == 0) continue;
@@ -575,18 +632,26 @@ static int get_workspace(Plan* pl, int nrhs, Workspace& w) {
     }
     w.b64 = (double*)pl->ws[WS_B].p;
     w.x64 = (double*)pl->ws[WS_X].p;
-    w.r64 = (double*)pl->ws[WS_R].p;
-    w.xlo = pl->ws[WS_XLO].p;
-    w.rlo = pl->ws[WS_RLO].p;
-    w.p = pl->ws[WS_P].p;
-    w.q = pl->ws[WS_Q].p;
     w.xbest = (double*)pl->ws[WS_XBEST].p;
+    w.xlo = pl->ws[WS_XLO].p;
+    w.r[0] = pl->ws[WS_R0].p;
+    w.r[1] = pl->ws[WS_R1].p;
+    w.q[0] = pl->ws[WS_Q0].p;
+    w.q[1] = pl->ws[WS_Q1].p;
+    w.p = pl->ws[WS_P].p;
     double* pp = (double*)pl->ws[WS_PART].p;
-    w.P.pq = pp;
-    w.P.rr0 = pp + npart;
-    w.P.rr1 = pp + 2 * npart;
-    w.P.b = pp + 3 * npart;
-    w.P.t = pp + 4 * npart;
+    for (int a = 0; a < 2; ++a) {
+        w.P.pq[a] = pp + (0 + a) * npart;
+        w.P.rq[a] = pp + (2 + a) * npart;
+        w.P.qq[a] = pp + (4 + a) * npart;
+        w.P.rrf[a] = pp + (6 + a) * npart;
+    }
+    w.P.b = pp + 8 * npart;
+    w.P.t = pp + 9 * npart;
+    w.P.rtrue = pp + 10 * npart;
+    w.P.rr0 = pp + 11 * npart;
+    w.P.rr1 = pp + 12 * npart;
+    w.P.pqp = pp + 13 * npart;
     double* sd = (double*)pl->ws[WS_SD].p;
     w.st.bnorm2 = sd;
     w.st.relres = sd + nsys;
@@ -596,7 +661,6 @@ static int get_workspace(Plan* pl, int nrhs, Workspace& w) {
     w.st.status = si + nsys;
     w.st.stall = si + 2 * nsys;
     w.st.save = si + 3 * nsys;
-    w.st.restart = si + 4 * nsys;   // [2][nsys]
     return JSSO_OK;
 }
 
@@ -631,32 +695,41 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
     const dim3 gridD((nsys + 127) / 128), blockD(128);
     const MT* A = nullptr;
     if constexpr (sizeof(MT) == 4) A = (const MT*)pl->d_Ke32; else A = (const MT*)pl->d_Ke64;
-    VT *xlo = (VT*)w.xlo, *rlo = (VT*)w.rlo, *p = (VT*)w.p, *q = (VT*)w.q;
+    VT *xlo = (VT*)w.xlo, *p = (VT*)w.p;
+    VT* rb[2] = {(VT*)w.r[0], (VT*)w.r[1]};
+    VT* qb[2] = {(VT*)w.q[0], (VT*)w.q[1]};
     hipLaunchKernelGGL((k_init<VT, NR>), gridS, blockS, 0, s, sl, w.P, pl->d_fixed, pl->d_Linv, f, w.b64, w.x64,
-                       w.xbest, xlo, rlo, p, q);
+                       w.xbest, xlo, rb[0], p);
     hipLaunchKernelGGL((k_decide<NR>), gridD, blockD, 0, s, sl, w.P, w.st, pl->n_sys, 0, tol, 0);
+    hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_START>), gridS, block64, 0, s, sl, w.P, w.st, 0, A,
+                       (const VT*)rb[1], (const VT*)qb[1], rb[0], qb[0], p, xlo);
     JSSO_HIP(hipGetLastError());
     std::vector<int32_t> act(nsys);
     int it = 0, repl = 0;
     bool done = false;
     while (!done && it < maxit) {
-        const int steps = std::min(replace_every, maxit - it);
-        for (int j = 0; j < steps; ++j) {
+        const int steps = std::max(1, std::min(replace_every, maxit - it));
+        for (int j = 0; j + 1 < steps; ++j) {
             ++it;
             std::pair<hipEvent_t, hipEvent_t>* ev;
             timing_begin(s, ev);
-            hipLaunchKernelGGL((k_cg1<MT, VT, NR>), gridS, block64, 0, s, sl, w.P, w.st, nsys, it, A,
-                               (const VT*)rlo, p, q);
+            hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_STEP>), gridS, block64, 0, s, sl, w.P, w.st, it, A,
+                               (const VT*)rb[(it - 1) & 1], (const VT*)qb[(it - 1) & 1], rb[it & 1], qb[it & 1], p,
+                               xlo);
             timing_end(s, ev);
-            hipLaunchKernelGGL((k_cg2<VT, NR>), gridS, blockS, 0, s, sl, w.P, w.st, nsys, it, (const VT*)p,
-                               (const VT*)q, xlo, rlo);
         }
+        // the last iteration of the chunk: update, fold, replace, then continue fresh
+        ++it;
+        hipLaunchKernelGGL((k_upd<VT, NR>), gridS, blockS, 0, s, sl, w.P, w.st, it, (const VT*)rb[(it - 1) & 1],
+                           (const VT*)qb[(it - 1) & 1], rb[it & 1], (const VT*)p, xlo);
         hipLaunchKernelGGL((k_fold<VT, NR>), gridS, blockS, 0, s, sl, (const int32_t*)w.st.active, xlo, w.x64);
-        hipLaunchKernelGGL((k_replace<VT, NR>), gridS, block64, 0, s, sl, w.P, w.st, it, (const double*)pl->d_Ke64,
-                           (const double*)w.b64, (const double*)w.x64, w.r64, rlo);
+        hipLaunchKernelGGL((k_replace<VT, NR>), gridS, block64, 0, s, sl, w.P, w.st, (const double*)pl->d_Ke64,
+                           (const double*)w.b64, (const double*)w.x64, rb[it & 1]);
         hipLaunchKernelGGL((k_decide<NR>), gridD, blockD, 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, it);
         hipLaunchKernelGGL((k_save<NR>), gridS, blockS, 0, s, sl, (const int32_t*)w.st.save, (const double*)w.x64,
                            w.xbest);
+        hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_FRESH>), gridS, block64, 0, s, sl, w.P, w.st, it, A,
+                           (const VT*)rb[(it - 1) & 1], (const VT*)qb[(it - 1) & 1], rb[it & 1], qb[it & 1], p, xlo);
         JSSO_HIP(hipGetLastError());
         ++repl;
         JSSO_HIP(hipMemcpyAsync(act.data(), w.st.active, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -792,7 +865,7 @@ __global__ __launch_bounds__(SOLVER_WG) void kp_spmv(Slices sl, Partials P, cons
         q[R] = acc;
         v[0] = pp[R] * acc;
     }
-    write_partials<1, NWAVES>(v, lds, P.pq, w);
+    write_partials<1, NWAVES>(v, lds, P.pqp, w);
 }
 
 // alpha = rz / <p,q>; x += alpha p; r -= alpha q; z = M r; partial <r,z> into rr(it).
@@ -806,7 +879,7 @@ __global__ __launch_bounds__(SOLVER_WG) void kp_update(Slices sl, Partials P, co
     const int w = slice_of_block(sl);
     if (w < 0) return;
     double rz[1], pq[1];
-    system_totals<1, NWAVES>(sl, sl.sys[w], P.rr(it - 1), P.pq, rz, pq, lds);
+    system_totals<1, NWAVES>(sl, sl.sys[w], P.rr(it - 1), P.pqp, rz, pq, lds);
     const bool act = active[0] != 0;
     const double alpha = (pq[0] > 0.0 && isfinite(pq[0])) ? rz[0] / pq[0] : 0.0;
     const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
@@ -875,7 +948,7 @@ int solve_plain(Plan* pl, const double* f, double* u, int nrhs, double tol, int 
     Workspace w;
     rc = get_workspace(pl, 1, w);
     if (rc) return rc;
-    double *x = w.x64, *r = w.r64, *z = w.b64, *pp = (double*)w.p, *q = (double*)w.q;
+    double *x = w.x64, *r = w.xbest, *z = w.b64, *pp = (double*)w.p, *q = (double*)w.q[0];
     int32_t* active = w.st.active;
     double* relres = w.st.relres;
     const int nsys = pl->n_sys * nrhs;

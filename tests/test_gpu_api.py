@@ -49,9 +49,9 @@ def test_arch_notebook_pins(precision):
     st = wl.notebook_arch()
     m = build_model(st)
     m.precision = precision
-    # the slender arch is the worst case for FP32 recurrences: the mixed path
-    # needs a 1e-11 residual for a 1e-8 gradient (FP64 reaches it at 1e-10)
-    m.tol = 1e-11 if precision == "mixed" else 1e-10
+    # the slender arch is the worst-conditioned case: its error at a 1e-11 scaled
+    # residual sits at the 1e-8 gradient bar on the mixed path, so ask for 1e-12
+    m.tol = 1e-12 if precision == "mixed" else 1e-10
     m.solve(which_solver="sparse")
     u = m.u.cpu().numpy()
     C = float(m.strain_energy())
@@ -108,10 +108,11 @@ def test_cfg1_shape_and_thickness_gradient_vs_oracle_and_fd():
     gt = g[len(st.design_nodes):]
     assert rel(gz, ref["dcrds"][st.design_nodes, 2]) < 1e-8
     assert rel(gt, ref["dprop_quad"][:, 0]) < 1e-8
-    # central finite differences through the GPU FP64 path; C is good to ~1e-12
-    # relative at this tolerance, so the FD noise is ~1e-12 C / h
+    # central finite differences through the GPU FP64 path; a CG (Galerkin)
+    # iterate's error in C is ||e||_K^2 / 2, quadratic in the solve error, so
+    # the FD noise at this tolerance stays below 1e-11 C / h
     m.precision = "fp64"
-    m.tol = 1e-11
+    m.tol = 1e-10
     p0 = sso.parameter_values.copy()
     for k in [5, 400, len(st.design_nodes) + 17]:
         h = 1e-5
@@ -204,7 +205,7 @@ def test_precisions_agree_cfg1():
     from jaxsso_amd.plan import FEPlan
     plan = FEPlan([0, st.n_node], st.cnct_beamcols, st.cnct_quads, st.known_id)
     plan.assemble(torch.tensor(st.crds), None, torch.tensor(st.prop_quads))
-    us = {p: plan.solve(torch.tensor(st.nodal_loads), precision=p, tol=1e-11).cpu().numpy()
+    us = {p: plan.solve(torch.tensor(st.nodal_loads), precision=p, tol=1e-10).cpu().numpy()
           for p in ("fp64", "mixed", "plain")}
     assert rel(us["mixed"], us["fp64"]) < 1e-9
     assert rel(us["plain"], us["fp64"]) < 1e-9

@@ -117,7 +117,10 @@ def test_solve_matches_oracle(case, precision):
     st = getattr(wl, case)() if case.startswith("cfg") else getattr(wl, "notebook_" + case)()
     plan = FEPlan([0, st.n_node], st.cnct_beamcols, st.cnct_quads, st.known_id)
     plan.assemble(torch.tensor(st.crds), torch.tensor(st.prop_beamcols), torch.tensor(st.prop_quads))
-    u = plan.solve(torch.tensor(st.nodal_loads), precision=precision, tol=1e-10).cpu().numpy()
+    # the slender arch amplifies the residual ~150x into u: a 1e-8 error bar
+    # needs a 1e-11 scaled residual there
+    tol = 1e-11 if case == "arch" else 1e-10
+    u = plan.solve(torch.tensor(st.nodal_loads), precision=precision, tol=tol).cpu().numpy()
     K = system.assemble_K(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads)
     ur = osolve.solve_reduced(K, st.known_id, st.nodal_loads)
     info = plan.last_info
