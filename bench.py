@@ -11,8 +11,10 @@ are resident on the GPU before timing starts.
 
 Also reported: the plain FP64 one-structure-per-launch GPU path, the batched
 FP64 path, the error of the timed path against the plain FP64 path and the FP64
-oracle, the dominant kernel's roofline (HIP events on the solver stream), and
-the oracle timed on this host's cores.
+oracle, the dominant kernel's roofline (HIP events on the solver stream),
+element stiffness + assembly throughput (elements/s), the oracle timed on this
+host's cores, and the 200x200 MITC-4 shell (242,406 DOF) as a single structure
+on every solver path with its error against the committed oracle fixture.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
 """
@@ -40,6 +42,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle-check", action="store_true")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--no-shell200", action="store_true", help="skip the 200x200 single-structure lines")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs every structure of a batch on a single "
@@ -98,6 +101,15 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
                 "avg_launch_us": avg_ms[0] * 1e3, "sampled_launches": int(nsamp[0]),
                 "alg_bytes_per_launch": int(alg_bytes)}
+
+    # -- element stiffness + assembly alone (elements/s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        batch.assemble()
+    torch.cuda.synchronize()
+    asm_ms = (time.perf_counter() - t0) * 1e3 / 5
+    n_el = batch.plan.n_quad + batch.plan.n_beam
 
     # -- comparison paths on a few designs
     sub = StructureBatch(designs[:args.plain_designs])
@@ -164,8 +176,48 @@ def main():
         "fp64_batched_evals_per_s": fp64_eps, "fp64_batched_cg_iterations": fp64_iters,
         "speedup_vs_plain_fp64": value / plain_eps,
         "rel_err_vs_plain_fp64_gpu": err_plain, "rel_err_vs_fp64_oracle": err_oracle,
+        "assembly_ms_per_step": asm_ms, "elements_per_s": n_el / (asm_ms * 1e-3),
+        "shell200": None if args.no_shell200 else shell200(args.tol),
     }
     print(json.dumps(out), flush=True)
+
+
+def shell200(tol):
+    """The 200x200 MITC-4 shell (242,406 DOF) as one structure: compliance + full
+    gradient per second for the mixed, batched-FP64 and plain FP64 paths, and the
+    error against the committed FP64 oracle fixture."""
+    import torch
+    from jaxsso_amd import workloads as wl
+    from jaxsso_amd.batch import StructureBatch
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "tests", "golden", "shell200_oracle.json")) as fh:
+        fx = json.load(fh)
+    st = wl.cfg2()
+    b = StructureBatch([st])
+    res = {"workload": "MITC-4 shell 200x200 over 20 m (40,401 nodes, 40,000 quads, 242,406 DOF), "
+                       "edges pinned, smooth Fourier thickness; one structure per evaluation"}
+    rel = lambda a, r: float(np.linalg.norm(np.asarray(a) - np.asarray(r)) / np.linalg.norm(np.asarray(r)))
+    for prec in ("mixed", "fp64", "plain"):
+        b.compliance_and_gradient(precision=prec, tol=tol)     # warm-up (and kernel tuning)
+        torch.cuda.synchronize()
+        reps = 2
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            C, (dc, _, dq) = b.compliance_and_gradient(precision=prec, tol=tol)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        dcn = dc.cpu().numpy().reshape(st.n_node, 3)
+        dqn = dq.cpu().numpy().reshape(st.n_quad, 5)
+        res[prec] = {"evals_per_s": 1.0 / dt, "ms_per_eval": dt * 1e3,
+                     "cg_iterations": b.plan.last_info["iterations"],
+                     "waves_per_slice": b.plan.last_info.get("waves_per_slice"),
+                     "rel_err_vs_fp64_oracle": {
+                         "C": abs(float(C[0, 0]) - fx["C"]) / fx["C"],
+                         "u_sample": rel(b.u.cpu().numpy().ravel()[fx["u_index"]], fx["u"]),
+                         "grad_z_sample": rel(dcn[fx["dz_node"], 2], fx["dz"]),
+                         "grad_t_sample": rel(dqn[fx["dt_elem"], 0], fx["dt"])}}
+    res["speedup_mixed_vs_plain_fp64"] = res["mixed"]["evals_per_s"] / res["plain"]["evals_per_s"]
+    return res
 
 
 def _cpu_model():

@@ -70,6 +70,8 @@ typedef struct {
     int32_t converged;    /* systems (structure x rhs) that met the tolerance       */
     int32_t n_systems;    /* structures x rhs                                       */
     double rel_residual;  /* max over systems of ||L^-1 (f - K u)|| / ||L^-1 f||    */
+    int32_t waves_per_slice; /* batched CG kernel variant used (1, 2 or 4 wavefronts
+                                per 64-block-row slice); 0 on the plain path      */
 } jsso_info;
 
 /* plan size indices for jsso_plan_sizes */
@@ -132,6 +134,15 @@ int jsso_adjoint_grad(jsso_plan* plan, const double* d_crds, const double* d_pro
 
 /* y = K x with the assembled, unconstrained K (nrhs = 1). */
 int jsso_spmv(jsso_plan* plan, const double* d_x, double* d_y, void* stream);
+
+/* Tuning of the batched CG kernel (no counterpart in the reference: the
+ * reference hands the solve to SuperLU).  The first batched solve on a plan for a
+ * (precision, nrhs) pair times the 1-, 2- and 4-wavefront-per-slice variants on
+ * the live problem and keeps the fastest for the plan's later solves.
+ * set_wps >= 0 overrides the stored choice (0 = tune again on the next solve,
+ * 1/2/4 = force); h_wps, if not NULL, receives the stored choice (0 = untuned). */
+int jsso_plan_cg_tuning(jsso_plan* plan, int32_t precision, int32_t nrhs, int32_t set_wps,
+                        int32_t* h_wps);
 
 /* Measurement hook (no counterpart in the reference): sampled HIP-event timing
  * of the batched solver's dominant kernel, the fused CG iteration that applies

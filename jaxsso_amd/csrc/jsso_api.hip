@@ -192,13 +192,14 @@ static int build_plan(Plan* p, int32_t n_sys, const int32_t* node_off, int32_t n
     // rows of one structure), slot j of lane l = j-th off-diagonal block of row
     // row0 + l; padding slots point at the row itself with zero values
     std::vector<int32_t> slice_voff(p->n_wg + 1, 0), ell_col;
-    std::vector<int64_t> ell_base(offcol.size(), -1);
+    std::vector<int32_t> ell_src;
     for (int w = 0; w < p->n_wg; ++w) {
         int nslot = 0;
         for (int i = wg_row0[w]; i < wg_row1[w]; ++i) nslot = std::max(nslot, offrowptr[i + 1] - offrowptr[i]);
         slice_voff[w + 1] = slice_voff[w] + nslot * ROWS_PER_WG;
         const size_t base = ell_col.size();
         ell_col.resize(base + (size_t)nslot * ROWS_PER_WG);
+        ell_src.resize(base + (size_t)nslot * ROWS_PER_WG, -1);
         for (int j = 0; j < nslot; ++j)
             for (int l = 0; l < ROWS_PER_WG; ++l) {
                 const int i = wg_row0[w] + l;
@@ -206,7 +207,7 @@ static int build_plan(Plan* p, int32_t n_sys, const int32_t* node_off, int32_t n
                 if (i < wg_row1[w] && j < offrowptr[i + 1] - offrowptr[i]) {
                     const int k = offrowptr[i] + j;
                     c = offcol[k];
-                    ell_base[k] = 36 * (int64_t)slice_voff[w] + (int64_t)j * 36 * ROWS_PER_WG + l;
+                    ell_src[base + (size_t)j * ROWS_PER_WG + l] = offsrc[k];
                 }
                 ell_col[base + (size_t)j * ROWS_PER_WG + l] = c;
             }
@@ -214,12 +215,9 @@ static int build_plan(Plan* p, int32_t n_sys, const int32_t* node_off, int32_t n
     p->ell_slots = slice_voff[p->n_wg];
     p->h_slice_voff.swap(slice_voff);
     p->h_ell_col.swap(ell_col);
-    p->h_ell_base.swap(ell_base);
+    p->h_ell_src.swap(ell_src);
     p->h_browof.swap(browof);
     p->h_diag.swap(diag);
-    p->h_offrowptr.swap(offrowptr);
-    p->h_offcol.swap(offcol);
-    p->h_offsrc.swap(offsrc);
     p->h_cptr.swap(cptr);
     p->h_csrc.swap(csrc);
     p->h_nptr.swap(nptr);
@@ -239,21 +237,19 @@ static int ensure_device(Plan* p) {
     int rc = 0;
     if ((rc = upload(p, &p->d_rowptr, p->h_rowptr)) || (rc = upload(p, &p->d_colind, p->h_colind)) ||
         (rc = upload(p, &p->d_browof, p->h_browof)) || (rc = upload(p, &p->d_diag, p->h_diag)) ||
-        (rc = upload(p, &p->d_offrowptr, p->h_offrowptr)) || (rc = upload(p, &p->d_offcol, p->h_offcol)) ||
-        (rc = upload(p, &p->d_offsrc, p->h_offsrc)) || (rc = upload(p, &p->d_cptr, p->h_cptr)) ||
+        (rc = upload(p, &p->d_cptr, p->h_cptr)) ||
         (rc = upload(p, &p->d_csrc, p->h_csrc)) || (rc = upload(p, &p->d_nptr, p->h_nptr)) ||
         (rc = upload(p, &p->d_nsrc, p->h_nsrc)) || (rc = upload(p, &p->d_cnct_beam, p->h_cnct_beam)) ||
         (rc = upload(p, &p->d_cnct_quad, p->h_cnct_quad)) || (rc = upload(p, &p->d_fixed, p->h_fixed)) ||
         (rc = upload(p, &p->d_wg_sys, p->h_wg_sys)) || (rc = upload(p, &p->d_wg_row0, p->h_wg_row0)) ||
         (rc = upload(p, &p->d_wg_row1, p->h_wg_row1)) || (rc = upload(p, &p->d_sys_wg_off, p->h_sys_wg_off)) ||
         (rc = upload(p, &p->d_slice_voff, p->h_slice_voff)) || (rc = upload(p, &p->d_ell_col, p->h_ell_col)) ||
-        (rc = upload(p, &p->d_ell_base, p->h_ell_base)))
+        (rc = upload(p, &p->d_ell_src, p->h_ell_src)))
         return rc;
     const size_t n = p->n_nodes;
     if ((rc = dalloc(p, &p->d_ke, (size_t)p->n_contrib * 36)) || (rc = dalloc(p, &p->d_K, (size_t)p->nnzb * 36)) ||
         (rc = dalloc(p, &p->d_Kc, (size_t)p->nnzb * 36)) || (rc = dalloc(p, &p->d_Linv, n * 36)) ||
-        (rc = dalloc(p, &p->d_Minv, n * 36)) || (rc = dalloc(p, &p->d_Ks64, (size_t)p->nnzb_off * 36)) ||
-        (rc = dalloc(p, &p->d_Ks32, (size_t)p->nnzb_off * 36)) || (rc = dalloc(p, &p->d_status, 1)) ||
+        (rc = dalloc(p, &p->d_Minv, n * 36)) || (rc = dalloc(p, &p->d_status, 1)) ||
         (rc = dalloc(p, &p->d_Ke64, (size_t)p->ell_slots * 36)) || (rc = dalloc(p, &p->d_Ke32, (size_t)p->ell_slots * 36)))
         return rc;
     JSSO_HIP(hipMemset(p->d_status, 0, sizeof(int32_t)));
@@ -463,6 +459,18 @@ int jsso_timing_enable(int32_t every) { return timing_enable(every); }
 
 int jsso_timing_read(double* h_avg_ms, int64_t* h_samples, int32_t reset) {
     return timing_read(h_avg_ms, h_samples, reset);
+}
+
+int jsso_plan_cg_tuning(jsso_plan* plan, int32_t precision, int32_t nrhs, int32_t set_wps, int32_t* h_wps) {
+    if (!plan || (precision != JSSO_PREC_FP64 && precision != JSSO_PREC_MIXED) || nrhs < 1 || nrhs > 8 ||
+        !(set_wps < 0 || set_wps == 0 || set_wps == 1 || set_wps == 2 || set_wps == 4)) {
+        set_error("invalid tuning arguments (precision FP64 or MIXED, nrhs 1..8, wps -1, 0, 1, 2 or 4)");
+        return JSSO_EINVAL;
+    }
+    int32_t& slot = P(plan)->cg_wps[precision == JSSO_PREC_MIXED ? 0 : 1][nrhs];
+    if (set_wps >= 0) slot = set_wps;
+    if (h_wps) *h_wps = slot;
+    return JSSO_OK;
 }
 
 }  // extern "C"

@@ -103,19 +103,20 @@ __global__ void diag_factor_kernel(int32_t n_nodes, const int32_t* __restrict__ 
     if (!ok) atomicOr(status, 1);
 }
 
-// One thread per off-diagonal block: Ks = Linv_i Kc_ij Linv_j^T in FP64, also
-// rounded to FP32.
-__global__ __launch_bounds__(256) void scale_blocks_kernel(int64_t nnzb_off, const int32_t* __restrict__ offsrc,
+// One thread per sliced-ELL position (slot-major, lane fastest): the scaled
+// block Ks_ij = Linv_i Kc_ij Linv_j^T of that position in FP64, also rounded to
+// FP32, written entry by entry so that each store of a wavefront is one
+// contiguous run of 64 lanes.  Padding positions (src < 0) keep their zeros.
+__global__ __launch_bounds__(256) void scale_blocks_kernel(int64_t n_pos, const int32_t* __restrict__ ell_src,
                                                            const int32_t* __restrict__ browof,
                                                            const int32_t* __restrict__ colind,
                                                            const double* __restrict__ Kc,
                                                            const double* __restrict__ Linv,
-                                                           const int64_t* __restrict__ ell_base,
-                                                           double* __restrict__ Ks64, float* __restrict__ Ks32,
                                                            double* __restrict__ Ke64, float* __restrict__ Ke32) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nnzb_off) return;
-    const int64_t src = offsrc[k];
+    const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= n_pos) return;
+    const int64_t src = ell_src[pos];
+    if (src < 0) return;
     const int i = browof[src], j = colind[src];
     const double* B = Kc + src * 36;
     const double* Li = Linv + (int64_t)i * 36;
@@ -130,6 +131,7 @@ __global__ __launch_bounds__(256) void scale_blocks_kernel(int64_t nnzb_off, con
             for (int k2 = 0; k2 <= c; ++k2) s += B[r * 6 + k2] * Lj[c * 6 + k2];  // B Lj^T
             T[r][c] = s;
         }
+    const int64_t base = (pos / ROWS_PER_WG) * 36 * ROWS_PER_WG + pos % ROWS_PER_WG;
 #pragma unroll
     for (int r = 0; r < 6; ++r)
 #pragma unroll
@@ -137,9 +139,7 @@ __global__ __launch_bounds__(256) void scale_blocks_kernel(int64_t nnzb_off, con
             double s = 0.0;
 #pragma unroll
             for (int k2 = 0; k2 <= r; ++k2) s += Li[r * 6 + k2] * T[k2][c];
-            Ks64[k * 36 + r * 6 + c] = s;
-            Ks32[k * 36 + r * 6 + c] = (float)s;
-            const int64_t e = ell_base[k] + (int64_t)(r * 6 + c) * ROWS_PER_WG;
+            const int64_t e = base + (int64_t)(r * 6 + c) * ROWS_PER_WG;
             Ke64[e] = s;
             Ke32[e] = (float)s;
         }
@@ -159,10 +159,10 @@ int launch_assemble(Plan* p, hipStream_t s) {
                            p->d_diag, p->d_Kc, p->d_Linv, p->d_Minv, p->d_status);
         JSSO_HIP(hipGetLastError());
     }
-    if (p->nnzb_off > 0) {
-        hipLaunchKernelGGL(scale_blocks_kernel, dim3((unsigned)((p->nnzb_off + 255) / 256)), dim3(256), 0, s,
-                           p->nnzb_off, p->d_offsrc, p->d_browof, p->d_colind, p->d_Kc, p->d_Linv,
-                           p->d_ell_base, p->d_Ks64, p->d_Ks32, p->d_Ke64, p->d_Ke32);
+    if (p->ell_slots > 0) {
+        hipLaunchKernelGGL(scale_blocks_kernel, dim3((unsigned)((p->ell_slots + 255) / 256)), dim3(256), 0, s,
+                           p->ell_slots, p->d_ell_src, p->d_browof, p->d_colind, p->d_Kc, p->d_Linv, p->d_Ke64,
+                           p->d_Ke32);
         JSSO_HIP(hipGetLastError());
     }
     return JSSO_OK;

@@ -51,15 +51,14 @@ struct Plan {
     int64_t n_contrib = 0; // element blocks = 4*n_beam + 16*n_quad
     // host copies (symbolic analysis runs on the host; uploaded on first use)
     std::vector<int32_t> h_node_off, h_rowptr, h_colind;
-    std::vector<int32_t> h_browof, h_diag, h_offrowptr, h_offcol, h_offsrc, h_cptr, h_csrc, h_nptr, h_nsrc;
+    std::vector<int32_t> h_browof, h_diag, h_cptr, h_csrc, h_nptr, h_nsrc;
     std::vector<int32_t> h_cnct_beam, h_cnct_quad, h_wg_sys, h_wg_row0, h_wg_row1, h_sys_wg_off;
     std::vector<int32_t> h_slice_voff, h_ell_col;   // sliced-ELL layout of the off-diagonal blocks
-    std::vector<int64_t> h_ell_base;
+    std::vector<int32_t> h_ell_src;  // per ELL position: source block in the block CSR, -1 = padding
     std::vector<uint8_t> h_fixed;
     bool on_device = false;
     // pattern (device)
     int32_t *d_rowptr = nullptr, *d_colind = nullptr, *d_browof = nullptr, *d_diag = nullptr;
-    int32_t *d_offrowptr = nullptr, *d_offcol = nullptr, *d_offsrc = nullptr;
     int32_t *d_cptr = nullptr, *d_csrc = nullptr;       // block -> element-block contributions
     int32_t *d_nptr = nullptr, *d_nsrc = nullptr;       // node -> adjoint output offsets
     int32_t *d_cnct_beam = nullptr, *d_cnct_quad = nullptr;
@@ -69,16 +68,17 @@ struct Plan {
     int32_t n_wg = 0;
     // sliced ELL: slice = workgroup above; values [slot][36][64 lanes] from 36*slice_voff[sl]
     int32_t *d_slice_voff = nullptr, *d_ell_col = nullptr;
-    int64_t* d_ell_base = nullptr;   // per off-diagonal block: index of its entry 0 in the ELL arrays
+    int32_t* d_ell_src = nullptr;    // per ELL position (slot-major, lane fastest): source block or -1
     int64_t ell_slots = 0;           // sum over slices of (slots x 64)
+    // waves per slice of the batched CG kernel, per [FP32 | FP64][load cases]:
+    // 0 = not tuned yet (the next solve measures the variants and stores the fastest)
+    int32_t cg_wps[2][9] = {};
     double* d_Ke64 = nullptr;        // scaled off-diagonal blocks, sliced ELL, FP64
     float* d_Ke32 = nullptr;         // same, FP32
     // numeric state (device)
     double* d_ke = nullptr;       // element blocks, 36 doubles each
     double* d_K = nullptr;        // K, 36 per block (unconstrained)
     double* d_Linv = nullptr;     // inverse Cholesky factor of each constrained diagonal block
-    double* d_Ks64 = nullptr;     // scaled off-diagonal blocks, FP64
-    float* d_Ks32 = nullptr;      // scaled off-diagonal blocks, FP32
     double* d_Minv = nullptr;     // block-Jacobi inverse (plain path)
     double* d_Kc = nullptr;       // constrained K incl. diagonal (plain path)
     int32_t* d_status = nullptr;  // device error flags

@@ -318,6 +318,14 @@ __global__ void k_decide(Slices sl, Partials P, SysState st, int nsys_struct, in
 
 enum { CG_START = 0, CG_STEP = 1, CG_FRESH = 2 };
 
+// Waves-per-slice variants built for VT and NR: the cross-wave LDS buffer stays
+// <= 24 KB, and more waves only pay while one slot group fits the registers.
+template <typename VT, int NR, int WPS>
+struct WpsFits {
+    static constexpr bool value =
+        WPS == 1 || ((size_t)(WPS - 1) * 6 * NR * 64 * sizeof(VT) <= 24576 && NR * sizeof(VT) <= 16);
+};
+
 // Fused CG iteration k (one lane per block row, sliced ELL).
 //   CG_STEP : alpha = rr_{k-1}/<p,q>_{k-1}; r_k = r_{k-1} - alpha q_{k-1} (own and
 //             gathered rows, on the fly); x += alpha p_{k-1};
@@ -329,68 +337,98 @@ enum { CG_START = 0, CG_STEP = 1, CG_FRESH = 2 };
 // then s = A r_k; p_k = r_k + beta p_{k-1}; q_k = s + beta q_{k-1}; partials
 // <p,q>, <r,q>, <q,q>, <r,r> of iteration k.  A non-positive curvature or a
 // negative recurrence restarts the direction (alpha = beta = 0).
-template <typename MT, typename VT, int NR, int MODE>
-__global__ __launch_bounds__(64) void k_cgf(Slices sl, Partials P, SysState st, int k, const MT* __restrict__ A,
-                                            const VT* __restrict__ rprev, const VT* __restrict__ qprev,
-                                            VT* __restrict__ rcur, VT* __restrict__ qcur, VT* __restrict__ p,
-                                            VT* __restrict__ xlo) {
-    __shared__ double lds[1];
+//
+// WPS waves share a slice: wave v takes ELL slots v, v + WPS, ...; waves
+// 1..WPS-1 hand their partial products to wave 0 through LDS, and wave 0
+// adds them in wave order and finishes the row.  More waves per slice means
+// more loads in flight for structures too small to fill the GPU.
+template <typename MT, typename VT, int NR, int MODE, int WPS>
+__global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysState st, int k,
+                                                  const MT* __restrict__ A, const VT* __restrict__ rprev,
+                                                  const VT* __restrict__ qprev, VT* __restrict__ rcur,
+                                                  VT* __restrict__ qcur, VT* __restrict__ p,
+                                                  VT* __restrict__ xlo) {
+    constexpr bool STEP = MODE == CG_STEP;
+    __shared__ double lds[4 * NR * WPS];
+    __shared__ VT red[WPS > 1 ? WPS - 1 : 1][6 * NR][64];
     const int w = slice_of_block(sl);
     if (w < 0) return;
     const int sys = sl.sys[w];
     if (structure_done<NR>(st, sys)) return;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t i = (int64_t)sl.row0[w] + lane;
     const bool on = i < sl.row1[w];
+    const int64_t ii = on ? i : (int64_t)sl.row0[w];
     const int pk = (k - 1) & 1, ck = k & 1;
-    VT alpha[NR], beta[NR];
-    bool act[NR];
+    const VT* rsrc = STEP ? rprev : rcur;   // where r_{k-1} (STEP) or r_k lives
+    const int v0 = sl.voff[w], v1 = sl.voff[w + 1];
+    const int nslot = (v1 - v0) / ROWS_PER_WG;
+    const MT* Aw = A + 36 * (int64_t)v0 + lane;
+    const int32_t* cw = sl.ell_col + v0 + lane;
+
+    // own row (wave 0): r_{k-1} / r_k and q_{k-1}
+    VT rown[6][NR], qown[6][NR];
+    if (wave == 0) {
 #pragma unroll
-    for (int h = 0; h < NR; ++h) {
-        alpha[h] = VT(0);
-        beta[h] = VT(0);
-        act[h] = st.active[sys * NR + h] != 0;
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int h = 0; h < NR; ++h) {
+                const int64_t o = (6 * ii + c) * NR + h;
+                rown[c][h] = rsrc[o];
+                qown[c][h] = MODE == CG_START ? VT(0) : qprev[o];
+            }
     }
-    if (MODE == CG_STEP) {
-        const double* const src[4] = {P.rrf[pk], P.pq[pk], P.rq[pk], P.qq[pk]};
-        double tot[4][NR];
-        system_totals_k<4, NR, 1>(sl, sys, src, tot, lds);
+
+    // CG scalars of this structure (identical in every thread of every slice)
+    VT alpha[NR], beta[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) alpha[h] = beta[h] = VT(0);
+    if (MODE != CG_START) {
+        const int s0 = sl.sys_off[sys], s1 = sl.sys_off[sys + 1];
+        constexpr int K = STEP ? 4 : 2;
+        const double* src[K];
+        if (STEP) {
+            src[0] = P.rrf[pk];
+            src[1] = P.pq[pk];
+            src[2] = P.rq[pk];
+            src[3] = P.qq[pk];
+        } else {
+            src[0] = P.rtrue;
+            src[1] = P.rrf[pk];
+        }
+        double v[K * NR];
+#pragma unroll
+        for (int a = 0; a < K * NR; ++a) v[a] = 0.0;
+        for (int q = s0 + (int)threadIdx.x; q < s1; q += 64 * WPS) {
+#pragma unroll
+            for (int a = 0; a < K; ++a)
+#pragma unroll
+                for (int h = 0; h < NR; ++h) v[a * NR + h] += src[a][(int64_t)q * NR + h];
+        }
+        group_sum<K * NR, WPS>(v, lds);
 #pragma unroll
         for (int h = 0; h < NR; ++h) {
-            const double rr = tot[0][h], pq = tot[1][h], rq = tot[2][h], qq = tot[3][h];
-            if (pq > 0.0 && isfinite(pq) && rr > 0.0) {
-                const double a = rr / pq;
-                const double rrn = rr - 2.0 * a * rq + a * a * qq;
-                alpha[h] = VT(a);
-                beta[h] = rrn > 0.0 ? VT(rrn / rr) : VT(0);
+            if (STEP) {
+                const double rr = v[h], pq = v[NR + h], rq = v[2 * NR + h], qq = v[3 * NR + h];
+                if (pq > 0.0 && isfinite(pq) && rr > 0.0) {
+                    const double a = rr / pq;
+                    const double rrn = rr - 2.0 * a * rq + a * a * qq;
+                    alpha[h] = VT(a);
+                    beta[h] = rrn > 0.0 ? VT(rrn / rr) : VT(0);
+                }
+            } else {
+                beta[h] = v[NR + h] > 0.0 ? VT(v[h] / v[NR + h]) : VT(0);
             }
         }
-    } else if (MODE == CG_FRESH) {
-        const double* const src[2] = {P.rtrue, P.rrf[pk]};
-        double tot[2][NR];
-        system_totals_k<2, NR, 1>(sl, sys, src, tot, lds);
-#pragma unroll
-        for (int h = 0; h < NR; ++h) beta[h] = tot[1][h] > 0.0 ? VT(tot[0][h] / tot[1][h]) : VT(0);
     }
-    const VT* rsrc = (MODE == CG_STEP) ? rprev : rcur;   // where r_{k-1} (STEP) or r_k lives
-    // own row: r_k, x update
-    const int64_t ii = on ? i : (int64_t)sl.row0[w];
-    VT rk[6][NR], acc[6][NR];
+
+    // neighbour products: acc = sum_j A_ij r_k(j)
+    VT acc[6][NR];
 #pragma unroll
     for (int c = 0; c < 6; ++c)
 #pragma unroll
-        for (int h = 0; h < NR; ++h) {
-            const int64_t o = (6 * ii + c) * NR + h;
-            VT v = rsrc[o];
-            if (MODE == CG_STEP) v = v - alpha[h] * qprev[o];
-            rk[c][h] = v;
-            acc[c][h] = v;
-        }
-    const int v0 = sl.voff[w], v1 = sl.voff[w + 1];
-    const MT* Aw = A + 36 * (int64_t)v0 + lane;
-    const int32_t* cw = sl.ell_col + v0 + lane;
-    const int nslot = (v1 - v0) / ROWS_PER_WG;
-    for (int j = 0; j < nslot; ++j) {
+        for (int h = 0; h < NR; ++h) acc[c][h] = VT(0);
+    for (int j = wave; j < nslot; j += WPS) {
         const int64_t col = cw[(int64_t)j * ROWS_PER_WG];
         VT xj[6][NR];
 #pragma unroll
@@ -399,7 +437,7 @@ __global__ __launch_bounds__(64) void k_cgf(Slices sl, Partials P, SysState st, 
             for (int h = 0; h < NR; ++h) {
                 const int64_t o = (6 * col + c) * NR + h;
                 VT v = rsrc[o];
-                if (MODE == CG_STEP) v = v - alpha[h] * qprev[o];
+                if (STEP) v = v - alpha[h] * qprev[o];
                 xj[c][h] = v;
             }
         const MT* a = Aw + (int64_t)j * 36 * ROWS_PER_WG;
@@ -410,9 +448,30 @@ __global__ __launch_bounds__(64) void k_cgf(Slices sl, Partials P, SysState st, 
             for (int h = 0; h < NR; ++h) acc[e / 6][h] += av * xj[e % 6][h];
         }
     }
+    if (WPS > 1) {
+        if (wave > 0) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+#pragma unroll
+                for (int h = 0; h < NR; ++h) red[wave - 1][c * NR + h][lane] = acc[c][h];
+        }
+        __syncthreads();
+        if (wave > 0) return;
+#pragma unroll
+        for (int u = 0; u < WPS - 1; ++u)
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+#pragma unroll
+                for (int h = 0; h < NR; ++h) acc[c][h] += red[u][c * NR + h][lane];
+    }
+
+    // wave 0: r_k, s = r_k + acc (unit diagonal blocks), p, q, x, partials
     double vpq[NR], vrq[NR], vqq[NR], vrr[NR];
 #pragma unroll
     for (int h = 0; h < NR; ++h) vpq[h] = vrq[h] = vqq[h] = vrr[h] = 0.0;
+    bool act[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) act[h] = st.active[sys * NR + h] != 0;
     if (on) {
 #pragma unroll
         for (int c = 0; c < 6; ++c)
@@ -421,16 +480,16 @@ __global__ __launch_bounds__(64) void k_cgf(Slices sl, Partials P, SysState st, 
                 if (!act[h]) continue;
                 const int64_t o = (6 * i + c) * NR + h;
                 const VT pold = MODE == CG_START ? VT(0) : p[o];
-                const VT qold = MODE == CG_START ? VT(0) : qprev[o];
-                if (MODE == CG_STEP) {
+                const VT rk = STEP ? rown[c][h] - alpha[h] * qown[c][h] : rown[c][h];
+                if (STEP) {
                     xlo[o] += alpha[h] * pold;
-                    rcur[o] = rk[c][h];
+                    rcur[o] = rk;
                 }
-                const VT pn = rk[c][h] + beta[h] * pold;
-                const VT qn = acc[c][h] + beta[h] * qold;
+                const VT pn = rk + beta[h] * pold;
+                const VT qn = (rk + acc[c][h]) + beta[h] * qown[c][h];
                 p[o] = pn;
                 qcur[o] = qn;
-                const double r_ = (double)rk[c][h], q_ = (double)qn;
+                const double r_ = (double)rk, q_ = (double)qn;
                 vpq[h] += (double)pn * q_;
                 vrq[h] += r_ * q_;
                 vqq[h] += q_ * q_;
@@ -447,7 +506,7 @@ __global__ __launch_bounds__(64) void k_cgf(Slices sl, Partials P, SysState st, 
     }
 #pragma unroll
     for (int a = 0; a < 4 * NR; ++a) v[a] = wave_sum(v[a]);
-    if (threadIdx.x == 0) {
+    if (lane == 0) {
 #pragma unroll
         for (int h = 0; h < NR; ++h) {
             P.pq[ck][(int64_t)w * NR + h] = v[h];
@@ -683,6 +742,86 @@ static Slices make_slices(Plan* pl, int base, int count) {
                   base, count, (count + 7) / 8};
 }
 
+// Launch one ordinary CG iteration with `wps` waves per slice (falls back to
+// the largest variant whose LDS fits).
+template <typename MT, typename VT, int NR>
+static void launch_step(int wps, const Slices& sl, const Workspace& w, int it, const MT* A, VT* const (&rb)[2],
+                        VT* const (&qb)[2], VT* p, VT* xlo, hipStream_t s) {
+    const dim3 grid = slice_grid(sl.count);
+    const VT* rp = rb[(it - 1) & 1];
+    const VT* qp = qb[(it - 1) & 1];
+    if constexpr (WpsFits<VT, NR, 4>::value) {
+        if (wps >= 4) {
+            hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_STEP, 4>), grid, dim3(256), 0, s, sl, w.P, w.st, it, A, rp, qp,
+                               rb[it & 1], qb[it & 1], p, xlo);
+            return;
+        }
+    }
+    if constexpr (WpsFits<VT, NR, 2>::value) {
+        if (wps >= 2) {
+            hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_STEP, 2>), grid, dim3(128), 0, s, sl, w.P, w.st, it, A, rp, qp,
+                               rb[it & 1], qb[it & 1], p, xlo);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_STEP, 1>), grid, dim3(64), 0, s, sl, w.P, w.st, it, A, rp, qp,
+                       rb[it & 1], qb[it & 1], p, xlo);
+}
+
+template <typename MT, typename VT, int NR>
+static void launch_init(Plan* pl, const Slices& sl, const Workspace& w, const double* f, const MT* A,
+                        VT* const (&rb)[2], VT* const (&qb)[2], VT* p, VT* xlo, double tol, hipStream_t s) {
+    const dim3 gridS = slice_grid(sl.count);
+    const int nsys = pl->n_sys * NR;
+    hipLaunchKernelGGL((k_init<VT, NR>), gridS, dim3(SOLVER_WG), 0, s, sl, w.P, pl->d_fixed, pl->d_Linv, f, w.b64,
+                       w.x64, w.xbest, xlo, rb[0], p);
+    hipLaunchKernelGGL((k_decide<NR>), dim3((nsys + 127) / 128), dim3(128), 0, s, sl, w.P, w.st, pl->n_sys, 0, tol,
+                       0);
+    hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_START, 1>), gridS, dim3(64), 0, s, sl, w.P, w.st, 0, A,
+                       (const VT*)rb[1], (const VT*)qb[1], rb[0], qb[0], p, xlo);
+}
+
+// Autotuner for the waves-per-slice count: time `reps` ordinary iterations of
+// each variant (after one untimed launch) on the live problem and keep the
+// fastest; a larger count must win by 3% to be chosen.  The caller
+// re-initialises the solve afterwards, so the result does not depend on the
+// iterations run here.
+template <typename MT, typename VT, int NR>
+static int tune_wps(Plan* pl, const Slices& sl, const Workspace& w, const MT* A, VT* const (&rb)[2],
+                    VT* const (&qb)[2], VT* p, VT* xlo, hipStream_t s, int& chosen) {
+    const int reps = 8;
+    hipEvent_t e0, e1;
+    JSSO_HIP(hipEventCreate(&e0));
+    JSSO_HIP(hipEventCreate(&e1));
+    int it = 0;
+    float best_ms = 0.f;
+    chosen = 1;
+    for (int wps : {1, 2, 4}) {
+        if ((wps == 2 && !WpsFits<VT, NR, 2>::value) || (wps == 4 && !WpsFits<VT, NR, 4>::value)) continue;
+        launch_step<MT, VT, NR>(wps, sl, w, ++it, A, rb, qb, p, xlo, s);
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps; ++r) launch_step<MT, VT, NR>(wps, sl, w, ++it, A, rb, qb, p, xlo, s);
+        (void)hipEventRecord(e1, s);
+        hipError_t e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            set_error("autotune: %s", hipGetErrorString(e));
+            return JSSO_EHIP;
+        }
+        if (wps == 1 || ms < 0.97f * best_ms) {
+            best_ms = ms;
+            chosen = wps;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    JSSO_HIP(hipGetLastError());
+    return JSSO_OK;
+}
+
 template <typename MT, typename VT, int NR>
 static int run_batched(Plan* pl, const double* f, double* u, double tol, int maxit, int replace_every,
                        jsso_info* info, hipStream_t s) {
@@ -696,14 +835,17 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
     const MT* A = nullptr;
     if constexpr (sizeof(MT) == 4) A = (const MT*)pl->d_Ke32; else A = (const MT*)pl->d_Ke64;
     VT *xlo = (VT*)w.xlo, *p = (VT*)w.p;
-    VT* rb[2] = {(VT*)w.r[0], (VT*)w.r[1]};
-    VT* qb[2] = {(VT*)w.q[0], (VT*)w.q[1]};
-    hipLaunchKernelGGL((k_init<VT, NR>), gridS, blockS, 0, s, sl, w.P, pl->d_fixed, pl->d_Linv, f, w.b64, w.x64,
-                       w.xbest, xlo, rb[0], p);
-    hipLaunchKernelGGL((k_decide<NR>), gridD, blockD, 0, s, sl, w.P, w.st, pl->n_sys, 0, tol, 0);
-    hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_START>), gridS, block64, 0, s, sl, w.P, w.st, 0, A,
-                       (const VT*)rb[1], (const VT*)qb[1], rb[0], qb[0], p, xlo);
+    VT* const rb[2] = {(VT*)w.r[0], (VT*)w.r[1]};
+    VT* const qb[2] = {(VT*)w.q[0], (VT*)w.q[1]};
+    int& wps = pl->cg_wps[sizeof(MT) == 4 ? 0 : 1][NR];
+    launch_init<MT, VT, NR>(pl, sl, w, f, A, rb, qb, p, xlo, tol, s);
     JSSO_HIP(hipGetLastError());
+    if (wps <= 0) {
+        rc = tune_wps<MT, VT, NR>(pl, sl, w, A, rb, qb, p, xlo, s, wps);
+        if (rc) return rc;
+        launch_init<MT, VT, NR>(pl, sl, w, f, A, rb, qb, p, xlo, tol, s);
+        JSSO_HIP(hipGetLastError());
+    }
     std::vector<int32_t> act(nsys);
     int it = 0, repl = 0;
     bool done = false;
@@ -713,9 +855,7 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
             ++it;
             std::pair<hipEvent_t, hipEvent_t>* ev;
             timing_begin(s, ev);
-            hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_STEP>), gridS, block64, 0, s, sl, w.P, w.st, it, A,
-                               (const VT*)rb[(it - 1) & 1], (const VT*)qb[(it - 1) & 1], rb[it & 1], qb[it & 1], p,
-                               xlo);
+            launch_step<MT, VT, NR>(wps, sl, w, it, A, rb, qb, p, xlo, s);
             timing_end(s, ev);
         }
         // the last iteration of the chunk: update, fold, replace, then continue fresh
@@ -728,7 +868,7 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
         hipLaunchKernelGGL((k_decide<NR>), gridD, blockD, 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, it);
         hipLaunchKernelGGL((k_save<NR>), gridS, blockS, 0, s, sl, (const int32_t*)w.st.save, (const double*)w.x64,
                            w.xbest);
-        hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_FRESH>), gridS, block64, 0, s, sl, w.P, w.st, it, A,
+        hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_FRESH, 1>), gridS, block64, 0, s, sl, w.P, w.st, it, A,
                            (const VT*)rb[(it - 1) & 1], (const VT*)qb[(it - 1) & 1], rb[it & 1], qb[it & 1], p, xlo);
         JSSO_HIP(hipGetLastError());
         ++repl;
@@ -759,6 +899,7 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
         info->converged = conv;
         info->n_systems = nsys;
         info->rel_residual = mx;
+        info->waves_per_slice = wps;
     }
     return conv == nsys ? JSSO_OK : JSSO_NOT_CONVERGED;
 }

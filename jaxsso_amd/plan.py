@@ -96,6 +96,14 @@ class FEPlan:
             warnings.warn(f"CG did not converge: {self.last_info}", RuntimeWarning, stacklevel=2)
         return u.reshape(f.shape) if vec else u
 
+    def cg_tuning(self, precision: str = "mixed", nrhs: int = 1, set_wps: int = -1) -> int:
+        """Waves per slice the batched CG kernel uses for (precision, nrhs): 0 until
+        the first solve has tuned it.  set_wps 1/2/4 forces a variant, 0 re-tunes."""
+        L = nat.lib()
+        out = ctypes.c_int32(0)
+        nat.check(L.jsso_plan_cg_tuning(self._h, nat.PREC[precision], int(nrhs), int(set_wps), ctypes.byref(out)))
+        return int(out.value)
+
     def adjoint(self, crds, prop_beam, prop_quad, u, lam=None, lam_scale: float = 0.5, want=("crds", "props")):
         """g = -sum_e lam_e^T dK_e/dp u_e for coordinates and all element properties.
 

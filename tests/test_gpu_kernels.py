@@ -169,3 +169,48 @@ def test_adjoint_general_lambda():
                              lam=torch.tensor(lam))
     assert np.abs(dc.cpu().numpy() - gref).max() <= 1e-9 * np.abs(gref).max()
     assert np.abs(dq.cpu().numpy() - eg["quad_props"]).max() <= 1e-9 * np.abs(eg["quad_props"]).max()
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("nrhs", [1, 2, 4])
+def test_cg_kernel_variants_agree_with_oracle(precision, nrhs):
+    """Every waves-per-slice variant of the batched CG kernel (forced through the
+    tuning entry point) solves a batch of two uneven designs to the oracle."""
+    designs = [wl.cfg1(0), wl.gridshell(9, 10.0, 2.0, 0.05, cfg=3, design=5, half_span_cases=True)]
+    from jaxsso_amd.batch import StructureBatch
+    b = StructureBatch(designs, n_cases=nrhs)
+    refs = []
+    for i, st in enumerate(designs):
+        K = system.assemble_K(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads)
+        cases = st.load_cases()
+        refs.append([osolve.solve_reduced(K, st.known_id, cases[c]) if c < cases.shape[0] else np.zeros(st.ndof)
+                     for c in range(nrhs)])
+    sols = {}
+    for wps in (1, 2, 4):
+        b.plan.cg_tuning(precision, nrhs, set_wps=wps)
+        u = b.solve(precision=precision, tol=1e-10).cpu().numpy().reshape(-1, nrhs)
+        info = b.plan.last_info
+        assert info["converged"] == info["n_systems"] == 2 * nrhs, (wps, info)
+        assert info["waves_per_slice"] in (1, 2, 4)
+        for i, st in enumerate(designs):
+            o = 6 * int(b.node_off[i])
+            for c in range(nrhs):
+                if np.any(refs[i][c]):
+                    assert rel(u[o:o + st.ndof, c], refs[i][c]) < 1e-8, (wps, i, c)
+        sols[wps] = u
+    # the variants differ only in summation order
+    assert rel(sols[2], sols[1]) < 1e-9 and rel(sols[4], sols[1]) < 1e-9
+
+
+def test_cg_autotuner_records_a_choice():
+    st = wl.cfg1()
+    plan = FEPlan([0, st.n_node], st.cnct_beamcols, st.cnct_quads, st.known_id)
+    plan.assemble(torch.tensor(st.crds), None, torch.tensor(st.prop_quads))
+    assert plan.cg_tuning("mixed", 1) == 0
+    u1 = plan.solve(torch.tensor(st.nodal_loads), precision="mixed").cpu().numpy()
+    chosen = plan.cg_tuning("mixed", 1)
+    assert chosen in (1, 2, 4) and plan.last_info["waves_per_slice"] == chosen
+    # a second solve reuses the choice and repeats the first bit for bit
+    u2 = plan.solve(torch.tensor(st.nodal_loads), precision="mixed").cpu().numpy()
+    assert np.array_equal(u1, u2)
+    assert plan.cg_tuning("fp64", 1) == 0

@@ -299,3 +299,18 @@ def test_config_sizes(cfg, nodes, elements, dof, cons, nnzb):
     assert len(st.known_id) == cons
     P = system.pattern_K(st.n_node, st.cnct_beamcols, st.cnct_quads)
     assert P.nnz == 36 * nnzb
+
+
+def test_shell200_fixture_is_consistent():
+    """The committed 200x200 oracle fixture: homogeneity in E holds to rounding
+    (sum_e E_e dC/dE_e = -C), supported DOFs carry zero displacement, and the
+    long-double refinement converged."""
+    with open(os.path.join(os.path.dirname(__file__), "golden", "shell200_oracle.json")) as fh:
+        fx = json.load(fh)
+    assert abs(fx["sum_E_dE"] + fx["C"]) / fx["C"] < 1e-10
+    st = wl.cfg2()
+    known = set(st.known_id.tolist())
+    u = dict(zip(fx["u_index"], fx["u"]))
+    assert any(i in known for i in u) and all(v == 0.0 for i, v in u.items() if i in known)
+    assert fx["refinement_residuals"][-1] < 1e-14
+    assert fx["C"] > 0
