@@ -135,6 +135,14 @@ int jsso_adjoint_grad(jsso_plan* plan, const double* d_crds, const double* d_pro
 /* y = K x with the assembled, unconstrained K (nrhs = 1). */
 int jsso_spmv(jsso_plan* plan, const double* d_x, double* d_y, void* stream);
 
+/* Compliance-type reduction per structure and load case:
+ *     d_C[s][h] = scale * sum over the DOFs of structure s of f[dof][h] u[dof][h]
+ * (d_C: n_sys x nrhs; scale 1/2 gives the strain-energy objective C = 1/2 f.u of
+ * JaxSSO/SSO_model.py:275-306, per packed structure).  Deterministic: per-slice
+ * partial sums added in a fixed order, no atomics. */
+int jsso_compliance(jsso_plan* plan, const double* d_f, const double* d_u, int32_t nrhs,
+                    double scale, double* d_C, void* stream);
+
 /* Tuning of the batched CG kernel (no counterpart in the reference: the
  * reference hands the solve to SuperLU).  The first batched solve on a plan for a
  * (precision, nrhs) pair times the 1-, 2- and 4-wavefront-per-slice variants on

@@ -19,8 +19,8 @@ PREC = {"fp64": 0, "mixed": 1, "plain": 2}
 
 EXPORTS = ("jsso_version", "jsso_last_error", "jsso_plan_create", "jsso_plan_destroy", "jsso_plan_sizes",
            "jsso_plan_pattern", "jsso_element_ke_beam", "jsso_element_ke_quad", "jsso_assemble",
-           "jsso_solve", "jsso_adjoint_grad", "jsso_spmv", "jsso_plan_cg_tuning", "jsso_timing_enable",
-           "jsso_timing_read")
+           "jsso_solve", "jsso_adjoint_grad", "jsso_spmv", "jsso_compliance", "jsso_plan_cg_tuning",
+           "jsso_timing_enable", "jsso_timing_read")
 
 
 class NativeUnavailable(RuntimeError):
@@ -67,6 +67,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.jsso_solve.argtypes = [P, P, P, I32, I32, D, I32, ctypes.POINTER(Info), V]
     lib.jsso_adjoint_grad.argtypes = [P, P, P, P, P, P, D, I32, P, P, P, V]
     lib.jsso_spmv.argtypes = [P, P, P, V]
+    lib.jsso_compliance.argtypes = [P, P, P, I32, D, P, V]
     lib.jsso_plan_cg_tuning.argtypes = [P, I32, I32, I32, P]
     lib.jsso_timing_enable.argtypes = [I32]
     lib.jsso_timing_read.argtypes = [P, P, I32]

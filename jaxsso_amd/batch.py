@@ -49,7 +49,6 @@ class StructureBatch:
             F[6 * o:6 * o + c.shape[1], :c.shape[0]] = c[:m].T
         self.n_cases = m
         self.f = torch.as_tensor(F, device=dev).contiguous()
-        self.sys_of_dof = torch.as_tensor(np.repeat(np.arange(len(n)), 6 * np.asarray(n)), device=dev)
 
     @property
     def n_structures(self):
@@ -66,12 +65,10 @@ class StructureBatch:
         return self.u
 
     def compliance(self, u=None):
-        """(n_structures, n_cases) compliance 1/2 f.u, reduced on the GPU in FP64."""
+        """(n_structures, n_cases) compliance 1/2 f.u, reduced on the GPU in FP64
+        (per-slice partial sums added in a fixed order)."""
         u = self.u if u is None else u
-        fu = (self.f.reshape(u.shape) * u).reshape(-1, self.n_cases) if u.dim() == 2 else (self.f[:, 0] * u)[:, None]
-        C = torch.zeros((self.n_structures, self.n_cases), dtype=torch.float64, device=u.device)
-        C.index_add_(0, self.sys_of_dof, 0.5 * fu)
-        return C
+        return self.plan.compliance(self.f.reshape(u.shape), u, 0.5)
 
     def gradient(self, u=None):
         """Compliance gradients per structure and case: (dcrds, dprop_beam, dprop_quad),

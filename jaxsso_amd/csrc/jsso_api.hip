@@ -213,6 +213,18 @@ static int build_plan(Plan* p, int32_t n_sys, const int32_t* node_off, int32_t n
             }
     }
     p->ell_slots = slice_voff[p->n_wg];
+    std::vector<int32_t> meta(8 * (size_t)p->n_wg, 0);
+    for (int w = 0; w < p->n_wg; ++w) {
+        int32_t* m = &meta[8 * (size_t)w];
+        m[0] = wg_sys[w];
+        m[1] = wg_row0[w];
+        m[2] = wg_row1[w];
+        m[3] = slice_voff[w];
+        m[4] = slice_voff[w + 1];
+        m[5] = sys_wg_off[wg_sys[w]];
+        m[6] = sys_wg_off[wg_sys[w] + 1];
+    }
+    p->h_slice_meta.swap(meta);
     p->h_slice_voff.swap(slice_voff);
     p->h_ell_col.swap(ell_col);
     p->h_ell_src.swap(ell_src);
@@ -244,6 +256,7 @@ static int ensure_device(Plan* p) {
         (rc = upload(p, &p->d_wg_sys, p->h_wg_sys)) || (rc = upload(p, &p->d_wg_row0, p->h_wg_row0)) ||
         (rc = upload(p, &p->d_wg_row1, p->h_wg_row1)) || (rc = upload(p, &p->d_sys_wg_off, p->h_sys_wg_off)) ||
         (rc = upload(p, &p->d_slice_voff, p->h_slice_voff)) || (rc = upload(p, &p->d_ell_col, p->h_ell_col)) ||
+        (rc = upload(p, &p->d_slice_meta, p->h_slice_meta)) ||
         (rc = upload(p, &p->d_ell_src, p->h_ell_src)))
         return rc;
     const size_t n = p->n_nodes;
@@ -311,6 +324,10 @@ int jsso_plan_create(int32_t n_sys, const int32_t* h_node_off, int32_t n_beam, c
 int jsso_plan_destroy(jsso_plan* plan) {
     if (!plan) return JSSO_OK;
     Plan* q = P(plan);
+    for (auto& row : q->cg_graph)
+        for (auto& g : row)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (q->capture_stream) (void)hipStreamDestroy(q->capture_stream);
     for (void* a : q->allocs) (void)hipFree(a);
     for (auto& b : q->ws)
         if (b.p) (void)hipFree(b.p);
@@ -459,6 +476,18 @@ int jsso_timing_enable(int32_t every) { return timing_enable(every); }
 
 int jsso_timing_read(double* h_avg_ms, int64_t* h_samples, int32_t reset) {
     return timing_read(h_avg_ms, h_samples, reset);
+}
+
+int jsso_compliance(jsso_plan* plan, const double* d_f, const double* d_u, int32_t nrhs, double scale, double* d_C,
+                    void* stream) {
+    if (!plan || !d_f || !d_u || !d_C || nrhs < 1) {
+        set_error("invalid compliance arguments");
+        return JSSO_EINVAL;
+    }
+    Plan* p = P(plan);
+    int rc = ensure_device(p);
+    if (rc) return rc;
+    return launch_compliance(p, d_f, d_u, nrhs, scale, d_C, (hipStream_t)stream);
 }
 
 int jsso_plan_cg_tuning(jsso_plan* plan, int32_t precision, int32_t nrhs, int32_t set_wps, int32_t* h_wps) {

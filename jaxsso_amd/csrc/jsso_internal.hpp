@@ -34,6 +34,7 @@ void set_error(const char* fmt, ...);
 constexpr int BDIR = 12;   // beam adjoint directions: 6 coordinates + 6 properties
 constexpr int QDIR = 17;   // quad adjoint directions: 12 coordinates + 5 properties
 constexpr int WS_ADJ_SLOT = 12;  // Plan::ws slot of the adjoint's gradient scratch (solver slots are below it)
+constexpr int WS_FU_SLOT = 13;   // Plan::ws slot of the f.u reduction's per-slice partials
 constexpr int ROWS_PER_WG = 64;   // block rows per solver slice (one wavefront in the SpMV)
 constexpr int SOLVER_WG = 6 * ROWS_PER_WG;   // threads of the scalar-row solver kernels
 
@@ -41,6 +42,14 @@ constexpr int SOLVER_WG = 6 * ROWS_PER_WG;   // threads of the scalar-row solver
 struct DBuf {
     void* p = nullptr;
     size_t bytes = 0;
+};
+
+// An instantiated HIP graph of one chunk of ordinary CG iterations, valid for
+// the buffers and kernel variant it was captured with.
+struct CgGraph {
+    hipGraphExec_t exec = nullptr;
+    int wps = 0, steps = 0;
+    const void* key[10] = {};
 };
 
 struct Plan {
@@ -54,6 +63,7 @@ struct Plan {
     std::vector<int32_t> h_browof, h_diag, h_cptr, h_csrc, h_nptr, h_nsrc;
     std::vector<int32_t> h_cnct_beam, h_cnct_quad, h_wg_sys, h_wg_row0, h_wg_row1, h_sys_wg_off;
     std::vector<int32_t> h_slice_voff, h_ell_col;   // sliced-ELL layout of the off-diagonal blocks
+    std::vector<int32_t> h_slice_meta;  // per slice: sys, row0, row1, voff0, voff1, first/end slice of sys, 0
     std::vector<int32_t> h_ell_src;  // per ELL position: source block in the block CSR, -1 = padding
     std::vector<uint8_t> h_fixed;
     bool on_device = false;
@@ -67,12 +77,15 @@ struct Plan {
     int32_t* d_sys_wg_off = nullptr;
     int32_t n_wg = 0;
     // sliced ELL: slice = workgroup above; values [slot][36][64 lanes] from 36*slice_voff[sl]
-    int32_t *d_slice_voff = nullptr, *d_ell_col = nullptr;
+    int32_t *d_slice_voff = nullptr, *d_ell_col = nullptr, *d_slice_meta = nullptr;
     int32_t* d_ell_src = nullptr;    // per ELL position (slot-major, lane fastest): source block or -1
     int64_t ell_slots = 0;           // sum over slices of (slots x 64)
     // waves per slice of the batched CG kernel, per [FP32 | FP64][load cases]:
     // 0 = not tuned yet (the next solve measures the variants and stores the fastest)
     int32_t cg_wps[2][9] = {};
+    CgGraph cg_graph[2][9];          // captured CG chunks, per [FP32 | FP64][load cases]
+    hipStream_t capture_stream = nullptr;
+    bool graphs_ok = true;           // false after a failed capture: launch eagerly
     double* d_Ke64 = nullptr;        // scaled off-diagonal blocks, sliced ELL, FP64
     float* d_Ke32 = nullptr;         // same, FP32
     // numeric state (device)
@@ -119,6 +132,7 @@ int solve_batched(Plan* p, const double* f, double* u, int nrhs, bool mixed, dou
                   int maxit, jsso_info* info, hipStream_t s);
 int timing_enable(int every);
 int timing_read(double* avg_ms, int64_t* samples, int reset);
+int launch_compliance(Plan* p, const double* f, const double* u, int nrhs, double scale, double* C, hipStream_t s);
 int solve_plain(Plan* p, const double* f, double* u, int nrhs, double tol, int maxit,
                 jsso_info* info, hipStream_t s);
 

@@ -52,12 +52,11 @@ struct KernelTiming {
     int64_t counter = 0, samples = 0;
     double total_ms = 0.0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pool, pending;
+    std::vector<int> pending_launches;   // launches each pending event pair brackets
 };
 static KernelTiming g_timing;
 
-static void timing_begin(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev) {
-    ev = nullptr;
-    if (g_timing.every <= 0 || (g_timing.counter++ % g_timing.every) != 0) return;
+static void timing_start(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev, int launches) {
     std::pair<hipEvent_t, hipEvent_t> e;
     if (!g_timing.pool.empty()) {
         e = g_timing.pool.back();
@@ -66,8 +65,23 @@ static void timing_begin(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev) 
         return;
     }
     g_timing.pending.push_back(e);
+    g_timing.pending_launches.push_back(launches);
     ev = &g_timing.pending.back();
     (void)hipEventRecord(ev->first, s);
+}
+
+// one launch in `every` is bracketed by events
+static void timing_begin(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev) {
+    ev = nullptr;
+    if (g_timing.every <= 0 || (g_timing.counter++ % g_timing.every) != 0) return;
+    timing_start(s, ev, 1);
+}
+
+// a graph of `launches` kernels back to back: bracketed whenever timing is on
+static void timing_begin_batch(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev, int launches) {
+    ev = nullptr;
+    if (g_timing.every <= 0) return;
+    timing_start(s, ev, launches);
 }
 
 static void timing_end(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>* ev) {
@@ -75,15 +89,17 @@ static void timing_end(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>* ev) {
 }
 
 static void timing_collect() {
-    for (auto& e : g_timing.pending) {
+    for (size_t i = 0; i < g_timing.pending.size(); ++i) {
+        auto& e = g_timing.pending[i];
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
             g_timing.total_ms += ms;
-            g_timing.samples++;
+            g_timing.samples += g_timing.pending_launches[i];
         }
         g_timing.pool.push_back(e);
     }
     g_timing.pending.clear();
+    g_timing.pending_launches.clear();
 }
 
 int timing_enable(int every) {
@@ -108,9 +124,10 @@ int timing_read(double* avg_ms, int64_t* samples, int reset) {
 struct Slices {
     const int32_t *sys, *row0, *row1, *sys_off;  // slice -> structure, rows; structure -> first slice
     const int32_t *voff, *ell_col;                // sliced ELL
-    int base;                                     // first slice of the launch (plain path)
-    int count;                                    // slices in the launch
-    int chunk;                                    // ceil(count / 8): slices per XCD
+    const int4* meta;   // per slice, two int4: {sys, row0, row1, voff0}, {voff1, first, end slice of sys, 0}
+    int base;           // first slice of the launch (plain path)
+    int count;          // slices in the launch
+    int chunk;          // ceil(count / 8): slices per XCD
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs; map them so that each
@@ -129,15 +146,17 @@ static inline dim3 slice_grid(int count) { return dim3(8 * ((count + 7) / 8)); }
 struct SysState {
     double *bnorm2, *relres, *best;
     int32_t *active, *status, *stall, *save;
+    int32_t* slice_act;   // per slice: bit h set while load case h of its structure runs
 };
 
-// Partial-sum buffers, one value per slice and rhs.  The fused CG kernel of
-// iteration k writes pq/rq/qq/rr into slot k & 1 and reads slot (k - 1) & 1.
+// Partial-sum buffers, per slice.  The fused CG kernel of iteration k writes
+// cg[k & 1] and reads cg[(k - 1) & 1]; each slice holds [4][rhs] values.
+enum { CG_RR = 0, CG_PQ = 1, CG_RQ = 2, CG_QQ = 3 };
 struct Partials {
-    double *pq[2], *rq[2], *qq[2], *rrf[2];   // fused CG: <p,q>, <r,q>, <q,q>, <r,r>
-    double *b, *t, *rtrue;                     // <b,b>, true FP64 <r,r>, low-precision <r,r> after replacement
-    double *rr0, *rr1;                          // plain path: alternating <r,z>
-    double* pqp;                                // plain path: <p,q>
+    double* cg[2];           // fused CG: <r,r>, <p,q>, <r,q>, <q,q>
+    double *b, *t, *rtrue;   // <b,b>; true FP64 <r,r> and low-precision <r,r> after a replacement
+    double *rr0, *rr1;       // plain path: alternating <r,z>
+    double* pqp;             // plain path: <p,q>
     __host__ __device__ double* rr(int it) const { return (it & 1) ? rr1 : rr0; }
 };
 
@@ -180,38 +199,27 @@ __device__ __forceinline__ void group_sum(double (&v)[NV], double* lds) {
     }
 }
 
-// Totals over the slices of `sys` of K partial buffers (NR values per slice
-// each), identical in every thread of every workgroup of the structure.
-template <int K, int NR, int NW>
-__device__ __forceinline__ void system_totals_k(const Slices& sl, int sys, const double* const (&src)[K],
-                                                double (&tot)[K][NR], double* lds) {
-    const int s0 = sl.sys_off[sys], s1 = sl.sys_off[sys + 1];
-    double v[K * NR];
-#pragma unroll
-    for (int i = 0; i < K * NR; ++i) v[i] = 0.0;
-    for (int w = s0 + (int)threadIdx.x; w < s1; w += NW * 64) {
-#pragma unroll
-        for (int a = 0; a < K; ++a)
-#pragma unroll
-            for (int h = 0; h < NR; ++h) v[a * NR + h] += src[a][(int64_t)w * NR + h];
-    }
-    group_sum<K * NR, NW>(v, lds);
-#pragma unroll
-    for (int a = 0; a < K; ++a)
-#pragma unroll
-        for (int h = 0; h < NR; ++h) tot[a][h] = v[a * NR + h];
-}
-
+// Totals over the slices of `sys` of one or two per-slice partial arrays (NR
+// values per slice), identical in every thread of every workgroup of the structure.
 template <int NR, int NW>
 __device__ __forceinline__ void system_totals(const Slices& sl, int sys, const double* a, const double* b,
                                               double (&ta)[NR], double (&tb)[NR], double* lds) {
-    const double* const src[2] = {a, b ? b : a};
-    double tot[2][NR];
-    system_totals_k<2, NR, NW>(sl, sys, src, tot, lds);
+    const int s0 = sl.sys_off[sys], s1 = sl.sys_off[sys + 1];
+    double v[2 * NR];
+#pragma unroll
+    for (int i = 0; i < 2 * NR; ++i) v[i] = 0.0;
+    for (int w = s0 + (int)threadIdx.x; w < s1; w += NW * 64) {
+#pragma unroll
+        for (int h = 0; h < NR; ++h) {
+            v[h] += a[(int64_t)w * NR + h];
+            if (b) v[NR + h] += b[(int64_t)w * NR + h];
+        }
+    }
+    group_sum<2 * NR, NW>(v, lds);
 #pragma unroll
     for (int h = 0; h < NR; ++h) {
-        ta[h] = tot[0][h];
-        tb[h] = tot[1][h];
+        ta[h] = v[h];
+        tb[h] = v[NR + h];
     }
 }
 
@@ -269,51 +277,84 @@ __global__ __launch_bounds__(SOLVER_WG) void k_init(Slices sl, Partials P, const
     write_partials<NR, NWAVES>(vb, lds, P.b, w);
 }
 
-// One thread per (structure, rhs): convergence bookkeeping from the partials.
+// One wavefront per structure: convergence bookkeeping of its load cases from
+// the slice partials, and the per-slice activity masks the CG kernels test.
 // mode 0 (after k_init): bnorm2, activity.  mode 1 (after k_replace): true
-// relative residual, best iterate, stagnation, status.
+// relative residual, best iterate, stagnation, status.  Every lane computes
+// the same (butterfly) totals; lane 0 writes the per-system state.
 template <int NR>
-__global__ void k_decide(Slices sl, Partials P, SysState st, int nsys_struct, int mode, double tol, int it) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nsys_struct * NR) return;
-    const int sys = k / NR, h = k % NR;
+__global__ __launch_bounds__(64) void k_decide(Slices sl, Partials P, SysState st, int nsys_struct, int mode,
+                                               double tol, int it) {
+    const int sys = blockIdx.x;
+    if (sys >= nsys_struct) return;
+    const int lane = threadIdx.x;
     const int s0 = sl.sys_off[sys], s1 = sl.sys_off[sys + 1];
-    if (mode == 0) {
-        double bb = 0.0;
-        for (int w = s0; w < s1; ++w) bb += P.b[(int64_t)w * NR + h];
-        st.bnorm2[k] = bb;
-        st.relres[k] = bb > 0.0 ? 1.0 : 0.0;
-        st.best[k] = st.relres[k];
-        st.active[k] = bb > 0.0 ? 1 : 0;
-        st.status[k] = bb > 0.0 ? SYS_RUNNING : SYS_CONVERGED;
-        st.stall[k] = 0;
-        st.save[k] = 0;
-        return;
+    double ta[NR], tb[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) ta[h] = tb[h] = 0.0;
+    for (int w = s0 + lane; w < s1; w += 64) {
+#pragma unroll
+        for (int h = 0; h < NR; ++h) {
+            if (mode == 0) {
+                ta[h] += P.b[(int64_t)w * NR + h];
+            } else {
+                ta[h] += P.t[(int64_t)w * NR + h];
+                tb[h] += P.rtrue[(int64_t)w * NR + h];
+            }
+        }
     }
-    if (!st.active[k]) {
-        st.save[k] = 0;
-        return;
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+        ta[h] = wave_sum(ta[h]);
+        tb[h] = wave_sum(tb[h]);
     }
-    double tt = 0.0, rr = 0.0;
-    for (int w = s0; w < s1; ++w) {
-        tt += P.t[(int64_t)w * NR + h];
-        rr += P.rtrue[(int64_t)w * NR + h];
+    int mask = 0;
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+        const int k = sys * NR + h;
+        if (mode == 0) {
+            const double bb = ta[h];
+            if (lane == 0) {
+                st.bnorm2[k] = bb;
+                st.relres[k] = bb > 0.0 ? 1.0 : 0.0;
+                st.best[k] = bb > 0.0 ? 1.0 : 0.0;
+                st.active[k] = bb > 0.0 ? 1 : 0;
+                st.status[k] = bb > 0.0 ? SYS_RUNNING : SYS_CONVERGED;
+                st.stall[k] = 0;
+                st.save[k] = 0;
+            }
+            mask |= bb > 0.0 ? 1 << h : 0;
+            continue;
+        }
+        if (!st.active[k]) {
+            if (lane == 0) st.save[k] = 0;
+            continue;
+        }
+        const double rel = sqrt(ta[h] / st.bnorm2[k]);
+        const double best = st.best[k];
+        const bool improved = rel < best;
+        const double nbest = improved ? rel : best;
+        const int stall = rel <= 0.9 * best ? it : st.stall[k];   // iteration of the last 10% gain
+        const int window = it / 2 > 2048 ? it / 2 : 2048;
+        int active = 1, status = SYS_RUNNING;
+        if (rel <= tol || tb[h] == 0.0) {
+            active = 0;
+            status = SYS_CONVERGED;
+        } else if (!isfinite(rel) || rel > 1e3 * nbest || it - stall > window) {
+            active = 0;
+            status = SYS_STALLED;
+        }
+        if (lane == 0) {
+            st.relres[k] = rel;
+            st.save[k] = improved ? 1 : 0;
+            st.best[k] = nbest;
+            st.stall[k] = stall;
+            st.active[k] = active;
+            st.status[k] = status;
+        }
+        mask |= active << h;
     }
-    const double rel = sqrt(tt / st.bnorm2[k]);
-    st.relres[k] = rel;
-    const double best = st.best[k];
-    const bool improved = rel < best;
-    st.save[k] = improved ? 1 : 0;
-    if (improved) st.best[k] = rel;
-    if (rel <= 0.9 * best) st.stall[k] = it;   // iteration of the last 10% gain
-    const int window = it / 2 > 2048 ? it / 2 : 2048;
-    if (rel <= tol || rr == 0.0) {
-        st.active[k] = 0;
-        st.status[k] = SYS_CONVERGED;
-    } else if (!isfinite(rel) || rel > 1e3 * st.best[k] || it - st.stall[k] > window) {
-        st.active[k] = 0;
-        st.status[k] = SYS_STALLED;
-    }
+    for (int w = s0 + lane; w < s1; w += 64) st.slice_act[w] = mask;
 }
 
 enum { CG_START = 0, CG_STEP = 1, CG_FRESH = 2 };
@@ -335,8 +376,12 @@ struct WpsFits {
 //   CG_FRESH: r_k already in buffer k & 1 (a replacement), alpha = 0,
 //             beta = <r_k,r_k> (replacement partials) / rr_{k-1}
 // then s = A r_k; p_k = r_k + beta p_{k-1}; q_k = s + beta q_{k-1}; partials
-// <p,q>, <r,q>, <q,q>, <r,r> of iteration k.  A non-positive curvature or a
+// <r,r>, <p,q>, <r,q>, <q,q> of iteration k.  A non-positive curvature or a
 // negative recurrence restarts the direction (alpha = beta = 0).
+//
+// The slice's metadata and activity come in one dependent load; the own row,
+// the first ELL slot and the scalar partials are then all requested before
+// any of them is used, so the scalar reduction overlaps the first loads.
 //
 // WPS waves share a slice: wave v takes ELL slots v, v + WPS, ...; waves
 // 1..WPS-1 hand their partial products to wave 0 through LDS, and wave 0
@@ -353,21 +398,23 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
     __shared__ VT red[WPS > 1 ? WPS - 1 : 1][6 * NR][64];
     const int w = slice_of_block(sl);
     if (w < 0) return;
-    const int sys = sl.sys[w];
-    if (structure_done<NR>(st, sys)) return;
+    const int4 ma = sl.meta[2 * w];
+    const int4 mb = sl.meta[2 * w + 1];
+    const int amask = st.slice_act[w];
+    if (amask == 0) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t i = (int64_t)sl.row0[w] + lane;
-    const bool on = i < sl.row1[w];
-    const int64_t ii = on ? i : (int64_t)sl.row0[w];
+    const int64_t i = (int64_t)ma.y + lane;
+    const bool on = i < ma.z;
+    const int64_t ii = on ? i : (int64_t)ma.y;
+    const int v0 = ma.w, v1 = mb.x, s0 = mb.y, s1 = mb.z;
+    const int nslot = (v1 - v0) / ROWS_PER_WG;
     const int pk = (k - 1) & 1, ck = k & 1;
     const VT* rsrc = STEP ? rprev : rcur;   // where r_{k-1} (STEP) or r_k lives
-    const int v0 = sl.voff[w], v1 = sl.voff[w + 1];
-    const int nslot = (v1 - v0) / ROWS_PER_WG;
     const MT* Aw = A + 36 * (int64_t)v0 + lane;
     const int32_t* cw = sl.ell_col + v0 + lane;
 
-    // own row (wave 0): r_{k-1} / r_k and q_{k-1}
-    VT rown[6][NR], qown[6][NR];
+    // own row (wave 0): r, q, p and x of this row, all needed by the update
+    VT rown[6][NR], qown[6][NR], pown[6][NR], xown[6][NR];
     if (wave == 0) {
 #pragma unroll
         for (int c = 0; c < 6; ++c)
@@ -376,40 +423,43 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
                 const int64_t o = (6 * ii + c) * NR + h;
                 rown[c][h] = rsrc[o];
                 qown[c][h] = MODE == CG_START ? VT(0) : qprev[o];
+                pown[c][h] = MODE == CG_START ? VT(0) : p[o];
+                xown[c][h] = STEP ? xlo[o] : VT(0);
             }
     }
+    // column of this wave's first ELL slot
+    int j = wave;
+    int64_t col = j < nslot ? (int64_t)cw[(int64_t)j * ROWS_PER_WG] : ii;
 
     // CG scalars of this structure (identical in every thread of every slice)
     VT alpha[NR], beta[NR];
 #pragma unroll
     for (int h = 0; h < NR; ++h) alpha[h] = beta[h] = VT(0);
     if (MODE != CG_START) {
-        const int s0 = sl.sys_off[sys], s1 = sl.sys_off[sys + 1];
         constexpr int K = STEP ? 4 : 2;
-        const double* src[K];
-        if (STEP) {
-            src[0] = P.rrf[pk];
-            src[1] = P.pq[pk];
-            src[2] = P.rq[pk];
-            src[3] = P.qq[pk];
-        } else {
-            src[0] = P.rtrue;
-            src[1] = P.rrf[pk];
-        }
         double v[K * NR];
 #pragma unroll
         for (int a = 0; a < K * NR; ++a) v[a] = 0.0;
+        const double* cgp = P.cg[pk];
         for (int q = s0 + (int)threadIdx.x; q < s1; q += 64 * WPS) {
+            if (STEP) {
+                const double* src = cgp + (int64_t)q * 4 * NR;
 #pragma unroll
-            for (int a = 0; a < K; ++a)
+                for (int a = 0; a < 4 * NR; ++a) v[a] += src[a];
+            } else {
 #pragma unroll
-                for (int h = 0; h < NR; ++h) v[a * NR + h] += src[a][(int64_t)q * NR + h];
+                for (int h = 0; h < NR; ++h) {
+                    v[h] += P.rtrue[(int64_t)q * NR + h];
+                    v[NR + h] += cgp[((int64_t)q * 4 + CG_RR) * NR + h];
+                }
+            }
         }
         group_sum<K * NR, WPS>(v, lds);
 #pragma unroll
         for (int h = 0; h < NR; ++h) {
             if (STEP) {
-                const double rr = v[h], pq = v[NR + h], rq = v[2 * NR + h], qq = v[3 * NR + h];
+                const double rr = v[CG_RR * NR + h], pq = v[CG_PQ * NR + h];
+                const double rq = v[CG_RQ * NR + h], qq = v[CG_QQ * NR + h];
                 if (pq > 0.0 && isfinite(pq) && rr > 0.0) {
                     const double a = rr / pq;
                     const double rrn = rr - 2.0 * a * rq + a * a * qq;
@@ -428,8 +478,7 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
     for (int c = 0; c < 6; ++c)
 #pragma unroll
         for (int h = 0; h < NR; ++h) acc[c][h] = VT(0);
-    for (int j = wave; j < nslot; j += WPS) {
-        const int64_t col = cw[(int64_t)j * ROWS_PER_WG];
+    for (; j < nslot; j += WPS) {
         VT xj[6][NR];
 #pragma unroll
         for (int c = 0; c < 6; ++c)
@@ -441,12 +490,16 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
                 xj[c][h] = v;
             }
         const MT* a = Aw + (int64_t)j * 36 * ROWS_PER_WG;
+        // next slot's column before this slot's products
+        const int jn = j + WPS;
+        const int64_t coln = jn < nslot ? (int64_t)cw[(int64_t)jn * ROWS_PER_WG] : ii;
 #pragma unroll
         for (int e = 0; e < 36; ++e) {
             const VT av = (VT)a[e * ROWS_PER_WG];
 #pragma unroll
             for (int h = 0; h < NR; ++h) acc[e / 6][h] += av * xj[e % 6][h];
         }
+        col = coln;
     }
     if (WPS > 1) {
         if (wave > 0) {
@@ -466,23 +519,20 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
     }
 
     // wave 0: r_k, s = r_k + acc (unit diagonal blocks), p, q, x, partials
-    double vpq[NR], vrq[NR], vqq[NR], vrr[NR];
+    double v[4 * NR];
 #pragma unroll
-    for (int h = 0; h < NR; ++h) vpq[h] = vrq[h] = vqq[h] = vrr[h] = 0.0;
-    bool act[NR];
-#pragma unroll
-    for (int h = 0; h < NR; ++h) act[h] = st.active[sys * NR + h] != 0;
+    for (int a = 0; a < 4 * NR; ++a) v[a] = 0.0;
     if (on) {
 #pragma unroll
         for (int c = 0; c < 6; ++c)
 #pragma unroll
             for (int h = 0; h < NR; ++h) {
-                if (!act[h]) continue;
+                if (!((amask >> h) & 1)) continue;
                 const int64_t o = (6 * i + c) * NR + h;
-                const VT pold = MODE == CG_START ? VT(0) : p[o];
+                const VT pold = pown[c][h];
                 const VT rk = STEP ? rown[c][h] - alpha[h] * qown[c][h] : rown[c][h];
                 if (STEP) {
-                    xlo[o] += alpha[h] * pold;
+                    xlo[o] = xown[c][h] + alpha[h] * pold;
                     rcur[o] = rk;
                 }
                 const VT pn = rk + beta[h] * pold;
@@ -490,30 +540,18 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
                 p[o] = pn;
                 qcur[o] = qn;
                 const double r_ = (double)rk, q_ = (double)qn;
-                vpq[h] += (double)pn * q_;
-                vrq[h] += r_ * q_;
-                vqq[h] += q_ * q_;
-                vrr[h] += r_ * r_;
+                v[CG_RR * NR + h] += r_ * r_;
+                v[CG_PQ * NR + h] += (double)pn * q_;
+                v[CG_RQ * NR + h] += r_ * q_;
+                v[CG_QQ * NR + h] += q_ * q_;
             }
-    }
-    double v[4 * NR];
-#pragma unroll
-    for (int h = 0; h < NR; ++h) {
-        v[h] = vpq[h];
-        v[NR + h] = vrq[h];
-        v[2 * NR + h] = vqq[h];
-        v[3 * NR + h] = vrr[h];
     }
 #pragma unroll
     for (int a = 0; a < 4 * NR; ++a) v[a] = wave_sum(v[a]);
     if (lane == 0) {
+        double* dst = P.cg[ck] + (int64_t)w * 4 * NR;
 #pragma unroll
-        for (int h = 0; h < NR; ++h) {
-            P.pq[ck][(int64_t)w * NR + h] = v[h];
-            P.rq[ck][(int64_t)w * NR + h] = v[NR + h];
-            P.qq[ck][(int64_t)w * NR + h] = v[2 * NR + h];
-            P.rrf[ck][(int64_t)w * NR + h] = v[3 * NR + h];
-        }
+        for (int a = 0; a < 4 * NR; ++a) dst[a] = v[a];
     }
 }
 
@@ -529,16 +567,27 @@ __global__ __launch_bounds__(SOLVER_WG) void k_upd(Slices sl, Partials P, SysSta
     if (w < 0) return;
     const int sys = sl.sys[w];
     if (structure_done<NR>(st, sys)) return;
-    const int pk = (k - 1) & 1;
-    double rr[NR], pq[NR];
-    system_totals<NR, NWAVES>(sl, sys, P.rrf[pk], P.pq[pk], rr, pq, lds);
+    const double* cgp = P.cg[(k - 1) & 1];
+    const int s0 = sl.sys_off[sys], s1 = sl.sys_off[sys + 1];
+    double v[2 * NR];
+#pragma unroll
+    for (int a = 0; a < 2 * NR; ++a) v[a] = 0.0;
+    for (int q = s0 + (int)threadIdx.x; q < s1; q += SOLVER_WG) {
+#pragma unroll
+        for (int h = 0; h < NR; ++h) {
+            v[h] += cgp[((int64_t)q * 4 + CG_RR) * NR + h];
+            v[NR + h] += cgp[((int64_t)q * 4 + CG_PQ) * NR + h];
+        }
+    }
+    group_sum<2 * NR, NWAVES>(v, lds);
     const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
     if (R >= 6 * (int64_t)sl.row1[w]) return;
 #pragma unroll
     for (int h = 0; h < NR; ++h) {
         if (!st.active[sys * NR + h]) continue;
-        const bool ok = pq[h] > 0.0 && isfinite(pq[h]) && rr[h] > 0.0;
-        const VT alpha = ok ? VT(rr[h] / pq[h]) : VT(0);
+        const double rr = v[h], pq = v[NR + h];
+        const bool ok = pq > 0.0 && isfinite(pq) && rr > 0.0;
+        const VT alpha = ok ? VT(rr / pq) : VT(0);
         const int64_t o = R * NR + h;
         xlo[o] += alpha * p[o];
         rcur[o] = rprev[o] - alpha * qprev[o];
@@ -655,6 +704,43 @@ __global__ __launch_bounds__(256) void k_unscale(int32_t n_nodes, const double* 
     }
 }
 
+// Compliance-type reductions: part[slice][rhs] = sum over the slice's DOFs of
+// f u; then C[sys][rhs] = scale * (sum of its slices' partials, in order).
+template <int NR>
+__global__ __launch_bounds__(SOLVER_WG) void k_fu_partial(Slices sl, const double* __restrict__ f,
+                                                          const double* __restrict__ u, double* __restrict__ part) {
+    __shared__ double lds[NWAVES * NR];
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
+    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
+    const bool on = R < 6 * (int64_t)sl.row1[w];
+    double v[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) v[h] = on ? f[R * NR + h] * u[R * NR + h] : 0.0;
+    write_partials<NR, NWAVES>(v, lds, part, w);
+}
+
+template <int NR>
+__global__ __launch_bounds__(64) void k_fu_total(Slices sl, int nsys, const double* __restrict__ part, double scale,
+                                                 double* __restrict__ C) {
+    const int sys = blockIdx.x;
+    if (sys >= nsys) return;
+    const int s0 = sl.sys_off[sys], s1 = sl.sys_off[sys + 1];
+    double v[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) v[h] = 0.0;
+    for (int w = s0 + (int)threadIdx.x; w < s1; w += 64) {
+#pragma unroll
+        for (int h = 0; h < NR; ++h) v[h] += part[(int64_t)w * NR + h];
+    }
+#pragma unroll
+    for (int h = 0; h < NR; ++h) v[h] = wave_sum(v[h]);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int h = 0; h < NR; ++h) C[(int64_t)sys * NR + h] = scale * v[h];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // workspace
 // ---------------------------------------------------------------------------
@@ -676,7 +762,7 @@ static int get_workspace(Plan* pl, int nrhs, Workspace& w) {
     for (int i = WS_B; i <= WS_P; ++i) need[i] = n * 8;
     need[WS_PART] = npart * 8 * 14;
     need[WS_SD] = nsys * 8 * 3;
-    need[WS_SI] = nsys * 4 * 4;
+    need[WS_SI] = (nsys * 4 + (size_t)pl->n_wg) * 4;
     if (pl->ws.size() < WS_N) pl->ws.resize(WS_N);
     for (int i = 0; i < WS_N; ++i) {
         if (i == WS_ADJ || need[i] == 0) continue;
@@ -699,12 +785,8 @@ static int get_workspace(Plan* pl, int nrhs, Workspace& w) {
     w.q[1] = pl->ws[WS_Q1].p;
     w.p = pl->ws[WS_P].p;
     double* pp = (double*)pl->ws[WS_PART].p;
-    for (int a = 0; a < 2; ++a) {
-        w.P.pq[a] = pp + (0 + a) * npart;
-        w.P.rq[a] = pp + (2 + a) * npart;
-        w.P.qq[a] = pp + (4 + a) * npart;
-        w.P.rrf[a] = pp + (6 + a) * npart;
-    }
+    w.P.cg[0] = pp;
+    w.P.cg[1] = pp + 4 * npart;
     w.P.b = pp + 8 * npart;
     w.P.t = pp + 9 * npart;
     w.P.rtrue = pp + 10 * npart;
@@ -720,6 +802,7 @@ static int get_workspace(Plan* pl, int nrhs, Workspace& w) {
     w.st.status = si + nsys;
     w.st.stall = si + 2 * nsys;
     w.st.save = si + 3 * nsys;
+    w.st.slice_act = si + 4 * nsys;
     return JSSO_OK;
 }
 
@@ -738,8 +821,9 @@ static int check_status(Plan* p, hipStream_t s) {
 }
 
 static Slices make_slices(Plan* pl, int base, int count) {
-    return Slices{pl->d_wg_sys, pl->d_wg_row0, pl->d_wg_row1, pl->d_sys_wg_off, pl->d_slice_voff, pl->d_ell_col,
-                  base, count, (count + 7) / 8};
+    return Slices{pl->d_wg_sys,     pl->d_wg_row0, pl->d_wg_row1, pl->d_sys_wg_off,
+                  pl->d_slice_voff, pl->d_ell_col, (const int4*)pl->d_slice_meta, base,
+                  count,            (count + 7) / 8};
 }
 
 // Launch one ordinary CG iteration with `wps` waves per slice (falls back to
@@ -775,8 +859,7 @@ static void launch_init(Plan* pl, const Slices& sl, const Workspace& w, const do
     const int nsys = pl->n_sys * NR;
     hipLaunchKernelGGL((k_init<VT, NR>), gridS, dim3(SOLVER_WG), 0, s, sl, w.P, pl->d_fixed, pl->d_Linv, f, w.b64,
                        w.x64, w.xbest, xlo, rb[0], p);
-    hipLaunchKernelGGL((k_decide<NR>), dim3((nsys + 127) / 128), dim3(128), 0, s, sl, w.P, w.st, pl->n_sys, 0, tol,
-                       0);
+    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 0, tol, 0);
     hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_START, 1>), gridS, dim3(64), 0, s, sl, w.P, w.st, 0, A,
                        (const VT*)rb[1], (const VT*)qb[1], rb[0], qb[0], p, xlo);
 }
@@ -822,6 +905,56 @@ static int tune_wps(Plan* pl, const Slices& sl, const Workspace& w, const MT* A,
     return JSSO_OK;
 }
 
+// HIP graphs for the ordinary iterations of a chunk: one graph launch replaces
+// `steps` kernel launches, so the host's launch rate no longer bounds short
+// iterations.  The graph is captured once per (buffers, variant, chunk length)
+// on a plan-owned stream and launched on the caller's stream.  Iteration k only
+// selects buffers by its parity, so with an even chunk length one graph serves
+// every chunk.  JSSO_CG_GRAPH=0 launches eagerly.
+static bool graphs_enabled() {
+    const char* e = getenv("JSSO_CG_GRAPH");
+    return !(e && e[0] == '0');
+}
+
+template <typename MT, typename VT, int NR>
+static hipGraphExec_t chunk_graph(Plan* pl, int wps, int steps, const Slices& sl, const Workspace& w, const MT* A,
+                                  VT* const (&rb)[2], VT* const (&qb)[2], VT* p, VT* xlo) {
+    if (!pl->graphs_ok || !graphs_enabled()) return nullptr;
+    CgGraph& g = pl->cg_graph[sizeof(MT) == 4 ? 0 : 1][NR];
+    const void* key[10] = {A, rb[0], rb[1], qb[0], qb[1], p, xlo, w.P.cg[0], w.P.cg[1], w.st.slice_act};
+    bool same = g.exec && g.wps == wps && g.steps == steps;
+    for (int i = 0; same && i < 10; ++i) same = g.key[i] == key[i];
+    if (same) return g.exec;
+    if (g.exec) {
+        (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+    }
+    if (!pl->capture_stream && hipStreamCreateWithFlags(&pl->capture_stream, hipStreamNonBlocking) != hipSuccess) {
+        pl->graphs_ok = false;
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    hipStream_t cs = pl->capture_stream;
+    hipGraph_t graph = nullptr;
+    bool ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        for (int it = 1; it <= steps; ++it) launch_step<MT, VT, NR>(wps, sl, w, it, A, rb, qb, p, xlo, cs);
+        ok = hipStreamEndCapture(cs, &graph) == hipSuccess && graph;
+    }
+    if (ok) ok = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (!ok) {
+        g.exec = nullptr;
+        pl->graphs_ok = false;
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    g.wps = wps;
+    g.steps = steps;
+    for (int i = 0; i < 10; ++i) g.key[i] = key[i];
+    return g.exec;
+}
+
 template <typename MT, typename VT, int NR>
 static int run_batched(Plan* pl, const double* f, double* u, double tol, int maxit, int replace_every,
                        jsso_info* info, hipStream_t s) {
@@ -831,7 +964,6 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
     const Slices sl = make_slices(pl, 0, pl->n_wg);
     const dim3 gridS = slice_grid(pl->n_wg), blockS(SOLVER_WG), block64(64);
     const int nsys = pl->n_sys * NR;
-    const dim3 gridD((nsys + 127) / 128), blockD(128);
     const MT* A = nullptr;
     if constexpr (sizeof(MT) == 4) A = (const MT*)pl->d_Ke32; else A = (const MT*)pl->d_Ke64;
     VT *xlo = (VT*)w.xlo, *p = (VT*)w.p;
@@ -851,12 +983,23 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
     bool done = false;
     while (!done && it < maxit) {
         const int steps = std::max(1, std::min(replace_every, maxit - it));
-        for (int j = 0; j + 1 < steps; ++j) {
-            ++it;
+        hipGraphExec_t gx = nullptr;
+        if (steps == replace_every && steps % 2 == 0 && steps > 1)
+            gx = chunk_graph<MT, VT, NR>(pl, wps, steps - 1, sl, w, A, rb, qb, p, xlo);
+        if (gx) {
             std::pair<hipEvent_t, hipEvent_t>* ev;
-            timing_begin(s, ev);
-            launch_step<MT, VT, NR>(wps, sl, w, it, A, rb, qb, p, xlo, s);
+            timing_begin_batch(s, ev, steps - 1);
+            JSSO_HIP(hipGraphLaunch(gx, s));
             timing_end(s, ev);
+            it += steps - 1;
+        } else {
+            for (int j = 0; j + 1 < steps; ++j) {
+                ++it;
+                std::pair<hipEvent_t, hipEvent_t>* ev;
+                timing_begin(s, ev);
+                launch_step<MT, VT, NR>(wps, sl, w, it, A, rb, qb, p, xlo, s);
+                timing_end(s, ev);
+            }
         }
         // the last iteration of the chunk: update, fold, replace, then continue fresh
         ++it;
@@ -865,7 +1008,7 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
         hipLaunchKernelGGL((k_fold<VT, NR>), gridS, blockS, 0, s, sl, (const int32_t*)w.st.active, xlo, w.x64);
         hipLaunchKernelGGL((k_replace<VT, NR>), gridS, block64, 0, s, sl, w.P, w.st, (const double*)pl->d_Ke64,
                            (const double*)w.b64, (const double*)w.x64, rb[it & 1]);
-        hipLaunchKernelGGL((k_decide<NR>), gridD, blockD, 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, it);
+        hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, it);
         hipLaunchKernelGGL((k_save<NR>), gridS, blockS, 0, s, sl, (const int32_t*)w.st.save, (const double*)w.x64,
                            w.xbest);
         hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_FRESH, 1>), gridS, block64, 0, s, sl, w.P, w.st, it, A,
@@ -925,6 +1068,41 @@ static int replace_interval(bool mixed) {
     const char* e = getenv("JSSO_REPLACE_EVERY");
     const int v = e ? atoi(e) : 0;
     return v > 0 ? v : (mixed ? 64 : 256);
+}
+
+template <int NR>
+static int run_compliance(Plan* pl, const double* f, const double* u, double scale, double* C, hipStream_t s) {
+    const size_t need = (size_t)pl->n_wg * NR * sizeof(double);
+    if (pl->ws.size() <= WS_FU_SLOT) pl->ws.resize(WS_FU_SLOT + 1);
+    DBuf& g = pl->ws[WS_FU_SLOT];
+    if (g.bytes < need) {
+        if (g.p) JSSO_HIP(hipFree(g.p));
+        g.p = nullptr;
+        g.bytes = 0;
+        hipError_t e = hipMalloc(&g.p, need);
+        JSSO_REQUIRE(e == hipSuccess, JSSO_ENOMEM, "f.u partials hipMalloc(%zu): %s", need, hipGetErrorString(e));
+        g.bytes = need;
+    }
+    const Slices sl = make_slices(pl, 0, pl->n_wg);
+    hipLaunchKernelGGL((k_fu_partial<NR>), slice_grid(pl->n_wg), dim3(SOLVER_WG), 0, s, sl, f, u, (double*)g.p);
+    hipLaunchKernelGGL((k_fu_total<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, pl->n_sys, (const double*)g.p, scale,
+                       C);
+    JSSO_HIP(hipGetLastError());
+    return JSSO_OK;
+}
+
+int launch_compliance(Plan* p, const double* f, const double* u, int nrhs, double scale, double* C, hipStream_t s) {
+    if (p->n_wg == 0) return JSSO_OK;
+    switch (nrhs) {
+        case 1: return run_compliance<1>(p, f, u, scale, C, s);
+        case 2: return run_compliance<2>(p, f, u, scale, C, s);
+        case 3: return run_compliance<3>(p, f, u, scale, C, s);
+        case 4: return run_compliance<4>(p, f, u, scale, C, s);
+        case 8: return run_compliance<8>(p, f, u, scale, C, s);
+        default:
+            set_error("nrhs=%d unsupported (1, 2, 3, 4 or 8)", nrhs);
+            return JSSO_EINVAL;
+    }
 }
 
 int solve_batched(Plan* p, const double* f, double* u, int nrhs, bool mixed, double tol, int maxit,

@@ -96,6 +96,19 @@ class FEPlan:
             warnings.warn(f"CG did not converge: {self.last_info}", RuntimeWarning, stacklevel=2)
         return u.reshape(f.shape) if vec else u
 
+    def compliance(self, f: torch.Tensor, u: torch.Tensor, scale: float = 0.5) -> torch.Tensor:
+        """scale * f.u per structure and load case, (n_sys, nrhs), reduced on the GPU
+        in FP64 in a fixed order."""
+        L = nat.lib()
+        f2 = _dev64(f).reshape(self.ndof, -1).contiguous()
+        u2 = _dev64(u).reshape(self.ndof, -1).contiguous()
+        if f2.shape != u2.shape:
+            raise ValueError(f"f {tuple(f2.shape)} and u {tuple(u2.shape)} differ")
+        C = torch.empty((self.n_sys, f2.shape[1]), dtype=torch.float64, device=u2.device)
+        nat.check(L.jsso_compliance(self._h, nat.ptr(f2), nat.ptr(u2), f2.shape[1], float(scale), nat.ptr(C),
+                                    nat.stream_ptr()))
+        return C
+
     def cg_tuning(self, precision: str = "mixed", nrhs: int = 1, set_wps: int = -1) -> int:
         """Waves per slice the batched CG kernel uses for (precision, nrhs): 0 until
         the first solve has tuned it.  set_wps 1/2/4 forces a variant, 0 re-tunes."""

@@ -214,3 +214,41 @@ def test_cg_autotuner_records_a_choice():
     u2 = plan.solve(torch.tensor(st.nodal_loads), precision="mixed").cpu().numpy()
     assert np.array_equal(u1, u2)
     assert plan.cg_tuning("fp64", 1) == 0
+
+
+def test_compliance_reduction_matches_per_structure_dot_products():
+    """jsso_compliance: scale * f.u per structure and load case, in a fixed order
+    (two calls agree bit for bit)."""
+    designs = [wl.cfg1(0), wl.gridshell(9, cfg=3, design=2, half_span_cases=True), wl.cfg0()]
+    from jaxsso_amd.batch import StructureBatch
+    b = StructureBatch(designs, n_cases=2)
+    rng = np.random.default_rng(3)
+    u = torch.tensor(rng.normal(size=b.f.shape), device=b.f.device)
+    C1 = b.plan.compliance(b.f, u, 0.5).cpu().numpy()
+    C2 = b.plan.compliance(b.f, u, 0.5).cpu().numpy()
+    assert np.array_equal(C1, C2) and C1.shape == (3, 2)
+    F, U = b.f.cpu().numpy(), u.cpu().numpy()
+    for i in range(3):
+        o0, o1 = 6 * int(b.node_off[i]), 6 * int(b.node_off[i + 1])
+        ref = 0.5 * np.einsum("ij,ij->j", F[o0:o1], U[o0:o1])
+        assert np.allclose(C1[i], ref, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_graph_and_eager_cg_agree_bitwise(precision, monkeypatch):
+    """The CG chunks run as captured HIP graphs or as eager launches: the same
+    kernels in the same order, so the same bits."""
+    designs = [wl.cfg1(d) for d in range(3)]
+    from jaxsso_amd.batch import StructureBatch
+    b = StructureBatch(designs)
+    b.plan.cg_tuning(precision, 1, set_wps=1)
+    monkeypatch.setenv("JSSO_CG_GRAPH", "1")
+    ug = b.solve(precision=precision, tol=1e-10).cpu().numpy()
+    ig = dict(b.plan.last_info)
+    monkeypatch.setenv("JSSO_CG_GRAPH", "0")
+    ue = b.solve(precision=precision, tol=1e-10).cpu().numpy()
+    assert ig["iterations"] == b.plan.last_info["iterations"] and ig["converged"] == 3
+    assert np.array_equal(ug, ue)
+    # a third solve reuses the captured graph
+    monkeypatch.setenv("JSSO_CG_GRAPH", "1")
+    assert np.array_equal(b.solve(precision=precision, tol=1e-10).cpu().numpy(), ug)
