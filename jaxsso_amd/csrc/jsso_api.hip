@@ -325,9 +325,13 @@ int jsso_plan_destroy(jsso_plan* plan) {
     if (!plan) return JSSO_OK;
     Plan* q = P(plan);
     for (auto& row : q->cg_graph)
-        for (auto& g : row)
-            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        for (auto& pair : row)
+            for (auto& g : pair)
+                if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (q->capture_stream) (void)hipStreamDestroy(q->capture_stream);
+    for (auto& e : q->flag_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (q->h_flags) (void)hipHostFree(q->h_flags);
     for (void* a : q->allocs) (void)hipFree(a);
     for (auto& b : q->ws)
         if (b.p) (void)hipFree(b.p);

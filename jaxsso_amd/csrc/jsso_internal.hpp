@@ -47,9 +47,11 @@ struct DBuf {
 // An instantiated HIP graph of one chunk of ordinary CG iterations, valid for
 // the buffers and kernel variant it was captured with.
 struct CgGraph {
+    static constexpr int NKEY = 16;
     hipGraphExec_t exec = nullptr;
     int wps = 0, steps = 0;
-    const void* key[10] = {};
+    double tol = 0.0;
+    const void* key[NKEY] = {};
 };
 
 struct Plan {
@@ -83,9 +85,12 @@ struct Plan {
     // waves per slice of the batched CG kernel, per [FP32 | FP64][load cases]:
     // 0 = not tuned yet (the next solve measures the variants and stores the fastest)
     int32_t cg_wps[2][9] = {};
-    CgGraph cg_graph[2][9];          // captured CG chunks, per [FP32 | FP64][load cases]
+    CgGraph cg_graph[2][9][2];       // captured CG chunks, per [FP32 | FP64][load cases][iterations | replacement]
     hipStream_t capture_stream = nullptr;
     bool graphs_ok = true;           // false after a failed capture: launch eagerly
+    int32_t* h_flags = nullptr;      // pinned host copies of the convergence flags, two chunks' worth
+    size_t h_flags_count = 0;
+    hipEvent_t flag_ev[2] = {nullptr, nullptr};
     double* d_Ke64 = nullptr;        // scaled off-diagonal blocks, sliced ELL, FP64
     float* d_Ke32 = nullptr;         // same, FP32
     // numeric state (device)

@@ -88,18 +88,28 @@ static void timing_end(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>* ev) {
     if (ev) (void)hipEventRecord(ev->second, s);
 }
 
+// fold finished samples into the totals; samples still in flight stay pending
 static void timing_collect() {
+    size_t keep = 0;
     for (size_t i = 0; i < g_timing.pending.size(); ++i) {
-        auto& e = g_timing.pending[i];
+        auto e = g_timing.pending[i];
+        const int n = g_timing.pending_launches[i];
+        if (hipEventQuery(e.second) == hipErrorNotReady) {
+            g_timing.pending[keep] = e;
+            g_timing.pending_launches[keep] = n;
+            ++keep;
+            continue;
+        }
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
             g_timing.total_ms += ms;
-            g_timing.samples += g_timing.pending_launches[i];
+            g_timing.samples += n;
         }
         g_timing.pool.push_back(e);
     }
-    g_timing.pending.clear();
-    g_timing.pending_launches.clear();
+    g_timing.pending.resize(keep);
+    g_timing.pending_launches.resize(keep);
+    (void)hipGetLastError();
 }
 
 int timing_enable(int every) {
@@ -147,6 +157,7 @@ struct SysState {
     double *bnorm2, *relres, *best;
     int32_t *active, *status, *stall, *save;
     int32_t* slice_act;   // per slice: bit h set while load case h of its structure runs
+    int32_t* iters;       // CG iterations run so far, per (structure, rhs)
 };
 
 // Partial-sum buffers, per slice.  The fused CG kernel of iteration k writes
@@ -280,11 +291,12 @@ __global__ __launch_bounds__(SOLVER_WG) void k_init(Slices sl, Partials P, const
 // One wavefront per structure: convergence bookkeeping of its load cases from
 // the slice partials, and the per-slice activity masks the CG kernels test.
 // mode 0 (after k_init): bnorm2, activity.  mode 1 (after k_replace): true
-// relative residual, best iterate, stagnation, status.  Every lane computes
-// the same (butterfly) totals; lane 0 writes the per-system state.
+// relative residual, best iterate, stagnation, status, after `steps` more
+// iterations (counted per system, so the kernel can sit in a captured graph).
+// Every lane computes the same (butterfly) totals; lane 0 writes the state.
 template <int NR>
 __global__ __launch_bounds__(64) void k_decide(Slices sl, Partials P, SysState st, int nsys_struct, int mode,
-                                               double tol, int it) {
+                                               double tol, int steps) {
     const int sys = blockIdx.x;
     if (sys >= nsys_struct) return;
     const int lane = threadIdx.x;
@@ -322,6 +334,7 @@ __global__ __launch_bounds__(64) void k_decide(Slices sl, Partials P, SysState s
                 st.status[k] = bb > 0.0 ? SYS_RUNNING : SYS_CONVERGED;
                 st.stall[k] = 0;
                 st.save[k] = 0;
+                st.iters[k] = 0;
             }
             mask |= bb > 0.0 ? 1 << h : 0;
             continue;
@@ -330,6 +343,7 @@ __global__ __launch_bounds__(64) void k_decide(Slices sl, Partials P, SysState s
             if (lane == 0) st.save[k] = 0;
             continue;
         }
+        const int it = st.iters[k] + steps;
         const double rel = sqrt(ta[h] / st.bnorm2[k]);
         const double best = st.best[k];
         const bool improved = rel < best;
@@ -351,6 +365,7 @@ __global__ __launch_bounds__(64) void k_decide(Slices sl, Partials P, SysState s
             st.stall[k] = stall;
             st.active[k] = active;
             st.status[k] = status;
+            st.iters[k] = it;
         }
         mask |= active << h;
     }
@@ -401,7 +416,8 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
     const int4 ma = sl.meta[2 * w];
     const int4 mb = sl.meta[2 * w + 1];
     const int amask = st.slice_act[w];
-    if (amask == 0) return;
+    // (every metadata field is >= 0: testing them only makes all loads one round trip)
+    if ((amask | ((ma.x | ma.y | ma.z | ma.w | mb.x | mb.y | mb.z) < 0)) == 0) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t i = (int64_t)ma.y + lane;
     const bool on = i < ma.z;
@@ -413,23 +429,26 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
     const MT* Aw = A + 36 * (int64_t)v0 + lane;
     const int32_t* cw = sl.ell_col + v0 + lane;
 
-    // own row (wave 0): r, q, p and x of this row, all needed by the update
+    // Straight-line loads, all requested before any is used: the own row (r, q,
+    // p and x, needed by the update; every wave loads it so that no branch
+    // splits the requests), the first ELL slot's column (a slot past the end
+    // reads the metadata word instead and is replaced), and the first scalar
+    // partials.
     VT rown[6][NR], qown[6][NR], pown[6][NR], xown[6][NR];
-    if (wave == 0) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c)
+    for (int c = 0; c < 6; ++c)
 #pragma unroll
-            for (int h = 0; h < NR; ++h) {
-                const int64_t o = (6 * ii + c) * NR + h;
-                rown[c][h] = rsrc[o];
-                qown[c][h] = MODE == CG_START ? VT(0) : qprev[o];
-                pown[c][h] = MODE == CG_START ? VT(0) : p[o];
-                xown[c][h] = STEP ? xlo[o] : VT(0);
-            }
-    }
-    // column of this wave's first ELL slot
+        for (int h = 0; h < NR; ++h) {
+            const int64_t o = (6 * ii + c) * NR + h;
+            rown[c][h] = rsrc[o];
+            qown[c][h] = MODE == CG_START ? VT(0) : qprev[o];
+            pown[c][h] = MODE == CG_START ? VT(0) : p[o];
+            xown[c][h] = STEP ? xlo[o] : VT(0);
+        }
     int j = wave;
-    int64_t col = j < nslot ? (int64_t)cw[(int64_t)j * ROWS_PER_WG] : ii;
+    const int32_t* cj = j < nslot ? cw + (int64_t)j * ROWS_PER_WG : (const int32_t*)&sl.meta[2 * w];
+    int64_t col = *cj;
+    col = j < nslot ? col : ii;
 
     // CG scalars of this structure (identical in every thread of every slice)
     VT alpha[NR], beta[NR];
@@ -437,23 +456,35 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
     for (int h = 0; h < NR; ++h) alpha[h] = beta[h] = VT(0);
     if (MODE != CG_START) {
         constexpr int K = STEP ? 4 : 2;
+        constexpr int NT = 64 * WPS;   // threads sharing the partial sums
+        constexpr int T = 4;           // slices per thread loaded without a loop
         double v[K * NR];
 #pragma unroll
         for (int a = 0; a < K * NR; ++a) v[a] = 0.0;
         const double* cgp = P.cg[pk];
-        for (int q = s0 + (int)threadIdx.x; q < s1; q += 64 * WPS) {
+        auto add_slice = [&](int q, bool valid) {
+            const int qq = valid ? q : s0;   // the structure's first slice: always a valid address
+            double x[K * NR];
             if (STEP) {
-                const double* src = cgp + (int64_t)q * 4 * NR;
+                const double* src = cgp + (int64_t)qq * 4 * NR;
 #pragma unroll
-                for (int a = 0; a < 4 * NR; ++a) v[a] += src[a];
+                for (int a = 0; a < 4 * NR; ++a) x[a] = src[a];
             } else {
 #pragma unroll
                 for (int h = 0; h < NR; ++h) {
-                    v[h] += P.rtrue[(int64_t)q * NR + h];
-                    v[NR + h] += cgp[((int64_t)q * 4 + CG_RR) * NR + h];
+                    x[h] = P.rtrue[(int64_t)qq * NR + h];
+                    x[NR + h] = cgp[((int64_t)qq * 4 + CG_RR) * NR + h];
                 }
             }
+#pragma unroll
+            for (int a = 0; a < K * NR; ++a) v[a] += valid ? x[a] : 0.0;
+        };
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int q = s0 + (int)threadIdx.x + t * NT;
+            add_slice(q, q < s1);
         }
+        for (int q = s0 + (int)threadIdx.x + T * NT; q < s1; q += NT) add_slice(q, true);
         group_sum<K * NR, WPS>(v, lds);
 #pragma unroll
         for (int h = 0; h < NR; ++h) {
@@ -492,7 +523,9 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
         const MT* a = Aw + (int64_t)j * 36 * ROWS_PER_WG;
         // next slot's column before this slot's products
         const int jn = j + WPS;
-        const int64_t coln = jn < nslot ? (int64_t)cw[(int64_t)jn * ROWS_PER_WG] : ii;
+        const int32_t* cn = jn < nslot ? cw + (int64_t)jn * ROWS_PER_WG : (const int32_t*)&sl.meta[2 * w];
+        int64_t coln = *cn;
+        coln = jn < nslot ? coln : ii;
 #pragma unroll
         for (int e = 0; e < 36; ++e) {
             const VT av = (VT)a[e * ROWS_PER_WG];
@@ -555,13 +588,14 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
     }
 }
 
-// Update half of iteration k before a replacement: alpha from the partials of
-// k - 1; x += alpha p; r_k = r_{k-1} - alpha q_{k-1} into buffer k & 1.
+// Update half of iteration k before a replacement, folded into the FP64
+// solution: alpha from the partials of k - 1; x64 += xlo + alpha p; xlo = 0;
+// r_k = r_{k-1} - alpha q_{k-1} into buffer k & 1.
 template <typename VT, int NR>
 __global__ __launch_bounds__(SOLVER_WG) void k_upd(Slices sl, Partials P, SysState st, int k,
                                                    const VT* __restrict__ rprev, const VT* __restrict__ qprev,
                                                    VT* __restrict__ rcur, const VT* __restrict__ p,
-                                                   VT* __restrict__ xlo) {
+                                                   VT* __restrict__ xlo, double* __restrict__ x64) {
     __shared__ double lds[NWAVES * 2 * NR];
     const int w = slice_of_block(sl);
     if (w < 0) return;
@@ -589,26 +623,10 @@ __global__ __launch_bounds__(SOLVER_WG) void k_upd(Slices sl, Partials P, SysSta
         const bool ok = pq > 0.0 && isfinite(pq) && rr > 0.0;
         const VT alpha = ok ? VT(rr / pq) : VT(0);
         const int64_t o = R * NR + h;
-        xlo[o] += alpha * p[o];
-        rcur[o] = rprev[o] - alpha * qprev[o];
-    }
-}
-
-// x64 += xlo; xlo = 0.
-template <typename VT, int NR>
-__global__ __launch_bounds__(SOLVER_WG) void k_fold(Slices sl, const int32_t* __restrict__ active,
-                                                    VT* __restrict__ xlo, double* __restrict__ x64) {
-    const int w = slice_of_block(sl);
-    if (w < 0) return;
-    const int sys = sl.sys[w];
-    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
-    if (R >= 6 * (int64_t)sl.row1[w]) return;
-#pragma unroll
-    for (int h = 0; h < NR; ++h) {
-        if (!active[sys * NR + h]) continue;
-        const int64_t o = R * NR + h;
-        x64[o] += (double)xlo[o];
+        const VT xn = xlo[o] + alpha * p[o];
+        x64[o] += (double)xn;
         xlo[o] = VT(0);
+        rcur[o] = rprev[o] - alpha * qprev[o];
     }
 }
 
@@ -762,7 +780,7 @@ static int get_workspace(Plan* pl, int nrhs, Workspace& w) {
     for (int i = WS_B; i <= WS_P; ++i) need[i] = n * 8;
     need[WS_PART] = npart * 8 * 14;
     need[WS_SD] = nsys * 8 * 3;
-    need[WS_SI] = (nsys * 4 + (size_t)pl->n_wg) * 4;
+    need[WS_SI] = (nsys * 5 + (size_t)pl->n_wg) * 4;
     if (pl->ws.size() < WS_N) pl->ws.resize(WS_N);
     for (int i = 0; i < WS_N; ++i) {
         if (i == WS_ADJ || need[i] == 0) continue;
@@ -802,7 +820,8 @@ static int get_workspace(Plan* pl, int nrhs, Workspace& w) {
     w.st.status = si + nsys;
     w.st.stall = si + 2 * nsys;
     w.st.save = si + 3 * nsys;
-    w.st.slice_act = si + 4 * nsys;
+    w.st.iters = si + 4 * nsys;
+    w.st.slice_act = si + 5 * nsys;
     return JSSO_OK;
 }
 
@@ -905,25 +924,42 @@ static int tune_wps(Plan* pl, const Slices& sl, const Workspace& w, const MT* A,
     return JSSO_OK;
 }
 
-// HIP graphs for the ordinary iterations of a chunk: one graph launch replaces
-// `steps` kernel launches, so the host's launch rate no longer bounds short
-// iterations.  The graph is captured once per (buffers, variant, chunk length)
-// on a plan-owned stream and launched on the caller's stream.  Iteration k only
-// selects buffers by its parity, so with an even chunk length one graph serves
-// every chunk.  JSSO_CG_GRAPH=0 launches eagerly.
+// The replacement step closing a chunk that ends at iteration `it` after
+// `steps` iterations: update + fold into FP64, true FP64 residual, convergence
+// test, best iterate, fresh direction.
+template <typename MT, typename VT, int NR>
+static void launch_replace(Plan* pl, const Slices& sl, const Workspace& w, int it, int steps, double tol,
+                           const MT* A, VT* const (&rb)[2], VT* const (&qb)[2], VT* p, VT* xlo, hipStream_t s) {
+    const dim3 gridS = slice_grid(sl.count);
+    hipLaunchKernelGGL((k_upd<VT, NR>), gridS, dim3(SOLVER_WG), 0, s, sl, w.P, w.st, it, (const VT*)rb[(it - 1) & 1],
+                       (const VT*)qb[(it - 1) & 1], rb[it & 1], (const VT*)p, xlo, w.x64);
+    hipLaunchKernelGGL((k_replace<VT, NR>), gridS, dim3(64), 0, s, sl, w.P, w.st, (const double*)pl->d_Ke64,
+                       (const double*)w.b64, (const double*)w.x64, rb[it & 1]);
+    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, steps);
+    hipLaunchKernelGGL((k_save<NR>), gridS, dim3(SOLVER_WG), 0, s, sl, (const int32_t*)w.st.save,
+                       (const double*)w.x64, w.xbest);
+    hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_FRESH, 1>), gridS, dim3(64), 0, s, sl, w.P, w.st, it, A,
+                       (const VT*)rb[(it - 1) & 1], (const VT*)qb[(it - 1) & 1], rb[it & 1], qb[it & 1], p, xlo);
+}
+
+// HIP graphs of a chunk: its ordinary iterations (one graph launch replaces
+// steps - 1 kernel launches, so the host's launch rate no longer bounds short
+// iterations) and its replacement step.  Captured once per (buffers, variant,
+// chunk length) on a plan-owned stream and launched on the caller's stream.
+// Kernels select buffers by the parity of the iteration only, and the
+// convergence test counts iterations per system on the device, so with an
+// even chunk length the same two graphs serve every chunk.  JSSO_CG_GRAPH=0
+// launches eagerly.
 static bool graphs_enabled() {
     const char* e = getenv("JSSO_CG_GRAPH");
     return !(e && e[0] == '0');
 }
 
-template <typename MT, typename VT, int NR>
-static hipGraphExec_t chunk_graph(Plan* pl, int wps, int steps, const Slices& sl, const Workspace& w, const MT* A,
-                                  VT* const (&rb)[2], VT* const (&qb)[2], VT* p, VT* xlo) {
-    if (!pl->graphs_ok || !graphs_enabled()) return nullptr;
-    CgGraph& g = pl->cg_graph[sizeof(MT) == 4 ? 0 : 1][NR];
-    const void* key[10] = {A, rb[0], rb[1], qb[0], qb[1], p, xlo, w.P.cg[0], w.P.cg[1], w.st.slice_act};
-    bool same = g.exec && g.wps == wps && g.steps == steps;
-    for (int i = 0; same && i < 10; ++i) same = g.key[i] == key[i];
+template <typename F>
+static hipGraphExec_t capture_graph(Plan* pl, CgGraph& g, const void* const (&key)[CgGraph::NKEY], int wps,
+                                    int steps, double tol, F&& enqueue) {
+    bool same = g.exec && g.wps == wps && g.steps == steps && g.tol == tol;
+    for (int i = 0; same && i < CgGraph::NKEY; ++i) same = g.key[i] == key[i];
     if (same) return g.exec;
     if (g.exec) {
         (void)hipGraphExecDestroy(g.exec);
@@ -938,7 +974,7 @@ static hipGraphExec_t chunk_graph(Plan* pl, int wps, int steps, const Slices& sl
     hipGraph_t graph = nullptr;
     bool ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
-        for (int it = 1; it <= steps; ++it) launch_step<MT, VT, NR>(wps, sl, w, it, A, rb, qb, p, xlo, cs);
+        enqueue(cs);
         ok = hipStreamEndCapture(cs, &graph) == hipSuccess && graph;
     }
     if (ok) ok = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) == hipSuccess;
@@ -951,8 +987,22 @@ static hipGraphExec_t chunk_graph(Plan* pl, int wps, int steps, const Slices& sl
     }
     g.wps = wps;
     g.steps = steps;
-    for (int i = 0; i < 10; ++i) g.key[i] = key[i];
+    g.tol = tol;
+    for (int i = 0; i < CgGraph::NKEY; ++i) g.key[i] = key[i];
     return g.exec;
+}
+
+static int ensure_flag_buffers(Plan* pl, size_t nsys) {
+    if (pl->h_flags_count < 2 * nsys) {
+        if (pl->h_flags) JSSO_HIP(hipHostFree(pl->h_flags));
+        pl->h_flags = nullptr;
+        pl->h_flags_count = 0;
+        JSSO_HIP(hipHostMalloc((void**)&pl->h_flags, 2 * nsys * sizeof(int32_t), hipHostMallocDefault));
+        pl->h_flags_count = 2 * nsys;
+    }
+    for (auto& e : pl->flag_ev)
+        if (!e) JSSO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return JSSO_OK;
 }
 
 template <typename MT, typename VT, int NR>
@@ -962,8 +1012,8 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
     int rc = get_workspace(pl, NR, w);
     if (rc) return rc;
     const Slices sl = make_slices(pl, 0, pl->n_wg);
-    const dim3 gridS = slice_grid(pl->n_wg), blockS(SOLVER_WG), block64(64);
     const int nsys = pl->n_sys * NR;
+    if ((rc = ensure_flag_buffers(pl, nsys))) return rc;
     const MT* A = nullptr;
     if constexpr (sizeof(MT) == 4) A = (const MT*)pl->d_Ke32; else A = (const MT*)pl->d_Ke64;
     VT *xlo = (VT*)w.xlo, *p = (VT*)w.p;
@@ -978,67 +1028,100 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
         launch_init<MT, VT, NR>(pl, sl, w, f, A, rb, qb, p, xlo, tol, s);
         JSSO_HIP(hipGetLastError());
     }
-    std::vector<int32_t> act(nsys);
-    int it = 0, repl = 0;
-    bool done = false;
-    while (!done && it < maxit) {
-        const int steps = std::max(1, std::min(replace_every, maxit - it));
-        hipGraphExec_t gx = nullptr;
-        if (steps == replace_every && steps % 2 == 0 && steps > 1)
-            gx = chunk_graph<MT, VT, NR>(pl, wps, steps - 1, sl, w, A, rb, qb, p, xlo);
-        if (gx) {
+    const int E = replace_every;
+    // everything the captured kernels' arguments point at
+    const void* key[CgGraph::NKEY] = {A,         rb[0],       rb[1],       qb[0],         qb[1],   p,
+                                      xlo,       w.P.cg[0],   w.P.cg[1],   w.st.slice_act, w.b64,  w.x64,
+                                      w.xbest,   w.st.best,   w.st.active, pl->d_Ke64};
+    CgGraph(&gc)[2] = pl->cg_graph[sizeof(MT) == 4 ? 0 : 1][NR];
+
+    // One chunk: iterations it0+1 .. it0+steps-1, then the replacement step at it0+steps.
+    auto enqueue_chunk = [&](int it0, int steps) -> int {
+        hipGraphExec_t ga = nullptr, gb = nullptr;
+        if (steps == E && E % 2 == 0 && it0 % 2 == 0 && pl->graphs_ok && graphs_enabled()) {
+            ga = capture_graph(pl, gc[0], key, wps, E, tol, [&](hipStream_t cs) {
+                for (int it = 1; it < E; ++it) launch_step<MT, VT, NR>(wps, sl, w, it, A, rb, qb, p, xlo, cs);
+            });
+            if (ga)
+                gb = capture_graph(pl, gc[1], key, wps, E, tol, [&](hipStream_t cs) {
+                    launch_replace<MT, VT, NR>(pl, sl, w, E, E, tol, A, rb, qb, p, xlo, cs);
+                });
+        }
+        if (ga && gb) {
             std::pair<hipEvent_t, hipEvent_t>* ev;
             timing_begin_batch(s, ev, steps - 1);
-            JSSO_HIP(hipGraphLaunch(gx, s));
+            JSSO_HIP(hipGraphLaunch(ga, s));
             timing_end(s, ev);
-            it += steps - 1;
+            JSSO_HIP(hipGraphLaunch(gb, s));
         } else {
-            for (int j = 0; j + 1 < steps; ++j) {
-                ++it;
+            for (int it = it0 + 1; it < it0 + steps; ++it) {
                 std::pair<hipEvent_t, hipEvent_t>* ev;
                 timing_begin(s, ev);
                 launch_step<MT, VT, NR>(wps, sl, w, it, A, rb, qb, p, xlo, s);
                 timing_end(s, ev);
             }
+            launch_replace<MT, VT, NR>(pl, sl, w, it0 + steps, steps, tol, A, rb, qb, p, xlo, s);
         }
-        // the last iteration of the chunk: update, fold, replace, then continue fresh
-        ++it;
-        hipLaunchKernelGGL((k_upd<VT, NR>), gridS, blockS, 0, s, sl, w.P, w.st, it, (const VT*)rb[(it - 1) & 1],
-                           (const VT*)qb[(it - 1) & 1], rb[it & 1], (const VT*)p, xlo);
-        hipLaunchKernelGGL((k_fold<VT, NR>), gridS, blockS, 0, s, sl, (const int32_t*)w.st.active, xlo, w.x64);
-        hipLaunchKernelGGL((k_replace<VT, NR>), gridS, block64, 0, s, sl, w.P, w.st, (const double*)pl->d_Ke64,
-                           (const double*)w.b64, (const double*)w.x64, rb[it & 1]);
-        hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, it);
-        hipLaunchKernelGGL((k_save<NR>), gridS, blockS, 0, s, sl, (const int32_t*)w.st.save, (const double*)w.x64,
-                           w.xbest);
-        hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_FRESH, 1>), gridS, block64, 0, s, sl, w.P, w.st, it, A,
-                           (const VT*)rb[(it - 1) & 1], (const VT*)qb[(it - 1) & 1], rb[it & 1], qb[it & 1], p, xlo);
         JSSO_HIP(hipGetLastError());
-        ++repl;
-        JSSO_HIP(hipMemcpyAsync(act.data(), w.st.active, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        JSSO_HIP(hipStreamSynchronize(s));
-        timing_collect();
-        done = true;
-        for (int k = 0; k < nsys; ++k)
-            if (act[k]) { done = false; break; }
+        return JSSO_OK;
+    };
+
+    // The host reads chunk c's convergence flags while chunk c + 1 runs (one
+    // chunk of lookahead): the GPU never waits for the host, and a chunk
+    // queued after everything converged finds every slice inactive and exits
+    // at once without touching the solution.
+    std::vector<int> chunk_end;
+    int it = 0, it_done = 0, chunks_done = 0, c = 0;
+    for (;;) {
+        const bool more = it < maxit;
+        if (more) {
+            const int steps = std::min(E, maxit - it);
+            if ((rc = enqueue_chunk(it, steps))) return rc;
+            it += steps;
+            chunk_end.push_back(it);
+            JSSO_HIP(hipMemcpyAsync(pl->h_flags + (size_t)(c & 1) * nsys, w.st.active, nsys * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, s));
+            JSSO_HIP(hipEventRecord(pl->flag_ev[c & 1], s));
+            ++c;
+        }
+        const int chk = more ? c - 2 : c - 1;
+        if (chk >= 0) {
+            JSSO_HIP(hipEventSynchronize(pl->flag_ev[chk & 1]));
+            timing_collect();
+            const int32_t* fl = pl->h_flags + (size_t)(chk & 1) * nsys;
+            bool any = false;
+            for (int k = 0; k < nsys && !any; ++k) any = fl[k] != 0;
+            if (!any) {
+                it_done = chunk_end[chk];
+                chunks_done = chk + 1;
+                break;
+            }
+        }
+        if (!more) {
+            it_done = it;
+            chunks_done = c;
+            break;
+        }
     }
     const int64_t nr = 6 * (int64_t)pl->n_nodes;
     hipLaunchKernelGGL((k_unscale<NR>), dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, pl->n_nodes,
                        pl->d_Linv, (const double*)w.xbest, u);
     JSSO_HIP(hipGetLastError());
     std::vector<double> rel(nsys);
+    std::vector<int32_t> status(nsys);
     JSSO_HIP(hipMemcpyAsync(rel.data(), w.st.best, nsys * sizeof(double), hipMemcpyDeviceToHost, s));
-    JSSO_HIP(hipMemcpyAsync(act.data(), w.st.status, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    JSSO_HIP(hipMemcpyAsync(status.data(), w.st.status, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     JSSO_HIP(hipStreamSynchronize(s));
+    timing_collect();
     int conv = 0;
     double mx = 0.0;
     for (int k = 0; k < nsys; ++k) {
-        conv += act[k] == SYS_CONVERGED ? 1 : 0;
+        conv += status[k] == SYS_CONVERGED ? 1 : 0;
         mx = std::max(mx, rel[k]);
     }
     if (info) {
-        info->iterations = it;
-        info->replacements = repl;
+        info->iterations = it_done;
+        info->replacements = chunks_done;
         info->converged = conv;
         info->n_systems = nsys;
         info->rel_residual = mx;

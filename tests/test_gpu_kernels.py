@@ -252,3 +252,8 @@ def test_graph_and_eager_cg_agree_bitwise(precision, monkeypatch):
     # a third solve reuses the captured graph
     monkeypatch.setenv("JSSO_CG_GRAPH", "1")
     assert np.array_equal(b.solve(precision=precision, tol=1e-10).cpu().numpy(), ug)
+    # another tolerance on the same plan is honoured (the graphs are re-captured)
+    b.solve(precision=precision, tol=1e-5)
+    loose = dict(b.plan.last_info)
+    assert loose["converged"] == 3 and loose["rel_residual"] <= 1e-5 and loose["iterations"] < ig["iterations"]
+    assert np.array_equal(b.solve(precision=precision, tol=1e-10).cpu().numpy(), ug)
