@@ -181,11 +181,39 @@ __device__ __forceinline__ bool structure_done(const SysState& st, int sys) {
     return !any;
 }
 
-// all-lanes sum over one wavefront (butterfly: identical result in every lane)
+// All-lanes sum over one wavefront, as a butterfly: every step adds a lane's
+// partner value, with partners paired by an involution, so paired lanes add
+// the same two numbers (IEEE addition is commutative) and after six steps
+// every lane holds the bitwise-identical total.  Steps 1-4 pair lanes inside a
+// 16-lane row with DPP moves (quad xor 1, quad xor 2, half-row mirror, row
+// mirror); steps 5-6 exchange rows with the gfx950 permlane swaps, whose two
+// results hold the lane's own value and its partner's (in either order).
+// Requires every lane of the wavefront to be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_add_partner(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return v + __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double join_halves(int lo, int hi) {
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
+    v = dpp_add_partner<0xB1>(v);    // quad_perm [1,0,3,2]
+    v = dpp_add_partner<0x4E>(v);    // quad_perm [2,3,0,1]
+    v = dpp_add_partner<0x141>(v);   // row_half_mirror
+    v = dpp_add_partner<0x140>(v);   // row_mirror
+    long long b = __double_as_longlong(v);
+    auto l16 = __builtin_amdgcn_permlane16_swap((int)b, (int)b, false, false);
+    auto h16 = __builtin_amdgcn_permlane16_swap((int)(b >> 32), (int)(b >> 32), false, false);
+    v = join_halves(l16[0], h16[0]) + join_halves(l16[1], h16[1]);
+    b = __double_as_longlong(v);
+    auto l32 = __builtin_amdgcn_permlane32_swap((int)b, (int)b, false, false);
+    auto h32 = __builtin_amdgcn_permlane32_swap((int)(b >> 32), (int)(b >> 32), false, false);
+    return join_halves(l32[0], h32[0]) + join_halves(l32[1], h32[1]);
 }
 
 // Sum over the whole workgroup, result in every thread.  `lds` holds >= NW*NV.
@@ -457,7 +485,7 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
     if (MODE != CG_START) {
         constexpr int K = STEP ? 4 : 2;
         constexpr int NT = 64 * WPS;   // threads sharing the partial sums
-        constexpr int T = 4;           // slices per thread loaded without a loop
+        constexpr int T = WPS == 1 ? 8 : (WPS == 2 ? 6 : 4);   // slices per thread loaded without a loop
         double v[K * NR];
 #pragma unroll
         for (int a = 0; a < K * NR; ++a) v[a] = 0.0;
@@ -631,37 +659,48 @@ __global__ __launch_bounds__(SOLVER_WG) void k_upd(Slices sl, Partials P, SysSta
 }
 
 // r64 = b - A64 x64; r_k = r64 (low precision, buffer k & 1); partials
-// <r64,r64> (FP64, convergence) and <r_k,r_k>.
-template <typename VT, int NR>
-__global__ __launch_bounds__(64) void k_replace(Slices sl, Partials P, SysState st,
-                                                const double* __restrict__ A, const double* __restrict__ b64,
-                                                const double* __restrict__ x64, VT* __restrict__ rcur) {
-    __shared__ double lds[2 * NR];
+// <r64,r64> (FP64, convergence) and <r_k,r_k>.  Same slice layout, wave split
+// and straight-line front end as k_cgf.
+template <typename VT, int NR, int WPS>
+__global__ __launch_bounds__(64 * WPS) void k_replace(Slices sl, Partials P, SysState st,
+                                                      const double* __restrict__ A, const double* __restrict__ b64,
+                                                      const double* __restrict__ x64, VT* __restrict__ rcur) {
+    __shared__ double red[WPS > 1 ? WPS - 1 : 1][6 * NR][64];
     const int w = slice_of_block(sl);
     if (w < 0) return;
-    const int sys = sl.sys[w];
-    if (structure_done<NR>(st, sys)) return;
-    const int lane = threadIdx.x;
-    const int64_t i = (int64_t)sl.row0[w] + lane;
-    const bool on = i < sl.row1[w];
-    const int64_t ii = on ? i : (int64_t)sl.row0[w];
+    const int4 ma = sl.meta[2 * w];
+    const int4 mb = sl.meta[2 * w + 1];
+    const int amask = st.slice_act[w];
+    if ((amask | ((ma.x | ma.y | ma.z | ma.w | mb.x) < 0)) == 0) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i = (int64_t)ma.y + lane;
+    const bool on = i < ma.z;
+    const int64_t ii = on ? i : (int64_t)ma.y;
+    const int v0 = ma.w, v1 = mb.x;
+    const int nslot = (v1 - v0) / ROWS_PER_WG;
+    const double* Aw = A + 36 * (int64_t)v0 + lane;
+    const int32_t* cw = sl.ell_col + v0 + lane;
+    double xown[6][NR], bown[6][NR];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int h = 0; h < NR; ++h) {
+            const int64_t o = (6 * ii + c) * NR + h;
+            xown[c][h] = x64[o];
+            bown[c][h] = b64[o];
+        }
     double acc[6][NR];
 #pragma unroll
     for (int c = 0; c < 6; ++c)
 #pragma unroll
-        for (int h = 0; h < NR; ++h) acc[c][h] = x64[(6 * ii + c) * NR + h];
-    const int v0 = sl.voff[w], v1 = sl.voff[w + 1];
-    const double* Aw = A + 36 * (int64_t)v0 + lane;
-    const int32_t* cw = sl.ell_col + v0 + lane;
-    const int nslot = (v1 - v0) / ROWS_PER_WG;
-    for (int j = 0; j < nslot; ++j) {
+        for (int h = 0; h < NR; ++h) acc[c][h] = 0.0;
+    for (int j = wave; j < nslot; j += WPS) {
         const int64_t col = cw[(int64_t)j * ROWS_PER_WG];
-        const double* xs = x64 + 6 * col * NR;
         double xj[6][NR];
 #pragma unroll
         for (int c = 0; c < 6; ++c)
 #pragma unroll
-            for (int h = 0; h < NR; ++h) xj[c][h] = xs[c * NR + h];
+            for (int h = 0; h < NR; ++h) xj[c][h] = x64[(6 * col + c) * NR + h];
         const double* a = Aw + (int64_t)j * 36 * ROWS_PER_WG;
 #pragma unroll
         for (int e = 0; e < 36; ++e) {
@@ -669,6 +708,22 @@ __global__ __launch_bounds__(64) void k_replace(Slices sl, Partials P, SysState 
 #pragma unroll
             for (int h = 0; h < NR; ++h) acc[e / 6][h] += av * xj[e % 6][h];
         }
+    }
+    if (WPS > 1) {
+        if (wave > 0) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+#pragma unroll
+                for (int h = 0; h < NR; ++h) red[wave - 1][c * NR + h][lane] = acc[c][h];
+        }
+        __syncthreads();
+        if (wave > 0) return;
+#pragma unroll
+        for (int u = 0; u < WPS - 1; ++u)
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+#pragma unroll
+                for (int h = 0; h < NR; ++h) acc[c][h] += red[u][c * NR + h][lane];
     }
     double vt[NR], vr[NR];
 #pragma unroll
@@ -678,17 +733,27 @@ __global__ __launch_bounds__(64) void k_replace(Slices sl, Partials P, SysState 
         for (int c = 0; c < 6; ++c)
 #pragma unroll
             for (int h = 0; h < NR; ++h) {
-                if (!st.active[sys * NR + h]) continue;
+                if (!((amask >> h) & 1)) continue;
                 const int64_t o = (6 * i + c) * NR + h;
-                const double rv = b64[o] - acc[c][h];
+                const double rv = bown[c][h] - (xown[c][h] + acc[c][h]);
                 const VT rl = VT(rv);
                 rcur[o] = rl;
                 vt[h] += rv * rv;
                 vr[h] += (double)rl * (double)rl;
             }
     }
-    write_partials<NR, 1>(vt, lds, P.t, w);
-    write_partials<NR, 1>(vr, lds, P.rtrue, w);
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+        vt[h] = wave_sum(vt[h]);
+        vr[h] = wave_sum(vr[h]);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int h = 0; h < NR; ++h) {
+            P.t[(int64_t)w * NR + h] = vt[h];
+            P.rtrue[(int64_t)w * NR + h] = vr[h];
+        }
+    }
 }
 
 // keep the best iterate: xbest = x64 where the last replacement improved it
@@ -928,18 +993,54 @@ static int tune_wps(Plan* pl, const Slices& sl, const Workspace& w, const MT* A,
 // `steps` iterations: update + fold into FP64, true FP64 residual, convergence
 // test, best iterate, fresh direction.
 template <typename MT, typename VT, int NR>
-static void launch_replace(Plan* pl, const Slices& sl, const Workspace& w, int it, int steps, double tol,
+static void launch_replace(Plan* pl, int wps, const Slices& sl, const Workspace& w, int it, int steps, double tol,
                            const MT* A, VT* const (&rb)[2], VT* const (&qb)[2], VT* p, VT* xlo, hipStream_t s) {
     const dim3 gridS = slice_grid(sl.count);
     hipLaunchKernelGGL((k_upd<VT, NR>), gridS, dim3(SOLVER_WG), 0, s, sl, w.P, w.st, it, (const VT*)rb[(it - 1) & 1],
                        (const VT*)qb[(it - 1) & 1], rb[it & 1], (const VT*)p, xlo, w.x64);
-    hipLaunchKernelGGL((k_replace<VT, NR>), gridS, dim3(64), 0, s, sl, w.P, w.st, (const double*)pl->d_Ke64,
-                       (const double*)w.b64, (const double*)w.x64, rb[it & 1]);
+    const double* A64 = pl->d_Ke64;
+    const double* b64 = w.b64;
+    const double* x64 = w.x64;
+    bool done = false;
+    if constexpr (WpsFits<double, NR, 4>::value) {
+        if (!done && wps >= 4) {
+            hipLaunchKernelGGL((k_replace<VT, NR, 4>), gridS, dim3(256), 0, s, sl, w.P, w.st, A64, b64, x64,
+                               rb[it & 1]);
+            done = true;
+        }
+    }
+    if constexpr (WpsFits<double, NR, 2>::value) {
+        if (!done && wps >= 2) {
+            hipLaunchKernelGGL((k_replace<VT, NR, 2>), gridS, dim3(128), 0, s, sl, w.P, w.st, A64, b64, x64,
+                               rb[it & 1]);
+            done = true;
+        }
+    }
+    if (!done)
+        hipLaunchKernelGGL((k_replace<VT, NR, 1>), gridS, dim3(64), 0, s, sl, w.P, w.st, A64, b64, x64, rb[it & 1]);
     hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, steps);
     hipLaunchKernelGGL((k_save<NR>), gridS, dim3(SOLVER_WG), 0, s, sl, (const int32_t*)w.st.save,
                        (const double*)w.x64, w.xbest);
-    hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_FRESH, 1>), gridS, dim3(64), 0, s, sl, w.P, w.st, it, A,
-                       (const VT*)rb[(it - 1) & 1], (const VT*)qb[(it - 1) & 1], rb[it & 1], qb[it & 1], p, xlo);
+    const VT* rp = rb[(it - 1) & 1];
+    const VT* qp = qb[(it - 1) & 1];
+    done = false;
+    if constexpr (WpsFits<VT, NR, 4>::value) {
+        if (!done && wps >= 4) {
+            hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_FRESH, 4>), gridS, dim3(256), 0, s, sl, w.P, w.st, it, A, rp, qp,
+                               rb[it & 1], qb[it & 1], p, xlo);
+            done = true;
+        }
+    }
+    if constexpr (WpsFits<VT, NR, 2>::value) {
+        if (!done && wps >= 2) {
+            hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_FRESH, 2>), gridS, dim3(128), 0, s, sl, w.P, w.st, it, A, rp, qp,
+                               rb[it & 1], qb[it & 1], p, xlo);
+            done = true;
+        }
+    }
+    if (!done)
+        hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_FRESH, 1>), gridS, dim3(64), 0, s, sl, w.P, w.st, it, A, rp, qp,
+                           rb[it & 1], qb[it & 1], p, xlo);
 }
 
 // HIP graphs of a chunk: its ordinary iterations (one graph launch replaces
@@ -1044,7 +1145,7 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
             });
             if (ga)
                 gb = capture_graph(pl, gc[1], key, wps, E, tol, [&](hipStream_t cs) {
-                    launch_replace<MT, VT, NR>(pl, sl, w, E, E, tol, A, rb, qb, p, xlo, cs);
+                    launch_replace<MT, VT, NR>(pl, wps, sl, w, E, E, tol, A, rb, qb, p, xlo, cs);
                 });
         }
         if (ga && gb) {
@@ -1060,7 +1161,7 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
                 launch_step<MT, VT, NR>(wps, sl, w, it, A, rb, qb, p, xlo, s);
                 timing_end(s, ev);
             }
-            launch_replace<MT, VT, NR>(pl, sl, w, it0 + steps, steps, tol, A, rb, qb, p, xlo, s);
+            launch_replace<MT, VT, NR>(pl, wps, sl, w, it0 + steps, steps, tol, A, rb, qb, p, xlo, s);
         }
         JSSO_HIP(hipGetLastError());
         return JSSO_OK;
@@ -1146,7 +1247,8 @@ static int dispatch_nr(Plan* pl, const double* f, double* u, int nrhs, double to
 }
 
 // Iterations between FP64 residual replacements; JSSO_REPLACE_EVERY overrides
-// the defaults (64 mixed, 256 FP64) for tuning.
+// the defaults (64 mixed, 256 FP64) for tuning.  (128 saves 3% on large
+// shells but lets the FP32 recurrences of slender beam systems drift too far.)
 static int replace_interval(bool mixed) {
     const char* e = getenv("JSSO_REPLACE_EVERY");
     const int v = e ? atoi(e) : 0;
