@@ -13,8 +13,9 @@ Also reported: the plain FP64 one-structure-per-launch GPU path, the batched
 FP64 path, the error of the timed path against the plain FP64 path and the FP64
 oracle, the dominant kernel's roofline (HIP events on the solver stream),
 element stiffness + assembly throughput (elements/s), the oracle timed on this
-host's cores, and the 200x200 MITC-4 shell (242,406 DOF) as a single structure
-on every solver path with its error against the committed oracle fixture.
+host's cores, the 200x200 MITC-4 shell (242,406 DOF) as a single structure on
+every solver path with its error against the committed oracle fixture, and two
+more batches: 256 uneven beam gridshells x 4 load cases, and 64 SIMP shells.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
 """
@@ -43,6 +44,8 @@ def main():
     ap.add_argument("--no-oracle-check", action="store_true")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-shell200", action="store_true", help="skip the 200x200 single-structure lines")
+    ap.add_argument("--no-batches", action="store_true",
+                    help="skip the gridshell-batch and SIMP-batch lines")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs every structure of a batch on a single "
@@ -178,6 +181,8 @@ def main():
         "rel_err_vs_plain_fp64_gpu": err_plain, "rel_err_vs_fp64_oracle": err_oracle,
         "assembly_ms_per_step": asm_ms, "elements_per_s": n_el / (asm_ms * 1e-3),
         "shell200": None if args.no_shell200 else shell200(args.tol),
+        "gridshell_batch": None if args.no_batches else gridshell_batch(args.tol),
+        "simp_batch": None if args.no_batches else simp_batch(args.tol),
     }
     print(json.dumps(out), flush=True)
 
@@ -218,6 +223,87 @@ def shell200(tol):
                          "grad_t_sample": rel(dqn[fx["dt_elem"], 0], fx["dt"])}}
     res["speedup_mixed_vs_plain_fp64"] = res["mixed"]["evals_per_s"] / res["plain"]["evals_per_s"]
     return res
+
+
+def _timed(fn, reps=1):
+    import torch
+    fn()                                      # warm-up (and kernel tuning)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = fn()
+    torch.cuda.synchronize()
+    return out, (time.perf_counter() - t0) / reps
+
+
+def gridshell_batch(tol, n_designs=256):
+    """256 beam gridshells of 8..32 bays (uneven sizes) with 4 load cases each in
+    one plan: compliance and dC/dz per design and case (an evaluation = one
+    design x one load case)."""
+    from jaxsso_amd import workloads as wl
+    from jaxsso_amd.batch import StructureBatch
+    import oracle.gradient as og
+    designs = wl.cfg3(n_designs)
+    b = StructureBatch(designs)
+    (C, (dc, _, _)), dt = _timed(lambda: b.compliance_and_gradient(precision="mixed", tol=tol))
+    st = designs[0]
+    C = C.cpu().numpy()
+    dz = b.split_nodes(dc.cpu().numpy())[0][:, 2, :]
+    err_c, err_g = 0.0, 0.0
+    for c in range(4):
+        ref = og.compliance_and_gradient(st, loads=st.load_cases()[c])
+        err_c = max(err_c, abs(C[0, c] - ref["C"]) / ref["C"])
+        g, r = dz[st.design_nodes, c], ref["dcrds"][st.design_nodes, 2]
+        err_g = max(err_g, float(np.linalg.norm(g - r) / np.linalg.norm(r)))
+    return {"workload": f"{n_designs} beam gridshells, 8..32 bays (seeded), perimeter pinned, 4 load cases each, "
+                        f"{b.plan.ndof} DOF in one plan", "precision": "mixed",
+            "evals_per_s": n_designs * 4 / dt, "ms_per_step": dt * 1e3,
+            "cg_iterations": b.plan.last_info["iterations"], "converged": b.plan.last_info["converged"],
+            "waves_per_slice": b.plan.last_info.get("waves_per_slice"),
+            "rel_err_vs_fp64_oracle_design0": {"C": err_c, "grad_z_free": err_g}}
+
+
+def simp_batch(tol, n_designs=64):
+    """64 SIMP shells (100x100 MITC-4, 61,206 DOF each, E = rho^3 30 GPa) in one
+    plan: compliance, dC/dz and dC/drho (chain rule through E) per design.  FP64
+    CG: with a 1e6 stiffness contrast the FP32 recurrences need ~5x the
+    iterations (61k vs 11.5k), which the halved bytes do not repay."""
+    import torch
+    from jaxsso_amd import workloads as wl
+    from jaxsso_amd.batch import StructureBatch
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "tests", "golden", "simp100_oracle.json")) as fh:
+        fx = json.load(fh)
+    designs = wl.cfg4(n_designs)
+    b = StructureBatch(designs)
+    rho = torch.as_tensor(np.concatenate([d.rho for d in designs]), device=b.f.device)
+
+    def step():
+        C, (dc, _, dq) = b.compliance_and_gradient(precision="fp64", tol=tol)
+        drho = dq.reshape(-1, 5)[:, 1] * 3.0 * rho ** 2 * 30e9
+        return C, dc, drho
+
+    (C, dc, drho), dt = _timed(step)
+    C = C.cpu().numpy()[:, 0]
+    dc = dc.cpu().numpy().reshape(-1, 3)
+    drho = drho.cpu().numpy()
+    rel = lambda a, r: float(np.linalg.norm(np.asarray(a) - np.asarray(r)) / np.linalg.norm(np.asarray(r)))
+    errs = {}
+    for r in fx["designs"]:
+        d = r["design"]
+        if d >= n_designs:
+            continue
+        o = int(b.node_off[d])
+        q0 = b.quad_off[d]
+        errs[f"design{d}"] = {"C": abs(C[d] - r["C"]) / r["C"],
+                              "grad_z_sample": rel(dc[o + np.asarray(r["dz_node"]), 2], r["dz"]),
+                              "grad_rho_sample": rel(drho[q0 + np.asarray(r["drho_elem"])], r["drho"])}
+    return {"workload": f"{n_designs} SIMP MITC-4 shells 100x100 over 15 m (10,201 nodes, 10,000 quads each), "
+                        f"{b.plan.ndof} DOF in one plan", "precision": "fp64",
+            "evals_per_s": n_designs / dt, "ms_per_step": dt * 1e3,
+            "cg_iterations": b.plan.last_info["iterations"], "converged": b.plan.last_info["converged"],
+            "waves_per_slice": b.plan.last_info.get("waves_per_slice"),
+            "rel_err_vs_fp64_oracle": errs}
 
 
 def _cpu_model():

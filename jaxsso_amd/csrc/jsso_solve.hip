@@ -28,7 +28,7 @@
 //
 // Safeguards: the iterate with the smallest true residual is kept and is what
 // the solve returns; a system stops when its true residual has not improved by
-// 10% for max(2048, it/2) iterations (the attainable-accuracy floor), grows
+// 10% for max(8192, it/2) iterations (the attainable-accuracy floor), grows
 // 1000-fold past its best, or is not finite; a non-positive or non-finite
 // curvature <p,q> restarts the search direction.
 //
@@ -377,7 +377,10 @@ __global__ __launch_bounds__(64) void k_decide(Slices sl, Partials P, SysState s
         const bool improved = rel < best;
         const double nbest = improved ? rel : best;
         const int stall = rel <= 0.9 * best ? it : st.stall[k];   // iteration of the last 10% gain
-        const int window = it / 2 > 2048 ? it / 2 : 2048;
+        // CG on strongly heterogeneous systems (SIMP densities) plateaus for a
+        // few thousand iterations before it moves on: only a longer flat stretch
+        // is taken as the attainable-accuracy floor
+        const int window = it / 2 > 8192 ? it / 2 : 8192;
         int active = 1, status = SYS_RUNNING;
         if (rel <= tol || tb[h] == 0.0) {
             active = 0;
@@ -439,6 +442,7 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
     constexpr bool STEP = MODE == CG_STEP;
     __shared__ double lds[4 * NR * WPS];
     __shared__ VT red[WPS > 1 ? WPS - 1 : 1][6 * NR][64];
+    __shared__ VT stage[64 * 6 * NR];
     const int w = slice_of_block(sl);
     if (w < 0) return;
     const int4 ma = sl.meta[2 * w];
@@ -462,7 +466,19 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
     // splits the requests), the first ELL slot's column (a slot past the end
     // reads the metadata word instead and is replaced), and the first scalar
     // partials.
+    // (p and x only needed at the end are loaded there for many load cases: registers)
+    constexpr bool EARLY = NR * sizeof(VT) <= 8;
     VT rown[6][NR], qown[6][NR], pown[6][NR], xown[6][NR];
+    auto load_px = [&]() {
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int h = 0; h < NR; ++h) {
+                const int64_t o = (6 * ii + c) * NR + h;
+                pown[c][h] = MODE == CG_START ? VT(0) : p[o];
+                xown[c][h] = STEP ? xlo[o] : VT(0);
+            }
+    };
 #pragma unroll
     for (int c = 0; c < 6; ++c)
 #pragma unroll
@@ -470,9 +486,8 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
             const int64_t o = (6 * ii + c) * NR + h;
             rown[c][h] = rsrc[o];
             qown[c][h] = MODE == CG_START ? VT(0) : qprev[o];
-            pown[c][h] = MODE == CG_START ? VT(0) : p[o];
-            xown[c][h] = STEP ? xlo[o] : VT(0);
         }
+    if (EARLY) load_px();
     int j = wave;
     const int32_t* cj = j < nslot ? cw + (int64_t)j * ROWS_PER_WG : (const int32_t*)&sl.meta[2 * w];
     int64_t col = *cj;
@@ -485,7 +500,9 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
     if (MODE != CG_START) {
         constexpr int K = STEP ? 4 : 2;
         constexpr int NT = 64 * WPS;   // threads sharing the partial sums
-        constexpr int T = WPS == 1 ? 8 : (WPS == 2 ? 6 : 4);   // slices per thread loaded without a loop
+        // slices per thread loaded without a loop (fewer for many load cases: registers)
+        constexpr int T0 = WPS == 1 ? 8 : (WPS == 2 ? 6 : 4);
+        constexpr int T = T0 / NR > 0 ? T0 / NR : 1;
         double v[K * NR];
 #pragma unroll
         for (int a = 0; a < K * NR; ++a) v[a] = 0.0;
@@ -570,7 +587,15 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
                 for (int h = 0; h < NR; ++h) red[wave - 1][c * NR + h][lane] = acc[c][h];
         }
         __syncthreads();
-        if (wave > 0) return;
+    }
+
+    // Wave 0 finishes the rows: r_k, s = r_k + acc (unit diagonal blocks), q, p,
+    // x, and the partial sums; inactive load cases keep their old values.  The
+    // slice's rows of each vector are one contiguous run of nrow * 6 * NR
+    // values: they leave through LDS, one vector at a time (values are
+    // recomputed rather than kept, to save registers), so that every store
+    // instruction writes consecutive addresses (whole cache lines).
+    if (WPS > 1 && wave == 0) {
 #pragma unroll
         for (int u = 0; u < WPS - 1; ++u)
 #pragma unroll
@@ -578,41 +603,54 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
 #pragma unroll
                 for (int h = 0; h < NR; ++h) acc[c][h] += red[u][c * NR + h][lane];
     }
-
-    // wave 0: r_k, s = r_k + acc (unit diagonal blocks), p, q, x, partials
+    if (!EARLY) load_px();
+    const int64_t base = (int64_t)ma.y * 6 * NR;
+    const int n = (ma.z - ma.y) * 6 * NR;
     double v[4 * NR];
 #pragma unroll
     for (int a = 0; a < 4 * NR; ++a) v[a] = 0.0;
-    if (on) {
+    auto flush = [&](auto&& value, VT* __restrict__ dst) {
+        if (wave == 0) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c)
+            for (int c = 0; c < 6; ++c)
 #pragma unroll
-            for (int h = 0; h < NR; ++h) {
-                if (!((amask >> h) & 1)) continue;
-                const int64_t o = (6 * i + c) * NR + h;
-                const VT pold = pown[c][h];
-                const VT rk = STEP ? rown[c][h] - alpha[h] * qown[c][h] : rown[c][h];
-                if (STEP) {
-                    xlo[o] = xown[c][h] + alpha[h] * pold;
-                    rcur[o] = rk;
-                }
-                const VT pn = rk + beta[h] * pold;
-                const VT qn = (rk + acc[c][h]) + beta[h] * qown[c][h];
-                p[o] = pn;
-                qcur[o] = qn;
-                const double r_ = (double)rk, q_ = (double)qn;
-                v[CG_RR * NR + h] += r_ * r_;
-                v[CG_PQ * NR + h] += (double)pn * q_;
-                v[CG_RQ * NR + h] += r_ * q_;
-                v[CG_QQ * NR + h] += q_ * q_;
+                for (int h = 0; h < NR; ++h) stage[lane * 6 * NR + c * NR + h] = value(c, h, on && ((amask >> h) & 1));
+        }
+        __syncthreads();
+        for (int u = (int)threadIdx.x; u < n; u += 64 * WPS) dst[base + u] = stage[u];
+        __syncthreads();
+    };
+    auto r_k = [&](int c, int h) { return STEP ? rown[c][h] - alpha[h] * qown[c][h] : rown[c][h]; };
+    auto q_k = [&](int c, int h) { return (r_k(c, h) + acc[c][h]) + beta[h] * qown[c][h]; };
+    auto p_k = [&](int c, int h) { return r_k(c, h) + beta[h] * pown[c][h]; };
+    if (STEP) flush([&](int c, int h, bool run) { return run ? r_k(c, h) : rown[c][h]; }, rcur);
+    flush(
+        [&](int c, int h, bool run) {
+            const VT rk = r_k(c, h), qn = q_k(c, h);
+            if (run) {
+                v[CG_RR * NR + h] += (double)rk * (double)rk;
+                v[CG_RQ * NR + h] += (double)rk * (double)qn;
+                v[CG_QQ * NR + h] += (double)qn * (double)qn;
             }
-    }
+            return run ? qn : qown[c][h];
+        },
+        qcur);
+    flush(
+        [&](int c, int h, bool run) {
+            const VT pn = p_k(c, h);
+            if (run) v[CG_PQ * NR + h] += (double)pn * (double)q_k(c, h);
+            return run ? pn : pown[c][h];
+        },
+        p);
+    if (STEP) flush([&](int c, int h, bool run) { return run ? xown[c][h] + alpha[h] * pown[c][h] : xown[c][h]; }, xlo);
+    if (wave == 0) {
 #pragma unroll
-    for (int a = 0; a < 4 * NR; ++a) v[a] = wave_sum(v[a]);
-    if (lane == 0) {
-        double* dst = P.cg[ck] + (int64_t)w * 4 * NR;
+        for (int a = 0; a < 4 * NR; ++a) v[a] = wave_sum(v[a]);
+        if (lane == 0) {
+            double* dst = P.cg[ck] + (int64_t)w * 4 * NR;
 #pragma unroll
-        for (int a = 0; a < 4 * NR; ++a) dst[a] = v[a];
+            for (int a = 0; a < 4 * NR; ++a) dst[a] = v[a];
+        }
     }
 }
 
