@@ -99,9 +99,11 @@ def main():
     ndof = p.ndof
     alg_bytes = nnzb_off * (36 * s_m + 4) + 4 * (p.n_nodes + 1) + 8 * ndof * s_v
     achieved = alg_bytes / (avg_ms[0] * 1e-3) / 1e9 if avg_ms[0] > 0 else None
+    traffic, traffic_src = _measured_traffic(args)
     roofline = {"kernel": f"k_cgf<{'float' if s_m == 4 else 'double'}>", "bound": "hbm",
                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                "traffic_source": traffic_src,
                 "avg_launch_us": avg_ms[0] * 1e3, "sampled_launches": int(nsamp[0]),
                 "alg_bytes_per_launch": int(alg_bytes)}
 
@@ -223,6 +225,21 @@ def shell200(tol):
                          "grad_t_sample": rel(dqn[fx["dt_elem"], 0], fx["dt"])}}
     res["speedup_mixed_vs_plain_fp64"] = res["mixed"]["evals_per_s"] / res["plain"]["evals_per_s"]
     return res
+
+
+def _measured_traffic(args):
+    """HBM bytes per launch of k_cgf<float> on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a copy-kernel
+    calibration); None for other workloads or precisions."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "cg_batch128_shell40_traffic.txt")
+    if args.batch != 128 or args.precision != "mixed" or not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch k_cgf<float>:"):
+                return int(line.split(":")[1]), "profiles/cg_batch128_shell40_traffic.txt"
+    return None, None
 
 
 def _timed(fn, reps=1):
