@@ -90,14 +90,16 @@ def main():
     print(f"# timed: {ms_per_step:.1f} ms/step, {value:.1f} evals/s, CG iters {iters}, info {info}", flush=True)
 
     # -- dominant kernel roofline: the fused CG iteration k_cgf (scaled off-diagonal
-    # blocks + column indices once; r_{k-1}, q_{k-1}, p, x read and r_k, q_k, p, x
+    # blocks once, their indices; r_{k-1}, q_{k-1}, p, x read and r_k, q_k, p, x
     # written once per DOF and load case)
     p = batch.plan
     nnzb_off = p.nnzb - p.n_nodes
     s_m = 4 if args.precision == "mixed" else 8
     s_v = s_m
     ndof = p.ndof
-    alg_bytes = nnzb_off * (36 * s_m + 4) + 4 * (p.n_nodes + 1) + 8 * ndof * s_v
+    # each off-diagonal block pair stored once (36 values), both halves indexed
+    # (a column each, plus the value position of the transposed half)
+    alg_bytes = (nnzb_off // 2) * 36 * s_m + nnzb_off * 4 + (nnzb_off // 2) * 4 + 8 * ndof * s_v
     achieved = alg_bytes / (avg_ms[0] * 1e-3) / 1e9 if avg_ms[0] > 0 else None
     traffic, traffic_src = _measured_traffic(args)
     roofline = {"kernel": f"k_cgf<{'float' if s_m == 4 else 'double'}>", "bound": "hbm",

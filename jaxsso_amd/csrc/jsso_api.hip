@@ -164,18 +164,18 @@ static int build_plan(Plan* p, int32_t n_sys, const int32_t* node_off, int32_t n
         }
     }
     // off-diagonal solver pattern
-    std::vector<int32_t> offrowptr(n + 1, 0), offcol, offsrc;
-    offcol.reserve(nnzb - n);
-    offsrc.reserve(nnzb - n);
-    for (int i = 0; i < n; ++i) {
-        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k)
-            if (colind[k] != i) {
-                offcol.push_back(colind[k]);
-                offsrc.push_back(k);
-            }
-        offrowptr[i + 1] = (int32_t)offcol.size();
-    }
-    p->nnzb_off = offcol.size();
+    int64_t n_off = 0;
+    for (int i = 0; i < n; ++i) n_off += rowptr[i + 1] - rowptr[i] - 1;
+    p->nnzb_off = n_off;
+    // position in the block CSR of the transposed block (j, i) of every block (i, j)
+    std::vector<int32_t> tr(nnzb);
+    for (int i = 0; i < n; ++i)
+        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+            const int j = colind[k];
+            const int32_t* b = colind.data() + rowptr[j];
+            const int32_t* e = colind.data() + rowptr[j + 1];
+            tr[k] = (int32_t)(std::lower_bound(b, e, i) - colind.data());
+        }
     // solver workgroups: contiguous chunks of ROWS_PER_WG block rows per structure
     std::vector<int32_t> wg_sys, wg_row0, wg_row1, sys_wg_off(n_sys + 1, 0);
     for (int s = 0; s < n_sys; ++s) {
@@ -188,45 +188,104 @@ static int build_plan(Plan* p, int32_t n_sys, const int32_t* node_off, int32_t n
     }
     p->n_wg = (int32_t)wg_sys.size();
     p->h_sys_wg_off = sys_wg_off;
-    // sliced ELL of the off-diagonal blocks: slice = solver workgroup (<= 64 block
-    // rows of one structure), slot j of lane l = j-th off-diagonal block of row
-    // row0 + l; padding slots point at the row itself with zero values
-    std::vector<int32_t> slice_voff(p->n_wg + 1, 0), ell_col;
-    std::vector<int32_t> ell_src;
+    // Symmetric sliced ELL.  A slice is one solver workgroup (<= 64 block rows of
+    // one structure); lane l is row row0 + l.  Only the upper blocks (i < j) are
+    // stored: value slot u of a slice holds the u-th upper block of every row,
+    // [36 entries][64 lanes].  The column list of a slice has nup upper slots
+    // (column j, block stored in this slice) followed by nlo lower slots (column
+    // j < i; the block is the transpose of the upper block (j, i), found through
+    // its value position vpos = value slot * 64 + lane).  Padding: upper slots
+    // hold zero values with the row's own column; lower slots point at an
+    // all-zero value slot appended after the last one.
+    std::vector<int32_t> ell_col, ell_vpos, ell_src;   // per column position / per value position
+    std::vector<int32_t> pos_off(p->n_wg), val_off(p->n_wg), nupv(p->n_wg), nlov(p->n_wg);
+    std::vector<int64_t> vpos_of(nnzb, -1);
+    int64_t npos = 0, nval = 0;
     for (int w = 0; w < p->n_wg; ++w) {
-        int nslot = 0;
-        for (int i = wg_row0[w]; i < wg_row1[w]; ++i) nslot = std::max(nslot, offrowptr[i + 1] - offrowptr[i]);
-        slice_voff[w + 1] = slice_voff[w] + nslot * ROWS_PER_WG;
-        const size_t base = ell_col.size();
-        ell_col.resize(base + (size_t)nslot * ROWS_PER_WG);
-        ell_src.resize(base + (size_t)nslot * ROWS_PER_WG, -1);
-        for (int j = 0; j < nslot; ++j)
+        int nup = 0, nlo = 0;
+        for (int i = wg_row0[w]; i < wg_row1[w]; ++i) {
+            nup = std::max(nup, rowptr[i + 1] - diag[i] - 1);
+            nlo = std::max(nlo, diag[i] - rowptr[i]);
+        }
+        JSSO_REQUIRE(nup < 65536 && nlo < 65536, JSSO_EINVAL, "a block row has more than 65535 neighbours");
+        pos_off[w] = (int32_t)npos;
+        val_off[w] = (int32_t)nval;
+        nupv[w] = nup;
+        nlov[w] = nlo;
+        ell_col.resize((size_t)(npos + (int64_t)nup * ROWS_PER_WG));
+        ell_vpos.resize(ell_col.size(), 0);
+        ell_src.resize((size_t)(nval + (int64_t)nup * ROWS_PER_WG), -1);
+        for (int u = 0; u < nup; ++u)
             for (int l = 0; l < ROWS_PER_WG; ++l) {
                 const int i = wg_row0[w] + l;
+                const int64_t pos = npos + (int64_t)u * ROWS_PER_WG + l;
+                const int64_t vp = nval + (int64_t)u * ROWS_PER_WG + l;
                 int32_t c = i < wg_row1[w] ? i : wg_row0[w];
-                if (i < wg_row1[w] && j < offrowptr[i + 1] - offrowptr[i]) {
-                    const int k = offrowptr[i] + j;
-                    c = offcol[k];
-                    ell_src[base + (size_t)j * ROWS_PER_WG + l] = offsrc[k];
+                if (i < wg_row1[w] && u < rowptr[i + 1] - diag[i] - 1) {
+                    const int k = diag[i] + 1 + u;
+                    c = colind[k];
+                    ell_src[vp] = k;
+                    vpos_of[k] = vp;
                 }
-                ell_col[base + (size_t)j * ROWS_PER_WG + l] = c;
+                ell_col[pos] = c;
+                ell_vpos[pos] = (int32_t)vp;
             }
+        npos += (int64_t)nup * ROWS_PER_WG;
+        nval += (int64_t)nup * ROWS_PER_WG;
+        JSSO_REQUIRE(npos < INT32_MAX / 2 && nval < INT32_MAX / 36, JSSO_EINVAL,
+                     "plan too large for 32-bit sliced-ELL offsets");
     }
-    p->ell_slots = slice_voff[p->n_wg];
-    std::vector<int32_t> meta(8 * (size_t)p->n_wg, 0);
+    // lower slots, once every upper block has its value position (the zero slot
+    // comes after the last value slot)
+    const int64_t zero_vp = nval;
+    std::vector<int32_t> lo_off(p->n_wg);
+    {
+        int64_t total = npos;
+        for (int w = 0; w < p->n_wg; ++w) total += (int64_t)nlov[w] * ROWS_PER_WG;
+        JSSO_REQUIRE(total < INT32_MAX, JSSO_EINVAL, "plan too large for 32-bit sliced-ELL offsets");
+        ell_col.resize((size_t)total);
+        ell_vpos.resize((size_t)total, 0);
+    }
     for (int w = 0; w < p->n_wg; ++w) {
-        int32_t* m = &meta[8 * (size_t)w];
+        lo_off[w] = (int32_t)npos;
+        const int nlo = nlov[w];
+        for (int u = 0; u < nlo; ++u)
+            for (int l = 0; l < ROWS_PER_WG; ++l) {
+                const int i = wg_row0[w] + l;
+                const int64_t pos = npos + (int64_t)u * ROWS_PER_WG + l;
+                int32_t c = i < wg_row1[w] ? i : wg_row0[w];
+                int64_t vp = zero_vp;
+                if (i < wg_row1[w] && u < diag[i] - rowptr[i]) {
+                    const int k = rowptr[i] + u;
+                    c = colind[k];
+                    vp = vpos_of[tr[k]];
+                }
+                ell_col[pos] = c;
+                ell_vpos[pos] = (int32_t)vp;
+            }
+        npos += (int64_t)nlo * ROWS_PER_WG;
+    }
+    p->ell_slots = nval + ROWS_PER_WG;   // value positions incl. the zero slot
+    // per slice, three int4: {sys, row0, row1, first value position},
+    // {first upper column position, upper slots, first lower column position, lower slots},
+    // {first slice of sys, end slice of sys, 0, 0}
+    std::vector<int32_t> meta(12 * (size_t)p->n_wg, 0);
+    for (int w = 0; w < p->n_wg; ++w) {
+        int32_t* m = &meta[12 * (size_t)w];
         m[0] = wg_sys[w];
         m[1] = wg_row0[w];
         m[2] = wg_row1[w];
-        m[3] = slice_voff[w];
-        m[4] = slice_voff[w + 1];
-        m[5] = sys_wg_off[wg_sys[w]];
-        m[6] = sys_wg_off[wg_sys[w] + 1];
+        m[3] = val_off[w];
+        m[4] = pos_off[w];
+        m[5] = nupv[w];
+        m[6] = lo_off[w];
+        m[7] = nlov[w];
+        m[8] = sys_wg_off[wg_sys[w]];
+        m[9] = sys_wg_off[wg_sys[w] + 1];
     }
     p->h_slice_meta.swap(meta);
-    p->h_slice_voff.swap(slice_voff);
     p->h_ell_col.swap(ell_col);
+    p->h_ell_vpos.swap(ell_vpos);
     p->h_ell_src.swap(ell_src);
     p->h_browof.swap(browof);
     p->h_diag.swap(diag);
@@ -255,7 +314,7 @@ static int ensure_device(Plan* p) {
         (rc = upload(p, &p->d_cnct_quad, p->h_cnct_quad)) || (rc = upload(p, &p->d_fixed, p->h_fixed)) ||
         (rc = upload(p, &p->d_wg_sys, p->h_wg_sys)) || (rc = upload(p, &p->d_wg_row0, p->h_wg_row0)) ||
         (rc = upload(p, &p->d_wg_row1, p->h_wg_row1)) || (rc = upload(p, &p->d_sys_wg_off, p->h_sys_wg_off)) ||
-        (rc = upload(p, &p->d_slice_voff, p->h_slice_voff)) || (rc = upload(p, &p->d_ell_col, p->h_ell_col)) ||
+        (rc = upload(p, &p->d_ell_vpos, p->h_ell_vpos)) || (rc = upload(p, &p->d_ell_col, p->h_ell_col)) ||
         (rc = upload(p, &p->d_slice_meta, p->h_slice_meta)) ||
         (rc = upload(p, &p->d_ell_src, p->h_ell_src)))
         return rc;
@@ -403,8 +462,10 @@ int jsso_assemble(jsso_plan* plan, const double* d_crds, const double* d_prop_be
     if (rc) return rc;
     rc = launch_assemble(p, s);
     if (rc) return rc;
-    if (d_K_out && p->nnzb)
+    if (d_K_out && p->nnzb) {
+        if ((rc = launch_full_blocks(p, s))) return rc;
         JSSO_HIP(hipMemcpyAsync(d_K_out, p->d_K, (size_t)p->nnzb * 36 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
     p->assembled = true;
     return JSSO_OK;
 }
@@ -473,6 +534,8 @@ int jsso_spmv(jsso_plan* plan, const double* d_x, double* d_y, void* stream) {
         set_error("jsso_spmv before jsso_assemble");
         return JSSO_EINVAL;
     }
+    int rc = launch_full_blocks(p, (hipStream_t)stream);
+    if (rc) return rc;
     return launch_spmv_full(p, p->d_K, d_x, d_y, (hipStream_t)stream);
 }
 

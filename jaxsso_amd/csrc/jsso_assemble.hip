@@ -1,6 +1,9 @@
-// Numeric assembly of K on the 6x6 block-CSR pattern (row gather over the
-// element blocks, fixed summation order, no atomics), constraint masking and
-// the block-Jacobi scaling the solver uses.
+// Numeric assembly on the 6x6 block pattern (gathers over the element blocks,
+// fixed summation order, no atomics), constraint masking and the block-Jacobi
+// scaling the solver uses.  The solver needs only the factors of the diagonal
+// blocks and the scaled off-diagonal blocks in sliced ELL; both are gathered
+// straight from the element blocks.  K and Kc in block CSR (export, y = K x,
+// the plain path) are gathered on demand.
 //
 //   K_ij       = sum over contributions (element e, local a, b) of Ke_e[a][b]
 //   Kc         = K with rows/columns of constrained DOFs replaced by identity
@@ -38,21 +41,39 @@ __global__ __launch_bounds__(256) void gather_blocks_kernel(int64_t nnzb, const 
     Kc[tid] = v;
 }
 
-// Per node: Cholesky of the constrained diagonal block, its inverse factor and
-// the block-Jacobi inverse Minv_i = Linv_i^T Linv_i.  status[0] |= 1 when a
-// block is not positive definite.
-__global__ void diag_factor_kernel(int32_t n_nodes, const int32_t* __restrict__ diag,
-                                   const double* __restrict__ Kc, double* __restrict__ Linv,
-                                   double* __restrict__ Minv, int32_t* status) {
+// Per node: the constrained diagonal block, summed from its element blocks in
+// the same order as K, its Cholesky factor, the inverse factor and the
+// block-Jacobi inverse Minv_i = Linv_i^T Linv_i.  status[0] |= 1 when a block
+// is not positive definite.
+__global__ __launch_bounds__(128) void diag_factor_kernel(int32_t n_nodes, const int32_t* __restrict__ diag,
+                                                          const int32_t* __restrict__ cptr,
+                                                          const int32_t* __restrict__ csrc,
+                                                          const uint8_t* __restrict__ fixed,
+                                                          const double* __restrict__ ke,
+                                                          double* __restrict__ Linv, double* __restrict__ Minv,
+                                                          int32_t* status) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_nodes) return;
     double A[6][6], L[6][6], Li[6][6];
-    const double* src = Kc + (int64_t)diag[i] * 36;
+    double B[36];
+#pragma unroll
+    for (int q = 0; q < 36; ++q) B[q] = 0.0;
+    const int k = diag[i];
+    for (int c = cptr[k]; c < cptr[k + 1]; ++c) {
+        const double* src = ke + (int64_t)csrc[c] * 36;
+#pragma unroll
+        for (int q = 0; q < 36; ++q) B[q] += src[q];
+    }
+    bool fx[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) fx[r] = fixed[6 * (int64_t)i + r] != 0;
 #pragma unroll
     for (int r = 0; r < 6; ++r)
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-            A[r][c] = 0.5 * (src[r * 6 + c] + src[c * 6 + r]);
+            const double brc = (fx[r] | fx[c]) ? (r == c ? 1.0 : 0.0) : B[r * 6 + c];
+            const double bcr = (fx[r] | fx[c]) ? (r == c ? 1.0 : 0.0) : B[c * 6 + r];
+            A[r][c] = 0.5 * (brc + bcr);
             L[r][c] = 0.0;
             Li[r][c] = 0.0;
         }
@@ -103,66 +124,138 @@ __global__ void diag_factor_kernel(int32_t n_nodes, const int32_t* __restrict__ 
     if (!ok) atomicOr(status, 1);
 }
 
-// One thread per sliced-ELL position (slot-major, lane fastest): the scaled
-// block Ks_ij = Linv_i Kc_ij Linv_j^T of that position in FP64, also rounded to
-// FP32, written entry by entry so that each store of a wavefront is one
-// contiguous run of 64 lanes.  Padding positions (src < 0) keep their zeros.
-__global__ __launch_bounds__(256) void scale_blocks_kernel(int64_t n_pos, const int32_t* __restrict__ ell_src,
-                                                           const int32_t* __restrict__ browof,
-                                                           const int32_t* __restrict__ colind,
-                                                           const double* __restrict__ Kc,
-                                                           const double* __restrict__ Linv,
-                                                           double* __restrict__ Ke64, float* __restrict__ Ke32) {
-    const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (pos >= n_pos) return;
-    const int64_t src = ell_src[pos];
-    if (src < 0) return;
-    const int i = browof[src], j = colind[src];
-    const double* B = Kc + src * 36;
-    const double* Li = Linv + (int64_t)i * 36;
-    const double* Lj = Linv + (int64_t)j * 36;
-    double T[6][6];
+// One workgroup per sliced-ELL slot (64 positions, one per block row of the
+// slice): the slot's off-diagonal blocks are summed from their element blocks
+// (same order as K), masked (a constrained row or column of an off-diagonal
+// block is zero), scaled Ks_ij = Linv_i Kc_ij Linv_j^T and written to the FP64
+// and FP32 ELL copies.  Padding positions (src < 0) keep their zeros.
+//   phase 1: all 256 threads read the element blocks as (position, entry) pairs,
+//            entry fastest (consecutive threads, consecutive values; every load
+//            of a thread issued before the first sum), and stage the blocks, the
+//            row factors L_i^-1 (64 consecutive rows: contiguous) and the column
+//            factors L_j^-1 in LDS;
+//   phase 2: wave w computes columns c = w, w + 4 of every position's Ks (lane =
+//            position) and stores them entry-major, 64 consecutive positions per
+//            store.
+constexpr int ASM_WG = 256;
+constexpr int MAX_CONTRIB = 2;   // element blocks per off-diagonal block read without a loop (2 on a quad mesh)
+
+__global__ __launch_bounds__(ASM_WG) void ell_assemble_kernel(
+    int64_t n_slots, const int32_t* __restrict__ ell_src, const int32_t* __restrict__ browof,
+    const int32_t* __restrict__ colind, const int32_t* __restrict__ cptr, const int32_t* __restrict__ csrc,
+    const uint8_t* __restrict__ fixed, const double* __restrict__ ke, const double* __restrict__ Linv,
+    double* __restrict__ Ke64, float* __restrict__ Ke32) {
+    constexpr int P = ROWS_PER_WG;
+    __shared__ double tile[P][37];
+    __shared__ double li[P][37], lj[P][37];
+    __shared__ int32_t cidx[P][MAX_CONTRIB], cstart[P], ccount[P], rowi[P], colj[P];
+    __shared__ uint8_t rfix[P][6], cfix[P][6];
+    const int64_t slot = blockIdx.x;
+    if (slot >= n_slots) return;
+    const int t = threadIdx.x;
+    if (t < P) {
+        const int32_t src = ell_src[slot * P + t];
+        int n = 0, c0 = 0, i = 0, j = 0;
+        int32_t e[MAX_CONTRIB] = {};
+        if (src >= 0) {
+            c0 = cptr[src];
+            n = cptr[src + 1] - c0;
+            for (int c = 0; c < n && c < MAX_CONTRIB; ++c) e[c] = csrc[c0 + c];
+            i = browof[src];
+            j = colind[src];
 #pragma unroll
-    for (int r = 0; r < 6; ++r)
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            double s = 0.0;
-#pragma unroll
-            for (int k2 = 0; k2 <= c; ++k2) s += B[r * 6 + k2] * Lj[c * 6 + k2];  // B Lj^T
-            T[r][c] = s;
+            for (int r = 0; r < 6; ++r) {
+                rfix[t][r] = fixed[6 * (int64_t)i + r];
+                cfix[t][r] = fixed[6 * (int64_t)j + r];
+            }
         }
-    const int64_t base = (pos / ROWS_PER_WG) * 36 * ROWS_PER_WG + pos % ROWS_PER_WG;
 #pragma unroll
-    for (int r = 0; r < 6; ++r)
+        for (int c = 0; c < MAX_CONTRIB; ++c) cidx[t][c] = e[c];   // 0 (a valid block) when unused
+        cstart[t] = c0;
+        ccount[t] = n;
+        rowi[t] = i;
+        colj[t] = j;
+    }
+    __syncthreads();
+    constexpr int NPAIR = P * 36 / ASM_WG;   // 9
+    double v[NPAIR][MAX_CONTRIB], lv[NPAIR][2];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
+    for (int it = 0; it < NPAIR; ++it) {
+        const int u = t + it * ASM_WG;
+        const int p = u / 36, q = u % 36;
+#pragma unroll
+        for (int c = 0; c < MAX_CONTRIB; ++c) v[it][c] = ke[(int64_t)cidx[p][c] * 36 + q];
+        lv[it][0] = Linv[(int64_t)rowi[p] * 36 + q];
+        lv[it][1] = Linv[(int64_t)colj[p] * 36 + q];
+    }
+#pragma unroll
+    for (int it = 0; it < NPAIR; ++it) {
+        const int u = t + it * ASM_WG;
+        const int p = u / 36, q = u % 36;
+        const int n = ccount[p];
+        double sum = 0.0;
+#pragma unroll
+        for (int c = 0; c < MAX_CONTRIB; ++c)
+            if (c < n) sum += v[it][c];
+        for (int c = MAX_CONTRIB; c < n; ++c) sum += ke[(int64_t)csrc[cstart[p] + c] * 36 + q];   // rare meshes
+        tile[p][q] = (n > 0 && !(rfix[p][q / 6] | cfix[p][q % 6])) ? sum : 0.0;
+        li[p][q] = lv[it][0];
+        lj[p][q] = lv[it][1];
+    }
+    __syncthreads();
+    const int wave = t >> 6, lane = t & 63;
+    if (ccount[lane] == 0) return;   // padding (or an empty block): its entries stay zero
+    const int64_t base = slot * 36 * P + lane;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int c = wave + 4 * h;
+        if (c >= 6) break;
+        // column c of T = B Lj^T, then column c of Ks = Li T (both factors lower triangular)
+        double Tc[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
             double s = 0.0;
 #pragma unroll
-            for (int k2 = 0; k2 <= r; ++k2) s += Li[r * 6 + k2] * T[k2][c];
-            const int64_t e = base + (int64_t)(r * 6 + c) * ROWS_PER_WG;
+            for (int k2 = 0; k2 <= c; ++k2) s += tile[lane][r * 6 + k2] * lj[lane][c * 6 + k2];
+            Tc[r] = s;
+        }
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            double s = 0.0;
+#pragma unroll
+            for (int k2 = 0; k2 <= r; ++k2) s += li[lane][r * 6 + k2] * Tc[k2];
+            const int64_t e = base + (int64_t)(r * 6 + c) * P;
             Ke64[e] = s;
             Ke32[e] = (float)s;
         }
+    }
+}
+
+// K and Kc in block CSR, for callers that need them (export, y = K x, the plain path).
+int launch_full_blocks(Plan* p, hipStream_t s) {
+    if (p->full_valid || p->nnzb == 0) return JSSO_OK;
+    const int64_t total = p->nnzb * 36;
+    hipLaunchKernelGGL(gather_blocks_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, p->nnzb,
+                       p->d_cptr, p->d_csrc, p->d_browof, p->d_colind, p->d_fixed, p->d_ke, p->d_K, p->d_Kc);
+    JSSO_HIP(hipGetLastError());
+    p->full_valid = true;
+    return JSSO_OK;
 }
 
 int launch_assemble(Plan* p, hipStream_t s) {
     JSSO_REQUIRE(p->on_device, JSSO_EINVAL, "plan not uploaded to the device");
-    if (p->nnzb > 0) {
-        const int64_t total = p->nnzb * 36;
-        hipLaunchKernelGGL(gather_blocks_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                           p->nnzb, p->d_cptr, p->d_csrc, p->d_browof, p->d_colind, p->d_fixed, p->d_ke,
-                           p->d_K, p->d_Kc);
-        JSSO_HIP(hipGetLastError());
-    }
+    p->full_valid = false;
     if (p->n_nodes > 0) {
         hipLaunchKernelGGL(diag_factor_kernel, dim3((p->n_nodes + 127) / 128), dim3(128), 0, s, p->n_nodes,
-                           p->d_diag, p->d_Kc, p->d_Linv, p->d_Minv, p->d_status);
+                           p->d_diag, p->d_cptr, p->d_csrc, p->d_fixed, p->d_ke, p->d_Linv, p->d_Minv,
+                           p->d_status);
         JSSO_HIP(hipGetLastError());
     }
-    if (p->ell_slots > 0) {
-        hipLaunchKernelGGL(scale_blocks_kernel, dim3((unsigned)((p->ell_slots + 255) / 256)), dim3(256), 0, s,
-                           p->ell_slots, p->d_ell_src, p->d_browof, p->d_colind, p->d_Kc, p->d_Linv, p->d_Ke64,
-                           p->d_Ke32);
+    const int64_t n_slots = p->ell_slots / ROWS_PER_WG - 1;   // value slots, without the zero slot
+    if (n_slots > 0) {
+        hipLaunchKernelGGL(ell_assemble_kernel, dim3((unsigned)n_slots), dim3(ASM_WG), 0, s, n_slots, p->d_ell_src,
+                           p->d_browof, p->d_colind, p->d_cptr, p->d_csrc, p->d_fixed, p->d_ke, p->d_Linv,
+                           p->d_Ke64, p->d_Ke32);
         JSSO_HIP(hipGetLastError());
     }
     return JSSO_OK;

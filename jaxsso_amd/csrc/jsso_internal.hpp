@@ -64,9 +64,11 @@ struct Plan {
     std::vector<int32_t> h_node_off, h_rowptr, h_colind;
     std::vector<int32_t> h_browof, h_diag, h_cptr, h_csrc, h_nptr, h_nsrc;
     std::vector<int32_t> h_cnct_beam, h_cnct_quad, h_wg_sys, h_wg_row0, h_wg_row1, h_sys_wg_off;
-    std::vector<int32_t> h_slice_voff, h_ell_col;   // sliced-ELL layout of the off-diagonal blocks
-    std::vector<int32_t> h_slice_meta;  // per slice: sys, row0, row1, voff0, voff1, first/end slice of sys, 0
-    std::vector<int32_t> h_ell_src;  // per ELL position: source block in the block CSR, -1 = padding
+    // symmetric sliced ELL of the off-diagonal blocks (upper blocks stored once)
+    std::vector<int32_t> h_ell_col;    // per column position: block column
+    std::vector<int32_t> h_ell_vpos;   // per column position: value position (slot * 64 + lane) of its block
+    std::vector<int32_t> h_slice_meta; // per slice: 12 ints, see build_plan
+    std::vector<int32_t> h_ell_src;    // per value position: source block in the block CSR, -1 = padding
     std::vector<uint8_t> h_fixed;
     bool on_device = false;
     // pattern (device)
@@ -78,10 +80,10 @@ struct Plan {
     int32_t *d_wg_sys = nullptr, *d_wg_row0 = nullptr, *d_wg_row1 = nullptr;
     int32_t* d_sys_wg_off = nullptr;
     int32_t n_wg = 0;
-    // sliced ELL: slice = workgroup above; values [slot][36][64 lanes] from 36*slice_voff[sl]
-    int32_t *d_slice_voff = nullptr, *d_ell_col = nullptr, *d_slice_meta = nullptr;
-    int32_t* d_ell_src = nullptr;    // per ELL position (slot-major, lane fastest): source block or -1
-    int64_t ell_slots = 0;           // sum over slices of (slots x 64)
+    // sliced ELL: slice = workgroup above; values [value slot][36][64 lanes]
+    int32_t *d_ell_vpos = nullptr, *d_ell_col = nullptr, *d_slice_meta = nullptr;
+    int32_t* d_ell_src = nullptr;    // per value position: source block or -1
+    int64_t ell_slots = 0;           // value positions (value slots x 64), incl. one all-zero slot
     // waves per slice of the batched CG kernel, per [FP32 | FP64][load cases]:
     // 0 = not tuned yet (the next solve measures the variants and stores the fastest)
     int32_t cg_wps[2][9] = {};
@@ -101,6 +103,7 @@ struct Plan {
     double* d_Kc = nullptr;       // constrained K incl. diagonal (plain path)
     int32_t* d_status = nullptr;  // device error flags
     bool assembled = false;
+    bool full_valid = false;      // d_K / d_Kc hold the current assembly
     // solver workspace, grown on demand
     std::vector<DBuf> ws;
     int ws_nrhs = 0;
@@ -128,6 +131,7 @@ int launch_ke_beam(int32_t n, const double* crds_e, const int32_t* cnct, const d
 int launch_ke_quad(int32_t n, const double* crds_e, const int32_t* cnct, const double* crds_nodes,
                    const double* props, double* ke, hipStream_t s);
 int launch_assemble(Plan* p, hipStream_t s);
+int launch_full_blocks(Plan* p, hipStream_t s);
 int launch_adjoint(Plan* p, const double* crds, const double* prop_beam, const double* prop_quad,
                    const double* u, const double* lam, double lam_scale, int nrhs, double* gbuf,
                    double* dprop_beam, double* dprop_quad, hipStream_t s);

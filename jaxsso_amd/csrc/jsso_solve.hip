@@ -133,8 +133,11 @@ int timing_read(double* avg_ms, int64_t* samples, int reset) {
 // ---------------------------------------------------------------------------
 struct Slices {
     const int32_t *sys, *row0, *row1, *sys_off;  // slice -> structure, rows; structure -> first slice
-    const int32_t *voff, *ell_col;                // sliced ELL
-    const int4* meta;   // per slice, two int4: {sys, row0, row1, voff0}, {voff1, first, end slice of sys, 0}
+    const int32_t *ell_vpos, *ell_col;           // symmetric sliced ELL: value position and column per position
+    // per slice, three int4: {sys, row0, row1, first value position},
+    // {first upper column position, upper slots, first lower column position, lower slots},
+    // {first slice of sys, end slice of sys, 0, 0}
+    const int4* meta;
     int base;           // first slice of the launch (plain path)
     int count;          // slices in the launch
     int chunk;          // ceil(count / 8): slices per XCD
@@ -445,21 +448,26 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
     __shared__ VT stage[64 * 6 * NR];
     const int w = slice_of_block(sl);
     if (w < 0) return;
-    const int4 ma = sl.meta[2 * w];
-    const int4 mb = sl.meta[2 * w + 1];
+    const int4 ma = sl.meta[3 * w];
+    const int4 mb = sl.meta[3 * w + 1];
+    const int4 mc = sl.meta[3 * w + 2];
     const int amask = st.slice_act[w];
     // (every metadata field is >= 0: testing them only makes all loads one round trip)
-    if ((amask | ((ma.x | ma.y | ma.z | ma.w | mb.x | mb.y | mb.z) < 0)) == 0) return;
+    if ((amask | ((ma.x | ma.y | ma.z | ma.w | mb.x | mb.y | mb.z | mb.w | mc.x | mc.y) < 0)) == 0) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t i = (int64_t)ma.y + lane;
     const bool on = i < ma.z;
     const int64_t ii = on ? i : (int64_t)ma.y;
-    const int v0 = ma.w, v1 = mb.x, s0 = mb.y, s1 = mb.z;
-    const int nslot = (v1 - v0) / ROWS_PER_WG;
+    const int pup = mb.x, nup = mb.y, plo = mb.z, nlo = mb.w, s0 = mc.x, s1 = mc.y;
+    const int nslot = nup + nlo;
     const int pk = (k - 1) & 1, ck = k & 1;
     const VT* rsrc = STEP ? rprev : rcur;   // where r_{k-1} (STEP) or r_k lives
-    const MT* Aw = A + 36 * (int64_t)v0 + lane;
-    const int32_t* cw = sl.ell_col + v0 + lane;
+    const MT* Aw = A + 36 * (int64_t)ma.w + lane;   // this slice's stored (upper) blocks
+    // column position of slot jj: upper slots first, then lower slots
+    auto cpos = [&](int jj) -> int64_t {
+        return jj < nup ? (int64_t)pup + (int64_t)jj * ROWS_PER_WG + lane
+                        : (int64_t)plo + (int64_t)(jj - nup) * ROWS_PER_WG + lane;
+    };
 
     // Straight-line loads, all requested before any is used: the own row (r, q,
     // p and x, needed by the update; every wave loads it so that no branch
@@ -489,8 +497,10 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
         }
     if (EARLY) load_px();
     int j = wave;
-    const int32_t* cj = j < nslot ? cw + (int64_t)j * ROWS_PER_WG : (const int32_t*)&sl.meta[2 * w];
-    int64_t col = *cj;
+    // (past the last slot, the loads read this slice's metadata record instead: a valid address)
+    const int32_t* dummy = (const int32_t*)&sl.meta[3 * w];
+    int64_t col = *(j < nslot ? sl.ell_col + cpos(j) : dummy);
+    int vp = *(j < nslot ? sl.ell_vpos + cpos(j) : dummy);
     col = j < nslot ? col : ii;
 
     // CG scalars of this structure (identical in every thread of every slice)
@@ -565,19 +575,32 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
                 if (STEP) v = v - alpha[h] * qprev[o];
                 xj[c][h] = v;
             }
-        const MT* a = Aw + (int64_t)j * 36 * ROWS_PER_WG;
+        const bool upper = j < nup;
+        // the block: stored here (upper), or the upper block (col, i) of another row, transposed
+        const MT* a = upper ? Aw + (int64_t)j * 36 * ROWS_PER_WG
+                            : A + (int64_t)(vp >> 6) * 36 * ROWS_PER_WG + (vp & (ROWS_PER_WG - 1));
         // next slot's column before this slot's products
         const int jn = j + WPS;
-        const int32_t* cn = jn < nslot ? cw + (int64_t)jn * ROWS_PER_WG : (const int32_t*)&sl.meta[2 * w];
-        int64_t coln = *cn;
+        int64_t coln = *(jn < nslot ? sl.ell_col + cpos(jn) : dummy);
+        const int vpn = *(jn < nslot ? sl.ell_vpos + cpos(jn) : dummy);
         coln = jn < nslot ? coln : ii;
+        if (upper) {
 #pragma unroll
-        for (int e = 0; e < 36; ++e) {
-            const VT av = (VT)a[e * ROWS_PER_WG];
+            for (int e = 0; e < 36; ++e) {
+                const VT av = (VT)a[e * ROWS_PER_WG];
 #pragma unroll
-            for (int h = 0; h < NR; ++h) acc[e / 6][h] += av * xj[e % 6][h];
+                for (int h = 0; h < NR; ++h) acc[e / 6][h] += av * xj[e % 6][h];
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 36; ++e) {
+                const VT av = (VT)a[e * ROWS_PER_WG];
+#pragma unroll
+                for (int h = 0; h < NR; ++h) acc[e % 6][h] += av * xj[e / 6][h];
+            }
         }
         col = coln;
+        vp = vpn;
     }
     if (WPS > 1) {
         if (wave > 0) {
@@ -706,18 +729,17 @@ __global__ __launch_bounds__(64 * WPS) void k_replace(Slices sl, Partials P, Sys
     __shared__ double red[WPS > 1 ? WPS - 1 : 1][6 * NR][64];
     const int w = slice_of_block(sl);
     if (w < 0) return;
-    const int4 ma = sl.meta[2 * w];
-    const int4 mb = sl.meta[2 * w + 1];
+    const int4 ma = sl.meta[3 * w];
+    const int4 mb = sl.meta[3 * w + 1];
     const int amask = st.slice_act[w];
-    if ((amask | ((ma.x | ma.y | ma.z | ma.w | mb.x) < 0)) == 0) return;
+    if ((amask | ((ma.x | ma.y | ma.z | ma.w | mb.x | mb.y | mb.z | mb.w) < 0)) == 0) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t i = (int64_t)ma.y + lane;
     const bool on = i < ma.z;
     const int64_t ii = on ? i : (int64_t)ma.y;
-    const int v0 = ma.w, v1 = mb.x;
-    const int nslot = (v1 - v0) / ROWS_PER_WG;
-    const double* Aw = A + 36 * (int64_t)v0 + lane;
-    const int32_t* cw = sl.ell_col + v0 + lane;
+    const int pup = mb.x, nup = mb.y, plo = mb.z, nlo = mb.w;
+    const int nslot = nup + nlo;
+    const double* Aw = A + 36 * (int64_t)ma.w + lane;
     double xown[6][NR], bown[6][NR];
 #pragma unroll
     for (int c = 0; c < 6; ++c)
@@ -733,18 +755,32 @@ __global__ __launch_bounds__(64 * WPS) void k_replace(Slices sl, Partials P, Sys
 #pragma unroll
         for (int h = 0; h < NR; ++h) acc[c][h] = 0.0;
     for (int j = wave; j < nslot; j += WPS) {
-        const int64_t col = cw[(int64_t)j * ROWS_PER_WG];
+        const bool upper = j < nup;
+        const int64_t pos = upper ? (int64_t)pup + (int64_t)j * ROWS_PER_WG + lane
+                                  : (int64_t)plo + (int64_t)(j - nup) * ROWS_PER_WG + lane;
+        const int64_t col = sl.ell_col[pos];
+        const int vp = sl.ell_vpos[pos];
         double xj[6][NR];
 #pragma unroll
         for (int c = 0; c < 6; ++c)
 #pragma unroll
             for (int h = 0; h < NR; ++h) xj[c][h] = x64[(6 * col + c) * NR + h];
-        const double* a = Aw + (int64_t)j * 36 * ROWS_PER_WG;
+        const double* a = upper ? Aw + (int64_t)j * 36 * ROWS_PER_WG
+                                : A + (int64_t)(vp >> 6) * 36 * ROWS_PER_WG + (vp & (ROWS_PER_WG - 1));
+        if (upper) {
 #pragma unroll
-        for (int e = 0; e < 36; ++e) {
-            const double av = a[e * ROWS_PER_WG];
+            for (int e = 0; e < 36; ++e) {
+                const double av = a[e * ROWS_PER_WG];
 #pragma unroll
-            for (int h = 0; h < NR; ++h) acc[e / 6][h] += av * xj[e % 6][h];
+                for (int h = 0; h < NR; ++h) acc[e / 6][h] += av * xj[e % 6][h];
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 36; ++e) {
+                const double av = a[e * ROWS_PER_WG];
+#pragma unroll
+                for (int h = 0; h < NR; ++h) acc[e % 6][h] += av * xj[e / 6][h];
+            }
         }
     }
     if (WPS > 1) {
@@ -943,9 +979,9 @@ static int check_status(Plan* p, hipStream_t s) {
 }
 
 static Slices make_slices(Plan* pl, int base, int count) {
-    return Slices{pl->d_wg_sys,     pl->d_wg_row0, pl->d_wg_row1, pl->d_sys_wg_off,
-                  pl->d_slice_voff, pl->d_ell_col, (const int4*)pl->d_slice_meta, base,
-                  count,            (count + 7) / 8};
+    return Slices{pl->d_wg_sys,  pl->d_wg_row0, pl->d_wg_row1, pl->d_sys_wg_off,
+                  pl->d_ell_vpos, pl->d_ell_col, (const int4*)pl->d_slice_meta, base,
+                  count,          (count + 7) / 8};
 }
 
 // Launch one ordinary CG iteration with `wps` waves per slice (falls back to
@@ -1487,6 +1523,7 @@ int solve_plain(Plan* pl, const double* f, double* u, int nrhs, double tol, int 
                 hipStream_t s) {
     int rc = check_status(pl, s);
     if (rc) return rc;
+    if ((rc = launch_full_blocks(pl, s))) return rc;   // the plain path runs on Kc in block CSR
     Workspace w;
     rc = get_workspace(pl, 1, w);
     if (rc) return rc;
