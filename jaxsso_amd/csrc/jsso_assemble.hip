@@ -135,8 +135,8 @@ __global__ __launch_bounds__(128) void diag_factor_kernel(int32_t n_nodes, const
 //            row factors L_i^-1 (64 consecutive rows: contiguous) and the column
 //            factors L_j^-1 in LDS;
 //   phase 2: wave w computes columns c = w, w + 4 of every position's Ks (lane =
-//            position) and stores them entry-major, 64 consecutive positions per
-//            store.
+//            position) and stores them into the value-slot layout (16-byte groups of
+//            entries, lanes interleaved per group).
 constexpr int ASM_WG = 256;
 constexpr int MAX_CONTRIB = 2;   // element blocks per off-diagonal block read without a loop (2 on a quad mesh)
 
@@ -205,7 +205,9 @@ __global__ __launch_bounds__(ASM_WG) void ell_assemble_kernel(
     __syncthreads();
     const int wave = t >> 6, lane = t & 63;
     if (ccount[lane] == 0) return;   // padding (or an empty block): its entries stay zero
-    const int64_t base = slot * 36 * P + lane;
+    // value-slot layout: 16-byte groups of entries, lanes interleaved per group
+    // (2 doubles / 4 floats per group)
+    const int64_t base = slot * 36 * P;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int c = wave + 4 * h;
@@ -224,9 +226,9 @@ __global__ __launch_bounds__(ASM_WG) void ell_assemble_kernel(
             double s = 0.0;
 #pragma unroll
             for (int k2 = 0; k2 <= r; ++k2) s += li[lane][r * 6 + k2] * Tc[k2];
-            const int64_t e = base + (int64_t)(r * 6 + c) * P;
-            Ke64[e] = s;
-            Ke32[e] = (float)s;
+            const int e = r * 6 + c;
+            Ke64[base + ((int64_t)(e / 2) * P + lane) * 2 + e % 2] = s;
+            Ke32[base + ((int64_t)(e / 4) * P + lane) * 4 + e % 4] = (float)s;
         }
     }
 }

@@ -219,6 +219,22 @@ __device__ __forceinline__ double wave_sum(double v) {
     return join_halves(l32[0], h32[0]) + join_halves(l32[1], h32[1]);
 }
 
+// Value slots of the sliced ELL: the 36 entries of lane l's block are stored in
+// 16-byte groups of G = 16 / sizeof(T) consecutive entries, lanes interleaved
+// group by group ([slot][36 / G][64 lanes][G]), so one vector load per lane
+// brings G entries and a wavefront's load covers 1 KB of consecutive bytes.
+template <typename T>
+__device__ __forceinline__ void load_block(const T* __restrict__ slot, int lane, T (&av)[36]) {
+    constexpr int G = 16 / sizeof(T);
+    typedef T V __attribute__((ext_vector_type(G)));
+#pragma unroll
+    for (int g = 0; g < 36 / G; ++g) {
+        const V v = *reinterpret_cast<const V*>(slot + ((int64_t)g * ROWS_PER_WG + lane) * G);
+#pragma unroll
+        for (int t = 0; t < G; ++t) av[g * G + t] = v[t];
+    }
+}
+
 // Sum over the whole workgroup, result in every thread.  `lds` holds >= NW*NV.
 template <int NV, int NW>
 __device__ __forceinline__ void group_sum(double (&v)[NV], double* lds) {
@@ -462,7 +478,7 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
     const int nslot = nup + nlo;
     const int pk = (k - 1) & 1, ck = k & 1;
     const VT* rsrc = STEP ? rprev : rcur;   // where r_{k-1} (STEP) or r_k lives
-    const MT* Aw = A + 36 * (int64_t)ma.w + lane;   // this slice's stored (upper) blocks
+    const MT* Aw = A + 36 * (int64_t)ma.w;   // this slice's stored (upper) value slots
     // column position of slot jj: upper slots first, then lower slots
     auto cpos = [&](int jj) -> int64_t {
         return jj < nup ? (int64_t)pup + (int64_t)jj * ROWS_PER_WG + lane
@@ -577,8 +593,11 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
             }
         const bool upper = j < nup;
         // the block: stored here (upper), or the upper block (col, i) of another row, transposed
-        const MT* a = upper ? Aw + (int64_t)j * 36 * ROWS_PER_WG
-                            : A + (int64_t)(vp >> 6) * 36 * ROWS_PER_WG + (vp & (ROWS_PER_WG - 1));
+        MT av[36];
+        if (upper)
+            load_block(Aw + (int64_t)j * 36 * ROWS_PER_WG, lane, av);
+        else
+            load_block(A + (int64_t)(vp >> 6) * 36 * ROWS_PER_WG, vp & (ROWS_PER_WG - 1), av);
         // next slot's column before this slot's products
         const int jn = j + WPS;
         int64_t coln = *(jn < nslot ? sl.ell_col + cpos(jn) : dummy);
@@ -586,18 +605,14 @@ __global__ __launch_bounds__(64 * WPS) void k_cgf(Slices sl, Partials P, SysStat
         coln = jn < nslot ? coln : ii;
         if (upper) {
 #pragma unroll
-            for (int e = 0; e < 36; ++e) {
-                const VT av = (VT)a[e * ROWS_PER_WG];
+            for (int e = 0; e < 36; ++e)
 #pragma unroll
-                for (int h = 0; h < NR; ++h) acc[e / 6][h] += av * xj[e % 6][h];
-            }
+                for (int h = 0; h < NR; ++h) acc[e / 6][h] += (VT)av[e] * xj[e % 6][h];
         } else {
 #pragma unroll
-            for (int e = 0; e < 36; ++e) {
-                const VT av = (VT)a[e * ROWS_PER_WG];
+            for (int e = 0; e < 36; ++e)
 #pragma unroll
-                for (int h = 0; h < NR; ++h) acc[e % 6][h] += av * xj[e / 6][h];
-            }
+                for (int h = 0; h < NR; ++h) acc[e % 6][h] += (VT)av[e] * xj[e / 6][h];
         }
         col = coln;
         vp = vpn;
@@ -739,7 +754,7 @@ __global__ __launch_bounds__(64 * WPS) void k_replace(Slices sl, Partials P, Sys
     const int64_t ii = on ? i : (int64_t)ma.y;
     const int pup = mb.x, nup = mb.y, plo = mb.z, nlo = mb.w;
     const int nslot = nup + nlo;
-    const double* Aw = A + 36 * (int64_t)ma.w + lane;
+    const double* Aw = A + 36 * (int64_t)ma.w;
     double xown[6][NR], bown[6][NR];
 #pragma unroll
     for (int c = 0; c < 6; ++c)
@@ -765,22 +780,21 @@ __global__ __launch_bounds__(64 * WPS) void k_replace(Slices sl, Partials P, Sys
         for (int c = 0; c < 6; ++c)
 #pragma unroll
             for (int h = 0; h < NR; ++h) xj[c][h] = x64[(6 * col + c) * NR + h];
-        const double* a = upper ? Aw + (int64_t)j * 36 * ROWS_PER_WG
-                                : A + (int64_t)(vp >> 6) * 36 * ROWS_PER_WG + (vp & (ROWS_PER_WG - 1));
+        double av[36];
+        if (upper)
+            load_block(Aw + (int64_t)j * 36 * ROWS_PER_WG, lane, av);
+        else
+            load_block(A + (int64_t)(vp >> 6) * 36 * ROWS_PER_WG, vp & (ROWS_PER_WG - 1), av);
         if (upper) {
 #pragma unroll
-            for (int e = 0; e < 36; ++e) {
-                const double av = a[e * ROWS_PER_WG];
+            for (int e = 0; e < 36; ++e)
 #pragma unroll
-                for (int h = 0; h < NR; ++h) acc[e / 6][h] += av * xj[e % 6][h];
-            }
+                for (int h = 0; h < NR; ++h) acc[e / 6][h] += av[e] * xj[e % 6][h];
         } else {
 #pragma unroll
-            for (int e = 0; e < 36; ++e) {
-                const double av = a[e * ROWS_PER_WG];
+            for (int e = 0; e < 36; ++e)
 #pragma unroll
-                for (int h = 0; h < NR; ++h) acc[e % 6][h] += av * xj[e / 6][h];
-            }
+                for (int h = 0; h < NR; ++h) acc[e % 6][h] += av[e] * xj[e / 6][h];
         }
     }
     if (WPS > 1) {
