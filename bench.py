@@ -101,12 +101,13 @@ def main():
     # (a column each, plus the value position of the transposed half)
     alg_bytes = (nnzb_off // 2) * 36 * s_m + nnzb_off * 4 + (nnzb_off // 2) * 4 + 8 * ndof * s_v
     achieved = alg_bytes / (avg_ms[0] * 1e-3) / 1e9 if avg_ms[0] > 0 else None
-    traffic, traffic_src = _measured_traffic(args)
+    traffic, traffic_src = _measured_traffic(args, info.get("waves_per_slice"))
     roofline = {"kernel": f"k_cgf<{'float' if s_m == 4 else 'double'}>", "bound": "hbm",
                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                 "traffic_source": traffic_src,
                 "avg_launch_us": avg_ms[0] * 1e3, "sampled_launches": int(nsamp[0]),
+                "waves_per_slice": info.get("waves_per_slice"),
                 "alg_bytes_per_launch": int(alg_bytes)}
 
     # -- element stiffness + assembly alone (elements/s)
@@ -229,17 +230,18 @@ def shell200(tol):
     return res
 
 
-def _measured_traffic(args):
+def _measured_traffic(args, wps):
     """HBM bytes per launch of k_cgf<float> on this workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a copy-kernel
-    calibration); None for other workloads or precisions."""
+    calibration), when it was measured for the variant (waves per slice) this run
+    chose; None for other workloads, precisions or variants."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         "cg_batch128_shell40_traffic.txt")
     if args.batch != 128 or args.precision != "mixed" or not os.path.exists(path):
         return None, None
     with open(path) as fh:
         for line in fh:
-            if line.startswith("traffic_bytes_per_launch k_cgf<float>:"):
+            if line.startswith(f"traffic_bytes_per_launch k_cgf<float, float, 1, 1, {wps}>:"):
                 return int(line.split(":")[1]), "profiles/cg_batch128_shell40_traffic.txt"
     return None, None
 
