@@ -64,6 +64,12 @@ enum {
     JSSO_PREC_PLAIN = 2  /* baseline: FP64 block-Jacobi PCG, one structure per launch */
 };
 
+/* preconditioner of the batched precisions (jsso_plan_set_preconditioner) */
+enum {
+    JSSO_PC_BLOCK_JACOBI = 0, /* split block-Jacobi scaling (the default)              */
+    JSSO_PC_AMG = 1           /* smoothed-aggregation AMG V-cycle on the scaled system  */
+};
+
 typedef struct {
     int32_t iterations;   /* CG iterations performed (max over systems)            */
     int32_t replacements; /* FP64 true-residual replacements performed             */
@@ -72,6 +78,7 @@ typedef struct {
     double rel_residual;  /* max over systems of ||L^-1 (f - K u)|| / ||L^-1 f||    */
     int32_t waves_per_slice; /* batched CG kernel variant used (1, 2 or 4 wavefronts
                                 per 64-block-row slice); 0 on the plain path      */
+    int32_t levels;          /* AMG levels (JSSO_PC_AMG), 0 otherwise                 */
 } jsso_info;
 
 /* plan size indices for jsso_plan_sizes */
@@ -151,6 +158,16 @@ int jsso_compliance(jsso_plan* plan, const double* d_f, const double* d_u, int32
  * 1/2/4 = force); h_wps, if not NULL, receives the stored choice (0 = untuned). */
 int jsso_plan_cg_tuning(jsso_plan* plan, int32_t precision, int32_t nrhs, int32_t set_wps,
                         int32_t* h_wps);
+
+/* Preconditioner of the batched solves (JSSO_PREC_FP64 / JSSO_PREC_MIXED); no
+ * counterpart in the reference, which factorises with SuperLU (JaxSSO/solver.py:
+ * 176-218).  JSSO_PC_AMG builds a smoothed-aggregation hierarchy on the plan's
+ * pattern (host, once) and its numeric operators after each jsso_assemble (device,
+ * at the next solve); the plain path always uses block-Jacobi. */
+int jsso_plan_set_preconditioner(jsso_plan* plan, int32_t pc);
+/* Symbolic AMG hierarchy of a plan (host only, no device work): *h_levels receives
+ * the number of levels, h_nodes[l] (up to cap entries) the block rows of level l. */
+int jsso_plan_amg_levels(jsso_plan* plan, int32_t* h_nodes, int32_t cap, int32_t* h_levels);
 
 /* Measurement hook (no counterpart in the reference): sampled HIP-event timing
  * of the batched solver's dominant kernel, the fused CG iteration that applies

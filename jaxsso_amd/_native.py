@@ -20,7 +20,8 @@ PREC = {"fp64": 0, "mixed": 1, "plain": 2}
 EXPORTS = ("jsso_version", "jsso_last_error", "jsso_plan_create", "jsso_plan_destroy", "jsso_plan_sizes",
            "jsso_plan_pattern", "jsso_element_ke_beam", "jsso_element_ke_quad", "jsso_assemble",
            "jsso_solve", "jsso_adjoint_grad", "jsso_spmv", "jsso_compliance", "jsso_plan_cg_tuning",
-           "jsso_timing_enable", "jsso_timing_read")
+           "jsso_timing_enable", "jsso_timing_read", "jsso_plan_set_preconditioner",
+           "jsso_plan_amg_levels")
 
 
 class NativeUnavailable(RuntimeError):
@@ -36,7 +37,8 @@ class JssoError(RuntimeError):
 class Info(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("replacements", ctypes.c_int32),
                 ("converged", ctypes.c_int32), ("n_systems", ctypes.c_int32),
-                ("rel_residual", ctypes.c_double), ("waves_per_slice", ctypes.c_int32)]
+                ("rel_residual", ctypes.c_double), ("waves_per_slice", ctypes.c_int32),
+                ("levels", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -71,6 +73,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.jsso_plan_cg_tuning.argtypes = [P, I32, I32, I32, P]
     lib.jsso_timing_enable.argtypes = [I32]
     lib.jsso_timing_read.argtypes = [P, P, I32]
+    lib.jsso_plan_set_preconditioner.argtypes = [P, I32]
+    lib.jsso_plan_amg_levels.argtypes = [P, P, I32, P]
     for name in EXPORTS:
         if name not in ("jsso_version",):
             getattr(lib, name).restype = ctypes.c_int

@@ -57,11 +57,11 @@ class StructureBatch:
     def assemble(self):
         self.plan.assemble(self.crds, self.prop_beam, self.prop_quad)
 
-    def solve(self, precision="fp64", tol=DEFAULT_TOL, maxit=DEFAULT_MAXIT, assemble=True):
+    def solve(self, precision="fp64", tol=DEFAULT_TOL, maxit=DEFAULT_MAXIT, assemble=True, preconditioner=None):
         if assemble:
             self.assemble()
         f = self.f[:, 0] if self.n_cases == 1 else self.f
-        self.u = self.plan.solve(f, precision=precision, tol=tol, maxit=maxit)
+        self.u = self.plan.solve(f, precision=precision, tol=tol, maxit=maxit, preconditioner=preconditioner)
         return self.u
 
     def compliance(self, u=None):
@@ -76,8 +76,8 @@ class StructureBatch:
         u = self.u if u is None else u
         return self.plan.adjoint(self.crds, self.prop_beam, self.prop_quad, u, lam=None, lam_scale=0.5)
 
-    def compliance_and_gradient(self, precision="fp64", tol=DEFAULT_TOL, maxit=DEFAULT_MAXIT):
-        u = self.solve(precision=precision, tol=tol, maxit=maxit)
+    def compliance_and_gradient(self, precision="fp64", tol=DEFAULT_TOL, maxit=DEFAULT_MAXIT, preconditioner=None):
+        u = self.solve(precision=precision, tol=tol, maxit=maxit, preconditioner=preconditioner)
         return self.compliance(u), self.gradient(u)
 
     def split_nodes(self, a):

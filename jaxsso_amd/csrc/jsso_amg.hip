@@ -1,0 +1,1975 @@
+// Smoothed-aggregation algebraic multigrid (SA-AMG) preconditioned CG for the
+// batched solves (no counterpart in the reference, which hands the system to
+// SuperLU: this is the "solver robustness beyond block-Jacobi" path).
+//
+// Every level works on a block-Jacobi-scaled operator with unit 6x6 diagonal
+// blocks, stored as the same symmetric sliced ELL as the fine system (only the
+// off-diagonal upper blocks; the diagonal is implicit).  Level 0 is the solver
+// matrix itself, A0 = L^-1 Kc L^-T.
+//
+// Hierarchy (Vanek, Mandel & Brezina 1996):
+//   symbolic, on the host, once per plan: greedy aggregation of each level's
+//   node graph (three passes: root nodes whose neighbours are all free take
+//   them; leftovers join a neighbouring aggregate; the rest form new ones),
+//   structure by structure; the block patterns of the prolongator P, of
+//   W = A P and of the Galerkin operator P^T A P, with the list of products
+//   that make each block;
+//   numeric, on the device, after every assembly: the near-null space (the six
+//   rigid-body modes of every node, in the scaled variables: L_i^T B_i) is
+//   orthonormalised per aggregate (tentative prolongator Q, coarse null space
+//   R); P = (I - omega A) Q with omega = 4 / (3 rho), rho from a power
+//   iteration; A_c = P^T (A P); the coarse diagonal blocks are factored
+//   (Cholesky) and A_c and P rescaled so the coarse level again has unit
+//   diagonal blocks.  A structure stops coarsening when it has <= AMG_MAXC
+//   nodes; its last level is inverted densely.
+//
+// V-cycle (symmetric, so M is SPD): one damped-Jacobi pre- and post-smoothing
+// step z = r / theta with theta = (rho + rho/30) / 2 (the one-step Chebyshev
+// smoother on [rho/30, rho]), restriction P^T, the coarse correction, and
+// prolongation P.  The V-cycle runs in FP32; level 0 runs in the solve's
+// precision.  The outer CG keeps the batched path's reliable-update scheme:
+// FP32 (or FP64) recurrences, the FP64 true residual every `replace_every`
+// iterations, convergence and the best iterate decided there.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <numeric>
+#include <type_traits>
+
+#include "solver_kernels.hpp"
+
+namespace jsso {
+
+constexpr int AMG_MAXC = 16;       // nodes of a structure's directly solved coarsest level (<= 96 DOF)
+constexpr int AMG_QR_RPL = 5;      // null-space rows per lane in the per-aggregate QR
+constexpr int AMG_MAX_AGG = AMG_QR_RPL * 64 / 6;   // nodes per aggregate (53)
+constexpr int AMG_POWER_STEPS = 10;
+constexpr double AMG_RHO_SAFETY = 1.12;   // the power iteration approaches rho from below
+constexpr int AMG_MAXLEVELS = 12;
+
+// ---------------------------------------------------------------------------
+// host: symbolic hierarchy
+// ---------------------------------------------------------------------------
+struct AmgHostLevel {
+    int32_t n = 0;                                 // nodes (all structures, packed)
+    std::vector<int32_t> node_off;                 // per structure: first node (n_sys + 1)
+    std::vector<int32_t> rowptr, colind;           // block pattern incl. the diagonal (level >= 1)
+    EllPattern ell;                                // level >= 1 (level 0: the plan's)
+    std::vector<int32_t> node_sys;                 // per node: structure
+    std::vector<uint8_t> cont;                     // per structure: 1 = has a coarser level (P), 0 = not here / terminal
+    std::vector<uint8_t> term;                     // per structure: 1 = solved densely at this level
+    // to the next level (nodes of continuing structures)
+    std::vector<int32_t> agg;                      // per node: aggregate (= coarse node) or -1
+    std::vector<int32_t> agg_ptr, agg_nodes;       // per aggregate: member nodes, in node order
+    std::vector<int32_t> p_ptr, p_col, p_row;      // P blocks by fine row: coarse column, fine row
+    std::vector<int32_t> p_cptr, p_cj, p_caref;    // per P block: products (node j, A block ref or -1 = diagonal)
+    std::vector<int32_t> pt_ptr, pt_blk, pt_row;   // per coarse node: the P blocks of its column, their fine rows
+    std::vector<int32_t> p_tpos;                   // per P block: its position in the pt lists
+    std::vector<int32_t> w_ptr, w_col, w_row;      // W = A P blocks by fine row
+    std::vector<int32_t> w_cptr, w_caref, w_cp;    // per W block: products (A block ref or -1, P block)
+    std::vector<int32_t> c_cptr, c_p, c_w;         // per block of the next level's CSR: products (P block, W block)
+};
+
+struct AmgLevelDev {
+    int32_t n = 0, n_wg = 0;
+    // pattern of this level (level >= 1)
+    int32_t *d_rowptr = nullptr, *d_colind = nullptr, *d_kref = nullptr, *d_browof = nullptr;
+    int32_t *d_wg_sys = nullptr, *d_wg_row0 = nullptr, *d_wg_row1 = nullptr, *d_sys_wg_off = nullptr;
+    int32_t *d_ell_col = nullptr, *d_ell_vpos = nullptr, *d_meta = nullptr, *d_ell_src = nullptr;
+    int64_t ell_slots = 0;
+    double* d_A64 = nullptr;
+    float* d_A32 = nullptr;
+    int32_t* d_node_sys = nullptr;
+    uint8_t* d_cont = nullptr;   // per structure
+    // smoothing / power iteration
+    double *d_rho = nullptr, *d_omega = nullptr;   // per structure
+    float* d_theta = nullptr;                      // per structure
+    double *d_pv[2] = {nullptr, nullptr};          // power iteration vectors (6 n)
+    double* d_ppart = nullptr;                     // per slice, per step parity: |y|^2
+    // transfer to the next level
+    int32_t n_agg = 0;
+    int32_t *d_agg = nullptr, *d_agg_ptr = nullptr, *d_agg_nodes = nullptr;
+    double *d_B = nullptr, *d_Q = nullptr, *d_R = nullptr;   // 36 per node / per aggregate
+    int64_t nP = 0, nW = 0, nPc = 0, nWc = 0, nCc = 0;
+    int32_t *d_p_ptr = nullptr, *d_p_col = nullptr, *d_p_row = nullptr;
+    int32_t *d_p_cptr = nullptr, *d_p_cj = nullptr, *d_p_caref = nullptr;
+    int32_t *d_pt_ptr = nullptr, *d_pt_row = nullptr, *d_p_tpos = nullptr;
+    int32_t *d_w_row = nullptr, *d_w_cptr = nullptr, *d_w_caref = nullptr, *d_w_cp = nullptr;
+    int32_t *d_c_cptr = nullptr, *d_c_p = nullptr, *d_c_w = nullptr;
+    double *d_P64 = nullptr, *d_W64 = nullptr;
+    float *d_P32 = nullptr, *d_PT32 = nullptr;   // by fine row / transposed by coarse node
+    // V-cycle vectors (level >= 1), 6 n x nrhs FP32
+    float *d_b = nullptr, *d_t = nullptr, *d_x = nullptr;
+};
+
+struct AmgState {
+    int nl = 0;                       // levels
+    std::vector<AmgHostLevel> host;
+    std::vector<AmgLevelDev> dev;
+    // next level's Galerkin storage lives with the coarse level: A_c (CSR order), Lc, Lc^-1
+    std::vector<double*> d_Ac, d_Lc, d_Lci;
+    std::vector<int32_t*> d_cdiag;
+    // dense coarsest solves: per structure {level, first node, nodes, inverse offset}
+    std::vector<int32_t> h_term;      // 4 per structure
+    int32_t* d_term = nullptr;
+    int32_t* d_node_off0 = nullptr;   // level-0 first node of every structure (the null space's origin)
+    std::vector<std::vector<int32_t>> term_list;   // per level: structures solved there
+    std::vector<int32_t*> d_term_list;
+    float* d_inv = nullptr;           // dense inverses, column-major, FP32
+    double* d_dense = nullptr;        // scratch for the factorisation, FP64
+    int64_t inv_total = 0;
+    int vec_nrhs = 0;                 // V-cycle vectors sized for this many load cases
+    int wps0 = 4;                     // waves per slice of the level-0 kernels
+    // level-0 extra vectors (VT-sized, 8 bytes per value)
+    void *d_z = nullptr, *d_t0 = nullptr, *d_p1 = nullptr;
+    size_t z_bytes = 0;
+    uint64_t gen = 1;                 // bumped whenever a buffer the graphs point at changes
+    std::vector<void*> allocs;
+    CgGraph graph[2][9][2];
+};
+
+// Greedy aggregation of one structure's nodes [n0, n1) of a level.  `active`
+// excludes nodes with no unknowns (every DOF constrained at level 0).
+static void aggregate_structure(const std::vector<int32_t>& rowptr, const std::vector<int32_t>& colind,
+                                const std::vector<uint8_t>& active, int n0, int n1, std::vector<int32_t>& agg,
+                                int32_t& na) {
+    std::vector<int32_t> size;   // nodes per aggregate of this structure (index a - first)
+    const int32_t first = na;
+    auto nbrs = [&](int i, auto&& f) {
+        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+            const int j = colind[k];
+            if (j != i && active[j]) f(j);
+        }
+    };
+    // pass 1: a node whose active neighbours are all free roots a new aggregate with them
+    for (int i = n0; i < n1; ++i) {
+        if (!active[i] || agg[i] >= 0) continue;
+        bool free = true;
+        int deg = 0;
+        nbrs(i, [&](int j) {
+            ++deg;
+            free &= agg[j] < 0;
+        });
+        if (deg == 0 || !free || deg + 1 > AMG_MAX_AGG) continue;
+        agg[i] = na;
+        nbrs(i, [&](int j) { agg[j] = na; });
+        size.push_back(deg + 1);
+        ++na;
+    }
+    // pass 2: leftovers join the aggregate of their first aggregated neighbour
+    std::vector<int32_t> pass2(n1 - n0, -1);
+    for (int i = n0; i < n1; ++i) {
+        if (!active[i] || agg[i] >= 0) continue;
+        int a = -1;
+        nbrs(i, [&](int j) {
+            if (a < 0 && agg[j] >= 0 && size[agg[j] - first] < AMG_MAX_AGG) a = agg[j];
+        });
+        pass2[i - n0] = a;
+        if (a >= 0) ++size[a - first];
+    }
+    for (int i = n0; i < n1; ++i)
+        if (pass2[i - n0] >= 0) agg[i] = pass2[i - n0];
+    // pass 3: the rest form new aggregates with their free neighbours
+    for (int i = n0; i < n1; ++i) {
+        if (!active[i] || agg[i] >= 0) continue;
+        agg[i] = na;
+        int cnt = 1;
+        nbrs(i, [&](int j) {
+            if (agg[j] < 0 && cnt < AMG_MAX_AGG) {
+                agg[j] = na;
+                ++cnt;
+            }
+        });
+        size.push_back(cnt);
+        ++na;
+    }
+}
+
+// Symbolic hierarchy.  Level 0 is the plan's pattern; each further level is
+// built while some structure still has more than AMG_MAXC nodes.
+static int amg_build_host(Plan* pl, AmgState& A) {
+    const int ns = pl->n_sys;
+    A.host.clear();
+    A.host.reserve(AMG_MAXLEVELS + 1);   // references to levels stay valid while levels are added
+    A.host.emplace_back();
+    {
+        AmgHostLevel& L0 = A.host[0];
+        L0.n = pl->n_nodes;
+        L0.node_off = pl->h_node_off;
+        L0.rowptr = pl->h_rowptr;
+        L0.colind = pl->h_colind;
+        L0.ell.kref = pl->h_kref;
+    }
+    for (int l = 0;; ++l) {
+        JSSO_REQUIRE(l < AMG_MAXLEVELS, JSSO_EUNSUPPORTED, "AMG: coarsening did not reach %d nodes in %d levels",
+                     AMG_MAXC, AMG_MAXLEVELS);
+        AmgHostLevel& F = A.host[l];
+        const int n = F.n;
+        F.node_sys.assign(n, 0);
+        for (int s = 0; s < ns; ++s)
+            for (int i = F.node_off[s]; i < F.node_off[s + 1]; ++i) F.node_sys[i] = s;
+        F.cont.assign(ns, 0);
+        F.term.assign(ns, 0);
+        bool any = false;
+        for (int s = 0; s < ns; ++s) {
+            const int cnt = F.node_off[s + 1] - F.node_off[s];
+            if (cnt == 0) continue;   // finished at a finer level
+            if (l > 0 && cnt <= AMG_MAXC) {
+                F.term[s] = 1;
+            } else {
+                F.cont[s] = 1;
+                any = true;
+            }
+        }
+        if (!any) break;
+        // aggregation
+        std::vector<uint8_t> active(n, 1);
+        if (l == 0)
+            for (int i = 0; i < n; ++i) {
+                bool all = true;
+                for (int c = 0; c < 6; ++c) all &= pl->h_fixed[6 * (size_t)i + c] != 0;
+                active[i] = all ? 0 : 1;
+            }
+        F.agg.assign(n, -1);
+        std::vector<int32_t> cnode_off(ns + 1, 0);
+        int32_t na = 0;
+        for (int s = 0; s < ns; ++s) {
+            cnode_off[s] = na;
+            if (!F.cont[s]) continue;
+            const int32_t before = na;
+            aggregate_structure(F.rowptr, F.colind, active, F.node_off[s], F.node_off[s + 1], F.agg, na);
+            const int cnt = F.node_off[s + 1] - F.node_off[s];
+            JSSO_REQUIRE(l == 0 || na - before < cnt, JSSO_EUNSUPPORTED,
+                         "AMG: aggregation of structure %d made no progress at level %d", s, l);
+        }
+        cnode_off[ns] = na;
+        // aggregate members in node order (the first is the origin of its rigid-body modes)
+        F.agg_ptr.assign(na + 1, 0);
+        for (int i = 0; i < n; ++i)
+            if (F.agg[i] >= 0) F.agg_ptr[F.agg[i] + 1]++;
+        for (int a = 0; a < na; ++a) F.agg_ptr[a + 1] += F.agg_ptr[a];
+        for (int a = 0; a < na; ++a)
+            JSSO_REQUIRE(F.agg_ptr[a + 1] - F.agg_ptr[a] <= AMG_MAX_AGG, JSSO_EUNSUPPORTED,
+                         "AMG: aggregate larger than %d nodes", AMG_MAX_AGG);
+        F.agg_nodes.assign(F.agg_ptr[na], 0);
+        {
+            std::vector<int32_t> fill(F.agg_ptr.begin(), F.agg_ptr.end() - 1);
+            for (int i = 0; i < n; ++i)
+                if (F.agg[i] >= 0) F.agg_nodes[fill[F.agg[i]]++] = i;
+        }
+        // P pattern and products: P(i, J) = sum over j in {i} u N(i), agg(j) = J
+        F.p_ptr.assign(n + 1, 0);
+        F.p_col.clear();
+        F.p_row.clear();
+        F.p_cptr.assign(1, 0);
+        F.p_cj.clear();
+        F.p_caref.clear();
+        std::vector<std::pair<int32_t, int32_t>> prod;   // (J, k)
+        for (int i = 0; i < n; ++i) {
+            prod.clear();
+            if (active[i] && F.cont[F.node_sys[i]]) {
+                for (int k = F.rowptr[i]; k < F.rowptr[i + 1]; ++k) {
+                    const int j = F.colind[k];
+                    if (F.agg[j] >= 0) prod.emplace_back(F.agg[j], k);
+                }
+            }
+            std::sort(prod.begin(), prod.end());
+            for (size_t t = 0; t < prod.size();) {
+                const int J = prod[t].first;
+                F.p_col.push_back(J);
+                F.p_row.push_back(i);
+                for (; t < prod.size() && prod[t].first == J; ++t) {
+                    const int k = prod[t].second;
+                    const int j = F.colind[k];
+                    F.p_cj.push_back(j);
+                    F.p_caref.push_back(j == i ? -1 : F.ell.kref[k]);
+                }
+                F.p_cptr.push_back((int32_t)F.p_cj.size());
+            }
+            F.p_ptr[i + 1] = (int32_t)F.p_col.size();
+        }
+        const int64_t nP = F.p_col.size();
+        JSSO_REQUIRE(nP < INT32_MAX / 36, JSSO_EUNSUPPORTED, "AMG: prolongator too large");
+        // P^T: per coarse node, its P blocks in fine-row order
+        F.pt_ptr.assign(na + 1, 0);
+        for (int64_t b = 0; b < nP; ++b) F.pt_ptr[F.p_col[b] + 1]++;
+        for (int a = 0; a < na; ++a) F.pt_ptr[a + 1] += F.pt_ptr[a];
+        F.pt_blk.assign(nP, 0);
+        {
+            std::vector<int32_t> fill(F.pt_ptr.begin(), F.pt_ptr.end() - 1);
+            for (int64_t b = 0; b < nP; ++b) F.pt_blk[fill[F.p_col[b]]++] = (int32_t)b;
+        }
+        F.pt_row.assign(nP, 0);
+        F.p_tpos.assign(nP, 0);
+        for (int64_t q = 0; q < nP; ++q) {
+            F.pt_row[q] = F.p_row[F.pt_blk[q]];
+            F.p_tpos[F.pt_blk[q]] = (int32_t)q;
+        }
+        // W = A P: W(i, J) = sum over j in {i} u N(i) of A(i, j) P(j, J)
+        F.w_ptr.assign(n + 1, 0);
+        F.w_col.clear();
+        F.w_row.clear();
+        F.w_cptr.assign(1, 0);
+        F.w_caref.clear();
+        F.w_cp.clear();
+        std::vector<std::pair<int32_t, std::pair<int32_t, int32_t>>> wprod;   // (J, (aref, P block))
+        for (int i = 0; i < n; ++i) {
+            wprod.clear();
+            if (active[i] && F.cont[F.node_sys[i]]) {
+                for (int k = F.rowptr[i]; k < F.rowptr[i + 1]; ++k) {
+                    const int j = F.colind[k];
+                    const int32_t aref = j == i ? -1 : F.ell.kref[k];
+                    for (int b = F.p_ptr[j]; b < F.p_ptr[j + 1]; ++b) wprod.push_back({F.p_col[b], {aref, b}});
+                }
+            }
+            std::sort(wprod.begin(), wprod.end());
+            for (size_t t = 0; t < wprod.size();) {
+                const int J = wprod[t].first;
+                F.w_col.push_back(J);
+                F.w_row.push_back(i);
+                for (; t < wprod.size() && wprod[t].first == J; ++t) {
+                    F.w_caref.push_back(wprod[t].second.first);
+                    F.w_cp.push_back(wprod[t].second.second);
+                }
+                F.w_cptr.push_back((int32_t)F.w_cp.size());
+            }
+            F.w_ptr[i + 1] = (int32_t)F.w_col.size();
+        }
+        JSSO_REQUIRE((int64_t)F.w_col.size() < INT32_MAX / 36, JSSO_EUNSUPPORTED, "AMG: A P too large");
+        // Galerkin pattern: (I, J) for I in P row i, J in W row i; products for I <= J
+        std::vector<std::vector<int32_t>> cols(na);
+        std::vector<std::pair<std::pair<int32_t, int32_t>, std::pair<int32_t, int32_t>>> cprod;   // ((I,J),(Pb,Wb))
+        for (int i = 0; i < n; ++i)
+            for (int pb = F.p_ptr[i]; pb < F.p_ptr[i + 1]; ++pb)
+                for (int wb = F.w_ptr[i]; wb < F.w_ptr[i + 1]; ++wb) {
+                    const int I = F.p_col[pb], J = F.w_col[wb];
+                    cols[I].push_back(J);
+                    cols[J].push_back(I);
+                    if (I <= J) cprod.push_back({{I, J}, {pb, wb}});
+                }
+        A.host.emplace_back();
+        AmgHostLevel& F0 = A.host[l];   // (emplace_back may have moved it)
+        AmgHostLevel& C = A.host[l + 1];
+        C.n = na;
+        C.node_off = cnode_off;
+        C.rowptr.assign(na + 1, 0);
+        C.colind.clear();
+        for (int I = 0; I < na; ++I) {
+            auto& v = cols[I];
+            v.push_back(I);
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            C.colind.insert(C.colind.end(), v.begin(), v.end());
+            C.rowptr[I + 1] = (int32_t)C.colind.size();
+            std::vector<int32_t>().swap(v);
+        }
+        std::sort(cprod.begin(), cprod.end());
+        const int64_t nnzc = C.colind.size();
+        F0.c_cptr.assign(nnzc + 1, 0);
+        F0.c_p.clear();
+        F0.c_w.clear();
+        {
+            size_t t = 0;
+            for (int I = 0; I < na; ++I)
+                for (int k = C.rowptr[I]; k < C.rowptr[I + 1]; ++k) {
+                    const int J = C.colind[k];
+                    while (t < cprod.size() && cprod[t].first < std::make_pair(I, J)) ++t;   // (never: every product has its block)
+                    for (; t < cprod.size() && cprod[t].first == std::make_pair(I, J); ++t) {
+                        F0.c_p.push_back(cprod[t].second.first);
+                        F0.c_w.push_back(cprod[t].second.second);
+                    }
+                    F0.c_cptr[k + 1] = (int32_t)F0.c_p.size();
+                }
+        }
+        int rc = build_ell(ns, C.node_off.data(), C.rowptr, C.colind, C.ell);
+        if (rc) return rc;
+    }
+    A.nl = (int)A.host.size();
+    return JSSO_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// device: numeric set-up (FP64), after every assembly
+// ---------------------------------------------------------------------------
+// entry e (row-major r * 6 + c) of the FP64 block stored at value position vp
+// (grouped sliced-ELL layout, two entries per 16-byte group)
+__device__ __forceinline__ double ell64(const double* __restrict__ A, int64_t vp, int e) {
+    return A[(vp >> 6) * 36 * ROWS_PER_WG + ((int64_t)(e >> 1) * ROWS_PER_WG + (vp & 63)) * 2 + (e & 1)];
+}
+__device__ __forceinline__ int64_t ell64_index(int64_t vp, int e) {
+    return (vp >> 6) * 36 * ROWS_PER_WG + ((int64_t)(e >> 1) * ROWS_PER_WG + (vp & 63)) * 2 + (e & 1);
+}
+__device__ __forceinline__ int64_t ell32_index(int64_t vp, int e) {
+    return (vp >> 6) * 36 * ROWS_PER_WG + ((int64_t)(e >> 2) * ROWS_PER_WG + (vp & 63)) * 4 + (e & 3);
+}
+// entry (r, c) of the off-diagonal block A(i, j) given its reference
+// (2 * value position, + 1 when A(i, j) is the transpose of the stored block);
+// -1 is the unit diagonal block
+__device__ __forceinline__ double ablk(const double* __restrict__ A, int32_t aref, int r, int c) {
+    if (aref < 0) return r == c ? 1.0 : 0.0;
+    const int64_t vp = aref >> 1;
+    return (aref & 1) ? ell64(A, vp, c * 6 + r) : ell64(A, vp, r * 6 + c);
+}
+
+// Rigid-body modes of every aggregated fine node relative to its aggregate's
+// root, constrained rows zeroed, in the scaled variables: B_i = L_i^T M_i with
+// L_i^T = (L_i^-1)^T solved for (upper triangular).  Columns: translations x, y,
+// z, rotations about x, y, z; rows: the node's DOFs [ux, uy, uz, rx, ry, rz].
+__global__ __launch_bounds__(256) void k_amg_null0(int32_t n, const int32_t* __restrict__ agg,
+                                                   const int32_t* __restrict__ agg_ptr,
+                                                   const int32_t* __restrict__ agg_nodes,
+                                                   const double* __restrict__ crds, const double* __restrict__ Linv,
+                                                   const uint8_t* __restrict__ fixed, double* __restrict__ B) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double* out = B + (int64_t)i * 36;
+    const int a = agg[i];
+    if (a < 0) {
+#pragma unroll
+        for (int e = 0; e < 36; ++e) out[e] = 0.0;
+        return;
+    }
+    const int root = agg_nodes[agg_ptr[a]];
+    const double dx = crds[3 * (int64_t)i] - crds[3 * (int64_t)root];
+    const double dy = crds[3 * (int64_t)i + 1] - crds[3 * (int64_t)root + 1];
+    const double dz = crds[3 * (int64_t)i + 2] - crds[3 * (int64_t)root + 2];
+    double M[6][6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) M[r][c] = r == c ? 1.0 : 0.0;
+    // translations of a rotation e_a x d
+    M[1][3] = -dz; M[2][3] = dy;
+    M[0][4] = dz;  M[2][4] = -dx;
+    M[0][5] = -dy; M[1][5] = dx;
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+        if (fixed[6 * (int64_t)i + r])
+#pragma unroll
+            for (int c = 0; c < 6; ++c) M[r][c] = 0.0;
+    const double* Li = Linv + (int64_t)i * 36;
+    double U[6][6];   // U = Linv^T
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) U[r][c] = Li[c * 6 + r];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        double x[6];
+#pragma unroll
+        for (int r = 5; r >= 0; --r) {
+            double v = M[r][c];
+#pragma unroll
+            for (int t = r + 1; t < 6; ++t) v -= U[r][t] * x[t];
+            x[r] = v / U[r][r];
+        }
+#pragma unroll
+        for (int r = 0; r < 6; ++r) out[r * 6 + c] = x[r];
+    }
+}
+
+// Orthonormalise the stacked null-space rows of each aggregate (modified
+// Gram-Schmidt, two passes), one wavefront per aggregate with its 6 m rows
+// spread over the lanes (up to AMG_QR_RPL rows per lane): Q (per member node,
+// 6 x 6) and R (6 x 6, upper triangular, B = Q R).  A column that is dependent
+// on the previous ones (relative norm below 1e-10, e.g. translations of an
+// aggregate whose translations are all constrained) is dropped: zero in Q and
+// zero pivot in R.
+// Level 0 builds the rotations about the aggregate's first node (well
+// conditioned); the coarse null space must hold the same global modes in every
+// aggregate, so R is then referred to one origin per structure (its first node
+// x_o): a rotation about x_o is the local one plus the translation e_a x d,
+// d = x_root - x_o, i.e. R <- R T with T = [[I, S], [0, I]], S e_a = e_a x d.
+// (crds == nullptr on coarser levels, whose null space is already global.)
+__global__ __launch_bounds__(64) void k_amg_qr(int32_t na, const int32_t* __restrict__ agg_ptr,
+                                               const int32_t* __restrict__ agg_nodes, const double* __restrict__ B,
+                                               const double* __restrict__ crds, const int32_t* __restrict__ node_sys,
+                                               const int32_t* __restrict__ node_off, double* __restrict__ Q,
+                                               double* __restrict__ R) {
+    const int a = blockIdx.x;
+    if (a >= na) return;
+    const int lane = threadIdx.x;
+    const int m0 = agg_ptr[a], nrow = 6 * (agg_ptr[a + 1] - m0);
+    double v[AMG_QR_RPL][6];
+    int64_t off[AMG_QR_RPL];
+#pragma unroll
+    for (int t = 0; t < AMG_QR_RPL; ++t) {
+        const int rho = lane + 64 * t;
+        off[t] = rho < nrow ? (int64_t)agg_nodes[m0 + rho / 6] * 36 + (rho % 6) * 6 : -1;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[t][k] = off[t] >= 0 ? B[off[t] + k] : 0.0;
+    }
+    double Rl[6][6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) Rl[r][c] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        double n0 = 0.0;
+#pragma unroll
+        for (int t = 0; t < AMG_QR_RPL; ++t) n0 += v[t][k] * v[t][k];
+        n0 = wave_sum(n0);
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass)
+#pragma unroll
+            for (int j = 0; j < k; ++j) {
+                double d = 0.0;
+#pragma unroll
+                for (int t = 0; t < AMG_QR_RPL; ++t) d += v[t][j] * v[t][k];
+                d = wave_sum(d);
+#pragma unroll
+                for (int t = 0; t < AMG_QR_RPL; ++t) v[t][k] -= d * v[t][j];
+                Rl[j][k] += d;
+            }
+        double nk = 0.0;
+#pragma unroll
+        for (int t = 0; t < AMG_QR_RPL; ++t) nk += v[t][k] * v[t][k];
+        nk = sqrt(wave_sum(nk));
+        const bool drop = !(nk > 1e-10 * sqrt(n0));
+        const double sc = drop ? 0.0 : 1.0 / nk;
+#pragma unroll
+        for (int t = 0; t < AMG_QR_RPL; ++t) v[t][k] *= sc;
+        Rl[k][k] = drop ? 0.0 : nk;
+    }
+    if (crds) {
+        const int64_t root = agg_nodes[m0];
+        const int64_t o = node_off[node_sys[root]];
+        const double dx = crds[3 * root] - crds[3 * o], dy = crds[3 * root + 1] - crds[3 * o + 1],
+                     dz = crds[3 * root + 2] - crds[3 * o + 2];
+        const double S[3][3] = {{0.0, dz, -dy}, {-dz, 0.0, dx}, {dy, -dx, 0.0}};
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                double t = 0.0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) t += Rl[r][k] * S[k][a];
+                Rl[r][3 + a] += t;
+            }
+    }
+#pragma unroll
+    for (int t = 0; t < AMG_QR_RPL; ++t)
+        if (off[t] >= 0)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) Q[off[t] + k] = v[t][k];
+    if (lane < 36) {
+        double val = 0.0;
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                if (r * 6 + c == lane) val = Rl[r][c];
+        R[(int64_t)a * 36 + lane] = val;
+    }
+}
+
+// deterministic start vector in [-1, 1) for the power iteration
+__device__ __forceinline__ double amg_hash(int64_t dof) {
+    uint32_t u = (uint32_t)dof * 2654435761u;
+    u ^= u >> 15;
+    u *= 2246822519u;
+    u ^= u >> 13;
+    return (double)(u >> 8) * (2.0 / 16777216.0) - 1.0;
+}
+
+// One power-iteration step y = A v on the slices of a level (lane = block row;
+// the FP32 copy of the matrix: only the largest eigenvalue is wanted): step 0
+// takes the hashed start vector, later steps v = y_prev / |y_prev| of their
+// structure (from the per-slice partials of the previous step).  Writes y and
+// this slice's |y|^2.  Structures without a coarser level are skipped.
+template <int WPS>
+__global__ __launch_bounds__(64 * WPS) void k_amg_power(Slices sl, const float* __restrict__ A, int step,
+                                                        const double* __restrict__ vin, double* __restrict__ vout,
+                                                        double* __restrict__ part, const uint8_t* __restrict__ cont,
+                                                        int32_t n_wg) {
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
+    const int4 ma = sl.meta[3 * w], mb = sl.meta[3 * w + 1], mc = sl.meta[3 * w + 2];
+    if (!cont[ma.x]) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double sc = 1.0;
+    if (step > 0) {
+        const double* pp = part + (int64_t)((step - 1) & 1) * n_wg;
+        double t = 0.0;
+        for (int q = mc.x + lane; q < mc.y; q += 64) t += pp[q];
+        t = wave_sum(t);
+        sc = t > 0.0 ? 1.0 / sqrt(t) : 0.0;
+    }
+    auto v = [&](int64_t node, int c, int) { return step == 0 ? amg_hash(6 * node + c) : vin[6 * node + c] * sc; };
+    const int64_t i = (int64_t)ma.y + lane;
+    const bool on = i < ma.z;
+    const int64_t ii = on ? i : (int64_t)ma.y;
+    double acc[6][1];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[c][0] = wave == 0 ? v(ii, c, 0) : 0.0;
+    ell_apply<float, double, 1, WPS>(sl, A, w, ma, mb, acc, v);
+    if (wave != 0) return;
+    double yy = 0.0;
+    if (on) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            vout[6 * i + c] = acc[c][0];
+            yy += acc[c][0] * acc[c][0];
+        }
+    }
+    yy = wave_sum(yy);
+    if (lane == 0) part[(int64_t)(step & 1) * n_wg + w] = yy;
+}
+
+// rho = safety * |A v| (|v| = 1) per continuing structure; omega = 4/(3 rho);
+// theta = (rho + rho/30) / 2
+__global__ void k_amg_theta(int32_t ns, const int32_t* __restrict__ sys_wg_off, const double* __restrict__ part,
+                            int last, const uint8_t* __restrict__ cont, int32_t n_wg, double* __restrict__ rho,
+                            double* __restrict__ omega, float* __restrict__ theta) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns || !cont[s]) return;
+    const double* pp = part + (int64_t)(last & 1) * n_wg;
+    double t = 0.0;
+    for (int q = sys_wg_off[s]; q < sys_wg_off[s + 1]; ++q) t += pp[q];
+    double r = AMG_RHO_SAFETY * sqrt(t);
+    if (!(r > 0.0) || !isfinite(r)) r = 1.0;
+    rho[s] = r;
+    omega[s] = 4.0 / (3.0 * r);
+    theta[s] = (float)(0.5 * (r + r / 30.0));
+}
+
+// P(i, J) = sum over its products of ((1 - omega) delta_ij I - omega A_off(i, j)) Q_j;
+// thread per (block, row r)
+__global__ __launch_bounds__(256) void k_amg_psmooth(int64_t nP, const int32_t* __restrict__ p_row,
+                                                     const int32_t* __restrict__ p_cptr,
+                                                     const int32_t* __restrict__ p_cj,
+                                                     const int32_t* __restrict__ p_caref,
+                                                     const int32_t* __restrict__ node_sys,
+                                                     const double* __restrict__ omega, const double* __restrict__ A,
+                                                     const double* __restrict__ Q, double* __restrict__ P) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nP * 6) return;
+    const int64_t b = t / 6;
+    const int r = (int)(t % 6);
+    const double om = omega[node_sys[p_row[b]]];
+    double out[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q = p_cptr[b]; q < p_cptr[b + 1]; ++q) {
+        const double* Qj = Q + (int64_t)p_cj[q] * 36;
+        const int32_t aref = p_caref[q];
+        double a[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) a[m] = aref < 0 ? (m == r ? 1.0 - om : 0.0) : -om * ablk(A, aref, r, m);
+#pragma unroll
+        for (int m = 0; m < 6; ++m)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) out[c] += a[m] * Qj[m * 6 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) P[b * 36 + r * 6 + c] = out[c];
+}
+
+// W(i, J) = sum over its products of A(i, j) P(j, J); thread per (block, row r)
+__global__ __launch_bounds__(256) void k_amg_w(int64_t nW, const int32_t* __restrict__ w_cptr,
+                                               const int32_t* __restrict__ w_caref, const int32_t* __restrict__ w_cp,
+                                               const double* __restrict__ A, const double* __restrict__ P,
+                                               double* __restrict__ W) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nW * 6) return;
+    const int64_t b = t / 6;
+    const int r = (int)(t % 6);
+    double out[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q = w_cptr[b]; q < w_cptr[b + 1]; ++q) {
+        const double* Pb = P + (int64_t)w_cp[q] * 36;
+        const int32_t aref = w_caref[q];
+        double a[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) a[m] = ablk(A, aref, r, m);
+#pragma unroll
+        for (int m = 0; m < 6; ++m)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) out[c] += a[m] * Pb[m * 6 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) W[b * 36 + r * 6 + c] = out[c];
+}
+
+// A_c(I, J) = sum over its products of P(i, I)^T W(i, J) (blocks with I <= J);
+// thread per (block, row r)
+__global__ __launch_bounds__(256) void k_amg_galerkin(int64_t nnzc, const int32_t* __restrict__ c_cptr,
+                                                      const int32_t* __restrict__ c_p, const int32_t* __restrict__ c_w,
+                                                      const double* __restrict__ P, const double* __restrict__ W,
+                                                      double* __restrict__ Ac) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nnzc * 6) return;
+    const int64_t k = t / 6;
+    const int r = (int)(t % 6);
+    double out[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int q = c_cptr[k]; q < c_cptr[k + 1]; ++q) {
+        const double* Pb = P + (int64_t)c_p[q] * 36;
+        const double* Wb = W + (int64_t)c_w[q] * 36;
+        double pc[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) pc[m] = Pb[m * 6 + r];
+#pragma unroll
+        for (int m = 0; m < 6; ++m)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) out[c] += pc[m] * Wb[m * 6 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) Ac[k * 36 + r * 6 + c] = out[c];
+}
+
+// Cholesky factor of each coarse diagonal block and its inverse.  A pivot that
+// is not positive (a null-space column dropped by the QR leaves a zero row and
+// column) is taken as a decoupled unknown: unit pivot, zero column.
+__global__ __launch_bounds__(128) void k_amg_cfactor(int32_t nc, const int32_t* __restrict__ cdiag,
+                                                     const double* __restrict__ Ac, double* __restrict__ Lc,
+                                                     double* __restrict__ Lci) {
+    const int J = blockIdx.x * blockDim.x + threadIdx.x;
+    if (J >= nc) return;
+    const double* a = Ac + (int64_t)cdiag[J] * 36;
+    double A[6][6], L[6][6], Li[6][6];
+    double dmax = 0.0;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            A[r][c] = 0.5 * (a[r * 6 + c] + a[c * 6 + r]);
+            L[r][c] = 0.0;
+            Li[r][c] = 0.0;
+        }
+        dmax = fmax(dmax, fabs(A[r][r]));
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        double d = A[j][j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
+        const bool zero = !(d > 1e-13 * dmax);
+        const double ljj = zero ? 1.0 : sqrt(d);
+        L[j][j] = ljj;
+#pragma unroll
+        for (int r = j + 1; r < 6; ++r) {
+            double s = A[r][j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) s -= L[r][k] * L[j][k];
+            L[r][j] = zero ? 0.0 : s / ljj;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        Li[j][j] = 1.0 / L[j][j];
+#pragma unroll
+        for (int r = j + 1; r < 6; ++r) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = j; k < r; ++k) s -= L[r][k] * Li[k][j];
+            Li[r][j] = s / L[r][r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            Lc[(int64_t)J * 36 + r * 6 + c] = L[r][c];
+            Lci[(int64_t)J * 36 + r * 6 + c] = Li[r][c];
+        }
+}
+
+// Scaled coarse operator into the coarse level's sliced ELL (FP64 and FP32):
+// As(I, J) = Lc_I^-1 A_c(I, J) Lc_J^-T for every stored (upper) block.
+__global__ __launch_bounds__(256) void k_amg_cscale(int64_t nval, const int32_t* __restrict__ ell_src,
+                                                    const int32_t* __restrict__ browof,
+                                                    const int32_t* __restrict__ colind, const double* __restrict__ Ac,
+                                                    const double* __restrict__ Lci, double* __restrict__ A64,
+                                                    float* __restrict__ A32) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nval * 36) return;
+    const int64_t vp = t / 36;
+    const int e = (int)(t % 36), r = e / 6, c = e % 6;
+    const int32_t src = ell_src[vp];
+    double v = 0.0;
+    if (src >= 0) {
+        const double* a = Ac + (int64_t)src * 36;
+        const double* li = Lci + (int64_t)browof[src] * 36;
+        const double* lj = Lci + (int64_t)colind[src] * 36;
+#pragma unroll
+        for (int p = 0; p < 6; ++p) {
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) s += a[p * 6 + q] * lj[c * 6 + q];
+            v += li[r * 6 + p] * s;
+        }
+    }
+    A64[ell64_index(vp, e)] = v;
+    A32[ell32_index(vp, e)] = (float)v;
+}
+
+// P scaled to the coarse variables (FP32 for the V-cycle): Ps(i, J) = P(i, J) Lc_J^-T,
+// by fine row (prolongation) and transposed by coarse node (restriction)
+__global__ __launch_bounds__(256) void k_amg_pscale(int64_t nP, const int32_t* __restrict__ p_col,
+                                                    const int32_t* __restrict__ p_tpos,
+                                                    const double* __restrict__ P, const double* __restrict__ Lci,
+                                                    float* __restrict__ P32, float* __restrict__ PT32) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nP * 36) return;
+    const int64_t b = t / 36;
+    const int e = (int)(t % 36), r = e / 6, c = e % 6;
+    const double* p = P + b * 36;
+    const double* l = Lci + (int64_t)p_col[b] * 36;
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) s += p[r * 6 + m] * l[c * 6 + m];
+    P32[t] = (float)s;
+    PT32[(int64_t)p_tpos[b] * 36 + c * 6 + r] = (float)s;   // per coarse node, block transposed
+}
+
+// coarse null space in the scaled coarse variables: B_J = Lc_J^T R_J
+__global__ __launch_bounds__(256) void k_amg_nullc(int32_t nc, const double* __restrict__ Lc,
+                                                   const double* __restrict__ R, double* __restrict__ B) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nc * 36) return;
+    const int64_t J = t / 36;
+    const int e = (int)(t % 36), r = e / 6, c = e % 6;
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) s += Lc[J * 36 + m * 6 + r] * R[J * 36 + m * 6 + c];
+    B[t] = s;
+}
+
+// Dense inverse of a structure's coarsest scaled operator (N = 6 nodes <= 96):
+// the matrix (unit diagonal blocks plus the stored off-diagonal blocks) is
+// factored in LDS (Cholesky), then each thread solves for one column of the
+// inverse (L y = e_c, L^T x = y, in the FP64 scratch) and stores it in FP32,
+// column-major.
+constexpr int AMG_DENSE_MAX = 6 * AMG_MAXC;
+__global__ __launch_bounds__(256) void k_amg_dense(const int32_t* __restrict__ list, const int32_t* __restrict__ term,
+                                                   const int32_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ colind,
+                                                   const int32_t* __restrict__ kref, const double* __restrict__ A,
+                                                   double* __restrict__ scratch, float* __restrict__ inv) {
+    __shared__ double M[AMG_DENSE_MAX * AMG_DENSE_MAX];
+    const int s = list[blockIdx.x];
+    const int node0 = term[4 * s + 1], nt = term[4 * s + 2];
+    const int64_t off = term[4 * s + 3];
+    const int N = 6 * nt;
+    for (int u = threadIdx.x; u < N * N; u += blockDim.x) {
+        const int R = u / N, C = u % N;
+        const int i = node0 + R / 6, j = node0 + C / 6;
+        double v = 0.0;
+        if (i == j) {
+            v = (R % 6) == (C % 6) ? 1.0 : 0.0;
+        } else {
+            for (int k = rowptr[i]; k < rowptr[i + 1]; ++k)
+                if (colind[k] == j) v = ablk(A, kref[k], R % 6, C % 6);
+        }
+        M[R * N + C] = v;
+    }
+    __syncthreads();
+    for (int j = 0; j < N; ++j) {
+        const double d = M[j * N + j];
+        const double ljj = d > 0.0 ? sqrt(d) : 1.0;
+        __syncthreads();
+        for (int r = j + 1 + threadIdx.x; r < N; r += blockDim.x) M[r * N + j] /= ljj;
+        if (threadIdx.x == 0) M[j * N + j] = ljj;
+        __syncthreads();
+        const int m = N - j - 1;
+        for (int u = threadIdx.x; u < m * m; u += blockDim.x) {
+            const int r = j + 1 + u / m, c = j + 1 + u % m;
+            if (c <= r) M[r * N + c] -= M[r * N + j] * M[c * N + j];
+        }
+        __syncthreads();
+    }
+    for (int c = threadIdx.x; c < N; c += blockDim.x) {
+        double* y = scratch + off + (int64_t)c * N;
+        for (int r = 0; r < N; ++r) {
+            double v = r == c ? 1.0 : 0.0;
+            for (int t = 0; t < r; ++t) v -= M[r * N + t] * y[t];
+            y[r] = v / M[r * N + r];
+        }
+        for (int r = N - 1; r >= 0; --r) {
+            double v = y[r];
+            for (int t = r + 1; t < N; ++t) v -= M[t * N + r] * y[t];
+            y[r] = v / M[r * N + r];
+        }
+        for (int r = 0; r < N; ++r) inv[off + (int64_t)c * N + r] = (float)y[r];
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// device: the preconditioned CG iteration and the V-cycle
+// ---------------------------------------------------------------------------
+// Partial slots of the AMG iteration k (per slice, in P.cg[k & 1]): CG_RR holds
+// <r_k, z_k>, CG_RQ holds <r_{k-1}, z_k>, CG_PQ holds <p_k, q_k>.  The
+// direction uses the Polak-Ribiere (flexible CG) coefficient
+//     beta_k = (<r_{k-1}, z_{k-1}> - <r_{k-2}, z_{k-1}>) / <r_{k-2}, z_{k-2}>, clipped at 0,
+// which keeps the iteration convergent with the FP32 V-cycle (a preconditioner
+// that is symmetric only to FP32 rounding) and across residual replacements.
+
+// Waves-per-slice variants whose cross-wave LDS buffer stays <= 24 KB.
+template <typename VT, int NR, int WPS>
+struct AmgWpsFits {
+    static constexpr bool value = WPS == 1 || (size_t)(WPS - 1) * 6 * NR * 64 * sizeof(VT) <= 24576;
+};
+
+// Totals over the slices [s0, s1) of one structure of slot `slot` of cg, in one
+// wavefront (every lane holds the same butterfly total).
+template <int NR>
+__device__ __forceinline__ void wave_totals(const double* __restrict__ cg, int slot, int s0, int s1,
+                                            double (&t)[NR]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int h = 0; h < NR; ++h) t[h] = 0.0;
+    for (int q = s0 + lane; q < s1; q += 64)
+#pragma unroll
+        for (int h = 0; h < NR; ++h) t[h] += cg[((int64_t)q * 4 + slot) * NR + h];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) t[h] = wave_sum(t[h]);
+}
+
+// acc[c][h] += sum over the slice's ELL slots of A_off(i, j) x(j), x(j) given
+// by `xj(col, c, h)` (lane = block row).  With WPS wavefronts per slice, wave v
+// takes slots v, v + WPS, ... and waves 1..WPS-1 hand their sums to wave 0
+// through LDS, which adds them in wave order: only wave 0's acc is the result
+// (the caller puts the own-row term into wave 0's acc beforehand).  The next
+// slot's column and value position are requested before the current slot's
+// products; a slot past the end reads the slice's metadata word instead.
+template <typename MT, typename VT, int NR, int WPS, typename F>
+__device__ __forceinline__ void ell_apply(const Slices& sl, const MT* __restrict__ A, int w, const int4& ma,
+                                          const int4& mb, VT (&acc)[6][NR], F&& xj) {
+    __shared__ VT red[WPS > 1 ? WPS - 1 : 1][6 * NR][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nup = mb.y, nslot = mb.y + mb.w;
+    const MT* Aw = A + 36 * (int64_t)ma.w;
+    const int32_t* dummy = (const int32_t*)&sl.meta[3 * w];
+    auto cpos = [&](int jj) -> int64_t {
+        return jj < nup ? (int64_t)mb.x + (int64_t)jj * ROWS_PER_WG + lane
+                        : (int64_t)mb.z + (int64_t)(jj - nup) * ROWS_PER_WG + lane;
+    };
+    int j = wave;
+    int64_t col = *(j < nslot ? sl.ell_col + cpos(j) : dummy);
+    int vp = *(j < nslot ? sl.ell_vpos + cpos(j) : dummy);
+    for (; j < nslot; j += WPS) {
+        VT x[6][NR];
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int h = 0; h < NR; ++h) x[c][h] = xj(col, c, h);
+        const bool upper = j < nup;
+        MT av[36];
+        if (upper)
+            load_block(Aw + (int64_t)j * 36 * ROWS_PER_WG, lane, av);
+        else
+            load_block(A + (int64_t)(vp >> 6) * 36 * ROWS_PER_WG, vp & (ROWS_PER_WG - 1), av);
+        const int jn = j + WPS;
+        const int64_t coln = *(jn < nslot ? sl.ell_col + cpos(jn) : dummy);
+        const int vpn = *(jn < nslot ? sl.ell_vpos + cpos(jn) : dummy);
+        if (upper) {
+#pragma unroll
+            for (int e = 0; e < 36; ++e)
+#pragma unroll
+                for (int h = 0; h < NR; ++h) acc[e / 6][h] += (VT)av[e] * x[e % 6][h];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 36; ++e)
+#pragma unroll
+                for (int h = 0; h < NR; ++h) acc[e % 6][h] += (VT)av[e] * x[e / 6][h];
+        }
+        col = coln;
+        vp = vpn;
+    }
+    if (WPS > 1) {
+        if (wave > 0) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+#pragma unroll
+                for (int h = 0; h < NR; ++h) red[wave - 1][c * NR + h][lane] = acc[c][h];
+        }
+        __syncthreads();
+        if (wave == 0) {
+#pragma unroll
+            for (int u = 0; u < WPS - 1; ++u)
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+#pragma unroll
+                    for (int h = 0; h < NR; ++h) acc[c][h] += red[u][c * NR + h][lane];
+        }
+    }
+}
+
+// Direction and product of iteration k: beta from the partials of k - 1 and
+// k - 2; p_k = z + beta p_{k-1} (own and gathered rows); q = A p_k; partials <p_k, q>.
+template <typename MT, typename VT, int NR, int WPS>
+__global__ __launch_bounds__(64 * WPS) void k_pcg_dir(Slices sl, Partials P, SysState st, int k,
+                                                      const MT* __restrict__ A, const VT* __restrict__ z,
+                                                      const VT* __restrict__ pprev, VT* __restrict__ pcur,
+                                                      VT* __restrict__ q) {
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
+    const int amask = st.slice_act[w];
+    if (!amask) return;
+    const int4 ma = sl.meta[3 * w], mb = sl.meta[3 * w + 1], mc = sl.meta[3 * w + 2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i = (int64_t)ma.y + lane;
+    const bool on = i < ma.z;
+    const int64_t ii = on ? i : (int64_t)ma.y;
+    double t1[NR], t2[NR], t0[NR];
+    wave_totals<NR>(P.cg[(k - 1) & 1], CG_RR, mc.x, mc.y, t1);
+    wave_totals<NR>(P.cg[(k - 1) & 1], CG_RQ, mc.x, mc.y, t2);
+    wave_totals<NR>(P.cg[k & 1], CG_RR, mc.x, mc.y, t0);
+    VT beta[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+        const double b = t0[h] > 0.0 ? (t1[h] - t2[h]) / t0[h] : 0.0;
+        beta[h] = (isfinite(b) && b > 0.0) ? VT(b) : VT(0);
+    }
+    auto pk = [&](int64_t node, int c, int h) {
+        const int64_t o = (6 * node + c) * NR + h;
+        return z[o] + beta[h] * pprev[o];
+    };
+    VT acc[6][NR];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? pk(ii, c, h) : VT(0);
+    ell_apply<MT, VT, NR, WPS>(sl, A, w, ma, mb, acc, pk);
+    if (wave != 0) return;
+    double v[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) v[h] = 0.0;
+    if (on) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int h = 0; h < NR; ++h) {
+                const int64_t o = (6 * i + c) * NR + h;
+                if ((amask >> h) & 1) {
+                    const VT pn = pk(i, c, h);
+                    pcur[o] = pn;
+                    q[o] = acc[c][h];
+                    v[h] += (double)pn * (double)acc[c][h];
+                } else {
+                    pcur[o] = pprev[o];
+                }
+            }
+    }
+#pragma unroll
+    for (int h = 0; h < NR; ++h) v[h] = wave_sum(v[h]);
+    if (lane == 0) {
+#pragma unroll
+        for (int h = 0; h < NR; ++h) P.cg[k & 1][((int64_t)w * 4 + CG_PQ) * NR + h] = v[h];
+    }
+}
+
+// Update of iteration k and the V-cycle's first smoothing step on level 0:
+// alpha = <r,z>_{k-1} / <p,q>_k; r_k = r_{k-1} - alpha q (own and gathered
+// rows); x += alpha p_k; t0 = r_k - A (r_k / theta).  FRESH: r_k is given (a
+// replacement or the start), no update.
+template <typename MT, typename VT, int NR, bool FRESH, int WPS>
+__global__ __launch_bounds__(64 * WPS) void k_pcg_pre(Slices sl, Partials P, SysState st, int k,
+                                                      const MT* __restrict__ A, const float* __restrict__ theta,
+                                                      const VT* __restrict__ rprev, VT* __restrict__ rcur,
+                                                      const VT* __restrict__ q, const VT* __restrict__ p,
+                                                      VT* __restrict__ xlo, VT* __restrict__ t0) {
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
+    const int amask = st.slice_act[w];
+    if (!amask) return;
+    const int4 ma = sl.meta[3 * w], mb = sl.meta[3 * w + 1], mc = sl.meta[3 * w + 2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i = (int64_t)ma.y + lane;
+    const bool on = i < ma.z;
+    const int64_t ii = on ? i : (int64_t)ma.y;
+    VT alpha[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) alpha[h] = VT(0);
+    if (!FRESH) {
+        double rz[NR], pq[NR];
+        wave_totals<NR>(P.cg[(k - 1) & 1], CG_RR, mc.x, mc.y, rz);
+        wave_totals<NR>(P.cg[k & 1], CG_PQ, mc.x, mc.y, pq);
+#pragma unroll
+        for (int h = 0; h < NR; ++h)
+            alpha[h] = (pq[h] > 0.0 && isfinite(pq[h]) && rz[h] > 0.0) ? VT(rz[h] / pq[h]) : VT(0);
+    }
+    auto rk = [&](int64_t node, int c, int h) {
+        const int64_t o = (6 * node + c) * NR + h;
+        return FRESH ? rcur[o] : rprev[o] - alpha[h] * q[o];
+    };
+    const VT th = (VT)theta[ma.x];
+    VT acc[6][NR];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? rk(ii, c, h) : VT(0);
+    ell_apply<MT, VT, NR, WPS>(sl, A, w, ma, mb, acc, rk);
+    if (wave != 0 || !on) return;
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int h = 0; h < NR; ++h) {
+            const int64_t o = (6 * i + c) * NR + h;
+            const VT r = rk(i, c, h);
+            t0[o] = r - acc[c][h] / th;
+            if (!FRESH) {
+                const bool run = (amask >> h) & 1;
+                rcur[o] = run ? r : rprev[o];
+                if (run) xlo[o] = xlo[o] + alpha[h] * p[o];
+            }
+        }
+}
+
+// Last smoothing step on level 0: z = z1 + (r - A z1) / theta, with z1 (in
+// t0) the prolongated correction; partials <r_k, z> and <r_{k-1}, z>.
+template <typename MT, typename VT, int NR, int WPS>
+__global__ __launch_bounds__(64 * WPS) void k_pcg_post(Slices sl, Partials P, SysState st, int k,
+                                                       const MT* __restrict__ A, const float* __restrict__ theta,
+                                                       const VT* __restrict__ r, const VT* __restrict__ rprev,
+                                                       const VT* __restrict__ z1, VT* __restrict__ z) {
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
+    const int amask = st.slice_act[w];
+    if (!amask) return;
+    const int4 ma = sl.meta[3 * w], mb = sl.meta[3 * w + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t i = (int64_t)ma.y + lane;
+    const bool on = i < ma.z;
+    const int64_t ii = on ? i : (int64_t)ma.y;
+    auto zz = [&](int64_t node, int c, int h) { return z1[(6 * node + c) * NR + h]; };
+    const VT th = (VT)theta[ma.x];
+    VT acc[6][NR];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? zz(ii, c, h) : VT(0);
+    ell_apply<MT, VT, NR, WPS>(sl, A, w, ma, mb, acc, zz);
+    if (wave != 0) return;
+    double v[NR], vp[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) v[h] = vp[h] = 0.0;
+    if (on) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int h = 0; h < NR; ++h) {
+                if (!((amask >> h) & 1)) continue;
+                const int64_t o = (6 * i + c) * NR + h;
+                const VT rv = r[o];
+                const VT zn = zz(i, c, h) + (rv - acc[c][h]) / th;
+                z[o] = zn;
+                v[h] += (double)rv * (double)zn;
+                if (k > 0) vp[h] += (double)rprev[o] * (double)zn;   // (no r_{-1} at the start)
+            }
+    }
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+        v[h] = wave_sum(v[h]);
+        vp[h] = wave_sum(vp[h]);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int h = 0; h < NR; ++h) {
+            P.cg[k & 1][((int64_t)w * 4 + CG_RR) * NR + h] = v[h];
+            P.cg[k & 1][((int64_t)w * 4 + CG_RQ) * NR + h] = vp[h];
+        }
+    }
+}
+
+// Update of the iteration that closes a chunk, folded into the FP64 solution:
+// alpha as above; x64 += xlo + alpha p_k; xlo = 0.  (r is replaced next.)
+template <typename VT, int NR>
+__global__ __launch_bounds__(SOLVER_WG) void k_pcg_fold(Slices sl, Partials P, SysState st, int k,
+                                                        const VT* __restrict__ p, VT* __restrict__ xlo,
+                                                        double* __restrict__ x64) {
+    __shared__ double lds[NWAVES * 2 * NR];
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
+    const int sys = sl.sys[w];
+    if (structure_done<NR>(st, sys)) return;
+    const int s0 = sl.sys_off[sys], s1 = sl.sys_off[sys + 1];
+    const double* rzp = P.cg[(k - 1) & 1];
+    const double* pqp = P.cg[k & 1];
+    double v[2 * NR];
+#pragma unroll
+    for (int a = 0; a < 2 * NR; ++a) v[a] = 0.0;
+    for (int q = s0 + (int)threadIdx.x; q < s1; q += SOLVER_WG) {
+#pragma unroll
+        for (int h = 0; h < NR; ++h) {
+            v[h] += rzp[((int64_t)q * 4 + CG_RR) * NR + h];
+            v[NR + h] += pqp[((int64_t)q * 4 + CG_PQ) * NR + h];
+        }
+    }
+    group_sum<2 * NR, NWAVES>(v, lds);
+    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
+    if (R >= 6 * (int64_t)sl.row1[w]) return;
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+        if (!st.active[sys * NR + h]) continue;
+        const double rz = v[h], pq = v[NR + h];
+        const VT alpha = (pq > 0.0 && isfinite(pq) && rz > 0.0) ? VT(rz / pq) : VT(0);
+        const int64_t o = R * NR + h;
+        x64[o] += (double)(xlo[o] + alpha * p[o]);
+        xlo[o] = VT(0);
+    }
+}
+
+// Coarse level, first smoothing step: t = b - A (b / theta).
+template <int NR, int WPS>
+__global__ __launch_bounds__(64 * WPS) void k_mg_pre(Slices sl, const float* __restrict__ A,
+                                                     const float* __restrict__ theta, const uint8_t* __restrict__ cont,
+                                                     SysState st, const float* __restrict__ b,
+                                                     float* __restrict__ t) {
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
+    const int4 ma = sl.meta[3 * w], mb = sl.meta[3 * w + 1];
+    if (!cont[ma.x] || structure_done<NR>(st, ma.x)) return;
+    const int wave = threadIdx.x >> 6;
+    const int64_t i = (int64_t)ma.y + (threadIdx.x & 63);
+    const bool on = i < ma.z;
+    const int64_t ii = on ? i : (int64_t)ma.y;
+    auto bb = [&](int64_t node, int c, int h) { return b[(6 * node + c) * NR + h]; };
+    const float th = theta[ma.x];
+    float acc[6][NR];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? bb(ii, c, h) : 0.f;
+    ell_apply<float, float, NR, WPS>(sl, A, w, ma, mb, acc, bb);
+    if (wave != 0 || !on) return;
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int h = 0; h < NR; ++h) t[(6 * i + c) * NR + h] = bb(i, c, h) - acc[c][h] / th;
+}
+
+// Coarse level, last smoothing step: x = z1 + (b - A z1) / theta (z1 in t).
+template <int NR, int WPS>
+__global__ __launch_bounds__(64 * WPS) void k_mg_post(Slices sl, const float* __restrict__ A,
+                                                      const float* __restrict__ theta, const uint8_t* __restrict__ cont,
+                                                      SysState st, const float* __restrict__ b,
+                                                      const float* __restrict__ z1, float* __restrict__ x) {
+    const int w = slice_of_block(sl);
+    if (w < 0) return;
+    const int4 ma = sl.meta[3 * w], mb = sl.meta[3 * w + 1];
+    if (!cont[ma.x] || structure_done<NR>(st, ma.x)) return;
+    const int wave = threadIdx.x >> 6;
+    const int64_t i = (int64_t)ma.y + (threadIdx.x & 63);
+    const bool on = i < ma.z;
+    const int64_t ii = on ? i : (int64_t)ma.y;
+    auto zz = [&](int64_t node, int c, int h) { return z1[(6 * node + c) * NR + h]; };
+    const float th = theta[ma.x];
+    float acc[6][NR];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? zz(ii, c, h) : 0.f;
+    ell_apply<float, float, NR, WPS>(sl, A, w, ma, mb, acc, zz);
+    if (wave != 0 || !on) return;
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int h = 0; h < NR; ++h) {
+            const int64_t o = (6 * i + c) * NR + h;
+            x[o] = zz(i, c, h) + (b[o] - acc[c][h]) / th;
+        }
+}
+
+// sum over the 4 lanes of an aligned quad (butterfly: all four hold the same bits)
+__device__ __forceinline__ float quad_sum(float v) {
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+    return v;
+}
+
+// Restriction b_c = P^T t: four lanes per coarse DOF row (J, c), each taking
+// every fourth block of the column (stored transposed and contiguous per
+// coarse node: PT[q][c][r]), then a quad sum.
+template <typename VT, int NR>
+__global__ __launch_bounds__(256) void k_mg_restrict(int32_t nc, const int32_t* __restrict__ node_sys_c,
+                                                     const int32_t* __restrict__ pt_ptr,
+                                                     const int32_t* __restrict__ pt_row, const float* __restrict__ PT,
+                                                     SysState st, const VT* __restrict__ t, float* __restrict__ bc) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = (int)(u & 3);
+    const int64_t jc = u >> 2;
+    const bool valid = jc < 6 * (int64_t)nc;
+    const int J = valid ? (int)(jc / 6) : 0, c = (int)(jc % 6);
+    const bool run = valid && !structure_done<NR>(st, node_sys_c[J]);
+    float acc[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) acc[h] = 0.f;
+    if (run) {
+        const int q1 = pt_ptr[J + 1];
+        for (int q = pt_ptr[J] + g; q < q1; q += 4) {
+            const int64_t i = pt_row[q];
+            const float* pr = PT + (int64_t)q * 36 + c * 6;
+            float pv[6];
+#pragma unroll
+            for (int r = 0; r < 6; ++r) pv[r] = pr[r];
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+#pragma unroll
+                for (int h = 0; h < NR; ++h) acc[h] += pv[r] * (float)t[(6 * i + r) * NR + h];
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < NR; ++h) acc[h] = quad_sum(acc[h]);
+    if (run && g == 0) {
+#pragma unroll
+        for (int h = 0; h < NR; ++h) bc[jc * NR + h] = acc[h];
+    }
+}
+
+// Prolongation and the smoothing step before it: z1 = b / theta + P x_c,
+// thread per fine DOF row (i, r) of the structures that have a coarser level
+template <typename VT, int NR>
+__global__ __launch_bounds__(256) void k_mg_prolong(int32_t n, const int32_t* __restrict__ node_sys,
+                                                    const uint8_t* __restrict__ cont, const int32_t* __restrict__ p_ptr,
+                                                    const int32_t* __restrict__ p_col, const float* __restrict__ P,
+                                                    const float* __restrict__ theta, SysState st,
+                                                    const VT* __restrict__ b, const float* __restrict__ xc,
+                                                    VT* __restrict__ z1) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= 6 * (int64_t)n) return;
+    const int i = (int)(u / 6), r = (int)(u % 6);
+    const int s = node_sys[i];
+    if (!cont[s] || structure_done<NR>(st, s)) return;
+    const float th = theta[s];
+    float acc[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) acc[h] = (float)b[u * NR + h] / th;
+    for (int q = p_ptr[i]; q < p_ptr[i + 1]; ++q) {
+        const int64_t J = p_col[q];
+        const float* pb = P + (int64_t)q * 36 + r * 6;
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int h = 0; h < NR; ++h) acc[h] += pb[c] * xc[(6 * J + c) * NR + h];
+    }
+#pragma unroll
+    for (int h = 0; h < NR; ++h) z1[u * NR + h] = (VT)acc[h];
+}
+
+// Coarsest levels: x = A^-1 b with the dense inverse of each structure solved
+// at this level (workgroup per structure, thread per row)
+template <int NR>
+__global__ __launch_bounds__(128) void k_mg_direct(const int32_t* __restrict__ list, const int32_t* __restrict__ term,
+                                                   const float* __restrict__ inv, SysState st,
+                                                   const float* __restrict__ b, float* __restrict__ x) {
+    const int s = list[blockIdx.x];
+    if (structure_done<NR>(st, s)) return;
+    const int64_t node0 = term[4 * s + 1];
+    const int N = 6 * term[4 * s + 2];
+    const float* Ai = inv + term[4 * s + 3];
+    const float* bb = b + 6 * node0 * NR;
+    for (int r = threadIdx.x; r < N; r += blockDim.x) {
+        float acc[NR];
+#pragma unroll
+        for (int h = 0; h < NR; ++h) acc[h] = 0.f;
+        for (int c = 0; c < N; ++c) {
+            const float a = Ai[(int64_t)c * N + r];
+#pragma unroll
+            for (int h = 0; h < NR; ++h) acc[h] += a * bb[c * NR + h];
+        }
+#pragma unroll
+        for (int h = 0; h < NR; ++h) x[(6 * node0 + r) * NR + h] = acc[h];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host: device copy of the hierarchy, numeric set-up, V-cycle, driver
+// ---------------------------------------------------------------------------
+template <typename T>
+static int amg_alloc(AmgState& A, T** dst, size_t count) {
+    *dst = nullptr;
+    if (count == 0) return JSSO_OK;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    JSSO_REQUIRE(e == hipSuccess, JSSO_ENOMEM, "AMG hipMalloc(%zu bytes): %s", count * sizeof(T),
+                 hipGetErrorString(e));
+    A.allocs.push_back(q);
+    JSSO_HIP(hipMemset(q, 0, count * sizeof(T)));
+    *dst = (T*)q;
+    return JSSO_OK;
+}
+
+template <typename T>
+static int amg_upload(AmgState& A, T** dst, const std::vector<T>& src) {
+    int rc = amg_alloc(A, dst, src.size());
+    if (rc || src.empty()) return rc;
+    JSSO_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    return JSSO_OK;
+}
+
+static Slices level_slices(Plan* pl, AmgState& A, int l) {
+    if (l == 0) return make_slices(pl, 0, pl->n_wg);
+    const AmgLevelDev& D = A.dev[l];
+    return Slices{D.d_wg_sys, D.d_wg_row0, D.d_wg_row1, D.d_sys_wg_off, D.d_ell_vpos, D.d_ell_col,
+                  (const int4*)D.d_meta, 0, D.n_wg, (D.n_wg + 7) / 8};
+}
+
+static int amg_build_device(Plan* pl, AmgState& A) {
+    const int nl = A.nl, ns = pl->n_sys;
+    A.dev.assign(nl, AmgLevelDev{});
+    A.d_Ac.assign(nl, nullptr);
+    A.d_Lc.assign(nl, nullptr);
+    A.d_Lci.assign(nl, nullptr);
+    A.d_cdiag.assign(nl, nullptr);
+    int rc = 0;
+#define AMG_TRY(x) \
+    if ((rc = (x))) return rc
+    for (int l = 0; l < nl; ++l) {
+        AmgHostLevel& H = A.host[l];
+        AmgLevelDev& D = A.dev[l];
+        D.n = H.n;
+        AMG_TRY(amg_upload(A, &D.d_node_sys, H.node_sys));
+        AMG_TRY(amg_upload(A, &D.d_cont, H.cont));
+        if (l == 0) {
+            D.n_wg = pl->n_wg;
+        } else {
+            const EllPattern& E = H.ell;
+            D.n_wg = E.n_wg;
+            D.ell_slots = E.ell_slots;
+            std::vector<int32_t> browof(H.colind.size()), cdiag(H.n);
+            for (int i = 0; i < H.n; ++i)
+                for (int k = H.rowptr[i]; k < H.rowptr[i + 1]; ++k) {
+                    browof[k] = i;
+                    if (H.colind[k] == i) cdiag[i] = k;
+                }
+            AMG_TRY(amg_upload(A, &D.d_rowptr, H.rowptr));
+            AMG_TRY(amg_upload(A, &D.d_colind, H.colind));
+            AMG_TRY(amg_upload(A, &D.d_kref, E.kref));
+            AMG_TRY(amg_upload(A, &D.d_browof, browof));
+            AMG_TRY(amg_upload(A, &A.d_cdiag[l], cdiag));
+            AMG_TRY(amg_upload(A, &D.d_wg_sys, E.wg_sys));
+            AMG_TRY(amg_upload(A, &D.d_wg_row0, E.wg_row0));
+            AMG_TRY(amg_upload(A, &D.d_wg_row1, E.wg_row1));
+            AMG_TRY(amg_upload(A, &D.d_sys_wg_off, E.sys_wg_off));
+            AMG_TRY(amg_upload(A, &D.d_ell_col, E.ell_col));
+            AMG_TRY(amg_upload(A, &D.d_ell_vpos, E.ell_vpos));
+            AMG_TRY(amg_upload(A, &D.d_meta, E.meta));
+            AMG_TRY(amg_upload(A, &D.d_ell_src, E.ell_src));
+            AMG_TRY(amg_alloc(A, &D.d_A64, (size_t)E.ell_slots * 36));
+            AMG_TRY(amg_alloc(A, &D.d_A32, (size_t)E.ell_slots * 36));
+            AMG_TRY(amg_alloc(A, &A.d_Ac[l], H.colind.size() * 36));
+            AMG_TRY(amg_alloc(A, &A.d_Lc[l], (size_t)H.n * 36));
+            AMG_TRY(amg_alloc(A, &A.d_Lci[l], (size_t)H.n * 36));
+        }
+        const bool has_p = l + 1 < nl;
+        if (!has_p) continue;
+        D.n_agg = (int32_t)H.agg_ptr.size() - 1;
+        D.nP = H.p_col.size();
+        D.nW = H.w_col.size();
+        AMG_TRY(amg_upload(A, &D.d_agg, H.agg));
+        AMG_TRY(amg_upload(A, &D.d_agg_ptr, H.agg_ptr));
+        AMG_TRY(amg_upload(A, &D.d_agg_nodes, H.agg_nodes));
+        AMG_TRY(amg_upload(A, &D.d_p_ptr, H.p_ptr));
+        AMG_TRY(amg_upload(A, &D.d_p_col, H.p_col));
+        AMG_TRY(amg_upload(A, &D.d_p_row, H.p_row));
+        AMG_TRY(amg_upload(A, &D.d_p_cptr, H.p_cptr));
+        AMG_TRY(amg_upload(A, &D.d_p_cj, H.p_cj));
+        AMG_TRY(amg_upload(A, &D.d_p_caref, H.p_caref));
+        AMG_TRY(amg_upload(A, &D.d_pt_ptr, H.pt_ptr));
+        AMG_TRY(amg_upload(A, &D.d_pt_row, H.pt_row));
+        AMG_TRY(amg_upload(A, &D.d_p_tpos, H.p_tpos));
+        AMG_TRY(amg_upload(A, &D.d_w_cptr, H.w_cptr));
+        AMG_TRY(amg_upload(A, &D.d_w_caref, H.w_caref));
+        AMG_TRY(amg_upload(A, &D.d_w_cp, H.w_cp));
+        AMG_TRY(amg_upload(A, &D.d_c_cptr, H.c_cptr));
+        AMG_TRY(amg_upload(A, &D.d_c_p, H.c_p));
+        AMG_TRY(amg_upload(A, &D.d_c_w, H.c_w));
+        AMG_TRY(amg_alloc(A, &D.d_B, (size_t)H.n * 36));
+        AMG_TRY(amg_alloc(A, &D.d_Q, (size_t)H.n * 36));
+        AMG_TRY(amg_alloc(A, &D.d_R, (size_t)D.n_agg * 36));
+        AMG_TRY(amg_alloc(A, &D.d_P64, (size_t)D.nP * 36));
+        AMG_TRY(amg_alloc(A, &D.d_P32, (size_t)D.nP * 36));
+        AMG_TRY(amg_alloc(A, &D.d_PT32, (size_t)D.nP * 36));
+        AMG_TRY(amg_alloc(A, &D.d_W64, (size_t)D.nW * 36));
+        AMG_TRY(amg_alloc(A, &D.d_rho, (size_t)ns));
+        AMG_TRY(amg_alloc(A, &D.d_omega, (size_t)ns));
+        AMG_TRY(amg_alloc(A, &D.d_theta, (size_t)ns));
+        AMG_TRY(amg_alloc(A, &D.d_pv[0], 6 * (size_t)H.n));
+        AMG_TRY(amg_alloc(A, &D.d_pv[1], 6 * (size_t)H.n));
+        AMG_TRY(amg_alloc(A, &D.d_ppart, 2 * (size_t)D.n_wg));
+    }
+    AMG_TRY(amg_upload(A, &A.d_node_off0, pl->h_node_off));
+    // dense coarsest solves
+    A.h_term.assign(4 * (size_t)ns, -1);
+    A.term_list.assign(nl, {});
+    int64_t off = 0;
+    for (int l = 0; l < nl; ++l)
+        for (int s = 0; s < ns; ++s)
+            if (A.host[l].term[s]) {
+                const int cnt = A.host[l].node_off[s + 1] - A.host[l].node_off[s];
+                A.h_term[4 * s] = l;
+                A.h_term[4 * s + 1] = A.host[l].node_off[s];
+                A.h_term[4 * s + 2] = cnt;
+                A.h_term[4 * s + 3] = (int32_t)off;
+                A.term_list[l].push_back(s);
+                off += 36 * (int64_t)cnt * cnt;
+                JSSO_REQUIRE(off < INT32_MAX, JSSO_EUNSUPPORTED, "AMG: dense coarse solves too large");
+            }
+    A.inv_total = off;
+    AMG_TRY(amg_upload(A, &A.d_term, A.h_term));
+    A.d_term_list.assign(nl, nullptr);
+    for (int l = 0; l < nl; ++l) AMG_TRY(amg_upload(A, &A.d_term_list[l], A.term_list[l]));
+    AMG_TRY(amg_alloc(A, &A.d_inv, (size_t)off));
+    AMG_TRY(amg_alloc(A, &A.d_dense, (size_t)off));
+#undef AMG_TRY
+    return JSSO_OK;
+}
+
+static inline unsigned nblk(int64_t threads, int wg) { return (unsigned)((threads + wg - 1) / wg); }
+
+// Call f with the largest waves-per-slice count <= wps whose LDS buffer fits.
+template <typename VT, int NR, typename F>
+static void wps_dispatch(int wps, F&& f) {
+    if constexpr (AmgWpsFits<VT, NR, 4>::value) {
+        if (wps >= 4) {
+            f(std::integral_constant<int, 4>{});
+            return;
+        }
+    }
+    if constexpr (AmgWpsFits<VT, NR, 2>::value) {
+        if (wps >= 2) {
+            f(std::integral_constant<int, 2>{});
+            return;
+        }
+    }
+    f(std::integral_constant<int, 1>{});
+}
+
+// Numeric set-up on the current assembly (stream-ordered).
+static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
+    const int nl = A.nl, ns = pl->n_sys;
+    for (int l = 0; l + 1 < nl; ++l) {
+        AmgLevelDev& D = A.dev[l];
+        AmgLevelDev& C = A.dev[l + 1];
+        const AmgHostLevel& H = A.host[l];
+        const double* A64 = l == 0 ? pl->d_Ke64 : D.d_A64;
+        const Slices sl = level_slices(pl, A, l);
+        const int32_t* sys_wg_off = l == 0 ? pl->d_sys_wg_off : D.d_sys_wg_off;
+        if (l == 0)
+            hipLaunchKernelGGL(k_amg_null0, dim3(nblk(D.n, 256)), dim3(256), 0, s, D.n, D.d_agg, D.d_agg_ptr,
+                               D.d_agg_nodes, (const double*)pl->d_crds, (const double*)pl->d_Linv,
+                               (const uint8_t*)pl->d_fixed, D.d_B);
+        if (D.n_agg > 0)
+            hipLaunchKernelGGL(k_amg_qr, dim3((unsigned)D.n_agg), dim3(64), 0, s, D.n_agg, D.d_agg_ptr,
+                               D.d_agg_nodes, (const double*)D.d_B, l == 0 ? (const double*)pl->d_crds : nullptr,
+                               D.d_node_sys, A.d_node_off0, D.d_Q, D.d_R);
+        const float* A32 = l == 0 ? pl->d_Ke32 : D.d_A32;
+        for (int step = 0; step < AMG_POWER_STEPS; ++step)
+            hipLaunchKernelGGL(k_amg_power<4>, slice_grid(D.n_wg), dim3(256), 0, s, sl, A32, step,
+                               (const double*)D.d_pv[(step + 1) & 1], D.d_pv[step & 1], D.d_ppart,
+                               (const uint8_t*)D.d_cont, D.n_wg);
+        hipLaunchKernelGGL(k_amg_theta, dim3(nblk(ns, 64)), dim3(64), 0, s, ns, sys_wg_off,
+                           (const double*)D.d_ppart, AMG_POWER_STEPS - 1, (const uint8_t*)D.d_cont, D.n_wg, D.d_rho,
+                           D.d_omega, D.d_theta);
+        if (D.nP > 0) {
+            hipLaunchKernelGGL(k_amg_psmooth, dim3(nblk(D.nP * 6, 256)), dim3(256), 0, s, D.nP, D.d_p_row,
+                               D.d_p_cptr, D.d_p_cj, D.d_p_caref, D.d_node_sys, (const double*)D.d_omega, A64,
+                               (const double*)D.d_Q, D.d_P64);
+            hipLaunchKernelGGL(k_amg_w, dim3(nblk(D.nW * 6, 256)), dim3(256), 0, s, D.nW, D.d_w_cptr, D.d_w_caref,
+                               D.d_w_cp, A64, (const double*)D.d_P64, D.d_W64);
+        }
+        const int64_t nnzc = A.host[l + 1].colind.size();
+        if (nnzc > 0) {
+            hipLaunchKernelGGL(k_amg_galerkin, dim3(nblk(nnzc * 6, 256)), dim3(256), 0, s, nnzc, D.d_c_cptr, D.d_c_p,
+                               D.d_c_w, (const double*)D.d_P64, (const double*)D.d_W64, A.d_Ac[l + 1]);
+            hipLaunchKernelGGL(k_amg_cfactor, dim3(nblk(C.n, 128)), dim3(128), 0, s, C.n, A.d_cdiag[l + 1],
+                               (const double*)A.d_Ac[l + 1], A.d_Lc[l + 1], A.d_Lci[l + 1]);
+            const int64_t nval = C.ell_slots - ROWS_PER_WG;
+            if (nval > 0)
+                hipLaunchKernelGGL(k_amg_cscale, dim3(nblk(nval * 36, 256)), dim3(256), 0, s, nval, C.d_ell_src,
+                                   C.d_browof, C.d_colind, (const double*)A.d_Ac[l + 1],
+                                   (const double*)A.d_Lci[l + 1], C.d_A64, C.d_A32);
+        }
+        if (D.nP > 0)
+            hipLaunchKernelGGL(k_amg_pscale, dim3(nblk(D.nP * 36, 256)), dim3(256), 0, s, D.nP, D.d_p_col,
+                               D.d_p_tpos, (const double*)D.d_P64, (const double*)A.d_Lci[l + 1], D.d_P32,
+                               D.d_PT32);
+        if (C.d_B && C.n > 0)
+            hipLaunchKernelGGL(k_amg_nullc, dim3(nblk((int64_t)C.n * 36, 256)), dim3(256), 0, s, C.n,
+                               (const double*)A.d_Lc[l + 1], (const double*)D.d_R, C.d_B);
+        (void)H;
+    }
+    for (int l = 1; l < nl; ++l) {
+        if (A.term_list[l].empty()) continue;
+        const AmgLevelDev& D = A.dev[l];
+        hipLaunchKernelGGL(k_amg_dense, dim3((unsigned)A.term_list[l].size()), dim3(256), 0, s, A.d_term_list[l],
+                           A.d_term, D.d_rowptr, D.d_colind, D.d_kref, (const double*)D.d_A64, A.d_dense, A.d_inv);
+    }
+    JSSO_HIP(hipGetLastError());
+    return JSSO_OK;
+}
+
+// Level-0 extra vectors (8 bytes per value) and the coarse V-cycle vectors for nrhs load cases.
+static int amg_vectors(Plan* pl, AmgState& A, int nrhs) {
+    const size_t zb = 6 * (size_t)pl->n_nodes * nrhs * sizeof(double);
+    if (A.z_bytes < zb) {
+        for (void** v : {&A.d_z, &A.d_t0, &A.d_p1}) {
+            if (*v) JSSO_HIP(hipFree(*v));
+            *v = nullptr;
+            hipError_t e = hipMalloc(v, zb);
+            JSSO_REQUIRE(e == hipSuccess, JSSO_ENOMEM, "AMG vectors hipMalloc(%zu): %s", zb, hipGetErrorString(e));
+        }
+        A.z_bytes = zb;
+        ++A.gen;
+    }
+    if (A.vec_nrhs < nrhs) {
+        for (int l = 1; l < A.nl; ++l) {
+            AmgLevelDev& D = A.dev[l];
+            for (float** v : {&D.d_b, &D.d_t, &D.d_x}) {
+                if (*v) JSSO_HIP(hipFree(*v));
+                *v = nullptr;
+                const size_t bytes = 6 * (size_t)std::max(D.n, 1) * nrhs * sizeof(float);
+                hipError_t e = hipMalloc((void**)v, bytes);
+                JSSO_REQUIRE(e == hipSuccess, JSSO_ENOMEM, "AMG vectors hipMalloc(%zu): %s", bytes,
+                             hipGetErrorString(e));
+                JSSO_HIP(hipMemset(*v, 0, bytes));
+            }
+        }
+        A.vec_nrhs = nrhs;
+        ++A.gen;
+    }
+    return JSSO_OK;
+}
+
+// Level-0 buffers of one AMG solve
+template <typename VT>
+struct AmgBufs {
+    VT* r[2];
+    VT* p[2];
+    VT *q, *z, *t0, *xlo;
+};
+
+// The V-cycle after the level-0 pre-smoothing of iteration k (t0 holds
+// r_k - A r_k / theta): restriction and pre-smoothing down the levels, the
+// dense coarsest solves, prolongation and post-smoothing up; ends with z = M r_k
+// and the partials <r_k, z>.
+template <typename MT, typename VT, int NR>
+static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs<VT>& bf, const MT* A0, int k,
+                            hipStream_t s) {
+    const int nl = A.nl;
+    for (int l = 0; l + 1 < nl; ++l) {
+        const AmgLevelDev& D = A.dev[l];
+        const AmgLevelDev& C = A.dev[l + 1];
+        if (C.n == 0) continue;
+        const dim3 gr(nblk(24 * (int64_t)C.n, 256));
+        if (l == 0)
+            hipLaunchKernelGGL((k_mg_restrict<VT, NR>), gr, dim3(256), 0, s, C.n, C.d_node_sys, D.d_pt_ptr,
+                               D.d_pt_row, (const float*)D.d_PT32, w.st, (const VT*)bf.t0, C.d_b);
+        else
+            hipLaunchKernelGGL((k_mg_restrict<float, NR>), gr, dim3(256), 0, s, C.n, C.d_node_sys, D.d_pt_ptr,
+                               D.d_pt_row, (const float*)D.d_PT32, w.st, (const float*)D.d_t, C.d_b);
+        if (l + 2 < nl)
+            wps_dispatch<float, NR>(4, [&](auto W) {
+                constexpr int WP = decltype(W)::value;
+                hipLaunchKernelGGL((k_mg_pre<NR, WP>), slice_grid(C.n_wg), dim3(64 * WP), 0, s,
+                                   level_slices(pl, A, l + 1), (const float*)C.d_A32, (const float*)C.d_theta,
+                                   (const uint8_t*)C.d_cont, w.st, (const float*)C.d_b, C.d_t);
+            });
+    }
+    for (int l = 1; l < nl; ++l) {
+        if (A.term_list[l].empty()) continue;
+        const AmgLevelDev& D = A.dev[l];
+        hipLaunchKernelGGL((k_mg_direct<NR>), dim3((unsigned)A.term_list[l].size()), dim3(128), 0, s,
+                           A.d_term_list[l], A.d_term, (const float*)A.d_inv, w.st, (const float*)D.d_b, D.d_x);
+    }
+    for (int l = nl - 2; l >= 0; --l) {
+        const AmgLevelDev& D = A.dev[l];
+        const AmgLevelDev& C = A.dev[l + 1];
+        if (l == 0) {
+            hipLaunchKernelGGL((k_mg_prolong<VT, NR>), dim3(nblk(6 * (int64_t)D.n, 256)), dim3(256), 0, s, D.n,
+                               D.d_node_sys, (const uint8_t*)D.d_cont, D.d_p_ptr, D.d_p_col, (const float*)D.d_P32,
+                               (const float*)D.d_theta, w.st, (const VT*)bf.r[k & 1], (const float*)C.d_x, bf.t0);
+            wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
+                constexpr int WP = decltype(W)::value;
+                hipLaunchKernelGGL((k_pcg_post<MT, VT, NR, WP>), slice_grid(pl->n_wg), dim3(64 * WP), 0, s,
+                                   level_slices(pl, A, 0), w.P, w.st, k, A0, (const float*)D.d_theta,
+                                   (const VT*)bf.r[k & 1], (const VT*)bf.r[(k - 1) & 1], (const VT*)bf.t0, bf.z);
+            });
+        } else {
+            hipLaunchKernelGGL((k_mg_prolong<float, NR>), dim3(nblk(6 * (int64_t)D.n, 256)), dim3(256), 0, s, D.n,
+                               D.d_node_sys, (const uint8_t*)D.d_cont, D.d_p_ptr, D.d_p_col, (const float*)D.d_P32,
+                               (const float*)D.d_theta, w.st, (const float*)D.d_b, (const float*)C.d_x, D.d_t);
+            wps_dispatch<float, NR>(4, [&](auto W) {
+                constexpr int WP = decltype(W)::value;
+                hipLaunchKernelGGL((k_mg_post<NR, WP>), slice_grid(D.n_wg), dim3(64 * WP), 0, s,
+                                   level_slices(pl, A, l), (const float*)D.d_A32, (const float*)D.d_theta,
+                                   (const uint8_t*)D.d_cont, w.st, (const float*)D.d_b, (const float*)D.d_t, D.d_x);
+            });
+        }
+    }
+}
+
+// one ordinary PCG iteration k
+template <typename MT, typename VT, int NR>
+static void amg_step(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs<VT>& bf, const MT* A0, int k,
+                     hipStream_t s) {
+    const Slices sl = level_slices(pl, A, 0);
+    const dim3 g = slice_grid(pl->n_wg);
+    wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
+        constexpr int WP = decltype(W)::value;
+        hipLaunchKernelGGL((k_pcg_dir<MT, VT, NR, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A0,
+                           (const VT*)bf.z, (const VT*)bf.p[(k - 1) & 1], bf.p[k & 1], bf.q);
+        hipLaunchKernelGGL((k_pcg_pre<MT, VT, NR, false, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A0,
+                           (const float*)A.dev[0].d_theta, (const VT*)bf.r[(k - 1) & 1], bf.r[k & 1],
+                           (const VT*)bf.q, (const VT*)bf.p[k & 1], bf.xlo, bf.t0);
+    });
+    amg_vcycle_rest<MT, VT, NR>(pl, A, w, bf, A0, k, s);
+}
+
+// The iteration k that closes a chunk of `steps` iterations: direction and
+// product, update folded into the FP64 solution, FP64 residual replacement,
+// convergence test, best iterate, and the V-cycle on the new residual.
+template <typename MT, typename VT, int NR>
+static void amg_replace(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs<VT>& bf, const MT* A0, int k,
+                        int steps, double tol, hipStream_t s) {
+    const Slices sl = level_slices(pl, A, 0);
+    const dim3 g = slice_grid(pl->n_wg);
+    wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
+        constexpr int WP = decltype(W)::value;
+        hipLaunchKernelGGL((k_pcg_dir<MT, VT, NR, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A0,
+                           (const VT*)bf.z, (const VT*)bf.p[(k - 1) & 1], bf.p[k & 1], bf.q);
+    });
+    hipLaunchKernelGGL((k_pcg_fold<VT, NR>), g, dim3(SOLVER_WG), 0, s, sl, w.P, w.st, k, (const VT*)bf.p[k & 1],
+                       bf.xlo, w.x64);
+    hipLaunchKernelGGL((k_replace<VT, NR, 1>), g, dim3(64), 0, s, sl, w.P, w.st, (const double*)pl->d_Ke64,
+                       (const double*)w.b64, (const double*)w.x64, bf.r[k & 1]);
+    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, steps);
+    hipLaunchKernelGGL((k_save<NR>), g, dim3(SOLVER_WG), 0, s, sl, (const int32_t*)w.st.save, (const double*)w.x64,
+                       w.xbest);
+    wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
+        constexpr int WP = decltype(W)::value;
+        hipLaunchKernelGGL((k_pcg_pre<MT, VT, NR, true, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A0,
+                           (const float*)A.dev[0].d_theta, (const VT*)bf.r[(k - 1) & 1], bf.r[k & 1],
+                           (const VT*)bf.q, (const VT*)bf.p[k & 1], bf.xlo, bf.t0);
+    });
+    amg_vcycle_rest<MT, VT, NR>(pl, A, w, bf, A0, k, s);
+}
+
+
+// Symbolic hierarchy on the host (no device work), then its device copy.
+static int amg_ensure_host(Plan* pl) {
+    if (pl->amg) return JSSO_OK;
+    AmgState* A = new AmgState();
+    int rc = amg_build_host(pl, *A);
+    if (rc) {
+        delete A;
+        return rc;
+    }
+    pl->amg = A;
+    return JSSO_OK;
+}
+
+static int amg_prepare(Plan* pl, hipStream_t s) {
+    int rc = amg_ensure_host(pl);
+    if (rc) return rc;
+    AmgState& A = *pl->amg;
+    if (A.dev.empty()) {
+        if ((rc = amg_build_device(pl, A))) {
+            A.dev.clear();
+            return rc;
+        }
+        ++A.gen;
+    }
+    if (!pl->amg_valid) {
+        JSSO_REQUIRE(pl->d_crds, JSSO_EINVAL, "AMG needs the node coordinates of an assembly");
+        if ((rc = amg_setup_numeric(pl, A, s))) return rc;
+        pl->amg_valid = true;
+    }
+    return JSSO_OK;
+}
+
+template <typename MT, typename VT, int NR>
+static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, int replace_every, jsso_info* info,
+                   hipStream_t s) {
+    Workspace w;
+    int rc = get_workspace(pl, NR, w);
+    if (rc) return rc;
+    AmgState& A = *pl->amg;
+    if ((rc = amg_vectors(pl, A, NR))) return rc;
+    const Slices sl = level_slices(pl, A, 0);
+    const int nsys = pl->n_sys * NR;
+    if ((rc = ensure_flag_buffers(pl, nsys))) return rc;
+    const MT* A0 = nullptr;
+    if constexpr (sizeof(MT) == 4) A0 = (const MT*)pl->d_Ke32; else A0 = (const MT*)pl->d_Ke64;
+    AmgBufs<VT> bf;
+    bf.r[0] = (VT*)w.r[0];
+    bf.r[1] = (VT*)w.r[1];
+    bf.p[0] = (VT*)w.p;
+    bf.p[1] = (VT*)A.d_p1;
+    bf.q = (VT*)w.q[0];
+    bf.z = (VT*)A.d_z;
+    bf.t0 = (VT*)A.d_t0;
+    bf.xlo = (VT*)w.xlo;
+    const dim3 gridS = slice_grid(pl->n_wg);
+    // start: b, x = 0, r_0 = b, p = 0; both partial buffers cleared (so the
+    // first direction has beta = 0); z_0 = M r_0
+    JSSO_HIP(hipMemsetAsync(w.P.cg[0], 0, 2 * 4 * (size_t)pl->n_wg * NR * sizeof(double), s));
+    hipLaunchKernelGGL((k_init<VT, NR>), gridS, dim3(SOLVER_WG), 0, s, sl, w.P, pl->d_fixed, pl->d_Linv, f, w.b64,
+                       w.x64, w.xbest, bf.xlo, bf.r[0], bf.p[0]);
+    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 0, tol, 0);
+    wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
+        constexpr int WP = decltype(W)::value;
+        hipLaunchKernelGGL((k_pcg_pre<MT, VT, NR, true, WP>), gridS, dim3(64 * WP), 0, s, sl, w.P, w.st, 0, A0,
+                           (const float*)A.dev[0].d_theta, (const VT*)bf.r[1], bf.r[0], (const VT*)bf.q,
+                           (const VT*)bf.p[0], bf.xlo, bf.t0);
+    });
+    amg_vcycle_rest<MT, VT, NR>(pl, A, w, bf, A0, 0, s);
+    JSSO_HIP(hipGetLastError());
+
+    const int E = replace_every;
+    const void* key[CgGraph::NKEY] = {A0,        w.r[0],   w.r[1],   w.q[0],         w.p,         A.d_p1,
+                                      A.d_z,     A.d_t0,   w.xlo,    w.P.cg[0],      w.st.slice_act, w.b64,
+                                      w.x64,     w.xbest,  w.st.active, (const void*)(uintptr_t)A.gen};
+    CgGraph(&gc)[2] = A.graph[sizeof(MT) == 4 ? 0 : 1][NR];
+    auto enqueue_chunk = [&](int it0, int steps) -> int {
+        hipGraphExec_t ga = nullptr, gb = nullptr;
+        if (steps == E && E % 2 == 0 && it0 % 2 == 0 && pl->graphs_ok && graphs_enabled()) {
+            ga = capture_graph(pl, gc[0], key, 0, E, tol, [&](hipStream_t cs) {
+                for (int it = 1; it < E; ++it) amg_step<MT, VT, NR>(pl, A, w, bf, A0, it, cs);
+            });
+            if (ga)
+                gb = capture_graph(pl, gc[1], key, 0, E, tol, [&](hipStream_t cs) {
+                    amg_replace<MT, VT, NR>(pl, A, w, bf, A0, E, E, tol, cs);
+                });
+        }
+        if (ga && gb) {
+            std::pair<hipEvent_t, hipEvent_t>* ev;
+            timing_begin_batch(s, ev, steps - 1);
+            JSSO_HIP(hipGraphLaunch(ga, s));
+            timing_end(s, ev);
+            JSSO_HIP(hipGraphLaunch(gb, s));
+        } else {
+            for (int it = it0 + 1; it < it0 + steps; ++it) {
+                std::pair<hipEvent_t, hipEvent_t>* ev;
+                timing_begin(s, ev);
+                amg_step<MT, VT, NR>(pl, A, w, bf, A0, it, s);
+                timing_end(s, ev);
+            }
+            amg_replace<MT, VT, NR>(pl, A, w, bf, A0, it0 + steps, steps, tol, s);
+        }
+        JSSO_HIP(hipGetLastError());
+        return JSSO_OK;
+    };
+
+    // convergence flags read one chunk behind the GPU (as in the block-Jacobi solver)
+    std::vector<int> chunk_end;
+    int it = 0, it_done = 0, chunks_done = 0, c = 0;
+    for (;;) {
+        const bool more = it < maxit;
+        if (more) {
+            const int steps = std::min(E, maxit - it);
+            if ((rc = enqueue_chunk(it, steps))) return rc;
+            it += steps;
+            chunk_end.push_back(it);
+            JSSO_HIP(hipMemcpyAsync(pl->h_flags + (size_t)(c & 1) * nsys, w.st.active, nsys * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, s));
+            JSSO_HIP(hipEventRecord(pl->flag_ev[c & 1], s));
+            ++c;
+        }
+        const int chk = more ? c - 2 : c - 1;
+        if (chk >= 0) {
+            JSSO_HIP(hipEventSynchronize(pl->flag_ev[chk & 1]));
+            timing_collect();
+            const int32_t* fl = pl->h_flags + (size_t)(chk & 1) * nsys;
+            bool any = false;
+            for (int k = 0; k < nsys && !any; ++k) any = fl[k] != 0;
+            if (!any) {
+                it_done = chunk_end[chk];
+                chunks_done = chk + 1;
+                break;
+            }
+        }
+        if (!more) {
+            it_done = it;
+            chunks_done = c;
+            break;
+        }
+    }
+    const int64_t nr = 6 * (int64_t)pl->n_nodes;
+    hipLaunchKernelGGL((k_unscale<NR>), dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, pl->n_nodes,
+                       pl->d_Linv, (const double*)w.xbest, u);
+    JSSO_HIP(hipGetLastError());
+    std::vector<double> rel(nsys);
+    std::vector<int32_t> status(nsys);
+    JSSO_HIP(hipMemcpyAsync(rel.data(), w.st.best, nsys * sizeof(double), hipMemcpyDeviceToHost, s));
+    JSSO_HIP(hipMemcpyAsync(status.data(), w.st.status, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    JSSO_HIP(hipStreamSynchronize(s));
+    timing_collect();
+    int conv = 0;
+    double mx = 0.0;
+    for (int k = 0; k < nsys; ++k) {
+        conv += status[k] == SYS_CONVERGED ? 1 : 0;
+        mx = std::max(mx, rel[k]);
+    }
+    if (info) {
+        info->iterations = it_done;
+        info->replacements = chunks_done;
+        info->converged = conv;
+        info->n_systems = nsys;
+        info->rel_residual = mx;
+        info->waves_per_slice = 1;
+        info->levels = A.nl;
+    }
+    return conv == nsys ? JSSO_OK : JSSO_NOT_CONVERGED;
+}
+
+// Iterations between FP64 residual replacements of the AMG solve
+// (JSSO_AMG_REPLACE_EVERY overrides; even, so the chunk graphs are reused).
+static int amg_replace_interval() {
+    const char* e = getenv("JSSO_AMG_REPLACE_EVERY");
+    const int v = e ? atoi(e) : 0;
+    return v > 0 ? v : 16;
+}
+
+template <typename MT, typename VT>
+static int amg_dispatch(Plan* pl, const double* f, double* u, int nrhs, double tol, int maxit, jsso_info* info,
+                        hipStream_t s) {
+    const int E = amg_replace_interval();
+    switch (nrhs) {
+        case 1: return run_amg<MT, VT, 1>(pl, f, u, tol, maxit, E, info, s);
+        case 2: return run_amg<MT, VT, 2>(pl, f, u, tol, maxit, E, info, s);
+        case 3: return run_amg<MT, VT, 3>(pl, f, u, tol, maxit, E, info, s);
+        case 4: return run_amg<MT, VT, 4>(pl, f, u, tol, maxit, E, info, s);
+        case 8: return run_amg<MT, VT, 8>(pl, f, u, tol, maxit, E, info, s);
+        default:
+            set_error("nrhs=%d unsupported (1, 2, 3, 4 or 8)", nrhs);
+            return JSSO_EINVAL;
+    }
+}
+
+int amg_solve(Plan* pl, const double* f, double* u, int nrhs, bool mixed, double tol, int maxit, jsso_info* info,
+              hipStream_t s) {
+    int rc = amg_prepare(pl, s);
+    if (rc) return rc;
+    if (mixed) return amg_dispatch<float, float>(pl, f, u, nrhs, tol, maxit, info, s);
+    return amg_dispatch<double, double>(pl, f, u, nrhs, tol, maxit, info, s);
+}
+
+int amg_levels(Plan* pl, int32_t* h_nodes, int32_t cap, int32_t* h_levels) {
+    int rc = amg_ensure_host(pl);
+    if (rc) return rc;
+    const AmgState& A = *pl->amg;
+    if (h_levels) *h_levels = A.nl;
+    for (int l = 0; l < A.nl && l < cap; ++l) h_nodes[l] = A.host[l].n;
+    return JSSO_OK;
+}
+
+void amg_destroy(Plan* pl) {
+    AmgState* A = pl->amg;
+    if (!A) return;
+    for (auto& row : A->graph)
+        for (auto& pair : row)
+            for (auto& g : pair)
+                if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    for (void* a : A->allocs) (void)hipFree(a);
+    for (void* v : {A->d_z, A->d_t0, A->d_p1})
+        if (v) (void)hipFree(v);
+    for (auto& D : A->dev)
+        for (float* v : {D.d_b, D.d_t, D.d_x})
+            if (v) (void)hipFree(v);
+    delete A;
+    pl->amg = nullptr;
+}
+
+}  // namespace jsso

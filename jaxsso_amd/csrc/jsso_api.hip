@@ -35,6 +35,158 @@ static int upload(Plan* p, T** dst, const std::vector<T>& src) {
     return JSSO_OK;
 }
 
+// Symmetric sliced ELL of the off-diagonal blocks of a block-CSR pattern
+// (sorted columns, diagonal present in every row) over packed structures
+// (structure s owns rows [node_off[s], node_off[s+1]); an empty range gives no
+// slice).  A slice is <= ROWS_PER_WG block rows of one structure; lane l is
+// row row0 + l.  Only the upper blocks (i < j) are stored: value slot u of a
+// slice holds the u-th upper block of every row.  The column list of a slice
+// has nup upper slots (column j, block stored in this slice) followed by nlo
+// lower slots (column j < i; the block is the transpose of the upper block
+// (j, i), found through its value position vpos = value slot * 64 + lane).
+// Padding: upper slots hold zero values with the row's own column; lower slots
+// point at an all-zero value slot appended after the last one.
+int build_ell(int32_t n_sys, const int32_t* node_off, const std::vector<int32_t>& rowptr,
+              const std::vector<int32_t>& colind, EllPattern& out) {
+    const int32_t n = node_off[n_sys];
+    const int64_t nnzb = colind.size();
+    std::vector<int32_t> diag(n);
+    for (int i = 0; i < n; ++i) {
+        auto b = colind.begin() + rowptr[i], e = colind.begin() + rowptr[i + 1];
+        auto it = std::lower_bound(b, e, i);
+        JSSO_REQUIRE(it != e && *it == i, JSSO_EINVAL, "block row %d has no diagonal block", i);
+        diag[i] = (int32_t)(it - colind.begin());
+    }
+    // position in the block CSR of the transposed block (j, i) of every block (i, j)
+    std::vector<int32_t> tr(nnzb);
+    for (int i = 0; i < n; ++i)
+        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+            const int j = colind[k];
+            const int32_t* b = colind.data() + rowptr[j];
+            const int32_t* e = colind.data() + rowptr[j + 1];
+            tr[k] = (int32_t)(std::lower_bound(b, e, i) - colind.data());
+        }
+    std::vector<int32_t>& wg_sys = out.wg_sys;
+    std::vector<int32_t>& wg_row0 = out.wg_row0;
+    std::vector<int32_t>& wg_row1 = out.wg_row1;
+    std::vector<int32_t>& sys_wg_off = out.sys_wg_off;
+    wg_sys.clear();
+    wg_row0.clear();
+    wg_row1.clear();
+    sys_wg_off.assign(n_sys + 1, 0);
+    for (int s = 0; s < n_sys; ++s) {
+        for (int r0 = node_off[s]; r0 < node_off[s + 1]; r0 += ROWS_PER_WG) {
+            wg_sys.push_back(s);
+            wg_row0.push_back(r0);
+            wg_row1.push_back(std::min(r0 + ROWS_PER_WG, node_off[s + 1]));
+        }
+        sys_wg_off[s + 1] = (int32_t)wg_sys.size();
+    }
+    const int n_wg = (int)wg_sys.size();
+    out.n_wg = n_wg;
+    std::vector<int32_t>& ell_col = out.ell_col;   // per column position
+    std::vector<int32_t>& ell_vpos = out.ell_vpos;
+    std::vector<int32_t>& ell_src = out.ell_src;   // per value position
+    ell_col.clear();
+    ell_vpos.clear();
+    ell_src.clear();
+    std::vector<int32_t> pos_off(n_wg), val_off(n_wg), nupv(n_wg), nlov(n_wg);
+    std::vector<int64_t> vpos_of(nnzb, -1);
+    int64_t npos = 0, nval = 0;
+    for (int w = 0; w < n_wg; ++w) {
+        int nup = 0, nlo = 0;
+        for (int i = wg_row0[w]; i < wg_row1[w]; ++i) {
+            nup = std::max(nup, rowptr[i + 1] - diag[i] - 1);
+            nlo = std::max(nlo, diag[i] - rowptr[i]);
+        }
+        JSSO_REQUIRE(nup < 65536 && nlo < 65536, JSSO_EINVAL, "a block row has more than 65535 neighbours");
+        pos_off[w] = (int32_t)npos;
+        val_off[w] = (int32_t)nval;
+        nupv[w] = nup;
+        nlov[w] = nlo;
+        ell_col.resize((size_t)(npos + (int64_t)nup * ROWS_PER_WG));
+        ell_vpos.resize(ell_col.size(), 0);
+        ell_src.resize((size_t)(nval + (int64_t)nup * ROWS_PER_WG), -1);
+        for (int u = 0; u < nup; ++u)
+            for (int l = 0; l < ROWS_PER_WG; ++l) {
+                const int i = wg_row0[w] + l;
+                const int64_t pos = npos + (int64_t)u * ROWS_PER_WG + l;
+                const int64_t vp = nval + (int64_t)u * ROWS_PER_WG + l;
+                int32_t c = i < wg_row1[w] ? i : wg_row0[w];
+                if (i < wg_row1[w] && u < rowptr[i + 1] - diag[i] - 1) {
+                    const int k = diag[i] + 1 + u;
+                    c = colind[k];
+                    ell_src[vp] = k;
+                    vpos_of[k] = vp;
+                }
+                ell_col[pos] = c;
+                ell_vpos[pos] = (int32_t)vp;
+            }
+        npos += (int64_t)nup * ROWS_PER_WG;
+        nval += (int64_t)nup * ROWS_PER_WG;
+        JSSO_REQUIRE(npos < INT32_MAX / 2 && nval < INT32_MAX / 36, JSSO_EINVAL,
+                     "plan too large for 32-bit sliced-ELL offsets");
+    }
+    // lower slots, once every upper block has its value position (the zero slot
+    // comes after the last value slot)
+    const int64_t zero_vp = nval;
+    std::vector<int32_t> lo_off(n_wg);
+    {
+        int64_t total = npos;
+        for (int w = 0; w < n_wg; ++w) total += (int64_t)nlov[w] * ROWS_PER_WG;
+        JSSO_REQUIRE(total < INT32_MAX, JSSO_EINVAL, "plan too large for 32-bit sliced-ELL offsets");
+        ell_col.resize((size_t)total);
+        ell_vpos.resize((size_t)total, 0);
+    }
+    for (int w = 0; w < n_wg; ++w) {
+        lo_off[w] = (int32_t)npos;
+        const int nlo = nlov[w];
+        for (int u = 0; u < nlo; ++u)
+            for (int l = 0; l < ROWS_PER_WG; ++l) {
+                const int i = wg_row0[w] + l;
+                const int64_t pos = npos + (int64_t)u * ROWS_PER_WG + l;
+                int32_t c = i < wg_row1[w] ? i : wg_row0[w];
+                int64_t vp = zero_vp;
+                if (i < wg_row1[w] && u < diag[i] - rowptr[i]) {
+                    const int k = rowptr[i] + u;
+                    c = colind[k];
+                    vp = vpos_of[tr[k]];
+                }
+                ell_col[pos] = c;
+                ell_vpos[pos] = (int32_t)vp;
+            }
+        npos += (int64_t)nlo * ROWS_PER_WG;
+    }
+    out.ell_slots = nval + ROWS_PER_WG;   // value positions incl. the zero slot
+    // per block of the CSR: 2 * value position of its stored block, + 1 when
+    // the block is the transpose of the stored one; -1 for diagonal blocks
+    out.kref.assign(nnzb, -1);
+    for (int i = 0; i < n; ++i)
+        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+            if (k == diag[i]) continue;
+            out.kref[k] = k > diag[i] ? (int32_t)(2 * vpos_of[k]) : (int32_t)(2 * vpos_of[tr[k]] + 1);
+        }
+    // per slice, three int4: {sys, row0, row1, first value position},
+    // {first upper column position, upper slots, first lower column position, lower slots},
+    // {first slice of sys, end slice of sys, 0, 0}
+    std::vector<int32_t>& meta = out.meta;
+    meta.assign(12 * (size_t)n_wg, 0);
+    for (int w = 0; w < n_wg; ++w) {
+        int32_t* m = &meta[12 * (size_t)w];
+        m[0] = wg_sys[w];
+        m[1] = wg_row0[w];
+        m[2] = wg_row1[w];
+        m[3] = val_off[w];
+        m[4] = pos_off[w];
+        m[5] = nupv[w];
+        m[6] = lo_off[w];
+        m[7] = nlov[w];
+        m[8] = sys_wg_off[wg_sys[w]];
+        m[9] = sys_wg_off[wg_sys[w] + 1];
+    }
+    return JSSO_OK;
+}
+
 static int build_plan(Plan* p, int32_t n_sys, const int32_t* node_off, int32_t n_beam, const int32_t* cb,
                       int32_t n_quad, const int32_t* cq, int32_t n_cons, const int32_t* cons) {
     JSSO_REQUIRE(n_sys >= 1 && node_off, JSSO_EINVAL, "n_sys must be >= 1 with node offsets");
@@ -164,125 +316,21 @@ static int build_plan(Plan* p, int32_t n_sys, const int32_t* node_off, int32_t n
         }
     }
     // off-diagonal solver pattern
-    int64_t n_off = 0;
-    for (int i = 0; i < n; ++i) n_off += rowptr[i + 1] - rowptr[i] - 1;
-    p->nnzb_off = n_off;
-    // position in the block CSR of the transposed block (j, i) of every block (i, j)
-    std::vector<int32_t> tr(nnzb);
-    for (int i = 0; i < n; ++i)
-        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-            const int j = colind[k];
-            const int32_t* b = colind.data() + rowptr[j];
-            const int32_t* e = colind.data() + rowptr[j + 1];
-            tr[k] = (int32_t)(std::lower_bound(b, e, i) - colind.data());
-        }
-    // solver workgroups: contiguous chunks of ROWS_PER_WG block rows per structure
-    std::vector<int32_t> wg_sys, wg_row0, wg_row1, sys_wg_off(n_sys + 1, 0);
-    for (int s = 0; s < n_sys; ++s) {
-        for (int r0 = node_off[s]; r0 < node_off[s + 1]; r0 += ROWS_PER_WG) {
-            wg_sys.push_back(s);
-            wg_row0.push_back(r0);
-            wg_row1.push_back(std::min(r0 + ROWS_PER_WG, node_off[s + 1]));
-        }
-        sys_wg_off[s + 1] = (int32_t)wg_sys.size();
-    }
-    p->n_wg = (int32_t)wg_sys.size();
-    p->h_sys_wg_off = sys_wg_off;
-    // Symmetric sliced ELL.  A slice is one solver workgroup (<= 64 block rows of
-    // one structure); lane l is row row0 + l.  Only the upper blocks (i < j) are
-    // stored: value slot u of a slice holds the u-th upper block of every row,
-    // [36 entries][64 lanes].  The column list of a slice has nup upper slots
-    // (column j, block stored in this slice) followed by nlo lower slots (column
-    // j < i; the block is the transpose of the upper block (j, i), found through
-    // its value position vpos = value slot * 64 + lane).  Padding: upper slots
-    // hold zero values with the row's own column; lower slots point at an
-    // all-zero value slot appended after the last one.
-    std::vector<int32_t> ell_col, ell_vpos, ell_src;   // per column position / per value position
-    std::vector<int32_t> pos_off(p->n_wg), val_off(p->n_wg), nupv(p->n_wg), nlov(p->n_wg);
-    std::vector<int64_t> vpos_of(nnzb, -1);
-    int64_t npos = 0, nval = 0;
-    for (int w = 0; w < p->n_wg; ++w) {
-        int nup = 0, nlo = 0;
-        for (int i = wg_row0[w]; i < wg_row1[w]; ++i) {
-            nup = std::max(nup, rowptr[i + 1] - diag[i] - 1);
-            nlo = std::max(nlo, diag[i] - rowptr[i]);
-        }
-        JSSO_REQUIRE(nup < 65536 && nlo < 65536, JSSO_EINVAL, "a block row has more than 65535 neighbours");
-        pos_off[w] = (int32_t)npos;
-        val_off[w] = (int32_t)nval;
-        nupv[w] = nup;
-        nlov[w] = nlo;
-        ell_col.resize((size_t)(npos + (int64_t)nup * ROWS_PER_WG));
-        ell_vpos.resize(ell_col.size(), 0);
-        ell_src.resize((size_t)(nval + (int64_t)nup * ROWS_PER_WG), -1);
-        for (int u = 0; u < nup; ++u)
-            for (int l = 0; l < ROWS_PER_WG; ++l) {
-                const int i = wg_row0[w] + l;
-                const int64_t pos = npos + (int64_t)u * ROWS_PER_WG + l;
-                const int64_t vp = nval + (int64_t)u * ROWS_PER_WG + l;
-                int32_t c = i < wg_row1[w] ? i : wg_row0[w];
-                if (i < wg_row1[w] && u < rowptr[i + 1] - diag[i] - 1) {
-                    const int k = diag[i] + 1 + u;
-                    c = colind[k];
-                    ell_src[vp] = k;
-                    vpos_of[k] = vp;
-                }
-                ell_col[pos] = c;
-                ell_vpos[pos] = (int32_t)vp;
-            }
-        npos += (int64_t)nup * ROWS_PER_WG;
-        nval += (int64_t)nup * ROWS_PER_WG;
-        JSSO_REQUIRE(npos < INT32_MAX / 2 && nval < INT32_MAX / 36, JSSO_EINVAL,
-                     "plan too large for 32-bit sliced-ELL offsets");
-    }
-    // lower slots, once every upper block has its value position (the zero slot
-    // comes after the last value slot)
-    const int64_t zero_vp = nval;
-    std::vector<int32_t> lo_off(p->n_wg);
-    {
-        int64_t total = npos;
-        for (int w = 0; w < p->n_wg; ++w) total += (int64_t)nlov[w] * ROWS_PER_WG;
-        JSSO_REQUIRE(total < INT32_MAX, JSSO_EINVAL, "plan too large for 32-bit sliced-ELL offsets");
-        ell_col.resize((size_t)total);
-        ell_vpos.resize((size_t)total, 0);
-    }
-    for (int w = 0; w < p->n_wg; ++w) {
-        lo_off[w] = (int32_t)npos;
-        const int nlo = nlov[w];
-        for (int u = 0; u < nlo; ++u)
-            for (int l = 0; l < ROWS_PER_WG; ++l) {
-                const int i = wg_row0[w] + l;
-                const int64_t pos = npos + (int64_t)u * ROWS_PER_WG + l;
-                int32_t c = i < wg_row1[w] ? i : wg_row0[w];
-                int64_t vp = zero_vp;
-                if (i < wg_row1[w] && u < diag[i] - rowptr[i]) {
-                    const int k = rowptr[i] + u;
-                    c = colind[k];
-                    vp = vpos_of[tr[k]];
-                }
-                ell_col[pos] = c;
-                ell_vpos[pos] = (int32_t)vp;
-            }
-        npos += (int64_t)nlo * ROWS_PER_WG;
-    }
-    p->ell_slots = nval + ROWS_PER_WG;   // value positions incl. the zero slot
-    // per slice, three int4: {sys, row0, row1, first value position},
-    // {first upper column position, upper slots, first lower column position, lower slots},
-    // {first slice of sys, end slice of sys, 0, 0}
-    std::vector<int32_t> meta(12 * (size_t)p->n_wg, 0);
-    for (int w = 0; w < p->n_wg; ++w) {
-        int32_t* m = &meta[12 * (size_t)w];
-        m[0] = wg_sys[w];
-        m[1] = wg_row0[w];
-        m[2] = wg_row1[w];
-        m[3] = val_off[w];
-        m[4] = pos_off[w];
-        m[5] = nupv[w];
-        m[6] = lo_off[w];
-        m[7] = nlov[w];
-        m[8] = sys_wg_off[wg_sys[w]];
-        m[9] = sys_wg_off[wg_sys[w] + 1];
-    }
+    p->nnzb_off = nnzb - n;
+    EllPattern ell;
+    int rc = build_ell(n_sys, node_off, rowptr, colind, ell);
+    if (rc) return rc;
+    p->n_wg = ell.n_wg;
+    p->h_sys_wg_off = ell.sys_wg_off;
+    p->ell_slots = ell.ell_slots;
+    p->h_kref.swap(ell.kref);
+    std::vector<int32_t>& meta = ell.meta;
+    std::vector<int32_t>& ell_col = ell.ell_col;
+    std::vector<int32_t>& ell_vpos = ell.ell_vpos;
+    std::vector<int32_t>& ell_src = ell.ell_src;
+    std::vector<int32_t>& wg_sys = ell.wg_sys;
+    std::vector<int32_t>& wg_row0 = ell.wg_row0;
+    std::vector<int32_t>& wg_row1 = ell.wg_row1;
     p->h_slice_meta.swap(meta);
     p->h_ell_col.swap(ell_col);
     p->h_ell_vpos.swap(ell_vpos);
@@ -322,7 +370,8 @@ static int ensure_device(Plan* p) {
     if ((rc = dalloc(p, &p->d_ke, (size_t)p->n_contrib * 36)) || (rc = dalloc(p, &p->d_K, (size_t)p->nnzb * 36)) ||
         (rc = dalloc(p, &p->d_Kc, (size_t)p->nnzb * 36)) || (rc = dalloc(p, &p->d_Linv, n * 36)) ||
         (rc = dalloc(p, &p->d_Minv, n * 36)) || (rc = dalloc(p, &p->d_status, 1)) ||
-        (rc = dalloc(p, &p->d_Ke64, (size_t)p->ell_slots * 36)) || (rc = dalloc(p, &p->d_Ke32, (size_t)p->ell_slots * 36)))
+        (rc = dalloc(p, &p->d_Ke64, (size_t)p->ell_slots * 36)) || (rc = dalloc(p, &p->d_Ke32, (size_t)p->ell_slots * 36)) ||
+        (rc = dalloc(p, &p->d_crds, n * 3)))
         return rc;
     JSSO_HIP(hipMemset(p->d_status, 0, sizeof(int32_t)));
     // padding slots of the ELL arrays stay zero
@@ -383,6 +432,7 @@ int jsso_plan_create(int32_t n_sys, const int32_t* h_node_off, int32_t n_beam, c
 int jsso_plan_destroy(jsso_plan* plan) {
     if (!plan) return JSSO_OK;
     Plan* q = P(plan);
+    amg_destroy(q);
     for (auto& row : q->cg_graph)
         for (auto& pair : row)
             for (auto& g : pair)
@@ -462,6 +512,8 @@ int jsso_assemble(jsso_plan* plan, const double* d_crds, const double* d_prop_be
     if (rc) return rc;
     rc = launch_assemble(p, s);
     if (rc) return rc;
+    JSSO_HIP(hipMemcpyAsync(p->d_crds, d_crds, (size_t)p->n_nodes * 3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    p->amg_valid = false;
     if (d_K_out && p->nnzb) {
         if ((rc = launch_full_blocks(p, s))) return rc;
         JSSO_HIP(hipMemcpyAsync(d_K_out, p->d_K, (size_t)p->nnzb * 36 * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -557,6 +609,21 @@ int jsso_compliance(jsso_plan* plan, const double* d_f, const double* d_u, int32
     return launch_compliance(p, d_f, d_u, nrhs, scale, d_C, (hipStream_t)stream);
 }
 
+int jsso_plan_set_preconditioner(jsso_plan* plan, int32_t pc) {
+    if (!plan || (pc != JSSO_PC_BLOCK_JACOBI && pc != JSSO_PC_AMG)) {
+        set_error("invalid preconditioner (JSSO_PC_BLOCK_JACOBI or JSSO_PC_AMG)");
+        return JSSO_EINVAL;
+    }
+    P(plan)->precond = pc;
+    return JSSO_OK;
+}
+int jsso_plan_amg_levels(jsso_plan* plan, int32_t* h_nodes, int32_t cap, int32_t* h_levels) {
+    if (!plan || cap < 0 || (cap > 0 && !h_nodes)) {
+        set_error("invalid arguments");
+        return JSSO_EINVAL;
+    }
+    return amg_levels(P(plan), h_nodes, cap, h_levels);
+}
 int jsso_plan_cg_tuning(jsso_plan* plan, int32_t precision, int32_t nrhs, int32_t set_wps, int32_t* h_wps) {
     if (!plan || (precision != JSSO_PREC_FP64 && precision != JSSO_PREC_MIXED) || nrhs < 1 || nrhs > 8 ||
         !(set_wps < 0 || set_wps == 0 || set_wps == 1 || set_wps == 2 || set_wps == 4)) {

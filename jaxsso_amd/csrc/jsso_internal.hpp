@@ -38,6 +38,20 @@ constexpr int WS_FU_SLOT = 13;   // Plan::ws slot of the f.u reduction's per-sli
 constexpr int ROWS_PER_WG = 64;   // block rows per solver slice (one wavefront in the SpMV)
 constexpr int SOLVER_WG = 6 * ROWS_PER_WG;   // threads of the scalar-row solver kernels
 
+// Symmetric sliced ELL of a block pattern (built on the host, see build_ell).
+struct EllPattern {
+    int32_t n_wg = 0;                                  // slices
+    std::vector<int32_t> wg_sys, wg_row0, wg_row1;     // per slice: structure, rows [row0, row1)
+    std::vector<int32_t> sys_wg_off;                   // per structure: first slice
+    std::vector<int32_t> ell_col, ell_vpos;            // per column position: block column, value position
+    std::vector<int32_t> ell_src;                      // per value position: block of the CSR, -1 = padding
+    std::vector<int32_t> meta;                         // per slice: 12 ints
+    std::vector<int32_t> kref;                         // per block of the CSR: 2*vpos (+1 transposed), -1 diagonal
+    int64_t ell_slots = 0;                             // value positions incl. the trailing zero slot
+};
+int build_ell(int32_t n_sys, const int32_t* node_off, const std::vector<int32_t>& rowptr,
+              const std::vector<int32_t>& colind, EllPattern& out);
+
 // Device buffer owned by the plan.
 struct DBuf {
     void* p = nullptr;
@@ -54,6 +68,8 @@ struct CgGraph {
     const void* key[NKEY] = {};
 };
 
+struct AmgState;   // smoothed-aggregation multigrid hierarchy (jsso_amg.hip)
+
 struct Plan {
     // sizes
     int32_t n_sys = 0, n_nodes = 0, n_beam = 0, n_quad = 0, n_cons = 0;
@@ -69,6 +85,7 @@ struct Plan {
     std::vector<int32_t> h_ell_vpos;   // per column position: value position (slot * 64 + lane) of its block
     std::vector<int32_t> h_slice_meta; // per slice: 12 ints, see build_plan
     std::vector<int32_t> h_ell_src;    // per value position: source block in the block CSR, -1 = padding
+    std::vector<int32_t> h_kref;       // per block of the CSR: 2 * value position (+1 when transposed), -1 diagonal
     std::vector<uint8_t> h_fixed;
     bool on_device = false;
     // pattern (device)
@@ -104,6 +121,11 @@ struct Plan {
     int32_t* d_status = nullptr;  // device error flags
     bool assembled = false;
     bool full_valid = false;      // d_K / d_Kc hold the current assembly
+    double* d_crds = nullptr;     // node coordinates of the current assembly (for the AMG null space)
+    // preconditioner of the batched solves (JSSO_PC_*) and the AMG hierarchy
+    int32_t precond = 0;
+    AmgState* amg = nullptr;
+    bool amg_valid = false;       // the AMG numeric set-up matches the current assembly
     // solver workspace, grown on demand
     std::vector<DBuf> ws;
     int ws_nrhs = 0;
@@ -142,6 +164,10 @@ int solve_batched(Plan* p, const double* f, double* u, int nrhs, bool mixed, dou
 int timing_enable(int every);
 int timing_read(double* avg_ms, int64_t* samples, int reset);
 int launch_compliance(Plan* p, const double* f, const double* u, int nrhs, double scale, double* C, hipStream_t s);
+int amg_solve(Plan* p, const double* f, double* u, int nrhs, bool mixed, double tol, int maxit, jsso_info* info,
+              hipStream_t s);
+int amg_levels(Plan* p, int32_t* h_nodes, int32_t cap, int32_t* h_levels);
+void amg_destroy(Plan* p);
 int solve_plain(Plan* p, const double* f, double* u, int nrhs, double tol, int maxit,
                 jsso_info* info, hipStream_t s);
 

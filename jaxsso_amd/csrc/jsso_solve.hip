@@ -37,12 +37,10 @@
 // per launch sequence -- the baseline the batched paths are measured against.
 #include <cstdlib>
 
-#include "jsso_internal.hpp"
+#include "solver_kernels.hpp"
 
 namespace jsso {
 
-constexpr int NWAVES = SOLVER_WG / 64;
-enum { SYS_RUNNING = 0, SYS_CONVERGED = 1, SYS_STALLED = 2 };
 
 // ---------------------------------------------------------------------------
 // sampled event timing of k_cgf (jsso_timing_enable / jsso_timing_read)
@@ -71,25 +69,25 @@ static void timing_start(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev, 
 }
 
 // one launch in `every` is bracketed by events
-static void timing_begin(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev) {
+void timing_begin(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev) {
     ev = nullptr;
     if (g_timing.every <= 0 || (g_timing.counter++ % g_timing.every) != 0) return;
     timing_start(s, ev, 1);
 }
 
 // a graph of `launches` kernels back to back: bracketed whenever timing is on
-static void timing_begin_batch(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev, int launches) {
+void timing_begin_batch(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev, int launches) {
     ev = nullptr;
     if (g_timing.every <= 0) return;
     timing_start(s, ev, launches);
 }
 
-static void timing_end(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>* ev) {
+void timing_end(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>* ev) {
     if (ev) (void)hipEventRecord(ev->second, s);
 }
 
 // fold finished samples into the totals; samples still in flight stay pending
-static void timing_collect() {
+void timing_collect() {
     size_t keep = 0;
     for (size_t i = 0; i < g_timing.pending.size(); ++i) {
         auto e = g_timing.pending[i];
@@ -128,299 +126,10 @@ int timing_read(double* avg_ms, int64_t* samples, int reset) {
     return JSSO_OK;
 }
 
-// ---------------------------------------------------------------------------
-// slices and reductions
-// ---------------------------------------------------------------------------
-struct Slices {
-    const int32_t *sys, *row0, *row1, *sys_off;  // slice -> structure, rows; structure -> first slice
-    const int32_t *ell_vpos, *ell_col;           // symmetric sliced ELL: value position and column per position
-    // per slice, three int4: {sys, row0, row1, first value position},
-    // {first upper column position, upper slots, first lower column position, lower slots},
-    // {first slice of sys, end slice of sys, 0, 0}
-    const int4* meta;
-    int base;           // first slice of the launch (plain path)
-    int count;          // slices in the launch
-    int chunk;          // ceil(count / 8): slices per XCD
-};
-
-// Workgroups are dispatched round-robin over the 8 XCDs; map them so that each
-// XCD processes a contiguous run of slices, identically in every kernel: the
-// vectors a slice touches then stay in one XCD's L2 from kernel to kernel.
-// Returns -1 for the (at most 7) padding workgroups of the grid.
-__device__ __forceinline__ int slice_of_block(const Slices& sl) {
-    const int b = blockIdx.x;
-    const int l = (b & 7) * sl.chunk + (b >> 3);
-    return l < sl.count ? sl.base + l : -1;
-}
-
-static inline dim3 slice_grid(int count) { return dim3(8 * ((count + 7) / 8)); }
-
-// per-(structure, rhs) state, written only by k_decide
-struct SysState {
-    double *bnorm2, *relres, *best;
-    int32_t *active, *status, *stall, *save;
-    int32_t* slice_act;   // per slice: bit h set while load case h of its structure runs
-    int32_t* iters;       // CG iterations run so far, per (structure, rhs)
-};
-
-// Partial-sum buffers, per slice.  The fused CG kernel of iteration k writes
-// cg[k & 1] and reads cg[(k - 1) & 1]; each slice holds [4][rhs] values.
-enum { CG_RR = 0, CG_PQ = 1, CG_RQ = 2, CG_QQ = 3 };
-struct Partials {
-    double* cg[2];           // fused CG: <r,r>, <p,q>, <r,q>, <q,q>
-    double *b, *t, *rtrue;   // <b,b>; true FP64 <r,r> and low-precision <r,r> after a replacement
-    double *rr0, *rr1;       // plain path: alternating <r,z>
-    double* pqp;             // plain path: <p,q>
-    __host__ __device__ double* rr(int it) const { return (it & 1) ? rr1 : rr0; }
-};
-
-// true when every load case of structure `sys` has stopped: its slices have
-// nothing to do (and nothing will read their partial sums)
-template <int NR>
-__device__ __forceinline__ bool structure_done(const SysState& st, int sys) {
-    bool any = false;
-#pragma unroll
-    for (int h = 0; h < NR; ++h) any |= st.active[sys * NR + h] != 0;
-    return !any;
-}
-
-// All-lanes sum over one wavefront, as a butterfly: every step adds a lane's
-// partner value, with partners paired by an involution, so paired lanes add
-// the same two numbers (IEEE addition is commutative) and after six steps
-// every lane holds the bitwise-identical total.  Steps 1-4 pair lanes inside a
-// 16-lane row with DPP moves (quad xor 1, quad xor 2, half-row mirror, row
-// mirror); steps 5-6 exchange rows with the gfx950 permlane swaps, whose two
-// results hold the lane's own value and its partner's (in either order).
-// Requires every lane of the wavefront to be active.
-template <int CTRL>
-__device__ __forceinline__ double dpp_add_partner(double v) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-    return v + __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-__device__ __forceinline__ double join_halves(int lo, int hi) {
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-    v = dpp_add_partner<0xB1>(v);    // quad_perm [1,0,3,2]
-    v = dpp_add_partner<0x4E>(v);    // quad_perm [2,3,0,1]
-    v = dpp_add_partner<0x141>(v);   // row_half_mirror
-    v = dpp_add_partner<0x140>(v);   // row_mirror
-    long long b = __double_as_longlong(v);
-    auto l16 = __builtin_amdgcn_permlane16_swap((int)b, (int)b, false, false);
-    auto h16 = __builtin_amdgcn_permlane16_swap((int)(b >> 32), (int)(b >> 32), false, false);
-    v = join_halves(l16[0], h16[0]) + join_halves(l16[1], h16[1]);
-    b = __double_as_longlong(v);
-    auto l32 = __builtin_amdgcn_permlane32_swap((int)b, (int)b, false, false);
-    auto h32 = __builtin_amdgcn_permlane32_swap((int)(b >> 32), (int)(b >> 32), false, false);
-    return join_halves(l32[0], h32[0]) + join_halves(l32[1], h32[1]);
-}
-
-// Value slots of the sliced ELL: the 36 entries of lane l's block are stored in
-// 16-byte groups of G = 16 / sizeof(T) consecutive entries, lanes interleaved
-// group by group ([slot][36 / G][64 lanes][G]), so one vector load per lane
-// brings G entries and a wavefront's load covers 1 KB of consecutive bytes.
-template <typename T>
-__device__ __forceinline__ void load_block(const T* __restrict__ slot, int lane, T (&av)[36]) {
-    constexpr int G = 16 / sizeof(T);
-    typedef T V __attribute__((ext_vector_type(G)));
-#pragma unroll
-    for (int g = 0; g < 36 / G; ++g) {
-        const V v = *reinterpret_cast<const V*>(slot + ((int64_t)g * ROWS_PER_WG + lane) * G);
-#pragma unroll
-        for (int t = 0; t < G; ++t) av[g * G + t] = v[t];
-    }
-}
-
-// Sum over the whole workgroup, result in every thread.  `lds` holds >= NW*NV.
-template <int NV, int NW>
-__device__ __forceinline__ void group_sum(double (&v)[NV], double* lds) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
-    if (NW == 1) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) lds[wave * NV + i] = v[i];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        double s = 0.0;
-#pragma unroll
-        for (int w = 0; w < NW; ++w) s += lds[w * NV + i];
-        v[i] = s;
-    }
-}
-
-// Totals over the slices of `sys` of one or two per-slice partial arrays (NR
-// values per slice), identical in every thread of every workgroup of the structure.
-template <int NR, int NW>
-__device__ __forceinline__ void system_totals(const Slices& sl, int sys, const double* a, const double* b,
-                                              double (&ta)[NR], double (&tb)[NR], double* lds) {
-    const int s0 = sl.sys_off[sys], s1 = sl.sys_off[sys + 1];
-    double v[2 * NR];
-#pragma unroll
-    for (int i = 0; i < 2 * NR; ++i) v[i] = 0.0;
-    for (int w = s0 + (int)threadIdx.x; w < s1; w += NW * 64) {
-#pragma unroll
-        for (int h = 0; h < NR; ++h) {
-            v[h] += a[(int64_t)w * NR + h];
-            if (b) v[NR + h] += b[(int64_t)w * NR + h];
-        }
-    }
-    group_sum<2 * NR, NW>(v, lds);
-#pragma unroll
-    for (int h = 0; h < NR; ++h) {
-        ta[h] = v[h];
-        tb[h] = v[NR + h];
-    }
-}
-
-// write this slice's partial sums (thread 0)
-template <int NV, int NW>
-__device__ __forceinline__ void write_partials(double (&v)[NV], double* lds, double* dst, int slice) {
-    group_sum<NV, NW>(v, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) dst[(int64_t)slice * NV + i] = v[i];
-    }
-}
 
 // ---------------------------------------------------------------------------
 // batched kernels
 // ---------------------------------------------------------------------------
-
-// b = L^-1 f (constrained DOFs zeroed), x = 0, r_0 = b (buffer 0); partials <b,b>.
-template <typename VT, int NR>
-__global__ __launch_bounds__(SOLVER_WG) void k_init(Slices sl, Partials P, const uint8_t* __restrict__ fixed,
-                                                    const double* __restrict__ Linv, const double* __restrict__ f,
-                                                    double* __restrict__ b64, double* __restrict__ x64,
-                                                    double* __restrict__ xbest, VT* __restrict__ xlo,
-                                                    VT* __restrict__ r0, VT* __restrict__ p) {
-    __shared__ double lds[NWAVES * NR];
-    const int w = slice_of_block(sl);
-    if (w < 0) return;
-    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
-    const bool on = R < 6 * (int64_t)sl.row1[w];
-    double vb[NR];
-#pragma unroll
-    for (int h = 0; h < NR; ++h) vb[h] = 0.0;
-    if (on) {
-        const int64_t i = R / 6;
-        const int r = (int)(R % 6);
-#pragma unroll
-        for (int h = 0; h < NR; ++h) {
-            double s = 0.0;
-#pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                const int64_t dof = 6 * i + c;
-                const double fv = fixed[dof] ? 0.0 : f[dof * NR + h];
-                s += Linv[i * 36 + r * 6 + c] * fv;
-            }
-            const int64_t o = R * NR + h;
-            b64[o] = s;
-            x64[o] = 0.0;
-            xbest[o] = 0.0;
-            xlo[o] = VT(0);
-            r0[o] = VT(s);
-            p[o] = VT(0);
-            vb[h] = s * s;
-        }
-    }
-    write_partials<NR, NWAVES>(vb, lds, P.b, w);
-}
-
-// One wavefront per structure: convergence bookkeeping of its load cases from
-// the slice partials, and the per-slice activity masks the CG kernels test.
-// mode 0 (after k_init): bnorm2, activity.  mode 1 (after k_replace): true
-// relative residual, best iterate, stagnation, status, after `steps` more
-// iterations (counted per system, so the kernel can sit in a captured graph).
-// Every lane computes the same (butterfly) totals; lane 0 writes the state.
-template <int NR>
-__global__ __launch_bounds__(64) void k_decide(Slices sl, Partials P, SysState st, int nsys_struct, int mode,
-                                               double tol, int steps) {
-    const int sys = blockIdx.x;
-    if (sys >= nsys_struct) return;
-    const int lane = threadIdx.x;
-    const int s0 = sl.sys_off[sys], s1 = sl.sys_off[sys + 1];
-    double ta[NR], tb[NR];
-#pragma unroll
-    for (int h = 0; h < NR; ++h) ta[h] = tb[h] = 0.0;
-    for (int w = s0 + lane; w < s1; w += 64) {
-#pragma unroll
-        for (int h = 0; h < NR; ++h) {
-            if (mode == 0) {
-                ta[h] += P.b[(int64_t)w * NR + h];
-            } else {
-                ta[h] += P.t[(int64_t)w * NR + h];
-                tb[h] += P.rtrue[(int64_t)w * NR + h];
-            }
-        }
-    }
-#pragma unroll
-    for (int h = 0; h < NR; ++h) {
-        ta[h] = wave_sum(ta[h]);
-        tb[h] = wave_sum(tb[h]);
-    }
-    int mask = 0;
-#pragma unroll
-    for (int h = 0; h < NR; ++h) {
-        const int k = sys * NR + h;
-        if (mode == 0) {
-            const double bb = ta[h];
-            if (lane == 0) {
-                st.bnorm2[k] = bb;
-                st.relres[k] = bb > 0.0 ? 1.0 : 0.0;
-                st.best[k] = bb > 0.0 ? 1.0 : 0.0;
-                st.active[k] = bb > 0.0 ? 1 : 0;
-                st.status[k] = bb > 0.0 ? SYS_RUNNING : SYS_CONVERGED;
-                st.stall[k] = 0;
-                st.save[k] = 0;
-                st.iters[k] = 0;
-            }
-            mask |= bb > 0.0 ? 1 << h : 0;
-            continue;
-        }
-        if (!st.active[k]) {
-            if (lane == 0) st.save[k] = 0;
-            continue;
-        }
-        const int it = st.iters[k] + steps;
-        const double rel = sqrt(ta[h] / st.bnorm2[k]);
-        const double best = st.best[k];
-        const bool improved = rel < best;
-        const double nbest = improved ? rel : best;
-        const int stall = rel <= 0.9 * best ? it : st.stall[k];   // iteration of the last 10% gain
-        // CG on strongly heterogeneous systems (SIMP densities) plateaus for a
-        // few thousand iterations before it moves on: only a longer flat stretch
-        // is taken as the attainable-accuracy floor
-        const int window = it / 2 > 8192 ? it / 2 : 8192;
-        int active = 1, status = SYS_RUNNING;
-        if (rel <= tol || tb[h] == 0.0) {
-            active = 0;
-            status = SYS_CONVERGED;
-        } else if (!isfinite(rel) || rel > 1e3 * nbest || it - stall > window) {
-            active = 0;
-            status = SYS_STALLED;
-        }
-        if (lane == 0) {
-            st.relres[k] = rel;
-            st.save[k] = improved ? 1 : 0;
-            st.best[k] = nbest;
-            st.stall[k] = stall;
-            st.active[k] = active;
-            st.status[k] = status;
-            st.iters[k] = it;
-        }
-        mask |= active << h;
-    }
-    for (int w = s0 + lane; w < s1; w += 64) st.slice_act[w] = mask;
-}
 
 enum { CG_START = 0, CG_STEP = 1, CG_FRESH = 2 };
 
@@ -734,146 +443,6 @@ __global__ __launch_bounds__(SOLVER_WG) void k_upd(Slices sl, Partials P, SysSta
     }
 }
 
-// r64 = b - A64 x64; r_k = r64 (low precision, buffer k & 1); partials
-// <r64,r64> (FP64, convergence) and <r_k,r_k>.  Same slice layout, wave split
-// and straight-line front end as k_cgf.
-template <typename VT, int NR, int WPS>
-__global__ __launch_bounds__(64 * WPS) void k_replace(Slices sl, Partials P, SysState st,
-                                                      const double* __restrict__ A, const double* __restrict__ b64,
-                                                      const double* __restrict__ x64, VT* __restrict__ rcur) {
-    __shared__ double red[WPS > 1 ? WPS - 1 : 1][6 * NR][64];
-    const int w = slice_of_block(sl);
-    if (w < 0) return;
-    const int4 ma = sl.meta[3 * w];
-    const int4 mb = sl.meta[3 * w + 1];
-    const int amask = st.slice_act[w];
-    if ((amask | ((ma.x | ma.y | ma.z | ma.w | mb.x | mb.y | mb.z | mb.w) < 0)) == 0) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t i = (int64_t)ma.y + lane;
-    const bool on = i < ma.z;
-    const int64_t ii = on ? i : (int64_t)ma.y;
-    const int pup = mb.x, nup = mb.y, plo = mb.z, nlo = mb.w;
-    const int nslot = nup + nlo;
-    const double* Aw = A + 36 * (int64_t)ma.w;
-    double xown[6][NR], bown[6][NR];
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
-#pragma unroll
-        for (int h = 0; h < NR; ++h) {
-            const int64_t o = (6 * ii + c) * NR + h;
-            xown[c][h] = x64[o];
-            bown[c][h] = b64[o];
-        }
-    double acc[6][NR];
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
-#pragma unroll
-        for (int h = 0; h < NR; ++h) acc[c][h] = 0.0;
-    for (int j = wave; j < nslot; j += WPS) {
-        const bool upper = j < nup;
-        const int64_t pos = upper ? (int64_t)pup + (int64_t)j * ROWS_PER_WG + lane
-                                  : (int64_t)plo + (int64_t)(j - nup) * ROWS_PER_WG + lane;
-        const int64_t col = sl.ell_col[pos];
-        const int vp = sl.ell_vpos[pos];
-        double xj[6][NR];
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-#pragma unroll
-            for (int h = 0; h < NR; ++h) xj[c][h] = x64[(6 * col + c) * NR + h];
-        double av[36];
-        if (upper)
-            load_block(Aw + (int64_t)j * 36 * ROWS_PER_WG, lane, av);
-        else
-            load_block(A + (int64_t)(vp >> 6) * 36 * ROWS_PER_WG, vp & (ROWS_PER_WG - 1), av);
-        if (upper) {
-#pragma unroll
-            for (int e = 0; e < 36; ++e)
-#pragma unroll
-                for (int h = 0; h < NR; ++h) acc[e / 6][h] += av[e] * xj[e % 6][h];
-        } else {
-#pragma unroll
-            for (int e = 0; e < 36; ++e)
-#pragma unroll
-                for (int h = 0; h < NR; ++h) acc[e % 6][h] += av[e] * xj[e / 6][h];
-        }
-    }
-    if (WPS > 1) {
-        if (wave > 0) {
-#pragma unroll
-            for (int c = 0; c < 6; ++c)
-#pragma unroll
-                for (int h = 0; h < NR; ++h) red[wave - 1][c * NR + h][lane] = acc[c][h];
-        }
-        __syncthreads();
-        if (wave > 0) return;
-#pragma unroll
-        for (int u = 0; u < WPS - 1; ++u)
-#pragma unroll
-            for (int c = 0; c < 6; ++c)
-#pragma unroll
-                for (int h = 0; h < NR; ++h) acc[c][h] += red[u][c * NR + h][lane];
-    }
-    double vt[NR], vr[NR];
-#pragma unroll
-    for (int h = 0; h < NR; ++h) vt[h] = vr[h] = 0.0;
-    if (on) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-#pragma unroll
-            for (int h = 0; h < NR; ++h) {
-                if (!((amask >> h) & 1)) continue;
-                const int64_t o = (6 * i + c) * NR + h;
-                const double rv = bown[c][h] - (xown[c][h] + acc[c][h]);
-                const VT rl = VT(rv);
-                rcur[o] = rl;
-                vt[h] += rv * rv;
-                vr[h] += (double)rl * (double)rl;
-            }
-    }
-#pragma unroll
-    for (int h = 0; h < NR; ++h) {
-        vt[h] = wave_sum(vt[h]);
-        vr[h] = wave_sum(vr[h]);
-    }
-    if (lane == 0) {
-#pragma unroll
-        for (int h = 0; h < NR; ++h) {
-            P.t[(int64_t)w * NR + h] = vt[h];
-            P.rtrue[(int64_t)w * NR + h] = vr[h];
-        }
-    }
-}
-
-// keep the best iterate: xbest = x64 where the last replacement improved it
-template <int NR>
-__global__ __launch_bounds__(SOLVER_WG) void k_save(Slices sl, const int32_t* __restrict__ save,
-                                                    const double* __restrict__ x64, double* __restrict__ xbest) {
-    const int w = slice_of_block(sl);
-    if (w < 0) return;
-    const int sys = sl.sys[w];
-    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
-    if (R >= 6 * (int64_t)sl.row1[w]) return;
-#pragma unroll
-    for (int h = 0; h < NR; ++h)
-        if (save[sys * NR + h]) xbest[R * NR + h] = x64[R * NR + h];
-}
-
-// u = L^-T x
-template <int NR>
-__global__ __launch_bounds__(256) void k_unscale(int32_t n_nodes, const double* __restrict__ Linv,
-                                                 const double* __restrict__ x64, double* __restrict__ u) {
-    const int64_t R = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (R >= 6 * (int64_t)n_nodes) return;
-    const int64_t i = R / 6;
-    const int r = (int)(R % 6);
-#pragma unroll
-    for (int h = 0; h < NR; ++h) {
-        double s = 0.0;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) s += Linv[i * 36 + c * 6 + r] * x64[(6 * i + c) * NR + h];
-        u[R * NR + h] = s;
-    }
-}
 
 // Compliance-type reductions: part[slice][rhs] = sum over the slice's DOFs of
 // f u; then C[sys][rhs] = scale * (sum of its slices' partials, in order).
@@ -915,17 +484,8 @@ __global__ __launch_bounds__(64) void k_fu_total(Slices sl, int nsys, const doub
 // ---------------------------------------------------------------------------
 // workspace
 // ---------------------------------------------------------------------------
-struct Workspace {
-    double *b64, *x64, *xbest;
-    void *xlo, *r[2], *q[2], *p;
-    SysState st;
-    Partials P;
-};
 
-enum { WS_B, WS_X, WS_XBEST, WS_XLO, WS_R0, WS_R1, WS_Q0, WS_Q1, WS_P, WS_PART, WS_SD, WS_SI, WS_ADJ, WS_N };
-static_assert(WS_ADJ == WS_ADJ_SLOT, "adjoint scratch slot");
-
-static int get_workspace(Plan* pl, int nrhs, Workspace& w) {
+int get_workspace(Plan* pl, int nrhs, Workspace& w) {
     const size_t n = 6 * (size_t)pl->n_nodes * nrhs;
     const size_t nsys = (size_t)pl->n_sys * nrhs;
     const size_t npart = (size_t)pl->n_wg * nrhs;
@@ -978,7 +538,7 @@ static int get_workspace(Plan* pl, int nrhs, Workspace& w) {
     return JSSO_OK;
 }
 
-static int check_status(Plan* p, hipStream_t s) {
+int check_status(Plan* p, hipStream_t s) {
     JSSO_REQUIRE(p->on_device, JSSO_EINVAL, "plan not uploaded to the device");
     int32_t status = 0;
     JSSO_HIP(hipMemcpyAsync(&status, p->d_status, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -992,7 +552,7 @@ static int check_status(Plan* p, hipStream_t s) {
     return JSSO_OK;
 }
 
-static Slices make_slices(Plan* pl, int base, int count) {
+Slices make_slices(Plan* pl, int base, int count) {
     return Slices{pl->d_wg_sys,  pl->d_wg_row0, pl->d_wg_row1, pl->d_sys_wg_off,
                   pl->d_ell_vpos, pl->d_ell_col, (const int4*)pl->d_slice_meta, base,
                   count,          (count + 7) / 8};
@@ -1139,49 +699,13 @@ static void launch_replace(Plan* pl, int wps, const Slices& sl, const Workspace&
 // convergence test counts iterations per system on the device, so with an
 // even chunk length the same two graphs serve every chunk.  JSSO_CG_GRAPH=0
 // launches eagerly.
-static bool graphs_enabled() {
+bool graphs_enabled() {
     const char* e = getenv("JSSO_CG_GRAPH");
     return !(e && e[0] == '0');
 }
 
-template <typename F>
-static hipGraphExec_t capture_graph(Plan* pl, CgGraph& g, const void* const (&key)[CgGraph::NKEY], int wps,
-                                    int steps, double tol, F&& enqueue) {
-    bool same = g.exec && g.wps == wps && g.steps == steps && g.tol == tol;
-    for (int i = 0; same && i < CgGraph::NKEY; ++i) same = g.key[i] == key[i];
-    if (same) return g.exec;
-    if (g.exec) {
-        (void)hipGraphExecDestroy(g.exec);
-        g.exec = nullptr;
-    }
-    if (!pl->capture_stream && hipStreamCreateWithFlags(&pl->capture_stream, hipStreamNonBlocking) != hipSuccess) {
-        pl->graphs_ok = false;
-        (void)hipGetLastError();
-        return nullptr;
-    }
-    hipStream_t cs = pl->capture_stream;
-    hipGraph_t graph = nullptr;
-    bool ok = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) == hipSuccess;
-    if (ok) {
-        enqueue(cs);
-        ok = hipStreamEndCapture(cs, &graph) == hipSuccess && graph;
-    }
-    if (ok) ok = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) == hipSuccess;
-    if (graph) (void)hipGraphDestroy(graph);
-    if (!ok) {
-        g.exec = nullptr;
-        pl->graphs_ok = false;
-        (void)hipGetLastError();
-        return nullptr;
-    }
-    g.wps = wps;
-    g.steps = steps;
-    g.tol = tol;
-    for (int i = 0; i < CgGraph::NKEY; ++i) g.key[i] = key[i];
-    return g.exec;
-}
 
-static int ensure_flag_buffers(Plan* pl, size_t nsys) {
+int ensure_flag_buffers(Plan* pl, size_t nsys) {
     if (pl->h_flags_count < 2 * nsys) {
         if (pl->h_flags) JSSO_HIP(hipHostFree(pl->h_flags));
         pl->h_flags = nullptr;
@@ -1382,6 +906,7 @@ int solve_batched(Plan* p, const double* f, double* u, int nrhs, bool mixed, dou
                   jsso_info* info, hipStream_t s) {
     int rc = check_status(p, s);
     if (rc) return rc;
+    if (p->precond == JSSO_PC_AMG) return amg_solve(p, f, u, nrhs, mixed, tol, maxit, info, s);
     if (mixed) return dispatch_nr<float, float>(p, f, u, nrhs, tol, maxit, replace_interval(true), info, s);
     return dispatch_nr<double, double>(p, f, u, nrhs, tol, maxit, replace_interval(false), info, s);
 }

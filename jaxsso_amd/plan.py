@@ -18,6 +18,7 @@ from . import _native as nat
 
 DEFAULT_TOL = 1e-10
 DEFAULT_MAXIT = 200000
+PRECONDITIONERS = {"block_jacobi": 0, "amg": 1}
 
 
 class FEPlan:
@@ -47,6 +48,7 @@ class FEPlan:
         sizes = np.zeros(6, np.int64)
         nat.check(L.jsso_plan_sizes(self._h, nat.ptr(sizes)))
         self.nnzb = int(sizes[1])
+        self.preconditioner = "block_jacobi"
         self.last_info = None
         self._assembled_key = None
 
@@ -78,10 +80,27 @@ class FEPlan:
         self._keep = (crds, pb, pq)
         return Kout
 
+    def set_preconditioner(self, name: str) -> None:
+        """Preconditioner of the batched precisions: 'block_jacobi' (split block-Jacobi
+        scaling) or 'amg' (smoothed-aggregation multigrid V-cycle)."""
+        if name not in PRECONDITIONERS:
+            raise ValueError(f"unknown preconditioner {name!r} ({sorted(PRECONDITIONERS)})")
+        nat.check(nat.load_library().jsso_plan_set_preconditioner(self._h, PRECONDITIONERS[name]))
+        self.preconditioner = name
+
+    def amg_levels(self) -> list:
+        """Block rows of every level of the plan's AMG hierarchy (built on the host)."""
+        nodes = np.zeros(32, np.int32)
+        nl = ctypes.c_int32(0)
+        nat.check(nat.load_library().jsso_plan_amg_levels(self._h, nat.ptr(nodes), 32, ctypes.byref(nl)))
+        return [int(v) for v in nodes[:nl.value]]
+
     def solve(self, f: torch.Tensor, precision: str = "fp64", tol: float = DEFAULT_TOL,
-              maxit: int = DEFAULT_MAXIT, warn: bool = True) -> torch.Tensor:
+              maxit: int = DEFAULT_MAXIT, warn: bool = True, preconditioner: str | None = None) -> torch.Tensor:
         """Solve K u = f (u = 0 at supports).  f: (ndof,) or (ndof, nrhs) FP64 on the device."""
         L = nat.lib()
+        if preconditioner is not None and preconditioner != self.preconditioner:
+            self.set_preconditioner(preconditioner)
         f = _dev64(f)
         vec = f.dim() == 1
         f2 = f.reshape(self.ndof, -1).contiguous()

@@ -1,0 +1,101 @@
+"""Smoothed-aggregation AMG preconditioned CG (JSSO_PC_AMG) against the oracle,
+through the C ABI."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from jaxsso_amd import workloads as wl
+from jaxsso_amd.batch import StructureBatch
+from jaxsso_amd.plan import FEPlan
+from oracle import gradient as og
+from oracle import solve as osolve
+from oracle import system
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=float)
+    b = np.asarray(b, dtype=float)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def oracle_u(st, f=None):
+    K = system.assemble_K(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads)
+    return osolve.solve_reduced(K, st.known_id, st.nodal_loads if f is None else f)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("case", ["cfg0", "cfg1", "barrel", "arch"])
+def test_amg_solve_matches_oracle(case, precision):
+    st = getattr(wl, case)() if case.startswith("cfg") else getattr(wl, "notebook_" + case)()
+    print(f"amg solve {case} {precision}", flush=True)
+    plan = FEPlan([0, st.n_node], st.cnct_beamcols, st.cnct_quads, st.known_id)
+    plan.set_preconditioner("amg")
+    plan.assemble(torch.tensor(st.crds), torch.tensor(st.prop_beamcols), torch.tensor(st.prop_quads))
+    tol = 1e-11 if case == "arch" else 1e-10
+    u = plan.solve(torch.tensor(st.nodal_loads), precision=precision, tol=tol).cpu().numpy()
+    info = plan.last_info
+    ur = oracle_u(st)
+    assert info["converged"] == info["n_systems"] == 1, info
+    assert info["levels"] == len(plan.amg_levels()) >= 2, info
+    assert rel(u, ur) < 1e-8, (rel(u, ur), info)
+    C, Cr = 0.5 * st.nodal_loads @ u, 0.5 * st.nodal_loads @ ur
+    assert abs(C - Cr) / abs(Cr) < 1e-10
+    assert np.all(u[st.known_id] == 0.0)
+    print(f"  iterations {info['iterations']}, levels {plan.amg_levels()}", flush=True)
+    if case == "cfg1":
+        # block-Jacobi CG needs ~2 900 (FP64) / ~3 700 (mixed) iterations here
+        assert info["iterations"] <= 400, info
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("nrhs", [1, 2, 4])
+def test_amg_batch_of_uneven_designs(precision, nrhs):
+    designs = [wl.cfg1(0), wl.gridshell(9, 10.0, 2.0, 0.05, cfg=3, design=5, half_span_cases=True), wl.cfg0()]
+    print(f"amg batch {precision} nrhs {nrhs}", flush=True)
+    b = StructureBatch(designs, n_cases=nrhs)
+    u = b.solve(precision=precision, tol=1e-10, preconditioner="amg").cpu().numpy().reshape(-1, nrhs)
+    info = b.plan.last_info
+    assert info["converged"] == info["n_systems"] == 3 * nrhs, info
+    for i, st in enumerate(designs):
+        cases = st.load_cases()
+        o = 6 * int(b.node_off[i])
+        for c in range(min(nrhs, cases.shape[0])):
+            assert rel(u[o:o + st.ndof, c], oracle_u(st, cases[c])) < 1e-8, (i, c, info)
+
+
+def test_amg_graph_and_eager_agree_bitwise(monkeypatch):
+    designs = [wl.cfg1(d) for d in range(3)]
+    b = StructureBatch(designs)
+    monkeypatch.setenv("JSSO_CG_GRAPH", "1")
+    ug = b.solve(precision="mixed", tol=1e-10, preconditioner="amg").cpu().numpy()
+    ig = dict(b.plan.last_info)
+    monkeypatch.setenv("JSSO_CG_GRAPH", "0")
+    ue = b.solve(precision="mixed", tol=1e-10, preconditioner="amg").cpu().numpy()
+    assert ig["converged"] == 3 and ig["iterations"] == b.plan.last_info["iterations"]
+    assert np.array_equal(ug, ue)
+
+
+def test_amg_set_up_follows_the_assembly():
+    """A new assembly on the same plan (other coordinates and thicknesses)
+    rebuilds the numeric hierarchy before the next solve."""
+    a, c = wl.cfg1(0), wl.cfg1(5)
+    plan = FEPlan([0, a.n_node], a.cnct_beamcols, a.cnct_quads, a.known_id)
+    plan.set_preconditioner("amg")
+    for st in (a, c, a):
+        plan.assemble(torch.tensor(st.crds), None, torch.tensor(st.prop_quads))
+        u = plan.solve(torch.tensor(st.nodal_loads), precision="mixed", tol=1e-10).cpu().numpy()
+        assert rel(u, oracle_u(st)) < 1e-8, plan.last_info
+
+
+def test_amg_compliance_gradient_matches_oracle():
+    st = wl.cfg1()
+    b = StructureBatch([st])
+    C, (dc, db, dq) = b.compliance_and_gradient(precision="mixed", tol=1e-10, preconditioner="amg")
+    ref = og.compliance_and_gradient(st)
+    assert abs(C.cpu().numpy()[0, 0] - ref["C"]) / ref["C"] < 1e-10
+    assert rel(dc.cpu().numpy().reshape(ref["dcrds"].shape), ref["dcrds"]) < 1e-8
+    assert rel(dq.cpu().numpy().reshape(ref["dprop_quad"].shape), ref["dprop_quad"]) < 1e-8

@@ -125,3 +125,30 @@ def test_quad_area_matches_reference_formula():
     from jaxsso_amd.element import Quad
     c = np.array([0, 0, 0, 2, 0, 0, 2, 3, 0, 0, 3, 0.0])
     assert abs(Quad.A(c) - 6.0) < 1e-14
+
+
+def test_amg_hierarchy_is_per_structure_and_coarsens():
+    """The symbolic AMG hierarchy (host only): every level is smaller than the one
+    above, a batched plan's levels are the sums of its structures' levels (the
+    aggregation never crosses structures), and the preconditioner setting is
+    validated."""
+    def plan_of(structs):
+        off = np.cumsum([0] + [s.n_node for s in structs])
+        cq = np.concatenate([s.cnct_quads + o for s, o in zip(structs, off)])
+        kn = np.concatenate([s.known_id + 6 * o for s, o in zip(structs, off)])
+        return FEPlan(off, np.zeros((0, 2)), cq, kn)
+
+    single = [plan_of([wl.cfg1(d)]).amg_levels() for d in range(3)]
+    assert single[0] == single[1] == single[2]
+    lv = single[0]
+    assert lv[0] == 1681 and all(a > b for a, b in zip(lv, lv[1:])) and lv[-1] <= 16
+    batch = plan_of([wl.cfg1(d) for d in range(3)]).amg_levels()
+    assert batch == [3 * v for v in lv]
+    gs = wl.gridshell(24)
+    g = FEPlan([0, gs.n_node], gs.cnct_beamcols, np.zeros((0, 4)), gs.known_id).amg_levels()
+    assert g[0] == 625 and g[-1] <= 16 and all(a > b for a, b in zip(g, g[1:]))
+    p = plan_of([wl.cfg1()])
+    with pytest.raises(ValueError):
+        p.set_preconditioner("ilu")
+    p.set_preconditioner("amg")
+    assert p.preconditioner == "amg"
