@@ -4,14 +4,17 @@
 Workload: the 40x40 MITC-4 shell (1,681 nodes, 1,600 quads, 10,086 DOF; four
 fully fixed corners, -2 kN/node, t = 0.10 U(0.9, 1.1) per element), as a batch
 of B seeded thickness designs (design 0 is the reference configuration).  One
-step = assemble K for every design, solve (FP32 CG with FP64 reliable updates,
-to a 1e-10 scaled residual), reduce the compliance in FP64 and run the adjoint
-kernel for d C / d(node coordinates) and d C / d(element properties).  Inputs
-are resident on the GPU before timing starts.
+step = assemble K for every design, build the numeric AMG hierarchy, solve
+(flexible CG preconditioned by the FP32 smoothed-aggregation V-cycle, FP32
+recurrences with FP64 residual replacement, to a 1e-10 scaled residual), reduce
+the compliance in FP64 and run the adjoint kernel for d C / d(node coordinates)
+and d C / d(element properties).  Inputs are resident on the GPU before timing
+starts.
 
 Also reported: the plain FP64 one-structure-per-launch GPU path, the batched
-FP64 path, the error of the timed path against the plain FP64 path and the FP64
-oracle, the dominant kernel's roofline (HIP events on the solver stream),
+FP64 path, the block-Jacobi-preconditioned batched path, the error of the timed
+path against the plain FP64 path and the FP64 oracle, the dominant kernel's
+roofline (HIP events on the solver stream),
 element stiffness + assembly throughput (elements/s), the oracle timed on this
 host's cores, the 200x200 MITC-4 shell (242,406 DOF) as a single structure on
 every solver path with its error against the committed oracle fixture, and two
@@ -89,25 +92,35 @@ def main():
     value = args.batch * args.steps / (t1 - t0)
     print(f"# timed: {ms_per_step:.1f} ms/step, {value:.1f} evals/s, CG iters {iters}, info {info}", flush=True)
 
-    # -- dominant kernel roofline: the fused CG iteration k_cgf (scaled off-diagonal
-    # blocks once, their indices; r_{k-1}, q_{k-1}, p, x read and r_k, q_k, p, x
-    # written once per DOF and load case)
+    # -- dominant kernel roofline: k_pcg_pre, the level-0 update and first
+    # smoothing step of every iteration (the stored upper blocks once, their
+    # indices; r_{k-1}, q, p, x read and r_k, x, t0 written once per DOF).  The
+    # timed steps run the iterations as captured HIP graphs; one more solve with
+    # eager launches brackets every k_pcg_pre launch with HIP events.
     p = batch.plan
     nnzb_off = p.nnzb - p.n_nodes
     s_m = 4 if args.precision == "mixed" else 8
     s_v = s_m
     ndof = p.ndof
+    os.environ["JSSO_CG_GRAPH"] = "0"
+    L.jsso_timing_read(None, None, 1)
+    L.jsso_timing_enable(1)
+    batch.solve(precision=args.precision, tol=args.tol, assemble=False)
+    torch.cuda.synchronize()
+    L.jsso_timing_enable(0)
+    os.environ.pop("JSSO_CG_GRAPH")
+    L.jsso_timing_read(nat.ptr(avg_ms), nat.ptr(nsamp), 1)
     # each off-diagonal block pair stored once (36 values), both halves indexed
     # (a column each, plus the value position of the transposed half)
-    alg_bytes = (nnzb_off // 2) * 36 * s_m + nnzb_off * 4 + (nnzb_off // 2) * 4 + 8 * ndof * s_v
+    alg_bytes = (nnzb_off // 2) * 36 * s_m + nnzb_off * 4 + (nnzb_off // 2) * 4 + 7 * ndof * s_v
     achieved = alg_bytes / (avg_ms[0] * 1e-3) / 1e9 if avg_ms[0] > 0 else None
-    traffic, traffic_src = _measured_traffic(args, info.get("waves_per_slice"))
-    roofline = {"kernel": f"k_cgf<{'float' if s_m == 4 else 'double'}>", "bound": "hbm",
+    kname = f"k_pcg_pre<{'float, float' if s_m == 4 else 'double, double'}, 1, false, 4>"
+    traffic, traffic_src = _measured_traffic(args, kname)
+    roofline = {"kernel": kname, "bound": "hbm",
                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                 "traffic_source": traffic_src,
                 "avg_launch_us": avg_ms[0] * 1e3, "sampled_launches": int(nsamp[0]),
-                "waves_per_slice": info.get("waves_per_slice"),
                 "alg_bytes_per_launch": int(alg_bytes)}
 
     # -- element stiffness + assembly alone (elements/s)
@@ -128,11 +141,24 @@ def main():
     torch.cuda.synchronize()
     plain_eps = args.plain_designs / (time.perf_counter() - t0)
     plain_iters = sub.plan.last_info["iterations"]
+    step(batch, "fp64", args.tol)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     Cf, dcf, dqf = step(batch, "fp64", args.tol)
     torch.cuda.synchronize()
     fp64_eps = args.batch / (time.perf_counter() - t0)
     fp64_iters = batch.plan.last_info["iterations"]
+    # the same batch with the block-Jacobi preconditioner (fused one-kernel CG iteration)
+    batch.plan.set_preconditioner("block_jacobi")
+    step(batch, args.precision, args.tol)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    step(batch, args.precision, args.tol)
+    torch.cuda.synchronize()
+    bj_eps = args.batch / (time.perf_counter() - t0)
+    bj_iters = batch.plan.last_info["iterations"]
+    batch.plan.set_preconditioner("amg")
+    C, dc, dq = step(batch, args.precision, args.tol)     # the timed path's results for the error checks
 
     nd = args.plain_designs
     n_nodes_sub = sub.plan.n_nodes
@@ -177,11 +203,14 @@ def main():
                                f"batch of {args.batch} seeded thickness designs per step; compliance + "
                                "gradient w.r.t. all node coordinates and all element properties",
                    "batch": args.batch, "tol": args.tol, "precision": args.precision,
+                   "preconditioner": "smoothed-aggregation AMG V-cycle (FP32)",
                    "cg_iterations": iters[-1], "parallelism": "batched structures, 1 GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_fp64_evals_per_s": plain_eps, "plain_fp64_cg_iterations": plain_iters,
         "fp64_batched_evals_per_s": fp64_eps, "fp64_batched_cg_iterations": fp64_iters,
+        "block_jacobi_batched_evals_per_s": bj_eps, "block_jacobi_batched_cg_iterations": bj_iters,
+        "amg_levels": batch.plan.amg_levels(),
         "speedup_vs_plain_fp64": value / plain_eps,
         "rel_err_vs_plain_fp64_gpu": err_plain, "rel_err_vs_fp64_oracle": err_oracle,
         "assembly_ms_per_step": asm_ms, "elements_per_s": n_el / (asm_ms * 1e-3),
@@ -207,18 +236,20 @@ def shell200(tol):
     res = {"workload": "MITC-4 shell 200x200 over 20 m (40,401 nodes, 40,000 quads, 242,406 DOF), "
                        "edges pinned, smooth Fourier thickness; one structure per evaluation"}
     rel = lambda a, r: float(np.linalg.norm(np.asarray(a) - np.asarray(r)) / np.linalg.norm(np.asarray(r)))
-    for prec in ("mixed", "fp64", "plain"):
-        b.compliance_and_gradient(precision=prec, tol=tol)     # warm-up (and kernel tuning)
+    for key, prec, pc in (("mixed", "mixed", "amg"), ("fp64", "fp64", "amg"),
+                          ("mixed_block_jacobi", "mixed", "block_jacobi"), ("plain", "plain", None)):
+        b.compliance_and_gradient(precision=prec, tol=tol, preconditioner=pc)     # warm-up (and kernel tuning)
         torch.cuda.synchronize()
-        reps = 2
+        reps = 3 if pc == "amg" else 2
         t0 = time.perf_counter()
         for _ in range(reps):
-            C, (dc, _, dq) = b.compliance_and_gradient(precision=prec, tol=tol)
+            C, (dc, _, dq) = b.compliance_and_gradient(precision=prec, tol=tol, preconditioner=pc)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / reps
         dcn = dc.cpu().numpy().reshape(st.n_node, 3)
         dqn = dq.cpu().numpy().reshape(st.n_quad, 5)
-        res[prec] = {"evals_per_s": 1.0 / dt, "ms_per_eval": dt * 1e3,
+        res[key] = {"preconditioner": pc or "block_jacobi (plain path)",
+                     "evals_per_s": 1.0 / dt, "ms_per_eval": dt * 1e3,
                      "cg_iterations": b.plan.last_info["iterations"],
                      "waves_per_slice": b.plan.last_info.get("waves_per_slice"),
                      "rel_err_vs_fp64_oracle": {
@@ -227,22 +258,23 @@ def shell200(tol):
                          "grad_z_sample": rel(dcn[fx["dz_node"], 2], fx["dz"]),
                          "grad_t_sample": rel(dqn[fx["dt_elem"], 0], fx["dt"])}}
     res["speedup_mixed_vs_plain_fp64"] = res["mixed"]["evals_per_s"] / res["plain"]["evals_per_s"]
+    res["speedup_mixed_vs_mixed_block_jacobi"] = (res["mixed"]["evals_per_s"] /
+                                                  res["mixed_block_jacobi"]["evals_per_s"])
     return res
 
 
-def _measured_traffic(args, wps):
-    """HBM bytes per launch of k_cgf<float> on this workload, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a copy-kernel
-    calibration), when it was measured for the variant (waves per slice) this run
-    chose; None for other workloads, precisions or variants."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "cg_batch128_shell40_traffic.txt")
-    if args.batch != 128 or args.precision != "mixed" or not os.path.exists(path):
+def _measured_traffic(args, kname):
+    """HBM bytes per launch of the roofline kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a
+    copy-kernel calibration); None for other workloads or kernels."""
+    name = "amg_batch128_shell40_traffic.txt"
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
+    if args.batch != 128 or not os.path.exists(path):
         return None, None
     with open(path) as fh:
         for line in fh:
-            if line.startswith(f"traffic_bytes_per_launch k_cgf<float, float, 1, 1, {wps}>:"):
-                return int(line.split(":")[1]), "profiles/cg_batch128_shell40_traffic.txt"
+            if line.startswith(f"traffic_bytes_per_launch {kname}:"):
+                return int(line.split(":")[1]), "profiles/" + name
     return None, None
 
 
@@ -266,7 +298,9 @@ def gridshell_batch(tol, n_designs=256):
     import oracle.gradient as og
     designs = wl.cfg3(n_designs)
     b = StructureBatch(designs)
-    (C, (dc, _, _)), dt = _timed(lambda: b.compliance_and_gradient(precision="mixed", tol=tol))
+    _, dt_bj = _timed(lambda: b.compliance_and_gradient(precision="mixed", tol=tol, preconditioner="block_jacobi"))
+    bj_iters = b.plan.last_info["iterations"]
+    (C, (dc, _, _)), dt = _timed(lambda: b.compliance_and_gradient(precision="mixed", tol=tol, preconditioner="amg"))
     st = designs[0]
     C = C.cpu().numpy()
     dz = b.split_nodes(dc.cpu().numpy())[0][:, 2, :]
@@ -278,17 +312,17 @@ def gridshell_batch(tol, n_designs=256):
         err_g = max(err_g, float(np.linalg.norm(g - r) / np.linalg.norm(r)))
     return {"workload": f"{n_designs} beam gridshells, 8..32 bays (seeded), perimeter pinned, 4 load cases each, "
                         f"{b.plan.ndof} DOF in one plan", "precision": "mixed",
-            "evals_per_s": n_designs * 4 / dt, "ms_per_step": dt * 1e3,
+            "preconditioner": "amg", "evals_per_s": n_designs * 4 / dt, "ms_per_step": dt * 1e3,
             "cg_iterations": b.plan.last_info["iterations"], "converged": b.plan.last_info["converged"],
-            "waves_per_slice": b.plan.last_info.get("waves_per_slice"),
+            "block_jacobi_evals_per_s": n_designs * 4 / dt_bj, "block_jacobi_cg_iterations": bj_iters,
             "rel_err_vs_fp64_oracle_design0": {"C": err_c, "grad_z_free": err_g}}
 
 
 def simp_batch(tol, n_designs=64):
     """64 SIMP shells (100x100 MITC-4, 61,206 DOF each, E = rho^3 30 GPa) in one
-    plan: compliance, dC/dz and dC/drho (chain rule through E) per design.  FP64
-    CG: with a 1e6 stiffness contrast the FP32 recurrences need ~5x the
-    iterations (61k vs 11.5k), which the halved bytes do not repay."""
+    plan: compliance, dC/dz and dC/drho (chain rule through E) per design, AMG
+    preconditioned; FP64 recurrences (with a 1e6 stiffness contrast the FP32
+    ones need ~1.7x the iterations)."""
     import torch
     from jaxsso_amd import workloads as wl
     from jaxsso_amd.batch import StructureBatch
@@ -304,7 +338,7 @@ def simp_batch(tol, n_designs=64):
         drho = dq.reshape(-1, 5)[:, 1] * 3.0 * rho ** 2 * 30e9
         return C, dc, drho
 
-    (C, dc, drho), dt = _timed(step)
+    (C, dc, drho), dt = _timed(step, reps=2)
     C = C.cpu().numpy()[:, 0]
     dc = dc.cpu().numpy().reshape(-1, 3)
     drho = drho.cpu().numpy()
@@ -320,7 +354,7 @@ def simp_batch(tol, n_designs=64):
                               "grad_z_sample": rel(dc[o + np.asarray(r["dz_node"]), 2], r["dz"]),
                               "grad_rho_sample": rel(drho[q0 + np.asarray(r["drho_elem"])], r["drho"])}
     return {"workload": f"{n_designs} SIMP MITC-4 shells 100x100 over 15 m (10,201 nodes, 10,000 quads each), "
-                        f"{b.plan.ndof} DOF in one plan", "precision": "fp64",
+                        f"{b.plan.ndof} DOF in one plan", "precision": "fp64", "preconditioner": "amg",
             "evals_per_s": n_designs / dt, "ms_per_step": dt * 1e3,
             "cg_iterations": b.plan.last_info["iterations"], "converged": b.plan.last_info["converged"],
             "waves_per_slice": b.plan.last_info.get("waves_per_slice"),

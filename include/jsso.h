@@ -66,8 +66,9 @@ enum {
 
 /* preconditioner of the batched precisions (jsso_plan_set_preconditioner) */
 enum {
-    JSSO_PC_BLOCK_JACOBI = 0, /* split block-Jacobi scaling (the default)              */
-    JSSO_PC_AMG = 1           /* smoothed-aggregation AMG V-cycle on the scaled system  */
+    JSSO_PC_BLOCK_JACOBI = 0, /* split block-Jacobi scaling                             */
+    JSSO_PC_AMG = 1           /* smoothed-aggregation AMG V-cycle on the scaled system
+                                 (the default)                                          */
 };
 
 typedef struct {
@@ -170,9 +171,11 @@ int jsso_plan_set_preconditioner(jsso_plan* plan, int32_t pc);
 int jsso_plan_amg_levels(jsso_plan* plan, int32_t* h_nodes, int32_t cap, int32_t* h_levels);
 
 /* Measurement hook (no counterpart in the reference): sampled HIP-event timing
- * of the batched solver's dominant kernel, the fused CG iteration that applies
- * the scaled stiffness.  every > 0 brackets one such launch in `every` with events on
- * the solver's stream; 0 disables.  jsso_timing_read returns the mean duration
+ * of the batched solver's dominant kernel: with block-Jacobi the fused CG
+ * iteration that applies the scaled stiffness (a captured chunk of iterations
+ * counts as that many launches); with AMG the level-0 update and pre-smoothing
+ * kernel, sampled only where it is launched eagerly (JSSO_CG_GRAPH=0).  every > 0
+ * brackets one such launch in `every` with events on the solver's stream; 0 disables.  jsso_timing_read returns the mean duration
  * (ms) and the number of sampled launches, optionally resetting the counters. */
 int jsso_timing_enable(int32_t every);
 int jsso_timing_read(double* h_avg_ms, int64_t* h_samples, int32_t reset);

@@ -24,9 +24,11 @@
 //   nodes; its last level is inverted densely.
 //
 // V-cycle (symmetric, so M is SPD): one damped-Jacobi pre- and post-smoothing
-// step z = r / theta with theta = (rho + rho/30) / 2 (the one-step Chebyshev
-// smoother on [rho/30, rho]), restriction P^T, the coarse correction, and
-// prolongation P.  The V-cycle runs in FP32; level 0 runs in the solve's
+// step z = r / theta with theta = 0.62 rho (the optimal one-step Chebyshev
+// smoother on [rho/30, rho] would take 0.517 rho, which needs rho to within a
+// few per cent: on heterogeneous SIMP shells a short power iteration reaches
+// only 0.83 rho, and the smoother must stay convergent, 2 theta > rho, there),
+// restriction P^T, the coarse correction, and prolongation P.  The V-cycle runs in FP32; level 0 runs in the solve's
 // precision.  The outer CG keeps the batched path's reliable-update scheme:
 // FP32 (or FP64) recurrences, the FP64 true residual every `replace_every`
 // iterations, convergence and the best iterate decided there.
@@ -43,7 +45,9 @@ namespace jsso {
 constexpr int AMG_MAXC = 16;       // nodes of a structure's directly solved coarsest level (<= 96 DOF)
 constexpr int AMG_QR_RPL = 5;      // null-space rows per lane in the per-aggregate QR
 constexpr int AMG_MAX_AGG = AMG_QR_RPL * 64 / 6;   // nodes per aggregate (53)
-constexpr int AMG_POWER_STEPS = 10;
+constexpr int AMG_POWER_STEPS = 12;
+constexpr double AMG_THETA = 0.62;        // smoothing step z = r / (AMG_THETA rho)
+constexpr int AMG_STALL_WINDOW = 256;     // iterations without a 10% gain that end a solve (floor)
 constexpr double AMG_RHO_SAFETY = 1.12;   // the power iteration approaches rho from below
 constexpr int AMG_MAXLEVELS = 12;
 
@@ -619,7 +623,7 @@ __global__ __launch_bounds__(64 * WPS) void k_amg_power(Slices sl, const float* 
 }
 
 // rho = safety * |A v| (|v| = 1) per continuing structure; omega = 4/(3 rho);
-// theta = (rho + rho/30) / 2
+// theta = AMG_THETA rho
 __global__ void k_amg_theta(int32_t ns, const int32_t* __restrict__ sys_wg_off, const double* __restrict__ part,
                             int last, const uint8_t* __restrict__ cont, int32_t n_wg, double* __restrict__ rho,
                             double* __restrict__ omega, float* __restrict__ theta) {
@@ -632,7 +636,7 @@ __global__ void k_amg_theta(int32_t ns, const int32_t* __restrict__ sys_wg_off, 
     if (!(r > 0.0) || !isfinite(r)) r = 1.0;
     rho[s] = r;
     omega[s] = 4.0 / (3.0 * r);
-    theta[s] = (float)(0.5 * (r + r / 30.0));
+    theta[s] = (float)(AMG_THETA * r);
 }
 
 // P(i, J) = sum over its products of ((1 - omega) delta_ij I - omega A_off(i, j)) Q_j;
@@ -1700,19 +1704,24 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
     }
 }
 
-// one ordinary PCG iteration k
+// one ordinary PCG iteration k; `timed` brackets the level-0 update and
+// pre-smoothing kernel (the iteration's largest) with the sampled events of
+// jsso_timing_enable (eager launches only)
 template <typename MT, typename VT, int NR>
 static void amg_step(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs<VT>& bf, const MT* A0, int k,
-                     hipStream_t s) {
+                     hipStream_t s, bool timed = false) {
     const Slices sl = level_slices(pl, A, 0);
     const dim3 g = slice_grid(pl->n_wg);
     wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
         constexpr int WP = decltype(W)::value;
         hipLaunchKernelGGL((k_pcg_dir<MT, VT, NR, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A0,
                            (const VT*)bf.z, (const VT*)bf.p[(k - 1) & 1], bf.p[k & 1], bf.q);
+        std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
+        if (timed) timing_begin(s, ev);
         hipLaunchKernelGGL((k_pcg_pre<MT, VT, NR, false, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A0,
                            (const float*)A.dev[0].d_theta, (const VT*)bf.r[(k - 1) & 1], bf.r[k & 1],
                            (const VT*)bf.q, (const VT*)bf.p[k & 1], bf.xlo, bf.t0);
+        if (timed) timing_end(s, ev);
     });
     amg_vcycle_rest<MT, VT, NR>(pl, A, w, bf, A0, k, s);
 }
@@ -1734,7 +1743,7 @@ static void amg_replace(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs
                        bf.xlo, w.x64);
     hipLaunchKernelGGL((k_replace<VT, NR, 1>), g, dim3(64), 0, s, sl, w.P, w.st, (const double*)pl->d_Ke64,
                        (const double*)w.b64, (const double*)w.x64, bf.r[k & 1]);
-    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, steps);
+    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, steps, AMG_STALL_WINDOW);
     hipLaunchKernelGGL((k_save<NR>), g, dim3(SOLVER_WG), 0, s, sl, (const int32_t*)w.st.save, (const double*)w.x64,
                        w.xbest);
     wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
@@ -1807,7 +1816,7 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
     JSSO_HIP(hipMemsetAsync(w.P.cg[0], 0, 2 * 4 * (size_t)pl->n_wg * NR * sizeof(double), s));
     hipLaunchKernelGGL((k_init<VT, NR>), gridS, dim3(SOLVER_WG), 0, s, sl, w.P, pl->d_fixed, pl->d_Linv, f, w.b64,
                        w.x64, w.xbest, bf.xlo, bf.r[0], bf.p[0]);
-    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 0, tol, 0);
+    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 0, tol, 0, 0);
     wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
         constexpr int WP = decltype(W)::value;
         hipLaunchKernelGGL((k_pcg_pre<MT, VT, NR, true, WP>), gridS, dim3(64 * WP), 0, s, sl, w.P, w.st, 0, A0,
@@ -1834,18 +1843,10 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
                 });
         }
         if (ga && gb) {
-            std::pair<hipEvent_t, hipEvent_t>* ev;
-            timing_begin_batch(s, ev, steps - 1);
             JSSO_HIP(hipGraphLaunch(ga, s));
-            timing_end(s, ev);
             JSSO_HIP(hipGraphLaunch(gb, s));
         } else {
-            for (int it = it0 + 1; it < it0 + steps; ++it) {
-                std::pair<hipEvent_t, hipEvent_t>* ev;
-                timing_begin(s, ev);
-                amg_step<MT, VT, NR>(pl, A, w, bf, A0, it, s);
-                timing_end(s, ev);
-            }
+            for (int it = it0 + 1; it < it0 + steps; ++it) amg_step<MT, VT, NR>(pl, A, w, bf, A0, it, s, true);
             amg_replace<MT, VT, NR>(pl, A, w, bf, A0, it0 + steps, steps, tol, s);
         }
         JSSO_HIP(hipGetLastError());

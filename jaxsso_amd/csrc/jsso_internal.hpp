@@ -123,7 +123,7 @@ struct Plan {
     bool full_valid = false;      // d_K / d_Kc hold the current assembly
     double* d_crds = nullptr;     // node coordinates of the current assembly (for the AMG null space)
     // preconditioner of the batched solves (JSSO_PC_*) and the AMG hierarchy
-    int32_t precond = 0;
+    int32_t precond = JSSO_PC_AMG;
     AmgState* amg = nullptr;
     bool amg_valid = false;       // the AMG numeric set-up matches the current assembly
     // solver workspace, grown on demand

@@ -591,7 +591,7 @@ static void launch_init(Plan* pl, const Slices& sl, const Workspace& w, const do
     const int nsys = pl->n_sys * NR;
     hipLaunchKernelGGL((k_init<VT, NR>), gridS, dim3(SOLVER_WG), 0, s, sl, w.P, pl->d_fixed, pl->d_Linv, f, w.b64,
                        w.x64, w.xbest, xlo, rb[0], p);
-    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 0, tol, 0);
+    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 0, tol, 0, 0);
     hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_START, 1>), gridS, dim3(64), 0, s, sl, w.P, w.st, 0, A,
                        (const VT*)rb[1], (const VT*)qb[1], rb[0], qb[0], p, xlo);
 }
@@ -666,7 +666,7 @@ static void launch_replace(Plan* pl, int wps, const Slices& sl, const Workspace&
     }
     if (!done)
         hipLaunchKernelGGL((k_replace<VT, NR, 1>), gridS, dim3(64), 0, s, sl, w.P, w.st, A64, b64, x64, rb[it & 1]);
-    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, steps);
+    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, steps, 8192);
     hipLaunchKernelGGL((k_save<NR>), gridS, dim3(SOLVER_WG), 0, s, sl, (const int32_t*)w.st.save,
                        (const double*)w.x64, w.xbest);
     const VT* rp = rb[(it - 1) & 1];

@@ -51,10 +51,12 @@ __global__ __launch_bounds__(SOLVER_WG) void k_init(Slices sl, Partials P, const
 // mode 0 (after k_init): bnorm2, activity.  mode 1 (after k_replace): true
 // relative residual, best iterate, stagnation, status, after `steps` more
 // iterations (counted per system, so the kernel can sit in a captured graph).
+// A system whose true residual has not improved by 10% for max(min_window,
+// it/2) iterations has reached its attainable accuracy and stops.
 // Every lane computes the same (butterfly) totals; lane 0 writes the state.
 template <int NR>
 __global__ __launch_bounds__(64) void k_decide(Slices sl, Partials P, SysState st, int nsys_struct, int mode,
-                                               double tol, int steps) {
+                                               double tol, int steps, int min_window) {
     const int sys = blockIdx.x;
     if (sys >= nsys_struct) return;
     const int lane = threadIdx.x;
@@ -107,10 +109,10 @@ __global__ __launch_bounds__(64) void k_decide(Slices sl, Partials P, SysState s
         const bool improved = rel < best;
         const double nbest = improved ? rel : best;
         const int stall = rel <= 0.9 * best ? it : st.stall[k];   // iteration of the last 10% gain
-        // CG on strongly heterogeneous systems (SIMP densities) plateaus for a
-        // few thousand iterations before it moves on: only a longer flat stretch
-        // is taken as the attainable-accuracy floor
-        const int window = it / 2 > 8192 ? it / 2 : 8192;
+        // block-Jacobi CG on strongly heterogeneous systems (SIMP densities)
+        // plateaus for a few thousand iterations before it moves on: only a
+        // longer flat stretch is taken as the attainable-accuracy floor there
+        const int window = it / 2 > min_window ? it / 2 : min_window;
         int active = 1, status = SYS_RUNNING;
         if (rel <= tol || tb[h] == 0.0) {
             active = 0;

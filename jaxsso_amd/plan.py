@@ -48,7 +48,7 @@ class FEPlan:
         sizes = np.zeros(6, np.int64)
         nat.check(L.jsso_plan_sizes(self._h, nat.ptr(sizes)))
         self.nnzb = int(sizes[1])
-        self.preconditioner = "block_jacobi"
+        self.preconditioner = "amg"   # the library's default
         self.last_info = None
         self._assembled_key = None
 
@@ -81,8 +81,8 @@ class FEPlan:
         return Kout
 
     def set_preconditioner(self, name: str) -> None:
-        """Preconditioner of the batched precisions: 'block_jacobi' (split block-Jacobi
-        scaling) or 'amg' (smoothed-aggregation multigrid V-cycle)."""
+        """Preconditioner of the batched precisions: 'amg' (smoothed-aggregation
+        multigrid V-cycle, the default) or 'block_jacobi' (split block-Jacobi scaling)."""
         if name not in PRECONDITIONERS:
             raise ValueError(f"unknown preconditioner {name!r} ({sorted(PRECONDITIONERS)})")
         nat.check(nat.load_library().jsso_plan_set_preconditioner(self._h, PRECONDITIONERS[name]))

@@ -126,11 +126,13 @@ def test_cfg1_shape_and_thickness_gradient_vs_oracle_and_fd():
         assert abs(fd - g[k]) <= 1e-5 * abs(g[k]) + 1e-11 * abs(val) / h, (k, fd, g[k])
 
 
+@pytest.mark.parametrize("pc", ["amg", "block_jacobi"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_unattainable_tolerance_stops_at_floor_with_best_iterate(precision):
+def test_unattainable_tolerance_stops_at_floor_with_best_iterate(precision, pc):
     st = wl.cfg1()
     from jaxsso_amd.plan import FEPlan
     plan = FEPlan([0, st.n_node], st.cnct_beamcols, st.cnct_quads, st.known_id)
+    plan.set_preconditioner(pc)
     plan.assemble(torch.tensor(st.crds), None, torch.tensor(st.prop_quads))
     with pytest.warns(RuntimeWarning, match="did not converge"):
         u = plan.solve(torch.tensor(st.nodal_loads), precision=precision, tol=1e-17, maxit=60000).cpu().numpy()

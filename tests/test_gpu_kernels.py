@@ -111,11 +111,14 @@ def test_assembly_matches_oracle(case):
     assert rel(y, R @ x) < 1e-13
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "plain"])
+@pytest.mark.parametrize("precision,pc", [("fp64", "block_jacobi"), ("mixed", "block_jacobi"), ("plain", None),
+                                          ("fp64", "amg"), ("mixed", "amg")])
 @pytest.mark.parametrize("case", ["cfg0", "cfg1", "barrel", "arch"])
-def test_solve_matches_oracle(case, precision):
+def test_solve_matches_oracle(case, precision, pc):
     st = getattr(wl, case)() if case.startswith("cfg") else getattr(wl, "notebook_" + case)()
     plan = FEPlan([0, st.n_node], st.cnct_beamcols, st.cnct_quads, st.known_id)
+    if pc:
+        plan.set_preconditioner(pc)
     plan.assemble(torch.tensor(st.crds), torch.tensor(st.prop_beamcols), torch.tensor(st.prop_quads))
     # the slender arch amplifies the residual ~150x into u: a 1e-8 error bar
     # needs a 1e-11 scaled residual there
@@ -179,6 +182,7 @@ def test_cg_kernel_variants_agree_with_oracle(precision, nrhs):
     designs = [wl.cfg1(0), wl.gridshell(9, 10.0, 2.0, 0.05, cfg=3, design=5, half_span_cases=True)]
     from jaxsso_amd.batch import StructureBatch
     b = StructureBatch(designs, n_cases=nrhs)
+    b.plan.set_preconditioner("block_jacobi")
     refs = []
     for i, st in enumerate(designs):
         K = system.assemble_K(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads)
@@ -205,6 +209,7 @@ def test_cg_kernel_variants_agree_with_oracle(precision, nrhs):
 def test_cg_autotuner_records_a_choice():
     st = wl.cfg1()
     plan = FEPlan([0, st.n_node], st.cnct_beamcols, st.cnct_quads, st.known_id)
+    plan.set_preconditioner("block_jacobi")
     plan.assemble(torch.tensor(st.crds), None, torch.tensor(st.prop_quads))
     assert plan.cg_tuning("mixed", 1) == 0
     u1 = plan.solve(torch.tensor(st.nodal_loads), precision="mixed").cpu().numpy()
@@ -241,6 +246,7 @@ def test_graph_and_eager_cg_agree_bitwise(precision, monkeypatch):
     designs = [wl.cfg1(d) for d in range(3)]
     from jaxsso_amd.batch import StructureBatch
     b = StructureBatch(designs)
+    b.plan.set_preconditioner("block_jacobi")
     b.plan.cg_tuning(precision, 1, set_wps=1)
     monkeypatch.setenv("JSSO_CG_GRAPH", "1")
     ug = b.solve(precision=precision, tol=1e-10).cpu().numpy()

@@ -28,10 +28,11 @@ def shell200():
     return st, StructureBatch([st]), fx
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp64", "plain"])
-def test_shell200_compliance_and_gradient_vs_oracle(shell200, precision):
+@pytest.mark.parametrize("precision,pc", [("mixed", "amg"), ("fp64", "amg"), ("mixed", "block_jacobi"),
+                                          ("fp64", "block_jacobi"), ("plain", None)])
+def test_shell200_compliance_and_gradient_vs_oracle(shell200, precision, pc):
     st, b, fx = shell200
-    C, (dc, _, dq) = b.compliance_and_gradient(precision=precision, tol=1e-10)
+    C, (dc, _, dq) = b.compliance_and_gradient(precision=precision, tol=1e-10, preconditioner=pc)
     info = b.plan.last_info
     assert info["converged"] == info["n_systems"] == 1, info
     C = float(C[0, 0])

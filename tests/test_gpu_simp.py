@@ -20,13 +20,14 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
 
 
+@pytest.mark.parametrize("pc", ["amg", "block_jacobi"])
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_simp_batch_vs_oracle(precision):
+def test_simp_batch_vs_oracle(precision, pc):
     with open(FIXTURE) as fh:
         fx = json.load(fh)
     designs = [wl.simp_shell(d) for d in range(4)]
     b = StructureBatch(designs)
-    C, (dc, _, dq) = b.compliance_and_gradient(precision=precision, tol=1e-10)
+    C, (dc, _, dq) = b.compliance_and_gradient(precision=precision, tol=1e-10, preconditioner=pc)
     info = b.plan.last_info
     assert info["converged"] == info["n_systems"] == 4, info
     C = C.cpu().numpy()[:, 0]
