@@ -999,6 +999,27 @@ __device__ __forceinline__ void ell_apply(const Slices& sl, const MT* __restrict
     }
 }
 
+// The rows of a slice form one contiguous run of (row1 - row0) * 6 * NR values
+// per vector: wave 0 puts value(c, h) of its lane's row into LDS, then every
+// thread of the workgroup copies the run, so each store instruction writes
+// consecutive addresses (whole cache lines rather than 4-byte pieces 24 bytes
+// apart).  Called by all threads.
+template <typename VT, int NR, int WPS, typename F>
+__device__ __forceinline__ void flush_rows(VT* stage, const int4& ma, F&& value, VT* __restrict__ dst) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (wave == 0) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int h = 0; h < NR; ++h) stage[lane * 6 * NR + c * NR + h] = value(c, h);
+    }
+    __syncthreads();
+    const int64_t base = (int64_t)ma.y * 6 * NR;
+    const int n = (ma.z - ma.y) * 6 * NR;
+    for (int u = threadIdx.x; u < n; u += 64 * WPS) dst[base + u] = stage[u];
+    __syncthreads();
+}
+
 // Direction and product of iteration k: beta from the partials of k - 1 and
 // k - 2; p_k = z + beta p_{k-1} (own and gathered rows); q = A p_k; partials <p_k, q>.
 template <typename MT, typename VT, int NR, int WPS>
@@ -1035,26 +1056,18 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_dir(Slices sl, Partials P, Sys
 #pragma unroll
         for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? pk(ii, c, h) : VT(0);
     ell_apply<MT, VT, NR, WPS>(sl, A, w, ma, mb, acc, pk);
-    if (wave != 0) return;
+    __shared__ VT stage[64 * 6 * NR];
     double v[NR];
 #pragma unroll
     for (int h = 0; h < NR; ++h) v[h] = 0.0;
-    if (on) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-#pragma unroll
-            for (int h = 0; h < NR; ++h) {
-                const int64_t o = (6 * i + c) * NR + h;
-                if ((amask >> h) & 1) {
-                    const VT pn = pk(i, c, h);
-                    pcur[o] = pn;
-                    q[o] = acc[c][h];
-                    v[h] += (double)pn * (double)acc[c][h];
-                } else {
-                    pcur[o] = pprev[o];
-                }
-            }
-    }
+    // (inactive load cases keep p; their q is never used)
+    flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) {
+        const VT pn = pk(ii, c, h);
+        if (on && ((amask >> h) & 1)) v[h] += (double)pn * (double)acc[c][h];
+        return ((amask >> h) & 1) ? pn : pprev[(6 * ii + c) * NR + h];
+    }, pcur);
+    flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) { return acc[c][h]; }, q);
+    if (wave != 0) return;
 #pragma unroll
     for (int h = 0; h < NR; ++h) v[h] = wave_sum(v[h]);
     if (lane == 0) {
@@ -1104,20 +1117,18 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_pre(Slices sl, Partials P, Sys
 #pragma unroll
         for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? rk(ii, c, h) : VT(0);
     ell_apply<MT, VT, NR, WPS>(sl, A, w, ma, mb, acc, rk);
-    if (wave != 0 || !on) return;
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
-#pragma unroll
-        for (int h = 0; h < NR; ++h) {
-            const int64_t o = (6 * i + c) * NR + h;
-            const VT r = rk(i, c, h);
-            t0[o] = r - acc[c][h] / th;
-            if (!FRESH) {
-                const bool run = (amask >> h) & 1;
-                rcur[o] = run ? r : rprev[o];
-                if (run) xlo[o] = xlo[o] + alpha[h] * p[o];
-            }
-        }
+    __shared__ VT stage[64 * 6 * NR];
+    flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) { return rk(ii, c, h) - acc[c][h] / th; }, t0);
+    if (!FRESH) {
+        // (inactive load cases keep r and x)
+        flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) {
+            return ((amask >> h) & 1) ? rk(ii, c, h) : rprev[(6 * ii + c) * NR + h];
+        }, rcur);
+        flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) {
+            const int64_t o = (6 * ii + c) * NR + h;
+            return ((amask >> h) & 1) ? xlo[o] + alpha[h] * p[o] : xlo[o];
+        }, xlo);
+    }
 }
 
 // Last smoothing step on level 0: z = z1 + (r - A z1) / theta, with z1 (in
@@ -1144,24 +1155,22 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_post(Slices sl, Partials P, Sy
 #pragma unroll
         for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? zz(ii, c, h) : VT(0);
     ell_apply<MT, VT, NR, WPS>(sl, A, w, ma, mb, acc, zz);
-    if (wave != 0) return;
+    __shared__ VT stage[64 * 6 * NR];
     double v[NR], vp[NR];
 #pragma unroll
     for (int h = 0; h < NR; ++h) v[h] = vp[h] = 0.0;
-    if (on) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-#pragma unroll
-            for (int h = 0; h < NR; ++h) {
-                if (!((amask >> h) & 1)) continue;
-                const int64_t o = (6 * i + c) * NR + h;
-                const VT rv = r[o];
-                const VT zn = zz(i, c, h) + (rv - acc[c][h]) / th;
-                z[o] = zn;
-                v[h] += (double)rv * (double)zn;
-                if (k > 0) vp[h] += (double)rprev[o] * (double)zn;   // (no r_{-1} at the start)
-            }
-    }
+    // (z of inactive load cases is never used)
+    flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) {
+        const int64_t o = (6 * ii + c) * NR + h;
+        const VT rv = r[o];
+        const VT zn = zz(ii, c, h) + (rv - acc[c][h]) / th;
+        if (on && ((amask >> h) & 1)) {
+            v[h] += (double)rv * (double)zn;
+            if (k > 0) vp[h] += (double)rprev[o] * (double)zn;   // (no r_{-1} at the start)
+        }
+        return zn;
+    }, z);
+    if (wave != 0) return;
 #pragma unroll
     for (int h = 0; h < NR; ++h) {
         v[h] = wave_sum(v[h]);
