@@ -1924,12 +1924,15 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
     return conv == nsys ? JSSO_OK : JSSO_NOT_CONVERGED;
 }
 
-// Iterations between FP64 residual replacements of the AMG solve
-// (JSSO_AMG_REPLACE_EVERY overrides; even, so the chunk graphs are reused).
+// Iterations between FP64 residual replacements (and convergence tests) of the
+// AMG solve; JSSO_AMG_REPLACE_EVERY overrides (an even count, so the chunk
+// graphs are reused).  8: the solve needs ~100 iterations, so the test that
+// ends it comes soon after convergence, and the FP32 recurrences stay close
+// to the FP64 residual.
 static int amg_replace_interval() {
     const char* e = getenv("JSSO_AMG_REPLACE_EVERY");
     const int v = e ? atoi(e) : 0;
-    return v > 0 ? v : 16;
+    return v > 0 ? v : 8;
 }
 
 template <typename MT, typename VT>
