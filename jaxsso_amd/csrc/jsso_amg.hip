@@ -42,7 +42,7 @@
 
 namespace jsso {
 
-constexpr int AMG_MAXC = 16;       // nodes of a structure's directly solved coarsest level (<= 96 DOF)
+constexpr int AMG_MAXC = 26;       // nodes of a structure's directly solved coarsest level (<= 156 DOF)
 constexpr int AMG_QR_RPL = 5;      // null-space rows per lane in the per-aggregate QR
 constexpr int AMG_MAX_AGG = AMG_QR_RPL * 64 / 6;   // nodes per aggregate (53)
 constexpr int AMG_POWER_STEPS = 12;
@@ -101,7 +101,7 @@ struct AmgLevelDev {
     int32_t *d_w_row = nullptr, *d_w_cptr = nullptr, *d_w_caref = nullptr, *d_w_cp = nullptr;
     int32_t *d_c_cptr = nullptr, *d_c_p = nullptr, *d_c_w = nullptr;
     double *d_P64 = nullptr, *d_W64 = nullptr;
-    float *d_P32 = nullptr, *d_PT32 = nullptr;   // by fine row / transposed by coarse node
+    _Float16 *d_P16 = nullptr, *d_PT16 = nullptr;   // FP16, by fine row / transposed by coarse node
     // V-cycle vectors (level >= 1), 6 n x nrhs FP32
     float *d_b = nullptr, *d_t = nullptr, *d_x = nullptr;
 };
@@ -120,7 +120,6 @@ struct AmgState {
     std::vector<std::vector<int32_t>> term_list;   // per level: structures solved there
     std::vector<int32_t*> d_term_list;
     float* d_inv = nullptr;           // dense inverses, column-major, FP32
-    double* d_dense = nullptr;        // scratch for the factorisation, FP64
     int64_t inv_total = 0;
     int vec_nrhs = 0;                 // V-cycle vectors sized for this many load cases
     int wps0 = 4;                     // waves per slice of the level-0 kernels
@@ -806,12 +805,13 @@ __global__ __launch_bounds__(256) void k_amg_cscale(int64_t nval, const int32_t*
     A32[ell32_index(vp, e)] = (float)v;
 }
 
-// P scaled to the coarse variables (FP32 for the V-cycle): Ps(i, J) = P(i, J) Lc_J^-T,
+// P scaled to the coarse variables, FP16 for the V-cycle (|entries| stay O(1): a
+// few units at most; the smallest underflow harmlessly): Ps(i, J) = P(i, J) Lc_J^-T,
 // by fine row (prolongation) and transposed by coarse node (restriction)
 __global__ __launch_bounds__(256) void k_amg_pscale(int64_t nP, const int32_t* __restrict__ p_col,
                                                     const int32_t* __restrict__ p_tpos,
                                                     const double* __restrict__ P, const double* __restrict__ Lci,
-                                                    float* __restrict__ P32, float* __restrict__ PT32) {
+                                                    _Float16* __restrict__ P16, _Float16* __restrict__ PT16) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nP * 36) return;
     const int64_t b = t / 36;
@@ -821,8 +821,8 @@ __global__ __launch_bounds__(256) void k_amg_pscale(int64_t nP, const int32_t* _
     double s = 0.0;
 #pragma unroll
     for (int m = 0; m < 6; ++m) s += p[r * 6 + m] * l[c * 6 + m];
-    P32[t] = (float)s;
-    PT32[(int64_t)p_tpos[b] * 36 + c * 6 + r] = (float)s;   // per coarse node, block transposed
+    P16[t] = (_Float16)s;
+    PT16[(int64_t)p_tpos[b] * 36 + c * 6 + r] = (_Float16)s;   // per coarse node, block transposed
 }
 
 // coarse null space in the scaled coarse variables: B_J = Lc_J^T R_J
@@ -838,65 +838,77 @@ __global__ __launch_bounds__(256) void k_amg_nullc(int32_t nc, const double* __r
     B[t] = s;
 }
 
-// Dense inverse of a structure's coarsest scaled operator (N = 6 nodes <= 96):
-// the matrix (unit diagonal blocks plus the stored off-diagonal blocks) is
-// factored in LDS (Cholesky), then each thread solves for one column of the
-// inverse (L y = e_c, L^T x = y, in the FP64 scratch) and stores it in FP32,
-// column-major.
+// Dense inverse of a structure's coarsest scaled operator (N = 6 nodes <= 156),
+// one workgroup per structure, all in LDS with the lower triangle packed row by
+// row (entry (r, c), c <= r, at r (r + 1) / 2 + c; 98 KB for N = 156): the
+// matrix (unit diagonal blocks plus the stored off-diagonal blocks) is factored
+// by a right-looking Cholesky A = L L^T, L is inverted in place column by column
+// from the right (X = L^-1: X_rj = -sum_{k=j+1..r} X_rk L_kj / L_jj), and
+// A^-1 = X^T X is written in FP32, row-major (it is symmetric: the solve kernel
+// reads a row contiguously).
 constexpr int AMG_DENSE_MAX = 6 * AMG_MAXC;
-__global__ __launch_bounds__(256) void k_amg_dense(const int32_t* __restrict__ list, const int32_t* __restrict__ term,
+__device__ __forceinline__ int tri(int r, int c) { return r * (r + 1) / 2 + c; }
+__global__ __launch_bounds__(1024) void k_amg_dense(const int32_t* __restrict__ list, const int32_t* __restrict__ term,
                                                    const int32_t* __restrict__ rowptr,
                                                    const int32_t* __restrict__ colind,
                                                    const int32_t* __restrict__ kref, const double* __restrict__ A,
-                                                   double* __restrict__ scratch, float* __restrict__ inv) {
-    __shared__ double M[AMG_DENSE_MAX * AMG_DENSE_MAX];
+                                                   float* __restrict__ inv) {
+    __shared__ double M[AMG_DENSE_MAX * (AMG_DENSE_MAX + 1) / 2];
+    __shared__ double col[AMG_DENSE_MAX];
     const int s = list[blockIdx.x];
     const int node0 = term[4 * s + 1], nt = term[4 * s + 2];
     const int64_t off = term[4 * s + 3];
     const int N = 6 * nt;
-    for (int u = threadIdx.x; u < N * N; u += blockDim.x) {
-        const int R = u / N, C = u % N;
-        const int i = node0 + R / 6, j = node0 + C / 6;
-        double v = 0.0;
-        if (i == j) {
-            v = (R % 6) == (C % 6) ? 1.0 : 0.0;
-        } else {
-            for (int k = rowptr[i]; k < rowptr[i + 1]; ++k)
-                if (colind[k] == j) v = ablk(A, kref[k], R % 6, C % 6);
-        }
-        M[R * N + C] = v;
+    const int NT = N * (N + 1) / 2;
+    for (int u = threadIdx.x; u < NT; u += blockDim.x) M[u] = 0.0;
+    __syncthreads();
+    // scatter the blocks of the structure's rows (lower triangle only): thread per (block, entry)
+    const int k0 = rowptr[node0], k1 = rowptr[node0 + nt];
+    for (int u = threadIdx.x; u < (k1 - k0) * 36; u += blockDim.x) {
+        const int k = k0 + u / 36, e = u % 36, r = e / 6, c = e % 6;
+        int i = node0;
+        while (rowptr[i + 1] <= k) ++i;
+        const int R = 6 * (i - node0) + r, C = 6 * (colind[k] - node0) + c;
+        if (C > R) continue;
+        M[tri(R, C)] = kref[k] < 0 ? (r == c ? 1.0 : 0.0) : ablk(A, kref[k], r, c);
     }
     __syncthreads();
     for (int j = 0; j < N; ++j) {
-        const double d = M[j * N + j];
+        const double d = M[tri(j, j)];
         const double ljj = d > 0.0 ? sqrt(d) : 1.0;
         __syncthreads();
-        for (int r = j + 1 + threadIdx.x; r < N; r += blockDim.x) M[r * N + j] /= ljj;
-        if (threadIdx.x == 0) M[j * N + j] = ljj;
+        for (int r = j + 1 + threadIdx.x; r < N; r += blockDim.x) M[tri(r, j)] /= ljj;
+        if (threadIdx.x == 0) M[tri(j, j)] = ljj;
         __syncthreads();
         const int m = N - j - 1;
         for (int u = threadIdx.x; u < m * m; u += blockDim.x) {
             const int r = j + 1 + u / m, c = j + 1 + u % m;
-            if (c <= r) M[r * N + c] -= M[r * N + j] * M[c * N + j];
+            if (c <= r) M[tri(r, c)] -= M[tri(r, j)] * M[tri(c, j)];
         }
         __syncthreads();
     }
-    for (int c = threadIdx.x; c < N; c += blockDim.x) {
-        double* y = scratch + off + (int64_t)c * N;
-        for (int r = 0; r < N; ++r) {
-            double v = r == c ? 1.0 : 0.0;
-            for (int t = 0; t < r; ++t) v -= M[r * N + t] * y[t];
-            y[r] = v / M[r * N + r];
+    // X = L^-1 in place, columns N-1 .. 0 (column j needs the finished columns > j of X
+    // and column j of L, which is overwritten only after the barrier)
+    for (int j = N - 1; j >= 0; --j) {
+        const double ljj = M[tri(j, j)];
+        for (int r = j + 1 + threadIdx.x; r < N; r += blockDim.x) {
+            double v = 0.0;
+            for (int k = j + 1; k <= r; ++k) v += M[tri(r, k)] * M[tri(k, j)];
+            col[r] = -v / ljj;
         }
-        for (int r = N - 1; r >= 0; --r) {
-            double v = y[r];
-            for (int t = r + 1; t < N; ++t) v -= M[t * N + r] * y[t];
-            y[r] = v / M[r * N + r];
-        }
-        for (int r = 0; r < N; ++r) inv[off + (int64_t)c * N + r] = (float)y[r];
+        __syncthreads();
+        for (int r = j + 1 + threadIdx.x; r < N; r += blockDim.x) M[tri(r, j)] = col[r];
+        if (threadIdx.x == 0) M[tri(j, j)] = 1.0 / ljj;
+        __syncthreads();
+    }
+    // A^-1 = X^T X
+    for (int u = threadIdx.x; u < N * N; u += blockDim.x) {
+        const int r = u / N, c = u % N;
+        double v = 0.0;
+        for (int k = r > c ? r : c; k < N; ++k) v += M[tri(k, r)] * M[tri(k, c)];
+        inv[off + u] = (float)v;
     }
 }
-
 
 // ---------------------------------------------------------------------------
 // device: the preconditioned CG iteration and the V-cycle
@@ -1079,10 +1091,11 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_dir(Slices sl, Partials P, Sys
 // Update of iteration k and the V-cycle's first smoothing step on level 0:
 // alpha = <r,z>_{k-1} / <p,q>_k; r_k = r_{k-1} - alpha q (own and gathered
 // rows); x += alpha p_k; t0 = r_k - A (r_k / theta).  FRESH: r_k is given (a
-// replacement or the start), no update.
-template <typename MT, typename VT, int NR, bool FRESH, int WPS>
+// replacement or the start), no update.  The smoothing product uses the FP16
+// copy of A (a preconditioner needs no more; halves the kernel's matrix bytes).
+template <typename VT, int NR, bool FRESH, int WPS>
 __global__ __launch_bounds__(64 * WPS) void k_pcg_pre(Slices sl, Partials P, SysState st, int k,
-                                                      const MT* __restrict__ A, const float* __restrict__ theta,
+                                                      const _Float16* __restrict__ A, const float* __restrict__ theta,
                                                       const VT* __restrict__ rprev, VT* __restrict__ rcur,
                                                       const VT* __restrict__ q, const VT* __restrict__ p,
                                                       VT* __restrict__ xlo, VT* __restrict__ t0) {
@@ -1116,7 +1129,7 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_pre(Slices sl, Partials P, Sys
     for (int c = 0; c < 6; ++c)
 #pragma unroll
         for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? rk(ii, c, h) : VT(0);
-    ell_apply<MT, VT, NR, WPS>(sl, A, w, ma, mb, acc, rk);
+    ell_apply<_Float16, VT, NR, WPS>(sl, A, w, ma, mb, acc, rk);
     __shared__ VT stage[64 * 6 * NR];
     flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) { return rk(ii, c, h) - acc[c][h] / th; }, t0);
     if (!FRESH) {
@@ -1132,10 +1145,11 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_pre(Slices sl, Partials P, Sys
 }
 
 // Last smoothing step on level 0: z = z1 + (r - A z1) / theta, with z1 (in
-// t0) the prolongated correction; partials <r_k, z> and <r_{k-1}, z>.
-template <typename MT, typename VT, int NR, int WPS>
+// t0) the prolongated correction (FP16 copy of A, as in k_pcg_pre); partials
+// <r_k, z> and <r_{k-1}, z>.
+template <typename VT, int NR, int WPS>
 __global__ __launch_bounds__(64 * WPS) void k_pcg_post(Slices sl, Partials P, SysState st, int k,
-                                                       const MT* __restrict__ A, const float* __restrict__ theta,
+                                                       const _Float16* __restrict__ A, const float* __restrict__ theta,
                                                        const VT* __restrict__ r, const VT* __restrict__ rprev,
                                                        const VT* __restrict__ z1, VT* __restrict__ z) {
     const int w = slice_of_block(sl);
@@ -1154,7 +1168,7 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_post(Slices sl, Partials P, Sy
     for (int c = 0; c < 6; ++c)
 #pragma unroll
         for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? zz(ii, c, h) : VT(0);
-    ell_apply<MT, VT, NR, WPS>(sl, A, w, ma, mb, acc, zz);
+    ell_apply<_Float16, VT, NR, WPS>(sl, A, w, ma, mb, acc, zz);
     __shared__ VT stage[64 * 6 * NR];
     double v[NR], vp[NR];
 #pragma unroll
@@ -1293,12 +1307,13 @@ __device__ __forceinline__ float quad_sum(float v) {
 
 // Restriction b_c = P^T t: four lanes per coarse DOF row (J, c), each taking
 // every fourth block of the column (stored transposed and contiguous per
-// coarse node: PT[q][c][r]), then a quad sum.
+// coarse node: PT[q][c][r], FP16), then a quad sum.
 template <typename VT, int NR>
 __global__ __launch_bounds__(256) void k_mg_restrict(int32_t nc, const int32_t* __restrict__ node_sys_c,
                                                      const int32_t* __restrict__ pt_ptr,
-                                                     const int32_t* __restrict__ pt_row, const float* __restrict__ PT,
-                                                     SysState st, const VT* __restrict__ t, float* __restrict__ bc) {
+                                                     const int32_t* __restrict__ pt_row,
+                                                     const _Float16* __restrict__ PT, SysState st,
+                                                     const VT* __restrict__ t, float* __restrict__ bc) {
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int g = (int)(u & 3);
     const int64_t jc = u >> 2;
@@ -1312,10 +1327,10 @@ __global__ __launch_bounds__(256) void k_mg_restrict(int32_t nc, const int32_t* 
         const int q1 = pt_ptr[J + 1];
         for (int q = pt_ptr[J] + g; q < q1; q += 4) {
             const int64_t i = pt_row[q];
-            const float* pr = PT + (int64_t)q * 36 + c * 6;
+            const _Float16* pr = PT + (int64_t)q * 36 + c * 6;
             float pv[6];
 #pragma unroll
-            for (int r = 0; r < 6; ++r) pv[r] = pr[r];
+            for (int r = 0; r < 6; ++r) pv[r] = (float)pr[r];
 #pragma unroll
             for (int r = 0; r < 6; ++r)
 #pragma unroll
@@ -1330,12 +1345,12 @@ __global__ __launch_bounds__(256) void k_mg_restrict(int32_t nc, const int32_t* 
     }
 }
 
-// Prolongation and the smoothing step before it: z1 = b / theta + P x_c,
-// thread per fine DOF row (i, r) of the structures that have a coarser level
+// Prolongation and the smoothing step before it: z1 = b / theta + P x_c (P in
+// FP16), thread per fine DOF row (i, r) of the structures that have a coarser level
 template <typename VT, int NR>
 __global__ __launch_bounds__(256) void k_mg_prolong(int32_t n, const int32_t* __restrict__ node_sys,
                                                     const uint8_t* __restrict__ cont, const int32_t* __restrict__ p_ptr,
-                                                    const int32_t* __restrict__ p_col, const float* __restrict__ P,
+                                                    const int32_t* __restrict__ p_col, const _Float16* __restrict__ P,
                                                     const float* __restrict__ theta, SysState st,
                                                     const VT* __restrict__ b, const float* __restrict__ xc,
                                                     VT* __restrict__ z1) {
@@ -1350,39 +1365,59 @@ __global__ __launch_bounds__(256) void k_mg_prolong(int32_t n, const int32_t* __
     for (int h = 0; h < NR; ++h) acc[h] = (float)b[u * NR + h] / th;
     for (int q = p_ptr[i]; q < p_ptr[i + 1]; ++q) {
         const int64_t J = p_col[q];
-        const float* pb = P + (int64_t)q * 36 + r * 6;
+        const _Float16* pb = P + (int64_t)q * 36 + r * 6;
 #pragma unroll
         for (int c = 0; c < 6; ++c)
 #pragma unroll
-            for (int h = 0; h < NR; ++h) acc[h] += pb[c] * xc[(6 * J + c) * NR + h];
+            for (int h = 0; h < NR; ++h) acc[h] += (float)pb[c] * xc[(6 * J + c) * NR + h];
     }
 #pragma unroll
     for (int h = 0; h < NR; ++h) z1[u * NR + h] = (VT)acc[h];
 }
 
-// Coarsest levels: x = A^-1 b with the dense inverse of each structure solved
-// at this level (workgroup per structure, thread per row)
+// Coarsest levels: x = A^-1 b with the dense (symmetric) inverse of each
+// structure solved at this level: workgroup per structure, b staged in LDS,
+// thread per row r reading A^-1 by columns (entry (c, r) = (r, c): coalesced
+// across the threads) with 8 independent partial sums
 template <int NR>
-__global__ __launch_bounds__(128) void k_mg_direct(const int32_t* __restrict__ list, const int32_t* __restrict__ term,
+__global__ __launch_bounds__(256) void k_mg_direct(const int32_t* __restrict__ list, const int32_t* __restrict__ term,
                                                    const float* __restrict__ inv, SysState st,
                                                    const float* __restrict__ b, float* __restrict__ x) {
+    __shared__ float bs[AMG_DENSE_MAX * NR];
     const int s = list[blockIdx.x];
     if (structure_done<NR>(st, s)) return;
     const int64_t node0 = term[4 * s + 1];
     const int N = 6 * term[4 * s + 2];
     const float* Ai = inv + term[4 * s + 3];
-    const float* bb = b + 6 * node0 * NR;
+    for (int u = threadIdx.x; u < N * NR; u += blockDim.x) bs[u] = b[6 * node0 * NR + u];
+    __syncthreads();
+    constexpr int U = 8;
     for (int r = threadIdx.x; r < N; r += blockDim.x) {
-        float acc[NR];
+        float acc[U][NR];
 #pragma unroll
-        for (int h = 0; h < NR; ++h) acc[h] = 0.f;
-        for (int c = 0; c < N; ++c) {
-            const float a = Ai[(int64_t)c * N + r];
+        for (int q = 0; q < U; ++q)
 #pragma unroll
-            for (int h = 0; h < NR; ++h) acc[h] += a * bb[c * NR + h];
+            for (int h = 0; h < NR; ++h) acc[q][h] = 0.f;
+        int c = 0;
+        for (; c + U <= N; c += U) {
+            float a[U];
+#pragma unroll
+            for (int q = 0; q < U; ++q) a[q] = Ai[(int64_t)(c + q) * N + r];
+#pragma unroll
+            for (int q = 0; q < U; ++q)
+#pragma unroll
+                for (int h = 0; h < NR; ++h) acc[q][h] += a[q] * bs[(c + q) * NR + h];
         }
+        for (; c < N; ++c)
 #pragma unroll
-        for (int h = 0; h < NR; ++h) x[(6 * node0 + r) * NR + h] = acc[h];
+            for (int h = 0; h < NR; ++h) acc[0][h] += Ai[(int64_t)c * N + r] * bs[c * NR + h];
+#pragma unroll
+        for (int h = 0; h < NR; ++h) {
+            float t = 0.f;
+#pragma unroll
+            for (int q = 0; q < U; ++q) t += acc[q][h];
+            x[(6 * node0 + r) * NR + h] = t;
+        }
     }
 }
 
@@ -1492,8 +1527,8 @@ static int amg_build_device(Plan* pl, AmgState& A) {
         AMG_TRY(amg_alloc(A, &D.d_Q, (size_t)H.n * 36));
         AMG_TRY(amg_alloc(A, &D.d_R, (size_t)D.n_agg * 36));
         AMG_TRY(amg_alloc(A, &D.d_P64, (size_t)D.nP * 36));
-        AMG_TRY(amg_alloc(A, &D.d_P32, (size_t)D.nP * 36));
-        AMG_TRY(amg_alloc(A, &D.d_PT32, (size_t)D.nP * 36));
+        AMG_TRY(amg_alloc(A, &D.d_P16, (size_t)D.nP * 36));
+        AMG_TRY(amg_alloc(A, &D.d_PT16, (size_t)D.nP * 36));
         AMG_TRY(amg_alloc(A, &D.d_W64, (size_t)D.nW * 36));
         AMG_TRY(amg_alloc(A, &D.d_rho, (size_t)ns));
         AMG_TRY(amg_alloc(A, &D.d_omega, (size_t)ns));
@@ -1524,7 +1559,6 @@ static int amg_build_device(Plan* pl, AmgState& A) {
     A.d_term_list.assign(nl, nullptr);
     for (int l = 0; l < nl; ++l) AMG_TRY(amg_upload(A, &A.d_term_list[l], A.term_list[l]));
     AMG_TRY(amg_alloc(A, &A.d_inv, (size_t)off));
-    AMG_TRY(amg_alloc(A, &A.d_dense, (size_t)off));
 #undef AMG_TRY
     return JSSO_OK;
 }
@@ -1596,8 +1630,8 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
         }
         if (D.nP > 0)
             hipLaunchKernelGGL(k_amg_pscale, dim3(nblk(D.nP * 36, 256)), dim3(256), 0, s, D.nP, D.d_p_col,
-                               D.d_p_tpos, (const double*)D.d_P64, (const double*)A.d_Lci[l + 1], D.d_P32,
-                               D.d_PT32);
+                               D.d_p_tpos, (const double*)D.d_P64, (const double*)A.d_Lci[l + 1], D.d_P16,
+                               D.d_PT16);
         if (C.d_B && C.n > 0)
             hipLaunchKernelGGL(k_amg_nullc, dim3(nblk((int64_t)C.n * 36, 256)), dim3(256), 0, s, C.n,
                                (const double*)A.d_Lc[l + 1], (const double*)D.d_R, C.d_B);
@@ -1606,8 +1640,8 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
     for (int l = 1; l < nl; ++l) {
         if (A.term_list[l].empty()) continue;
         const AmgLevelDev& D = A.dev[l];
-        hipLaunchKernelGGL(k_amg_dense, dim3((unsigned)A.term_list[l].size()), dim3(256), 0, s, A.d_term_list[l],
-                           A.d_term, D.d_rowptr, D.d_colind, D.d_kref, (const double*)D.d_A64, A.d_dense, A.d_inv);
+        hipLaunchKernelGGL(k_amg_dense, dim3((unsigned)A.term_list[l].size()), dim3(1024), 0, s, A.d_term_list[l],
+                           A.d_term, D.d_rowptr, D.d_colind, D.d_kref, (const double*)D.d_A64, A.d_inv);
     }
     JSSO_HIP(hipGetLastError());
     return JSSO_OK;
@@ -1668,10 +1702,10 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
         const dim3 gr(nblk(24 * (int64_t)C.n, 256));
         if (l == 0)
             hipLaunchKernelGGL((k_mg_restrict<VT, NR>), gr, dim3(256), 0, s, C.n, C.d_node_sys, D.d_pt_ptr,
-                               D.d_pt_row, (const float*)D.d_PT32, w.st, (const VT*)bf.t0, C.d_b);
+                               D.d_pt_row, (const _Float16*)D.d_PT16, w.st, (const VT*)bf.t0, C.d_b);
         else
             hipLaunchKernelGGL((k_mg_restrict<float, NR>), gr, dim3(256), 0, s, C.n, C.d_node_sys, D.d_pt_ptr,
-                               D.d_pt_row, (const float*)D.d_PT32, w.st, (const float*)D.d_t, C.d_b);
+                               D.d_pt_row, (const _Float16*)D.d_PT16, w.st, (const float*)D.d_t, C.d_b);
         if (l + 2 < nl)
             wps_dispatch<float, NR>(4, [&](auto W) {
                 constexpr int WP = decltype(W)::value;
@@ -1683,7 +1717,7 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
     for (int l = 1; l < nl; ++l) {
         if (A.term_list[l].empty()) continue;
         const AmgLevelDev& D = A.dev[l];
-        hipLaunchKernelGGL((k_mg_direct<NR>), dim3((unsigned)A.term_list[l].size()), dim3(128), 0, s,
+        hipLaunchKernelGGL((k_mg_direct<NR>), dim3((unsigned)A.term_list[l].size()), dim3(256), 0, s,
                            A.d_term_list[l], A.d_term, (const float*)A.d_inv, w.st, (const float*)D.d_b, D.d_x);
     }
     for (int l = nl - 2; l >= 0; --l) {
@@ -1691,17 +1725,18 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
         const AmgLevelDev& C = A.dev[l + 1];
         if (l == 0) {
             hipLaunchKernelGGL((k_mg_prolong<VT, NR>), dim3(nblk(6 * (int64_t)D.n, 256)), dim3(256), 0, s, D.n,
-                               D.d_node_sys, (const uint8_t*)D.d_cont, D.d_p_ptr, D.d_p_col, (const float*)D.d_P32,
+                               D.d_node_sys, (const uint8_t*)D.d_cont, D.d_p_ptr, D.d_p_col, (const _Float16*)D.d_P16,
                                (const float*)D.d_theta, w.st, (const VT*)bf.r[k & 1], (const float*)C.d_x, bf.t0);
             wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
                 constexpr int WP = decltype(W)::value;
-                hipLaunchKernelGGL((k_pcg_post<MT, VT, NR, WP>), slice_grid(pl->n_wg), dim3(64 * WP), 0, s,
-                                   level_slices(pl, A, 0), w.P, w.st, k, A0, (const float*)D.d_theta,
+                hipLaunchKernelGGL((k_pcg_post<VT, NR, WP>), slice_grid(pl->n_wg), dim3(64 * WP), 0, s,
+                                   level_slices(pl, A, 0), w.P, w.st, k, (const _Float16*)pl->d_Ke16,
+                                   (const float*)D.d_theta,
                                    (const VT*)bf.r[k & 1], (const VT*)bf.r[(k - 1) & 1], (const VT*)bf.t0, bf.z);
             });
         } else {
             hipLaunchKernelGGL((k_mg_prolong<float, NR>), dim3(nblk(6 * (int64_t)D.n, 256)), dim3(256), 0, s, D.n,
-                               D.d_node_sys, (const uint8_t*)D.d_cont, D.d_p_ptr, D.d_p_col, (const float*)D.d_P32,
+                               D.d_node_sys, (const uint8_t*)D.d_cont, D.d_p_ptr, D.d_p_col, (const _Float16*)D.d_P16,
                                (const float*)D.d_theta, w.st, (const float*)D.d_b, (const float*)C.d_x, D.d_t);
             wps_dispatch<float, NR>(4, [&](auto W) {
                 constexpr int WP = decltype(W)::value;
@@ -1721,13 +1756,14 @@ static void amg_step(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs<VT
                      hipStream_t s, bool timed = false) {
     const Slices sl = level_slices(pl, A, 0);
     const dim3 g = slice_grid(pl->n_wg);
+    const _Float16* A16 = (const _Float16*)pl->d_Ke16;
     wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
         constexpr int WP = decltype(W)::value;
         hipLaunchKernelGGL((k_pcg_dir<MT, VT, NR, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A0,
                            (const VT*)bf.z, (const VT*)bf.p[(k - 1) & 1], bf.p[k & 1], bf.q);
         std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
         if (timed) timing_begin(s, ev);
-        hipLaunchKernelGGL((k_pcg_pre<MT, VT, NR, false, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A0,
+        hipLaunchKernelGGL((k_pcg_pre<VT, NR, false, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A16,
                            (const float*)A.dev[0].d_theta, (const VT*)bf.r[(k - 1) & 1], bf.r[k & 1],
                            (const VT*)bf.q, (const VT*)bf.p[k & 1], bf.xlo, bf.t0);
         if (timed) timing_end(s, ev);
@@ -1743,6 +1779,7 @@ static void amg_replace(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs
                         int steps, double tol, hipStream_t s) {
     const Slices sl = level_slices(pl, A, 0);
     const dim3 g = slice_grid(pl->n_wg);
+    const _Float16* A16 = (const _Float16*)pl->d_Ke16;
     wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
         constexpr int WP = decltype(W)::value;
         hipLaunchKernelGGL((k_pcg_dir<MT, VT, NR, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A0,
@@ -1757,7 +1794,7 @@ static void amg_replace(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs
                        w.xbest);
     wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
         constexpr int WP = decltype(W)::value;
-        hipLaunchKernelGGL((k_pcg_pre<MT, VT, NR, true, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A0,
+        hipLaunchKernelGGL((k_pcg_pre<VT, NR, true, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A16,
                            (const float*)A.dev[0].d_theta, (const VT*)bf.r[(k - 1) & 1], bf.r[k & 1],
                            (const VT*)bf.q, (const VT*)bf.p[k & 1], bf.xlo, bf.t0);
     });
@@ -1828,7 +1865,8 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
     hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 0, tol, 0, 0);
     wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
         constexpr int WP = decltype(W)::value;
-        hipLaunchKernelGGL((k_pcg_pre<MT, VT, NR, true, WP>), gridS, dim3(64 * WP), 0, s, sl, w.P, w.st, 0, A0,
+        hipLaunchKernelGGL((k_pcg_pre<VT, NR, true, WP>), gridS, dim3(64 * WP), 0, s, sl, w.P, w.st, 0,
+                           (const _Float16*)pl->d_Ke16,
                            (const float*)A.dev[0].d_theta, (const VT*)bf.r[1], bf.r[0], (const VT*)bf.q,
                            (const VT*)bf.p[0], bf.xlo, bf.t0);
     });

@@ -371,6 +371,7 @@ static int ensure_device(Plan* p) {
         (rc = dalloc(p, &p->d_Kc, (size_t)p->nnzb * 36)) || (rc = dalloc(p, &p->d_Linv, n * 36)) ||
         (rc = dalloc(p, &p->d_Minv, n * 36)) || (rc = dalloc(p, &p->d_status, 1)) ||
         (rc = dalloc(p, &p->d_Ke64, (size_t)p->ell_slots * 36)) || (rc = dalloc(p, &p->d_Ke32, (size_t)p->ell_slots * 36)) ||
+        (rc = dalloc(p, &p->d_Ke16, (size_t)p->ell_slots * 36)) ||
         (rc = dalloc(p, &p->d_crds, n * 3)))
         return rc;
     JSSO_HIP(hipMemset(p->d_status, 0, sizeof(int32_t)));
@@ -378,6 +379,7 @@ static int ensure_device(Plan* p) {
     if (p->ell_slots) {
         JSSO_HIP(hipMemset(p->d_Ke64, 0, (size_t)p->ell_slots * 36 * sizeof(double)));
         JSSO_HIP(hipMemset(p->d_Ke32, 0, (size_t)p->ell_slots * 36 * sizeof(float)));
+        JSSO_HIP(hipMemset(p->d_Ke16, 0, (size_t)p->ell_slots * 36 * sizeof(uint16_t)));
     }
     p->on_device = true;
     return JSSO_OK;

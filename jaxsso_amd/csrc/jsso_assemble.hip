@@ -127,8 +127,9 @@ __global__ __launch_bounds__(128) void diag_factor_kernel(int32_t n_nodes, const
 // One workgroup per sliced-ELL slot (64 positions, one per block row of the
 // slice): the slot's off-diagonal blocks are summed from their element blocks
 // (same order as K), masked (a constrained row or column of an off-diagonal
-// block is zero), scaled Ks_ij = Linv_i Kc_ij Linv_j^T and written to the FP64
-// and FP32 ELL copies.  Padding positions (src < 0) keep their zeros.
+// block is zero), scaled Ks_ij = Linv_i Kc_ij Linv_j^T and written to the FP64,
+// FP32 and FP16 ELL copies (the solver's residual replacement, its iterations,
+// and the AMG smoothing products).  Padding positions (src < 0) keep their zeros.
 //   phase 1: all 256 threads read the element blocks as (position, entry) pairs,
 //            entry fastest (consecutive threads, consecutive values; every load
 //            of a thread issued before the first sum), and stage the blocks, the
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(ASM_WG) void ell_assemble_kernel(
     int64_t n_slots, const int32_t* __restrict__ ell_src, const int32_t* __restrict__ browof,
     const int32_t* __restrict__ colind, const int32_t* __restrict__ cptr, const int32_t* __restrict__ csrc,
     const uint8_t* __restrict__ fixed, const double* __restrict__ ke, const double* __restrict__ Linv,
-    double* __restrict__ Ke64, float* __restrict__ Ke32) {
+    double* __restrict__ Ke64, float* __restrict__ Ke32, _Float16* __restrict__ Ke16) {
     constexpr int P = ROWS_PER_WG;
     __shared__ double tile[P][37];
     __shared__ double li[P][37], lj[P][37];
@@ -229,6 +230,7 @@ __global__ __launch_bounds__(ASM_WG) void ell_assemble_kernel(
             const int e = r * 6 + c;
             Ke64[base + ((int64_t)(e / 2) * P + lane) * 2 + e % 2] = s;
             Ke32[base + ((int64_t)(e / 4) * P + lane) * 4 + e % 4] = (float)s;
+            Ke16[base + ((int64_t)(e / 4) * P + lane) * 4 + e % 4] = (_Float16)s;
         }
     }
 }
@@ -257,7 +259,7 @@ int launch_assemble(Plan* p, hipStream_t s) {
     if (n_slots > 0) {
         hipLaunchKernelGGL(ell_assemble_kernel, dim3((unsigned)n_slots), dim3(ASM_WG), 0, s, n_slots, p->d_ell_src,
                            p->d_browof, p->d_colind, p->d_cptr, p->d_csrc, p->d_fixed, p->d_ke, p->d_Linv,
-                           p->d_Ke64, p->d_Ke32);
+                           p->d_Ke64, p->d_Ke32, (_Float16*)p->d_Ke16);
         JSSO_HIP(hipGetLastError());
     }
     return JSSO_OK;

@@ -112,6 +112,7 @@ struct Plan {
     hipEvent_t flag_ev[2] = {nullptr, nullptr};
     double* d_Ke64 = nullptr;        // scaled off-diagonal blocks, sliced ELL, FP64
     float* d_Ke32 = nullptr;         // same, FP32
+    uint16_t* d_Ke16 = nullptr;      // same, FP16 bits (the AMG smoothing products)
     // numeric state (device)
     double* d_ke = nullptr;       // element blocks, 36 doubles each
     double* d_K = nullptr;        // K, 36 per block (unconstrained)

@@ -104,12 +104,13 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Value slots of the sliced ELL: the 36 entries of lane l's block are stored in
-// 16-byte groups of G = 16 / sizeof(T) consecutive entries, lanes interleaved
-// group by group ([slot][36 / G][64 lanes][G]), so one vector load per lane
-// brings G entries and a wavefront's load covers 1 KB of consecutive bytes.
+// 16-byte groups of G = 16 / sizeof(T) consecutive entries (FP16: 8-byte groups
+// of 4, the same index pattern as FP32), lanes interleaved group by group
+// ([slot][36 / G][64 lanes][G]), so one vector load per lane brings G entries
+// and a wavefront's load covers 512 B - 1 KB of consecutive bytes.
 template <typename T>
 __device__ __forceinline__ void load_block(const T* __restrict__ slot, int lane, T (&av)[36]) {
-    constexpr int G = 16 / sizeof(T);
+    constexpr int G = sizeof(T) == 2 ? 4 : 16 / sizeof(T);   // FP16: groups of 4 (8-byte loads, 36 = 9 x 4)
     typedef T V __attribute__((ext_vector_type(G)));
 #pragma unroll
     for (int g = 0; g < 36 / G; ++g) {
