@@ -92,11 +92,11 @@ def main():
     value = args.batch * args.steps / (t1 - t0)
     print(f"# timed: {ms_per_step:.1f} ms/step, {value:.1f} evals/s, CG iters {iters}, info {info}", flush=True)
 
-    # -- dominant kernel roofline: k_pcg_pre, the level-0 update and first
-    # smoothing step of every iteration (the stored upper blocks once in FP16,
-    # their indices; r_{k-1}, q, p, x read and r_k, x, t0 written once per DOF).  The
-    # timed steps run the iterations as captured HIP graphs; one more solve with
-    # eager launches brackets every k_pcg_pre launch with HIP events.
+    # -- dominant kernel roofline: k_pcg_dir, the direction and CG product of every
+    # iteration (the stored upper blocks once in the solve's precision, their
+    # indices; z and p_{k-1} read, p_k and q written once per DOF).  The timed
+    # steps run the iterations as captured HIP graphs; one more solve with eager
+    # launches brackets every k_pcg_dir launch with HIP events.
     p = batch.plan
     nnzb_off = p.nnzb - p.n_nodes
     s_m = 4 if args.precision == "mixed" else 8
@@ -110,12 +110,11 @@ def main():
     L.jsso_timing_enable(0)
     os.environ.pop("JSSO_CG_GRAPH")
     L.jsso_timing_read(nat.ptr(avg_ms), nat.ptr(nsamp), 1)
-    # each off-diagonal block pair stored once (36 values, the FP16 copy: the
-    # smoothing product), both halves indexed (a column each, plus the value
-    # position of the transposed half)
-    alg_bytes = (nnzb_off // 2) * 36 * 2 + nnzb_off * 4 + (nnzb_off // 2) * 4 + 7 * ndof * s_v
+    # each off-diagonal block pair stored once (36 values), both halves indexed
+    # (a column each, plus the value position of the transposed half)
+    alg_bytes = (nnzb_off // 2) * 36 * s_m + nnzb_off * 4 + (nnzb_off // 2) * 4 + 4 * ndof * s_v
     achieved = alg_bytes / (avg_ms[0] * 1e-3) / 1e9 if avg_ms[0] > 0 else None
-    kname = f"k_pcg_pre<{'float' if s_m == 4 else 'double'}, 1, false, 4>"
+    kname = f"k_pcg_dir<{'float, float' if s_m == 4 else 'double, double'}, 1, 4>"
     traffic, traffic_src = _measured_traffic(args, kname)
     roofline = {"kernel": kname, "bound": "hbm",
                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -173,8 +173,8 @@ int jsso_plan_amg_levels(jsso_plan* plan, int32_t* h_nodes, int32_t cap, int32_t
 /* Measurement hook (no counterpart in the reference): sampled HIP-event timing
  * of the batched solver's dominant kernel: with block-Jacobi the fused CG
  * iteration that applies the scaled stiffness (a captured chunk of iterations
- * counts as that many launches); with AMG the level-0 update and pre-smoothing
- * kernel, sampled only where it is launched eagerly (JSSO_CG_GRAPH=0).  every > 0
+ * counts as that many launches); with AMG the direction and CG-product kernel
+ * (k_pcg_dir), sampled only where it is launched eagerly (JSSO_CG_GRAPH=0).  every > 0
  * brackets one such launch in `every` with events on the solver's stream; 0 disables.  jsso_timing_read returns the mean duration
  * (ms) and the number of sampled launches, optionally resetting the counters. */
 int jsso_timing_enable(int32_t every);

@@ -119,6 +119,14 @@ struct AmgState {
     int32_t* d_node_off0 = nullptr;   // level-0 first node of every structure (the null space's origin)
     std::vector<std::vector<int32_t>> term_list;   // per level: structures solved there
     std::vector<int32_t*> d_term_list;
+    // A structure whose dense level T is >= 2 runs its level T - 1 and the dense
+    // solve in one kernel (k_mg_tail): per level, the structures fused there, the
+    // dense solves left to k_mg_direct, and per-structure flags for the level
+    // kernels (smoothed here by the level kernels; restriction from here done by
+    // the fused kernel)
+    std::vector<std::vector<int32_t>> tail_list, direct_list;
+    std::vector<int32_t*> d_tail_list, d_direct_list, d_node_off;
+    std::vector<uint8_t*> d_smooth, d_rskip;
     float* d_inv = nullptr;           // dense inverses, column-major, FP32
     int64_t inv_total = 0;
     int vec_nrhs = 0;                 // V-cycle vectors sized for this many load cases
@@ -839,18 +847,26 @@ __global__ __launch_bounds__(256) void k_amg_nullc(int32_t nc, const double* __r
 }
 
 // Dense inverse of a structure's coarsest scaled operator (N = 6 nodes <= 156),
-// one workgroup per structure, all in LDS with the lower triangle packed row by
-// row (entry (r, c), c <= r, at r (r + 1) / 2 + c; 98 KB for N = 156): the
-// matrix (unit diagonal blocks plus the stored off-diagonal blocks) is factored
-// by a right-looking Cholesky A = L L^T, L is inverted in place column by column
-// from the right (X = L^-1: X_rj = -sum_{k=j+1..r} X_rk L_kj / L_jj), and
-// A^-1 = X^T X is written in FP32, row-major (it is symmetric: the solve kernel
-// reads a row contiguously).
+// one workgroup of 1024 threads per structure, all in LDS with the lower
+// triangle packed row by row (entry (r, c), c <= r, at r (r + 1) / 2 + c; 98 KB
+// for N = 156):
+//  1. the matrix (unit diagonal blocks plus the stored off-diagonal blocks) is
+//     factored by a right-looking Cholesky A = L L^T, two barriers per column
+//     (scale the column into `col`, then the rank-1 update of the trailing
+//     triangle by a 32 x 32 thread tile);
+//  2. X = L^-1 replaces L in place by right-looking forward substitution on
+//     L X = I: at step k the row k of X is finished (divided by L_kk) and every
+//     later row i takes S_ij -= L_ik X_kj for j <= k, where S_ij (held at (i, j)
+//     since step j) becomes X_ij when its row is finished;
+//  3. A^-1 = X^T X by 4 x 4 register blocks of the lower block triangle, each
+//     written to both triangles (the result is exactly symmetric), in FP32,
+//     row-major.
 constexpr int AMG_DENSE_MAX = 6 * AMG_MAXC;
 __device__ __forceinline__ int tri(int r, int c) { return r * (r + 1) / 2 + c; }
 __global__ __launch_bounds__(1024) void k_amg_dense(const int32_t* __restrict__ list, const int32_t* __restrict__ term,
                                                    const int32_t* __restrict__ rowptr,
                                                    const int32_t* __restrict__ colind,
+                                                   const int32_t* __restrict__ browof,
                                                    const int32_t* __restrict__ kref, const double* __restrict__ A,
                                                    float* __restrict__ inv) {
     __shared__ double M[AMG_DENSE_MAX * (AMG_DENSE_MAX + 1) / 2];
@@ -860,53 +876,84 @@ __global__ __launch_bounds__(1024) void k_amg_dense(const int32_t* __restrict__ 
     const int64_t off = term[4 * s + 3];
     const int N = 6 * nt;
     const int NT = N * (N + 1) / 2;
-    for (int u = threadIdx.x; u < NT; u += blockDim.x) M[u] = 0.0;
+    const int tid = threadIdx.x, ty = tid >> 5, tx = tid & 31;
+    for (int u = tid; u < NT; u += blockDim.x) M[u] = 0.0;
     __syncthreads();
     // scatter the blocks of the structure's rows (lower triangle only): thread per (block, entry)
     const int k0 = rowptr[node0], k1 = rowptr[node0 + nt];
-    for (int u = threadIdx.x; u < (k1 - k0) * 36; u += blockDim.x) {
+    for (int u = tid; u < (k1 - k0) * 36; u += blockDim.x) {
         const int k = k0 + u / 36, e = u % 36, r = e / 6, c = e % 6;
-        int i = node0;
-        while (rowptr[i + 1] <= k) ++i;
-        const int R = 6 * (i - node0) + r, C = 6 * (colind[k] - node0) + c;
+        const int R = 6 * (browof[k] - node0) + r, C = 6 * (colind[k] - node0) + c;
         if (C > R) continue;
         M[tri(R, C)] = kref[k] < 0 ? (r == c ? 1.0 : 0.0) : ablk(A, kref[k], r, c);
     }
     __syncthreads();
+    // 1. Cholesky
     for (int j = 0; j < N; ++j) {
         const double d = M[tri(j, j)];
         const double ljj = d > 0.0 ? sqrt(d) : 1.0;
+        for (int r = j + 1 + tid; r < N; r += blockDim.x) {
+            const double v = M[tri(r, j)] / ljj;
+            col[r] = v;
+            M[tri(r, j)] = v;
+        }
         __syncthreads();
-        for (int r = j + 1 + threadIdx.x; r < N; r += blockDim.x) M[tri(r, j)] /= ljj;
-        if (threadIdx.x == 0) M[tri(j, j)] = ljj;
-        __syncthreads();
-        const int m = N - j - 1;
-        for (int u = threadIdx.x; u < m * m; u += blockDim.x) {
-            const int r = j + 1 + u / m, c = j + 1 + u % m;
-            if (c <= r) M[tri(r, c)] -= M[tri(r, j)] * M[tri(c, j)];
+        if (tid == 0) M[tri(j, j)] = ljj;
+        for (int r = j + 1 + ty; r < N; r += 32) {
+            const double cr = col[r];
+            for (int c = j + 1 + tx; c <= r; c += 32) M[tri(r, c)] -= cr * col[c];
         }
         __syncthreads();
     }
-    // X = L^-1 in place, columns N-1 .. 0 (column j needs the finished columns > j of X
-    // and column j of L, which is overwritten only after the barrier)
-    for (int j = N - 1; j >= 0; --j) {
-        const double ljj = M[tri(j, j)];
-        for (int r = j + 1 + threadIdx.x; r < N; r += blockDim.x) {
-            double v = 0.0;
-            for (int k = j + 1; k <= r; ++k) v += M[tri(r, k)] * M[tri(k, j)];
-            col[r] = -v / ljj;
+    // 2. X = L^-1 in place
+    for (int k = 0; k < N; ++k) {
+        const double lkk = M[tri(k, k)];
+        for (int j = tid; j < k; j += blockDim.x) M[tri(k, j)] /= lkk;
+        for (int i = k + 1 + tid; i < N; i += blockDim.x) col[i] = M[tri(i, k)];
+        __syncthreads();
+        const double xkk = 1.0 / lkk;
+        if (tid == 0) M[tri(k, k)] = xkk;
+        for (int i = k + 1 + ty; i < N; i += 32) {
+            const double li = col[i];
+            for (int j = tx; j <= k; j += 32) M[tri(i, j)] = j < k ? M[tri(i, j)] - li * M[tri(k, j)] : -li * xkk;
         }
         __syncthreads();
-        for (int r = j + 1 + threadIdx.x; r < N; r += blockDim.x) M[tri(r, j)] = col[r];
-        if (threadIdx.x == 0) M[tri(j, j)] = 1.0 / ljj;
-        __syncthreads();
     }
-    // A^-1 = X^T X
-    for (int u = threadIdx.x; u < N * N; u += blockDim.x) {
-        const int r = u / N, c = u % N;
-        double v = 0.0;
-        for (int k = r > c ? r : c; k < N; ++k) v += M[tri(k, r)] * M[tri(k, c)];
-        inv[off + u] = (float)v;
+    // 3. A^-1 = X^T X: (A^-1)_rc = sum_{k >= max(r, c)} X_kr X_kc
+    const int nb = (N + 3) / 4;
+    for (int t = tid; t < nb * (nb + 1) / 2; t += blockDim.x) {
+        int R = (int)((sqrtf(8.f * t + 1.f) - 1.f) * 0.5f);
+        while (R * (R + 1) / 2 > t) --R;
+        while ((R + 1) * (R + 2) / 2 <= t) ++R;
+        const int C = t - R * (R + 1) / 2;
+        const int r0 = 4 * R, c0 = 4 * C;
+        double acc[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+        for (int k = r0; k < N; ++k) {
+            double xr[4], xc[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                xr[a] = r0 + a <= k ? M[tri(k, r0 + a)] : 0.0;
+                xc[a] = c0 + a <= k ? M[tri(k, c0 + a)] : 0.0;
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] += xr[a] * xc[b];
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int r = r0 + a, c = c0 + b;
+                if (r < N && c < N) {
+                    inv[off + (int64_t)r * N + c] = (float)acc[a][b];
+                    inv[off + (int64_t)c * N + r] = (float)acc[a][b];
+                }
+            }
     }
 }
 
@@ -948,11 +995,13 @@ __device__ __forceinline__ void wave_totals(const double* __restrict__ cg, int s
 // (the caller puts the own-row term into wave 0's acc beforehand).  The next
 // slot's column and value position are requested before the current slot's
 // products; a slot past the end reads the slice's metadata word instead.
-template <typename MT, typename VT, int NR, int WPS, typename F>
+template <typename MT, typename VT, int NR, int WPS, int G = 1, typename F>
 __device__ __forceinline__ void ell_apply(const Slices& sl, const MT* __restrict__ A, int w, const int4& ma,
                                           const int4& mb, VT (&acc)[6][NR], F&& xj) {
-    __shared__ VT red[WPS > 1 ? WPS - 1 : 1][6 * NR][64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // G groups of WPS waves, each group on a slice of its own
+    __shared__ VT red[WPS > 1 ? G : 1][WPS > 1 ? WPS - 1 : 1][WPS > 1 ? 6 * NR : 1][64];
+    const int lane = threadIdx.x & 63, wave = WPS > 1 ? (threadIdx.x >> 6) % WPS : 0;
+    const int grp = G > 1 ? (threadIdx.x >> 6) / WPS : 0;
     const int nup = mb.y, nslot = mb.y + mb.w;
     const MT* Aw = A + 36 * (int64_t)ma.w;
     const int32_t* dummy = (const int32_t*)&sl.meta[3 * w];
@@ -997,7 +1046,7 @@ __device__ __forceinline__ void ell_apply(const Slices& sl, const MT* __restrict
 #pragma unroll
             for (int c = 0; c < 6; ++c)
 #pragma unroll
-                for (int h = 0; h < NR; ++h) red[wave - 1][c * NR + h][lane] = acc[c][h];
+                for (int h = 0; h < NR; ++h) red[grp][wave - 1][c * NR + h][lane] = acc[c][h];
         }
         __syncthreads();
         if (wave == 0) {
@@ -1006,7 +1055,7 @@ __device__ __forceinline__ void ell_apply(const Slices& sl, const MT* __restrict
 #pragma unroll
                 for (int c = 0; c < 6; ++c)
 #pragma unroll
-                    for (int h = 0; h < NR; ++h) acc[c][h] += red[u][c * NR + h][lane];
+                    for (int h = 0; h < NR; ++h) acc[c][h] += red[grp][u][c * NR + h][lane];
         }
     }
 }
@@ -1307,9 +1356,11 @@ __device__ __forceinline__ float quad_sum(float v) {
 
 // Restriction b_c = P^T t: four lanes per coarse DOF row (J, c), each taking
 // every fourth block of the column (stored transposed and contiguous per
-// coarse node: PT[q][c][r], FP16), then a quad sum.
+// coarse node: PT[q][c][r], FP16), then a quad sum.  Structures flagged in
+// `skip` restrict inside their fused kernel.
 template <typename VT, int NR>
 __global__ __launch_bounds__(256) void k_mg_restrict(int32_t nc, const int32_t* __restrict__ node_sys_c,
+                                                     const uint8_t* __restrict__ skip,
                                                      const int32_t* __restrict__ pt_ptr,
                                                      const int32_t* __restrict__ pt_row,
                                                      const _Float16* __restrict__ PT, SysState st,
@@ -1319,12 +1370,13 @@ __global__ __launch_bounds__(256) void k_mg_restrict(int32_t nc, const int32_t* 
     const int64_t jc = u >> 2;
     const bool valid = jc < 6 * (int64_t)nc;
     const int J = valid ? (int)(jc / 6) : 0, c = (int)(jc % 6);
-    const bool run = valid && !structure_done<NR>(st, node_sys_c[J]);
+    const bool run = valid && !skip[node_sys_c[J]] && !structure_done<NR>(st, node_sys_c[J]);
     float acc[NR];
 #pragma unroll
     for (int h = 0; h < NR; ++h) acc[h] = 0.f;
     if (run) {
         const int q1 = pt_ptr[J + 1];
+#pragma unroll 4
         for (int q = pt_ptr[J] + g; q < q1; q += 4) {
             const int64_t i = pt_row[q];
             const _Float16* pr = PT + (int64_t)q * 36 + c * 6;
@@ -1419,6 +1471,173 @@ __global__ __launch_bounds__(256) void k_mg_direct(const int32_t* __restrict__ l
             x[(6 * node0 + r) * NR + h] = t;
         }
     }
+}
+
+// A structure's last sparse level l and its dense level l + 1 in one kernel
+// (workgroup per structure; every step ends at a barrier, and the level-l
+// vectors, a few KB per structure, pass between the steps through global
+// memory inside the workgroup):
+//   t = b - A b / theta          (TG slices at a time, TW waves per slice)
+//   b_c = P^T t                  (four lanes per coarse row, into LDS)
+//   x_c = A_c^-1 b_c             (dense inverse, columns split over the workgroup)
+//   z1 = b / theta + P x_c       (into t)
+//   x = z1 + (b - A z1) / theta  (the level's result, read by the prolongation above)
+// It replaces five launches per iteration on a small, latency-bound level.
+template <int NR>
+struct TailShape {
+    static constexpr int TW = NR <= 2 ? 4 : (NR <= 4 ? 2 : 1);  // waves per slice
+    static constexpr int TG = 4;                                 // slices at a time
+    static constexpr int THREADS = 64 * TW * TG;
+    static constexpr int ROWS = 160;                             // >= AMG_DENSE_MAX
+    static constexpr int CHUNKS = THREADS / ROWS;                // column chunks of the dense product
+};
+static_assert(TailShape<1>::ROWS >= AMG_DENSE_MAX, "dense rows exceed the tail kernel's row span");
+
+template <int NR>
+__device__ __forceinline__ void tail_smooth(const Slices& sl, const float* __restrict__ A, int q0, int q1, float th,
+                                            const float* __restrict__ src, const float* __restrict__ b,
+                                            float* __restrict__ dst, bool post) {
+    using S = TailShape<NR>;
+    const int lane = threadIdx.x & 63, grp = (threadIdx.x >> 6) / S::TW, sub = (threadIdx.x >> 6) % S::TW;
+    auto xs = [&](int64_t node, int c, int h) { return src[(6 * node + c) * NR + h]; };
+    for (int base = q0; base < q1; base += S::TG) {
+        // every group runs a slice (groups past the end repeat the last one) so that
+        // all waves meet the same barriers; only owners store
+        const bool own = base + grp < q1;
+        const int q = own ? base + grp : q1 - 1;
+        const int4 ma = sl.meta[3 * q], mb = sl.meta[3 * q + 1];
+        const int64_t i = (int64_t)ma.y + lane;
+        const bool on = i < ma.z;
+        const int64_t ii = on ? i : (int64_t)ma.y;
+        float acc[6][NR];
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int h = 0; h < NR; ++h) acc[c][h] = sub == 0 ? xs(ii, c, h) : 0.f;
+        ell_apply<float, float, NR, S::TW, S::TG>(sl, A, q, ma, mb, acc, xs);
+        if (own && sub == 0 && on) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+#pragma unroll
+                for (int h = 0; h < NR; ++h) {
+                    const int64_t o = (6 * i + c) * NR + h;
+                    dst[o] = post ? src[o] + (b[o] - acc[c][h]) / th : b[o] - acc[c][h] / th;
+                }
+        }
+        __syncthreads();  // the cross-wave buffer is reused by the next round
+    }
+}
+
+template <int NR>
+__global__ __launch_bounds__(TailShape<NR>::THREADS) void k_mg_tail(
+    const int32_t* __restrict__ list, Slices sl, const float* __restrict__ A, const float* __restrict__ theta,
+    const int32_t* __restrict__ node_off, const int32_t* __restrict__ pt_ptr, const int32_t* __restrict__ pt_row,
+    const _Float16* __restrict__ PT, const int32_t* __restrict__ p_ptr, const int32_t* __restrict__ p_col,
+    const _Float16* __restrict__ P, const int32_t* __restrict__ term, const float* __restrict__ inv, SysState st,
+    const float* __restrict__ b, float* __restrict__ t, float* __restrict__ x) {
+    using S = TailShape<NR>;
+    __shared__ float bc[AMG_DENSE_MAX * NR], xc[AMG_DENSE_MAX * NR];
+    __shared__ float part[S::CHUNKS][S::ROWS * NR];
+    const int s = list[blockIdx.x];
+    if (structure_done<NR>(st, s)) return;
+    const int tid = threadIdx.x;
+    const int q0 = sl.sys_off[s], q1 = sl.sys_off[s + 1];
+    const float th = theta[s];
+    const int64_t n0 = node_off[s], n1 = node_off[s + 1];
+    const int64_t c0 = term[4 * s + 1];
+    const int N = 6 * term[4 * s + 2];
+    const float* Ai = inv + term[4 * s + 3];
+    // 1. pre-smoothing into t
+    tail_smooth<NR>(sl, A, q0, q1, th, b, b, t, false);
+    // 2. restriction into LDS: lanes 4u..4u+3 take every fourth block of coarse row u
+    for (int u0 = 0; u0 < 4 * N; u0 += S::THREADS) {
+        const int v = u0 + tid;
+        const bool valid = v < 4 * N;
+        const int u = valid ? v >> 2 : 0, g = v & 3;
+        const int64_t J = c0 + u / 6;
+        const int c = u % 6;
+        float acc[NR];
+#pragma unroll
+        for (int h = 0; h < NR; ++h) acc[h] = 0.f;
+        if (valid) {
+            const int e1 = pt_ptr[J + 1];
+#pragma unroll 4
+            for (int q = pt_ptr[J] + g; q < e1; q += 4) {
+                const int64_t i = pt_row[q];
+                const _Float16* pr = PT + (int64_t)q * 36 + c * 6;
+                float pv[6];
+#pragma unroll
+                for (int r = 0; r < 6; ++r) pv[r] = (float)pr[r];
+#pragma unroll
+                for (int r = 0; r < 6; ++r)
+#pragma unroll
+                    for (int h = 0; h < NR; ++h) acc[h] += pv[r] * t[(6 * i + r) * NR + h];
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < NR; ++h) acc[h] = quad_sum(acc[h]);
+        if (valid && g == 0) {
+#pragma unroll
+            for (int h = 0; h < NR; ++h) bc[u * NR + h] = acc[h];
+        }
+    }
+    __syncthreads();
+    // 3. dense solve: thread (chunk k, row r) sums columns k, k + CHUNKS, ... of
+    // row r (the inverse is symmetric: read by columns, coalesced over r)
+    {
+        const int r = tid % S::ROWS, k = tid / S::ROWS;
+        if (k < S::CHUNKS && r < N) {
+            float a0[NR], a1[NR];
+#pragma unroll
+            for (int h = 0; h < NR; ++h) a0[h] = a1[h] = 0.f;
+            int c = k;
+            for (; c + S::CHUNKS < N; c += 2 * S::CHUNKS) {
+                const float v0 = Ai[(int64_t)c * N + r], v1 = Ai[(int64_t)(c + S::CHUNKS) * N + r];
+#pragma unroll
+                for (int h = 0; h < NR; ++h) {
+                    a0[h] += v0 * bc[c * NR + h];
+                    a1[h] += v1 * bc[(c + S::CHUNKS) * NR + h];
+                }
+            }
+            if (c < N) {
+                const float v0 = Ai[(int64_t)c * N + r];
+#pragma unroll
+                for (int h = 0; h < NR; ++h) a0[h] += v0 * bc[c * NR + h];
+            }
+#pragma unroll
+            for (int h = 0; h < NR; ++h) part[k][r * NR + h] = a0[h] + a1[h];
+        }
+    }
+    __syncthreads();
+    for (int u = tid; u < N * NR; u += S::THREADS) {
+        float v = 0.f;
+#pragma unroll
+        for (int k = 0; k < S::CHUNKS; ++k) v += part[k][u];
+        xc[u] = v;
+    }
+    __syncthreads();
+    // 4. prolongation with the smoothing step before it, into t
+    for (int64_t u = 6 * n0 + tid; u < 6 * n1; u += S::THREADS) {
+        const int64_t i = u / 6;
+        const int r = (int)(u % 6);
+        float acc[NR];
+#pragma unroll
+        for (int h = 0; h < NR; ++h) acc[h] = b[u * NR + h] / th;
+#pragma unroll 4
+        for (int q = p_ptr[i]; q < p_ptr[i + 1]; ++q) {
+            const int J = p_col[q] - (int)c0;
+            const _Float16* pb = P + (int64_t)q * 36 + r * 6;
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+#pragma unroll
+                for (int h = 0; h < NR; ++h) acc[h] += (float)pb[c] * xc[(6 * J + c) * NR + h];
+        }
+#pragma unroll
+        for (int h = 0; h < NR; ++h) t[u * NR + h] = acc[h];
+    }
+    __syncthreads();
+    // 5. post-smoothing into x
+    tail_smooth<NR>(sl, A, q0, q1, th, t, b, x, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -1559,6 +1778,33 @@ static int amg_build_device(Plan* pl, AmgState& A) {
     A.d_term_list.assign(nl, nullptr);
     for (int l = 0; l < nl; ++l) AMG_TRY(amg_upload(A, &A.d_term_list[l], A.term_list[l]));
     AMG_TRY(amg_alloc(A, &A.d_inv, (size_t)off));
+    // fused last sparse levels (JSSO_AMG_FUSE=0 keeps every level on the per-level kernels)
+    const char* fe = getenv("JSSO_AMG_FUSE");
+    const bool fuse = !(fe && fe[0] == '0');
+    std::vector<int> fl(ns, -1);
+    for (int s = 0; s < ns; ++s)
+        if (fuse && A.h_term[4 * s] >= 2) fl[s] = A.h_term[4 * s] - 1;
+    A.tail_list.assign(nl, {});
+    A.direct_list.assign(nl, {});
+    A.d_tail_list.assign(nl, nullptr);
+    A.d_direct_list.assign(nl, nullptr);
+    A.d_node_off.assign(nl, nullptr);
+    A.d_smooth.assign(nl, nullptr);
+    A.d_rskip.assign(nl, nullptr);
+    for (int l = 0; l < nl; ++l) {
+        std::vector<uint8_t> smooth(ns, 0), rskip(ns, 0);
+        for (int s = 0; s < ns; ++s) {
+            smooth[s] = A.host[l].cont[s] && fl[s] != l ? 1 : 0;
+            rskip[s] = fl[s] == l ? 1 : 0;
+            if (fl[s] == l) A.tail_list[l].push_back(s);
+            if (A.host[l].term[s] && fl[s] != l - 1) A.direct_list[l].push_back(s);
+        }
+        AMG_TRY(amg_upload(A, &A.d_smooth[l], smooth));
+        AMG_TRY(amg_upload(A, &A.d_rskip[l], rskip));
+        AMG_TRY(amg_upload(A, &A.d_tail_list[l], A.tail_list[l]));
+        AMG_TRY(amg_upload(A, &A.d_direct_list[l], A.direct_list[l]));
+        AMG_TRY(amg_upload(A, &A.d_node_off[l], A.host[l].node_off));
+    }
 #undef AMG_TRY
     return JSSO_OK;
 }
@@ -1641,7 +1887,7 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
         if (A.term_list[l].empty()) continue;
         const AmgLevelDev& D = A.dev[l];
         hipLaunchKernelGGL(k_amg_dense, dim3((unsigned)A.term_list[l].size()), dim3(1024), 0, s, A.d_term_list[l],
-                           A.d_term, D.d_rowptr, D.d_colind, D.d_kref, (const double*)D.d_A64, A.d_inv);
+                           A.d_term, D.d_rowptr, D.d_colind, D.d_browof, D.d_kref, (const double*)D.d_A64, A.d_inv);
     }
     JSSO_HIP(hipGetLastError());
     return JSSO_OK;
@@ -1700,32 +1946,48 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
         const AmgLevelDev& C = A.dev[l + 1];
         if (C.n == 0) continue;
         const dim3 gr(nblk(24 * (int64_t)C.n, 256));
-        if (l == 0)
-            hipLaunchKernelGGL((k_mg_restrict<VT, NR>), gr, dim3(256), 0, s, C.n, C.d_node_sys, D.d_pt_ptr,
-                               D.d_pt_row, (const _Float16*)D.d_PT16, w.st, (const VT*)bf.t0, C.d_b);
-        else
-            hipLaunchKernelGGL((k_mg_restrict<float, NR>), gr, dim3(256), 0, s, C.n, C.d_node_sys, D.d_pt_ptr,
-                               D.d_pt_row, (const _Float16*)D.d_PT16, w.st, (const float*)D.d_t, C.d_b);
-        if (l + 2 < nl)
+        const bool any_restrict = A.tail_list[l].size() < (size_t)std::count(A.host[l].cont.begin(),
+                                                                               A.host[l].cont.end(), 1);
+        if (any_restrict) {
+            if (l == 0)
+                hipLaunchKernelGGL((k_mg_restrict<VT, NR>), gr, dim3(256), 0, s, C.n, C.d_node_sys,
+                                   (const uint8_t*)A.d_rskip[l], D.d_pt_ptr, D.d_pt_row, (const _Float16*)D.d_PT16,
+                                   w.st, (const VT*)bf.t0, C.d_b);
+            else
+                hipLaunchKernelGGL((k_mg_restrict<float, NR>), gr, dim3(256), 0, s, C.n, C.d_node_sys,
+                                   (const uint8_t*)A.d_rskip[l], D.d_pt_ptr, D.d_pt_row, (const _Float16*)D.d_PT16,
+                                   w.st, (const float*)D.d_t, C.d_b);
+        }
+        if (l + 2 < nl && A.tail_list[l + 1].size() < (size_t)std::count(A.host[l + 1].cont.begin(),
+                                                                          A.host[l + 1].cont.end(), 1))
             wps_dispatch<float, NR>(4, [&](auto W) {
                 constexpr int WP = decltype(W)::value;
                 hipLaunchKernelGGL((k_mg_pre<NR, WP>), slice_grid(C.n_wg), dim3(64 * WP), 0, s,
                                    level_slices(pl, A, l + 1), (const float*)C.d_A32, (const float*)C.d_theta,
-                                   (const uint8_t*)C.d_cont, w.st, (const float*)C.d_b, C.d_t);
+                                   (const uint8_t*)A.d_smooth[l + 1], w.st, (const float*)C.d_b, C.d_t);
             });
+        if (!A.tail_list[l + 1].empty()) {
+            const AmgLevelDev& T = A.dev[l + 1];
+            hipLaunchKernelGGL((k_mg_tail<NR>), dim3((unsigned)A.tail_list[l + 1].size()),
+                               dim3(TailShape<NR>::THREADS), 0, s,
+                               A.d_tail_list[l + 1], level_slices(pl, A, l + 1), (const float*)T.d_A32,
+                               (const float*)T.d_theta, A.d_node_off[l + 1], T.d_pt_ptr, T.d_pt_row,
+                               (const _Float16*)T.d_PT16, T.d_p_ptr, T.d_p_col, (const _Float16*)T.d_P16, A.d_term,
+                               (const float*)A.d_inv, w.st, (const float*)T.d_b, T.d_t, T.d_x);
+        }
     }
     for (int l = 1; l < nl; ++l) {
-        if (A.term_list[l].empty()) continue;
+        if (A.direct_list[l].empty()) continue;
         const AmgLevelDev& D = A.dev[l];
-        hipLaunchKernelGGL((k_mg_direct<NR>), dim3((unsigned)A.term_list[l].size()), dim3(256), 0, s,
-                           A.d_term_list[l], A.d_term, (const float*)A.d_inv, w.st, (const float*)D.d_b, D.d_x);
+        hipLaunchKernelGGL((k_mg_direct<NR>), dim3((unsigned)A.direct_list[l].size()), dim3(256), 0, s,
+                           A.d_direct_list[l], A.d_term, (const float*)A.d_inv, w.st, (const float*)D.d_b, D.d_x);
     }
     for (int l = nl - 2; l >= 0; --l) {
         const AmgLevelDev& D = A.dev[l];
         const AmgLevelDev& C = A.dev[l + 1];
         if (l == 0) {
             hipLaunchKernelGGL((k_mg_prolong<VT, NR>), dim3(nblk(6 * (int64_t)D.n, 256)), dim3(256), 0, s, D.n,
-                               D.d_node_sys, (const uint8_t*)D.d_cont, D.d_p_ptr, D.d_p_col, (const _Float16*)D.d_P16,
+                               D.d_node_sys, (const uint8_t*)A.d_smooth[0], D.d_p_ptr, D.d_p_col, (const _Float16*)D.d_P16,
                                (const float*)D.d_theta, w.st, (const VT*)bf.r[k & 1], (const float*)C.d_x, bf.t0);
             wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
                 constexpr int WP = decltype(W)::value;
@@ -1734,22 +1996,24 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
                                    (const float*)D.d_theta,
                                    (const VT*)bf.r[k & 1], (const VT*)bf.r[(k - 1) & 1], (const VT*)bf.t0, bf.z);
             });
-        } else {
+        } else if (A.tail_list[l].size() < (size_t)std::count(A.host[l].cont.begin(), A.host[l].cont.end(), 1)) {
             hipLaunchKernelGGL((k_mg_prolong<float, NR>), dim3(nblk(6 * (int64_t)D.n, 256)), dim3(256), 0, s, D.n,
-                               D.d_node_sys, (const uint8_t*)D.d_cont, D.d_p_ptr, D.d_p_col, (const _Float16*)D.d_P16,
-                               (const float*)D.d_theta, w.st, (const float*)D.d_b, (const float*)C.d_x, D.d_t);
+                               D.d_node_sys, (const uint8_t*)A.d_smooth[l], D.d_p_ptr, D.d_p_col,
+                               (const _Float16*)D.d_P16, (const float*)D.d_theta, w.st, (const float*)D.d_b,
+                               (const float*)C.d_x, D.d_t);
             wps_dispatch<float, NR>(4, [&](auto W) {
                 constexpr int WP = decltype(W)::value;
                 hipLaunchKernelGGL((k_mg_post<NR, WP>), slice_grid(D.n_wg), dim3(64 * WP), 0, s,
                                    level_slices(pl, A, l), (const float*)D.d_A32, (const float*)D.d_theta,
-                                   (const uint8_t*)D.d_cont, w.st, (const float*)D.d_b, (const float*)D.d_t, D.d_x);
+                                   (const uint8_t*)A.d_smooth[l], w.st, (const float*)D.d_b, (const float*)D.d_t,
+                                   D.d_x);
             });
         }
     }
 }
 
-// one ordinary PCG iteration k; `timed` brackets the level-0 update and
-// pre-smoothing kernel (the iteration's largest) with the sampled events of
+// one ordinary PCG iteration k; `timed` brackets the direction and CG-product
+// kernel (the iteration's largest) with the sampled events of
 // jsso_timing_enable (eager launches only)
 template <typename MT, typename VT, int NR>
 static void amg_step(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs<VT>& bf, const MT* A0, int k,
@@ -1759,14 +2023,14 @@ static void amg_step(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs<VT
     const _Float16* A16 = (const _Float16*)pl->d_Ke16;
     wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
         constexpr int WP = decltype(W)::value;
-        hipLaunchKernelGGL((k_pcg_dir<MT, VT, NR, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A0,
-                           (const VT*)bf.z, (const VT*)bf.p[(k - 1) & 1], bf.p[k & 1], bf.q);
         std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
         if (timed) timing_begin(s, ev);
+        hipLaunchKernelGGL((k_pcg_dir<MT, VT, NR, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A0,
+                           (const VT*)bf.z, (const VT*)bf.p[(k - 1) & 1], bf.p[k & 1], bf.q);
+        if (timed) timing_end(s, ev);
         hipLaunchKernelGGL((k_pcg_pre<VT, NR, false, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A16,
                            (const float*)A.dev[0].d_theta, (const VT*)bf.r[(k - 1) & 1], bf.r[k & 1],
                            (const VT*)bf.q, (const VT*)bf.p[k & 1], bf.xlo, bf.t0);
-        if (timed) timing_end(s, ev);
     });
     amg_vcycle_rest<MT, VT, NR>(pl, A, w, bf, A0, k, s);
 }
@@ -1787,7 +2051,8 @@ static void amg_replace(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs
     });
     hipLaunchKernelGGL((k_pcg_fold<VT, NR>), g, dim3(SOLVER_WG), 0, s, sl, w.P, w.st, k, (const VT*)bf.p[k & 1],
                        bf.xlo, w.x64);
-    hipLaunchKernelGGL((k_replace<VT, NR, 1>), g, dim3(64), 0, s, sl, w.P, w.st, (const double*)pl->d_Ke64,
+    constexpr int RW = WpsFits<double, NR, 4>::value ? 4 : 1;
+    hipLaunchKernelGGL((k_replace<VT, NR, RW>), g, dim3(64 * RW), 0, s, sl, w.P, w.st, (const double*)pl->d_Ke64,
                        (const double*)w.b64, (const double*)w.x64, bf.r[k & 1]);
     hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, steps, AMG_STALL_WINDOW);
     hipLaunchKernelGGL((k_save<NR>), g, dim3(SOLVER_WG), 0, s, sl, (const int32_t*)w.st.save, (const double*)w.x64,

@@ -133,14 +133,6 @@ int timing_read(double* avg_ms, int64_t* samples, int reset) {
 
 enum { CG_START = 0, CG_STEP = 1, CG_FRESH = 2 };
 
-// Waves-per-slice variants built for VT and NR: the cross-wave LDS buffer stays
-// <= 24 KB, and more waves only pay while one slot group fits the registers.
-template <typename VT, int NR, int WPS>
-struct WpsFits {
-    static constexpr bool value =
-        WPS == 1 || ((size_t)(WPS - 1) * 6 * NR * 64 * sizeof(VT) <= 24576 && NR * sizeof(VT) <= 16);
-};
-
 // Fused CG iteration k (one lane per block row, sliced ELL).
 //   CG_STEP : alpha = rr_{k-1}/<p,q>_{k-1}; r_k = r_{k-1} - alpha q_{k-1} (own and
 //             gathered rows, on the fly); x += alpha p_{k-1};

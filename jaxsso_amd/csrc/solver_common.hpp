@@ -176,6 +176,14 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double* lds, dou
     }
 }
 
+// Waves-per-slice variants built for VT and NR: the cross-wave LDS buffer stays
+// <= 24 KB, and more waves only pay while one slot group fits the registers.
+template <typename VT, int NR, int WPS>
+struct WpsFits {
+    static constexpr bool value =
+        WPS == 1 || ((size_t)(WPS - 1) * 6 * NR * 64 * sizeof(VT) <= 24576 && NR * sizeof(VT) <= 16);
+};
+
 struct Workspace {
     double *b64, *x64, *xbest;
     void *xlo, *r[2], *q[2], *p;
