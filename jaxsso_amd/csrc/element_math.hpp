@@ -274,9 +274,9 @@ JDEV void quad_plate_consts(T t, T E, T nu, T& cb, T& cs) {
     cs = E * t * (5.0 / 6.0) / (2.0 * (1.0 + nu));
 }
 
-// x^T Cb y for 3-vectors.
-template <typename T>
-JDEV T cb_form(T cb, T nu, T a0, T a1, T a2, T b0, T b1, T b2) {
+// x^T Cb y for 3-vectors (constants of type TC, vectors of type T).
+template <typename TC, typename T>
+JDEV auto cb_form(TC cb, TC nu, T a0, T a1, T a2, T b0, T b1, T b2) -> decltype(TC(0.0) * T(0.0)) {
     return cb * (a0 * b0 + nu * a0 * b1 + nu * a1 * b0 + a1 * b1 + (1.0 - nu) / 2.0 * a2 * b2);
 }
 
@@ -294,10 +294,12 @@ JDEV void quad_cm(T E, T nu, T kx, T ky, T Cm[3][3]) {
 // Diagonal entry d (of the 12x12 bending+shear block, d in {1,2,4,5,7,8,10,11})
 // summed over the Gauss points: candidates of the drilling spring
 // k_rz = min |diag| / 1000 (element.py:953-978).
-template <typename T>
-JDEV void quad_rot_diag(const T xy[8], T cb, T cs, T nu, T diag[8]) {
+template <typename T, typename TM>
+JDEV void quad_rot_diag(const T xy[8], TM cb, TM cs, TM nu, decltype(T(0.0) * TM(0.0)) diag[8]) {
+    using P = decltype(T(0.0) * TM(0.0));
 #pragma unroll
-    for (int m = 0; m < 8; ++m) diag[m] = T(0.0);
+    for (int m = 0; m < 8; ++m) diag[m] = P(0.0);
+#pragma unroll 1
     for (int g = 0; g < 4; ++g) {
         QuadGP<T> G;
         quad_gp(xy, g, G);
@@ -331,6 +333,219 @@ JDEV T quad_drill(const T diag[8]) {
     T out = T(m / 1000.0);
     set_tan(out, tsum / cnt / 1000.0);
     return out;
+}
+
+// ---------------------------------------------------------------------------
+// MITC-4 adjoint: phi = lambda_e^T K_e u_e for one load case (the element term
+// of the reference's VJP through K, solver.py:221-250, with K_e of
+// element.py:880-1084)
+// ---------------------------------------------------------------------------
+// phi depends on the 12 nodal coordinates only through the frame R (3 x 3) and
+// the in-plane coordinates xy (six free entries: node 3 is the origin), so
+// dphi/dc = dphi/dxy . dxy/dc + dphi/dR : dR/dc.
+
+// An empty statement the compiler must assume reads and writes memory: placed
+// at the top of a node's loop iteration it keeps that node's loads and products
+// inside the iteration (hoisted, the four nodes' values of every pass stay live
+// together and the adjoint kernels run out of registers).
+JDEV void keep_loads_here() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" ::: "memory");
+#endif
+}
+
+// Local values of node `node`: ul = R u, ll = R lambda per translation and
+// rotation triple (lambda = lam, or lam_scale u when lam is null).
+JDEV void quad_local_vals(const V3<double> R[3], const double* u, const double* lam, double lam_scale, int node,
+                          int nrhs, int rhs, double ul[6], double ll[6]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        double uv[3], lv[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int64_t gi = ((int64_t)6 * node + 3 * h + k) * nrhs + rhs;
+            uv[k] = u[gi];
+            lv[k] = lam ? lam[gi] : lam_scale * u[gi];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            ul[3 * h + i] = R[i].x * uv[0] + R[i].y * uv[1] + R[i].z * uv[2];
+            ll[3 * h + i] = R[i].x * lv[0] + R[i].y * lv[1] + R[i].z * lv[2];
+        }
+    }
+}
+
+// phi with the frame and the nodal values fixed; TX is the type of xy, TM that
+// of the properties (Dual for the one direction being differentiated).
+template <typename TX, typename TM>
+JDEV auto quad_phi(const V3<double> R[3], const TX xy[8], TM t, TM E, TM nu, TM kx, TM ky, const int nd[4],
+                   const double* u, const double* lam, double lam_scale, int nrhs, int rhs)
+    -> decltype(TX(0.0) * TM(0.0)) {
+    using P = decltype(TX(0.0) * TM(0.0));
+    TM cb, cs, Cm[3][3];
+    quad_plate_consts<TM>(t, E, nu, cb, cs);
+    quad_cm<TM>(E, nu, kx, ky, Cm);
+    P diag[8];
+    quad_rot_diag<TX, TM>(xy, cb, cs, nu, diag);
+    const P krz = quad_drill<P>(diag);
+    P em(0.0), eb(0.0), es(0.0);
+#pragma unroll 1
+    for (int g = 0; g < 4; ++g) {
+        QuadGP<TX> G;
+        quad_gp<TX>(xy, g, G);
+        TX mu[3] = {TX(0.0), TX(0.0), TX(0.0)}, ml[3] = {TX(0.0), TX(0.0), TX(0.0)};
+        TX ku[3] = {TX(0.0), TX(0.0), TX(0.0)}, kl[3] = {TX(0.0), TX(0.0), TX(0.0)};
+        TX gu[2] = {TX(0.0), TX(0.0)}, gl[2] = {TX(0.0), TX(0.0)};
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            keep_loads_here();
+            double ul[6], ll[6];
+            quad_local_vals(R, u, lam, lam_scale, nd[a], nrhs, rhs, ul, ll);
+            const TX h0 = G.dH[0][a], h1 = G.dH[1][a];
+            mu[0] = mu[0] + h0 * ul[0];
+            mu[1] = mu[1] + h1 * ul[1];
+            mu[2] = mu[2] + h1 * ul[0] + h0 * ul[1];
+            ml[0] = ml[0] + h0 * ll[0];
+            ml[1] = ml[1] + h1 * ll[1];
+            ml[2] = ml[2] + h1 * ll[0] + h0 * ll[1];
+            ku[0] = ku[0] - h0 * ul[4];
+            ku[1] = ku[1] + h1 * ul[3];
+            ku[2] = ku[2] + h0 * ul[3] - h1 * ul[4];
+            kl[0] = kl[0] - h0 * ll[4];
+            kl[1] = kl[1] + h1 * ll[3];
+            kl[2] = kl[2] + h0 * ll[3] - h1 * ll[4];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                TX b0, b1;
+                quad_bgamma_col<TX>(xy, G, 3 * a + cc, b0, b1);
+                gu[0] = gu[0] + b0 * ul[2 + cc];
+                gu[1] = gu[1] + b1 * ul[2 + cc];
+                gl[0] = gl[0] + b0 * ll[2 + cc];
+                gl[1] = gl[1] + b1 * ll[2 + cc];
+            }
+        }
+        P mform(0.0);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) mform = mform + ml[r] * (Cm[r][0] * mu[0] + Cm[r][1] * mu[1] + Cm[r][2] * mu[2]);
+        em = em + mform * G.detJ;
+        eb = eb + cb_form(cb, nu, kl[0], kl[1], kl[2], ku[0], ku[1], ku[2]) * G.detJ;
+        es = es + cs * (gl[0] * gu[0] + gl[1] * gu[1]) * G.detJ;
+    }
+    double ed = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        double ul[6], ll[6];
+        quad_local_vals(R, u, lam, lam_scale, nd[a], nrhs, rhs, ul, ll);
+        ed += ll[5] * ul[5];
+    }
+    return t * em + eb + es + krz * ed;
+}
+
+// dphi/dR (reverse mode through the local values): phi is bilinear in ul and ll,
+// dphi/dul = sum over the Gauss points of B^T (dphi/dstrain) plus the drilling
+// term krz ll, likewise for ll, and ul = R u gives dphi/dR_ij = sum over the
+// nodes and triples of dphi/dul_i u_j + dphi/dll_i lambda_j.
+JDEV void quad_frame_adjoint(const V3<double> R[3], const double xy[8], double t, double E, double nu, double kx,
+                             double ky, const int nd[4], const double* u, const double* lam, double lam_scale,
+                             int nrhs, int rhs, double dR[3][3]) {
+    double cb, cs, Cm[3][3], diag[8];
+    quad_plate_consts<double>(t, E, nu, cb, cs);
+    quad_cm<double>(E, nu, kx, ky, Cm);
+    quad_rot_diag<double, double>(xy, cb, cs, nu, diag);
+    const double krz = quad_drill<double>(diag);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) dR[i][j] = 0.0;
+    // adds dphi/d(ul, ll) of node a (gu, gl: the six local components) into dR
+    auto scatter = [&](int a, const double gu[6], const double gl[6]) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int64_t gi = ((int64_t)6 * nd[a] + 3 * h + j) * nrhs + rhs;
+                const double uj = u[gi], lj = lam ? lam[gi] : lam_scale * u[gi];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) dR[i][j] += gu[3 * h + i] * uj + gl[3 * h + i] * lj;
+            }
+    };
+#pragma unroll 1
+    for (int g = 0; g < 4; ++g) {
+        QuadGP<double> G;
+        quad_gp<double>(xy, g, G);
+        double mu[3] = {0.0, 0.0, 0.0}, ml[3] = {0.0, 0.0, 0.0}, ku[3] = {0.0, 0.0, 0.0}, kl[3] = {0.0, 0.0, 0.0};
+        double gu[2] = {0.0, 0.0}, gl[2] = {0.0, 0.0};
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            keep_loads_here();
+            double ul[6], ll[6];
+            quad_local_vals(R, u, lam, lam_scale, nd[a], nrhs, rhs, ul, ll);
+            const double h0 = G.dH[0][a], h1 = G.dH[1][a];
+            mu[0] += h0 * ul[0];
+            mu[1] += h1 * ul[1];
+            mu[2] += h1 * ul[0] + h0 * ul[1];
+            ml[0] += h0 * ll[0];
+            ml[1] += h1 * ll[1];
+            ml[2] += h1 * ll[0] + h0 * ll[1];
+            ku[0] -= h0 * ul[4];
+            ku[1] += h1 * ul[3];
+            ku[2] += h0 * ul[3] - h1 * ul[4];
+            kl[0] -= h0 * ll[4];
+            kl[1] += h1 * ll[3];
+            kl[2] += h0 * ll[3] - h1 * ll[4];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                double b0, b1;
+                quad_bgamma_col<double>(xy, G, 3 * a + cc, b0, b1);
+                gu[0] += b0 * ul[2 + cc];
+                gu[1] += b1 * ul[2 + cc];
+                gl[0] += b0 * ll[2 + cc];
+                gl[1] += b1 * ll[2 + cc];
+            }
+        }
+        // dphi/dstrain of u (s*u) and of lambda (s*l) at this Gauss point
+        const double tw = t * G.detJ, bw = cb * G.detJ, sw = cs * G.detJ;
+        double smu[3], sml[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            smu[c] = tw * (ml[0] * Cm[0][c] + ml[1] * Cm[1][c] + ml[2] * Cm[2][c]);
+            sml[c] = tw * (Cm[c][0] * mu[0] + Cm[c][1] * mu[1] + Cm[c][2] * mu[2]);
+        }
+        const double sku[3] = {bw * (kl[0] + nu * kl[1]), bw * (nu * kl[0] + kl[1]), bw * (1.0 - nu) / 2.0 * kl[2]};
+        const double skl[3] = {bw * (ku[0] + nu * ku[1]), bw * (nu * ku[0] + ku[1]), bw * (1.0 - nu) / 2.0 * ku[2]};
+        const double sgu[2] = {sw * gl[0], sw * gl[1]}, sgl[2] = {sw * gu[0], sw * gu[1]};
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            keep_loads_here();
+            const double h0 = G.dH[0][a], h1 = G.dH[1][a];
+            double du[6], dl[6];
+            du[0] = h0 * smu[0] + h1 * smu[2];
+            du[1] = h1 * smu[1] + h0 * smu[2];
+            du[3] = h1 * sku[1] + h0 * sku[2];
+            du[4] = -h0 * sku[0] - h1 * sku[2];
+            dl[0] = h0 * sml[0] + h1 * sml[2];
+            dl[1] = h1 * sml[1] + h0 * sml[2];
+            dl[3] = h1 * skl[1] + h0 * skl[2];
+            dl[4] = -h0 * skl[0] - h1 * skl[2];
+            du[2] = dl[2] = du[5] = dl[5] = 0.0;
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {   // shear: w, theta_x, theta_y
+                double b0, b1;
+                quad_bgamma_col<double>(xy, G, 3 * a + cc, b0, b1);
+                du[2 + cc] += b0 * sgu[0] + b1 * sgu[1];
+                dl[2 + cc] += b0 * sgl[0] + b1 * sgl[1];
+            }
+            scatter(a, du, dl);
+        }
+    }
+    // drilling: dphi/dul_5 = krz ll_5, dphi/dll_5 = krz ul_5
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        double ul[6], ll[6];
+        quad_local_vals(R, u, lam, lam_scale, nd[a], nrhs, rhs, ul, ll);
+        const double du[6] = {0.0, 0.0, 0.0, 0.0, 0.0, krz * ll[5]}, dl[6] = {0.0, 0.0, 0.0, 0.0, 0.0, krz * ul[5]};
+        scatter(a, du, dl);
+    }
 }
 
 }  // namespace jsso

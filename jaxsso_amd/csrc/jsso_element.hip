@@ -363,109 +363,125 @@ __global__ __launch_bounds__(256) void adj_beam_kernel(int32_t n, const int32_t*
     }
 }
 
-__global__ __launch_bounds__(128) void adj_quad_kernel(int32_t n, int32_t n_beam, const int32_t* __restrict__ cnct,
-                                                       const double* __restrict__ crds,
-                                                       const double* __restrict__ props,
-                                                       const double* __restrict__ u,
-                                                       const double* __restrict__ lam, double lam_scale,
-                                                       int nrhs, double* __restrict__ gbuf,
-                                                       double* __restrict__ dprop) {
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= (int64_t)n * QDIR) return;
-    const int e = (int)(tid / QDIR), p = (int)(tid % QDIR);
-    int nd[4];
-    Dual c[12];
+// Quad adjoint for load case `rhs`.  phi = lambda_e^T K_e u_e depends on the
+// coordinates only through the frame R and the in-plane coordinates xy
+// (element_math.hpp), so it is differentiated in three kernels that each run
+// uniform wavefronts:
+//   adj_quad_fwd:     forward mode in one free in-plane coordinate (k < 6: dphi/dxy
+//                     into the element's gbuf slot k) or one property (k >= 6:
+//                     dprop), with R and the local nodal values plain doubles;
+//   adj_quad_frame:   dphi/dR by reverse mode (gbuf slots 6..14);
+//   adj_quad_combine: thread (element, coordinate p) applies the frame's Jacobian
+//                     column dc_p (forward mode through quad_frame) to those 15
+//                     partials and overwrites slot p.
+// The result equals forward mode through the whole element per coordinate (to
+// rounding) at about half the work, and without the dual frame and nodal values
+// that made that form spill.
+__device__ constexpr int QA_XY_FREE[6] = {0, 1, 2, 3, 6, 7};   // xy[4] = xy[5] = 0: node 3 is the origin
+constexpr int QA_FWD = 11;                                      // 6 in-plane coordinates + 5 properties
+constexpr int QA_E = 32;                                        // elements per combine workgroup
+
+__device__ __forceinline__ void quad_geometry(int64_t e, const int32_t* __restrict__ cnct,
+                                              const double* __restrict__ crds, int nd[4], V3<double> R[3],
+                                              double xy[8]) {
+    double c[12];
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         nd[a] = cnct[4 * e + a];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) c[3 * a + k] = Dual(crds[3 * nd[a] + k]);
+        for (int q = 0; q < 3; ++q) c[3 * a + q] = crds[3 * nd[a] + q];
     }
-    Dual t(props[5 * e + 0]), E(props[5 * e + 1]), nu(props[5 * e + 2]), kx(props[5 * e + 3]),
-        ky(props[5 * e + 4]);
-    if (p < 12) c[p].d = 1.0;
-    else if (p == 12) t.d = 1.0;
-    else if (p == 13) E.d = 1.0;
-    else if (p == 14) nu.d = 1.0;
-    else if (p == 15) kx.d = 1.0;
-    else ky.d = 1.0;
-    V3<Dual> R[3];
-    Dual xy[8];
-    quad_frame<Dual>(c, R, xy);
-    Dual cb, cs, Cm[3][3];
-    quad_plate_consts<Dual>(t, E, nu, cb, cs);
-    quad_cm<Dual>(E, nu, kx, ky, Cm);
-    Dual diag[8];
-    quad_rot_diag<Dual>(xy, cb, cs, nu, diag);
-    const Dual krz = quad_drill<Dual>(diag);
-    for (int rhs = 0; rhs < nrhs; ++rhs) {
-        Dual ul[24], ll[24];
+    quad_frame<double>(c, R, xy);
+}
+
+__device__ __forceinline__ int64_t quad_slot(int32_t n_beam, int64_t e, int slot, int nrhs, int rhs) {
+    return ((int64_t)n_beam * BDIR + e * QDIR + slot) * nrhs + rhs;
+}
+
+__global__ __launch_bounds__(256) void adj_quad_fwd(int32_t n, int32_t n_beam, const int32_t* __restrict__ cnct,
+                                                    const double* __restrict__ crds,
+                                                    const double* __restrict__ props, const double* __restrict__ u,
+                                                    const double* __restrict__ lam, double lam_scale, int nrhs,
+                                                    int rhs, double* __restrict__ gbuf,
+                                                    double* __restrict__ dprop) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)QA_FWD * n) return;
+    const int k = (int)(t / n);
+    const int64_t e = t % n;
+    int nd[4];
+    V3<double> R[3];
+    double xy[8];
+    quad_geometry(e, cnct, crds, nd, R, xy);
+    const double* pr = props + 5 * e;
+    if (k < 6) {
+        Dual xd[8];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
+        for (int i = 0; i < 8; ++i) xd[i] = Dual(xy[i], i == QA_XY_FREE[k] ? 1.0 : 0.0);
+        gbuf[quad_slot(n_beam, e, k, nrhs, rhs)] =
+            quad_phi<Dual, double>(R, xd, pr[0], pr[1], pr[2], pr[3], pr[4], nd, u, lam, lam_scale, nrhs, rhs).d;
+    } else {
+        Dual m[5];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                double uv[3], lv[3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const int64_t gi = ((int64_t)6 * nd[a] + 3 * h + k) * nrhs + rhs;
-                    uv[k] = u[gi];
-                    lv[k] = load_lam(u, lam, lam_scale, gi);
-                }
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    ul[6 * a + 3 * h + i] = R[i].x * uv[0] + R[i].y * uv[1] + R[i].z * uv[2];
-                    ll[6 * a + 3 * h + i] = R[i].x * lv[0] + R[i].y * lv[1] + R[i].z * lv[2];
-                }
-            }
-        }
-        Dual em(0.0), eb(0.0), es(0.0), ed(0.0);
-        for (int g = 0; g < 4; ++g) {
-            QuadGP<Dual> G;
-            quad_gp<Dual>(xy, g, G);
-            Dual mu[3] = {Dual(0.0), Dual(0.0), Dual(0.0)}, ml[3] = {Dual(0.0), Dual(0.0), Dual(0.0)};
-            Dual ku[3] = {Dual(0.0), Dual(0.0), Dual(0.0)}, kl[3] = {Dual(0.0), Dual(0.0), Dual(0.0)};
-            Dual gu[2] = {Dual(0.0), Dual(0.0)}, gl[2] = {Dual(0.0), Dual(0.0)};
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const Dual h0 = G.dH[0][a], h1 = G.dH[1][a];
-                mu[0] = mu[0] + h0 * ul[6 * a];
-                mu[1] = mu[1] + h1 * ul[6 * a + 1];
-                mu[2] = mu[2] + h1 * ul[6 * a] + h0 * ul[6 * a + 1];
-                ml[0] = ml[0] + h0 * ll[6 * a];
-                ml[1] = ml[1] + h1 * ll[6 * a + 1];
-                ml[2] = ml[2] + h1 * ll[6 * a] + h0 * ll[6 * a + 1];
-                ku[0] = ku[0] - h0 * ul[6 * a + 4];
-                ku[1] = ku[1] + h1 * ul[6 * a + 3];
-                ku[2] = ku[2] + h0 * ul[6 * a + 3] - h1 * ul[6 * a + 4];
-                kl[0] = kl[0] - h0 * ll[6 * a + 4];
-                kl[1] = kl[1] + h1 * ll[6 * a + 3];
-                kl[2] = kl[2] + h0 * ll[6 * a + 3] - h1 * ll[6 * a + 4];
-#pragma unroll
-                for (int cc = 0; cc < 3; ++cc) {
-                    Dual b0, b1;
-                    quad_bgamma_col<Dual>(xy, G, 3 * a + cc, b0, b1);
-                    gu[0] = gu[0] + b0 * ul[6 * a + 2 + cc];
-                    gu[1] = gu[1] + b1 * ul[6 * a + 2 + cc];
-                    gl[0] = gl[0] + b0 * ll[6 * a + 2 + cc];
-                    gl[1] = gl[1] + b1 * ll[6 * a + 2 + cc];
-                }
-            }
-            Dual mform(0.0);
-#pragma unroll
-            for (int r = 0; r < 3; ++r) mform = mform + ml[r] * (Cm[r][0] * mu[0] + Cm[r][1] * mu[1] + Cm[r][2] * mu[2]);
-            em = em + mform * G.detJ;
-            eb = eb + cb_form<Dual>(cb, nu, kl[0], kl[1], kl[2], ku[0], ku[1], ku[2]) * G.detJ;
-            es = es + cs * (gl[0] * gu[0] + gl[1] * gu[1]) * G.detJ;
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a) ed = ed + ll[6 * a + 5] * ul[6 * a + 5];
-        const Dual phi = t * em + eb + es + krz * ed;
-        const double out = -phi.d;
-        if (p < 12)
-            gbuf[((int64_t)n_beam * BDIR + (int64_t)e * QDIR + p) * nrhs + rhs] = out;
-        else if (dprop)
-            dprop[((int64_t)e * 5 + (p - 12)) * nrhs + rhs] = out;
+        for (int i = 0; i < 5; ++i) m[i] = Dual(pr[i], i == k - 6 ? 1.0 : 0.0);
+        const double d =
+            quad_phi<double, Dual>(R, xy, m[0], m[1], m[2], m[3], m[4], nd, u, lam, lam_scale, nrhs, rhs).d;
+        if (dprop) dprop[(e * 5 + (k - 6)) * nrhs + rhs] = -d;
     }
+}
+
+__global__ __launch_bounds__(256) void adj_quad_frame(int32_t n, int32_t n_beam, const int32_t* __restrict__ cnct,
+                                                      const double* __restrict__ crds,
+                                                      const double* __restrict__ props,
+                                                      const double* __restrict__ u, const double* __restrict__ lam,
+                                                      double lam_scale, int nrhs, int rhs,
+                                                      double* __restrict__ gbuf) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    int nd[4];
+    V3<double> R[3];
+    double xy[8];
+    quad_geometry(e, cnct, crds, nd, R, xy);
+    const double* pr = props + 5 * e;
+    double dR[3][3];
+    quad_frame_adjoint(R, xy, pr[0], pr[1], pr[2], pr[3], pr[4], nd, u, lam, lam_scale, nrhs, rhs, dR);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) gbuf[quad_slot(n_beam, e, 6 + 3 * i + j, nrhs, rhs)] = dR[i][j];
+}
+
+__global__ __launch_bounds__(12 * QA_E) void adj_quad_combine(int32_t n, int32_t n_beam,
+                                                              const int32_t* __restrict__ cnct,
+                                                              const double* __restrict__ crds, int nrhs, int rhs,
+                                                              double* __restrict__ gbuf) {
+    __shared__ double part[QA_E][15];   // dphi/dxy (6 free entries), dphi/dR (9)
+    const int tid = threadIdx.x;
+    const int64_t e0 = (int64_t)blockIdx.x * QA_E;
+    for (int v = tid; v < QA_E * 15; v += blockDim.x) {
+        const int el = v / 15, q = v % 15;
+        if (e0 + el < n) part[el][q] = gbuf[quad_slot(n_beam, e0 + el, q, nrhs, rhs)];
+    }
+    __syncthreads();   // every partial is read before any slot is overwritten
+    const int el = tid % QA_E, p = tid / QA_E;   // coordinate p = 3 * node + component
+    const int64_t e = e0 + el;
+    if (e >= n) return;
+    Dual c[12];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int node = cnct[4 * e + a];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) c[3 * a + q] = Dual(crds[3 * node + q], 3 * a + q == p ? 1.0 : 0.0);
+    }
+    V3<Dual> Rd[3];
+    Dual xyd[8];
+    quad_frame<Dual>(c, Rd, xyd);
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) sum += part[el][q] * xyd[QA_XY_FREE[q]].d;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        sum += part[el][6 + 3 * i] * Rd[i].x.d + part[el][7 + 3 * i] * Rd[i].y.d + part[el][8 + 3 * i] * Rd[i].z.d;
+    gbuf[quad_slot(n_beam, e, p, nrhs, rhs)] = -sum;
 }
 
 // node gradient: sum of the coordinate derivatives of the elements incident to
@@ -495,10 +511,16 @@ int launch_adjoint(Plan* p, const double* crds, const double* prop_beam, const d
         JSSO_HIP(hipGetLastError());
     }
     if (p->n_quad > 0) {
-        const int64_t total = (int64_t)p->n_quad * QDIR;
-        hipLaunchKernelGGL(adj_quad_kernel, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s,
-                           p->n_quad, p->n_beam, p->d_cnct_quad, crds, prop_quad, u, lam, lam_scale,
-                           nrhs, gbuf, dprop_quad);
+        const int nq = p->n_quad;
+        for (int rhs = 0; rhs < nrhs; ++rhs) {
+            hipLaunchKernelGGL(adj_quad_fwd, dim3((unsigned)(((int64_t)QA_FWD * nq + 255) / 256)), dim3(256), 0, s,
+                               nq, p->n_beam, p->d_cnct_quad, crds, prop_quad, u, lam, lam_scale, nrhs, rhs, gbuf,
+                               dprop_quad);
+            hipLaunchKernelGGL(adj_quad_frame, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, nq, p->n_beam,
+                               p->d_cnct_quad, crds, prop_quad, u, lam, lam_scale, nrhs, rhs, gbuf);
+            hipLaunchKernelGGL(adj_quad_combine, dim3((unsigned)((nq + QA_E - 1) / QA_E)), dim3(12 * QA_E), 0, s,
+                               nq, p->n_beam, p->d_cnct_quad, crds, nrhs, rhs, gbuf);
+        }
         JSSO_HIP(hipGetLastError());
     }
     return JSSO_OK;
