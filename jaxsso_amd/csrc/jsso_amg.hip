@@ -46,7 +46,10 @@ constexpr int AMG_MAXC = 26;       // nodes of a structure's directly solved coa
 constexpr int AMG_QR_RPL = 5;      // null-space rows per lane in the per-aggregate QR
 constexpr int AMG_MAX_AGG = AMG_QR_RPL * 64 / 6;   // nodes per aggregate (53)
 constexpr int AMG_POWER_STEPS = 12;
-constexpr double AMG_THETA = 0.62;        // smoothing step z = r / (AMG_THETA rho)
+constexpr double AMG_THETA = 0.62;        // level-0 smoothing step z = r / (AMG_THETA rho)
+// coarse levels: degree-2 Chebyshev smoothing on [AMG_CHEB_LO hi, hi], hi = AMG_CHEB_HI rho
+constexpr double AMG_CHEB_HI = 1.1;
+constexpr double AMG_CHEB_LO = 0.1;
 constexpr int AMG_STALL_WINDOW = 256;     // iterations without a 10% gain that end a solve (floor)
 constexpr double AMG_RHO_SAFETY = 1.12;   // the power iteration approaches rho from below
 constexpr int AMG_MAXLEVELS = 12;
@@ -87,7 +90,8 @@ struct AmgLevelDev {
     uint8_t* d_cont = nullptr;   // per structure
     // smoothing / power iteration
     double *d_rho = nullptr, *d_omega = nullptr;   // per structure
-    float* d_theta = nullptr;                      // per structure
+    float* d_theta = nullptr;                      // per structure (coarse levels: the Chebyshev centre)
+    float* d_cheb = nullptr;                       // per structure, coarse levels: Chebyshev coefficient c
     double *d_pv[2] = {nullptr, nullptr};          // power iteration vectors (6 n)
     double* d_ppart = nullptr;                     // per slice, per step parity: |y|^2
     // transfer to the next level
@@ -103,7 +107,7 @@ struct AmgLevelDev {
     double *d_P64 = nullptr, *d_W64 = nullptr;
     _Float16 *d_P16 = nullptr, *d_PT16 = nullptr;   // FP16, by fine row / transposed by coarse node
     // V-cycle vectors (level >= 1), 6 n x nrhs FP32
-    float *d_b = nullptr, *d_t = nullptr, *d_x = nullptr;
+    float *d_b = nullptr, *d_t = nullptr, *d_r = nullptr, *d_x = nullptr;
 };
 
 struct AmgState {
@@ -630,10 +634,18 @@ __global__ __launch_bounds__(64 * WPS) void k_amg_power(Slices sl, const float* 
 }
 
 // rho = safety * |A v| (|v| = 1) per continuing structure; omega = 4/(3 rho);
-// theta = AMG_THETA rho
+// level 0: theta = AMG_THETA rho (one damped step).  Coarse levels (cheb set):
+// degree-2 Chebyshev on [lo, hi] = [AMG_CHEB_LO hi, hi], hi = AMG_CHEB_HI rho,
+// with centre theta = (hi + lo) / 2, half-width delta = (hi - lo) / 2 and
+// sigma = theta / delta; its two steps from x0 are
+//   x1 = x0 + r0 / theta,  x2 = x1 + rho1 rho0 r0 / theta + (2 rho1 / delta) r1,
+// rho0 = 1 / sigma, rho1 = 1 / (2 sigma - rho0), and both weights equal
+//   c = 2 sigma^2 / ((2 sigma^2 - 1) theta),
+// so x2 = x0 + c (r0 + r1), r1 = r0 - A r0 / theta (Saad, Iterative Methods for
+// Sparse Linear Systems, 2nd ed., ch. 12: the three-term Chebyshev recurrence).
 __global__ void k_amg_theta(int32_t ns, const int32_t* __restrict__ sys_wg_off, const double* __restrict__ part,
                             int last, const uint8_t* __restrict__ cont, int32_t n_wg, double* __restrict__ rho,
-                            double* __restrict__ omega, float* __restrict__ theta) {
+                            double* __restrict__ omega, float* __restrict__ theta, float* __restrict__ cheb) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns || !cont[s]) return;
     const double* pp = part + (int64_t)(last & 1) * n_wg;
@@ -643,7 +655,14 @@ __global__ void k_amg_theta(int32_t ns, const int32_t* __restrict__ sys_wg_off, 
     if (!(r > 0.0) || !isfinite(r)) r = 1.0;
     rho[s] = r;
     omega[s] = 4.0 / (3.0 * r);
-    theta[s] = (float)(AMG_THETA * r);
+    if (cheb) {
+        const double hi = AMG_CHEB_HI * r, lo = AMG_CHEB_LO * hi;
+        const double th = 0.5 * (hi + lo), sg = th / (0.5 * (hi - lo));
+        theta[s] = (float)th;
+        cheb[s] = (float)(2.0 * sg * sg / ((2.0 * sg * sg - 1.0) * th));
+    } else {
+        theta[s] = (float)(AMG_THETA * r);
+    }
 }
 
 // P(i, J) = sum over its products of ((1 - omega) delta_ij I - omega A_off(i, j)) Q_j;
@@ -1286,64 +1305,59 @@ __global__ __launch_bounds__(SOLVER_WG) void k_pcg_fold(Slices sl, Partials P, S
     }
 }
 
-// Coarse level, first smoothing step: t = b - A (b / theta).
-template <int NR, int WPS>
-__global__ __launch_bounds__(64 * WPS) void k_mg_pre(Slices sl, const float* __restrict__ A,
-                                                     const float* __restrict__ theta, const uint8_t* __restrict__ cont,
-                                                     SysState st, const float* __restrict__ b,
-                                                     float* __restrict__ t) {
-    const int w = slice_of_block(sl);
-    if (w < 0) return;
-    const int4 ma = sl.meta[3 * w], mb = sl.meta[3 * w + 1];
-    if (!cont[ma.x] || structure_done<NR>(st, ma.x)) return;
-    const int wave = threadIdx.x >> 6;
-    const int64_t i = (int64_t)ma.y + (threadIdx.x & 63);
-    const bool on = i < ma.z;
-    const int64_t ii = on ? i : (int64_t)ma.y;
-    auto bb = [&](int64_t node, int c, int h) { return b[(6 * node + c) * NR + h]; };
-    const float th = theta[ma.x];
-    float acc[6][NR];
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
-#pragma unroll
-        for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? bb(ii, c, h) : 0.f;
-    ell_apply<float, float, NR, WPS>(sl, A, w, ma, mb, acc, bb);
-    if (wave != 0 || !on) return;
-#pragma unroll
-    for (int c = 0; c < 6; ++c)
-#pragma unroll
-        for (int h = 0; h < NR; ++h) t[(6 * i + c) * NR + h] = bb(i, c, h) - acc[c][h] / th;
+// Coarse-level smoothing products (degree-2 Chebyshev, centre theta, weight c):
+//   SM_PRE1:  t = b - A b / theta                       (v = b)
+//   SM_PRE2:  r = b - c (theta (b - t) + A t)           (v = t): the residual of x = c (b + t)
+//   SM_POST1: r0 = b - A z1                             (v = z1)
+//   SM_POST2: x = z1 + c (2 r0 - A r0 / theta)          (v = r0, w = z1)
+// written as out = cb b + cv v + ca (A v) + w.
+enum { SM_PRE1 = 0, SM_PRE2 = 1, SM_POST1 = 2, SM_POST2 = 3 };
+struct SmoothCoef {
+    float cb, cv, ca;
+    bool use_b, use_w;
+};
+__device__ __forceinline__ SmoothCoef smooth_coef(int mode, float th, float c) {
+    switch (mode) {
+        case SM_PRE1: return {1.f, 0.f, -1.f / th, true, false};
+        case SM_PRE2: return {1.f - c * th, c * th, -c, true, false};
+        case SM_POST1: return {1.f, 0.f, -1.f, true, false};
+        default: return {0.f, 2.f * c, -c / th, false, true};
+    }
 }
 
-// Coarse level, last smoothing step: x = z1 + (b - A z1) / theta (z1 in t).
 template <int NR, int WPS>
-__global__ __launch_bounds__(64 * WPS) void k_mg_post(Slices sl, const float* __restrict__ A,
-                                                      const float* __restrict__ theta, const uint8_t* __restrict__ cont,
-                                                      SysState st, const float* __restrict__ b,
-                                                      const float* __restrict__ z1, float* __restrict__ x) {
-    const int w = slice_of_block(sl);
-    if (w < 0) return;
-    const int4 ma = sl.meta[3 * w], mb = sl.meta[3 * w + 1];
-    if (!cont[ma.x] || structure_done<NR>(st, ma.x)) return;
+__global__ __launch_bounds__(64 * WPS) void k_mg_smooth(Slices sl, const float* __restrict__ A,
+                                                        const float* __restrict__ theta,
+                                                        const float* __restrict__ cheb,
+                                                        const uint8_t* __restrict__ smooth, SysState st, int mode,
+                                                        const float* __restrict__ b, const float* __restrict__ v,
+                                                        const float* __restrict__ w, float* __restrict__ out) {
+    const int q = slice_of_block(sl);
+    if (q < 0) return;
+    const int4 ma = sl.meta[3 * q], mb = sl.meta[3 * q + 1];
+    if (!smooth[ma.x] || structure_done<NR>(st, ma.x)) return;
     const int wave = threadIdx.x >> 6;
     const int64_t i = (int64_t)ma.y + (threadIdx.x & 63);
     const bool on = i < ma.z;
     const int64_t ii = on ? i : (int64_t)ma.y;
-    auto zz = [&](int64_t node, int c, int h) { return z1[(6 * node + c) * NR + h]; };
-    const float th = theta[ma.x];
+    auto vv = [&](int64_t node, int c, int h) { return v[(6 * node + c) * NR + h]; };
+    const SmoothCoef k = smooth_coef(mode, theta[ma.x], cheb[ma.x]);
     float acc[6][NR];
 #pragma unroll
     for (int c = 0; c < 6; ++c)
 #pragma unroll
-        for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? zz(ii, c, h) : 0.f;
-    ell_apply<float, float, NR, WPS>(sl, A, w, ma, mb, acc, zz);
+        for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? vv(ii, c, h) : 0.f;
+    ell_apply<float, float, NR, WPS>(sl, A, q, ma, mb, acc, vv);
     if (wave != 0 || !on) return;
 #pragma unroll
     for (int c = 0; c < 6; ++c)
 #pragma unroll
         for (int h = 0; h < NR; ++h) {
             const int64_t o = (6 * i + c) * NR + h;
-            x[o] = zz(i, c, h) + (b[o] - acc[c][h]) / th;
+            float y = k.cv * vv(i, c, h) + k.ca * acc[c][h];
+            if (k.use_b) y += k.cb * b[o];
+            if (k.use_w) y += w[o];
+            out[o] = y;
         }
 }
 
@@ -1397,24 +1411,33 @@ __global__ __launch_bounds__(256) void k_mg_restrict(int32_t nc, const int32_t* 
     }
 }
 
-// Prolongation and the smoothing step before it: z1 = b / theta + P x_c (P in
-// FP16), thread per fine DOF row (i, r) of the structures that have a coarser level
+// Prolongation and the pre-smoothing result before it, P in FP16, thread per
+// fine DOF row (i, r) of the structures that have a coarser level:
+//   level 0 (cheb null):  z1 = b / theta + P x_c
+//   coarse levels:        z1 = c (b + t) + P x_c   (t = b - A b / theta from SM_PRE1)
 template <typename VT, int NR>
 __global__ __launch_bounds__(256) void k_mg_prolong(int32_t n, const int32_t* __restrict__ node_sys,
                                                     const uint8_t* __restrict__ cont, const int32_t* __restrict__ p_ptr,
                                                     const int32_t* __restrict__ p_col, const _Float16* __restrict__ P,
-                                                    const float* __restrict__ theta, SysState st,
-                                                    const VT* __restrict__ b, const float* __restrict__ xc,
+                                                    const float* __restrict__ theta, const float* __restrict__ cheb,
+                                                    SysState st, const VT* __restrict__ b,
+                                                    const float* __restrict__ t, const float* __restrict__ xc,
                                                     VT* __restrict__ z1) {
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= 6 * (int64_t)n) return;
     const int i = (int)(u / 6), r = (int)(u % 6);
     const int s = node_sys[i];
     if (!cont[s] || structure_done<NR>(st, s)) return;
-    const float th = theta[s];
     float acc[NR];
+    if (cheb) {
+        const float c = cheb[s];
 #pragma unroll
-    for (int h = 0; h < NR; ++h) acc[h] = (float)b[u * NR + h] / th;
+        for (int h = 0; h < NR; ++h) acc[h] = c * ((float)b[u * NR + h] + t[u * NR + h]);
+    } else {
+        const float th = theta[s];
+#pragma unroll
+        for (int h = 0; h < NR; ++h) acc[h] = (float)b[u * NR + h] / th;
+    }
     for (int q = p_ptr[i]; q < p_ptr[i + 1]; ++q) {
         const int64_t J = p_col[q];
         const _Float16* pb = P + (int64_t)q * 36 + r * 6;
@@ -1476,13 +1499,15 @@ __global__ __launch_bounds__(256) void k_mg_direct(const int32_t* __restrict__ l
 // A structure's last sparse level l and its dense level l + 1 in one kernel
 // (workgroup per structure; every step ends at a barrier, and the level-l
 // vectors, a few KB per structure, pass between the steps through global
-// memory inside the workgroup):
-//   t = b - A b / theta          (TG slices at a time, TW waves per slice)
-//   b_c = P^T t                  (four lanes per coarse row, into LDS)
+// memory inside the workgroup), with the coarse levels' degree-2 Chebyshev
+// smoothing (k_mg_smooth's modes; TG slices at a time, TW waves per slice):
+//   t = b - A b / theta,  r = b - c (theta (b - t) + A t)    (pre-smoothing)
+//   b_c = P^T r                  (four lanes per coarse row, into LDS)
 //   x_c = A_c^-1 b_c             (dense inverse, columns split over the workgroup)
-//   z1 = b / theta + P x_c       (into t)
-//   x = z1 + (b - A z1) / theta  (the level's result, read by the prolongation above)
-// It replaces five launches per iteration on a small, latency-bound level.
+//   z1 = c (b + t) + P x_c       (into r)
+//   r0 = b - A z1 (into t),  x = z1 + c (2 r0 - A r0 / theta)  (post-smoothing;
+//   x is the level's result, read by the prolongation above)
+// It replaces seven launches per iteration on a small, latency-bound level.
 template <int NR>
 struct TailShape {
     static constexpr int TW = NR <= 2 ? 4 : (NR <= 4 ? 2 : 1);  // waves per slice
@@ -1494,9 +1519,10 @@ struct TailShape {
 static_assert(TailShape<1>::ROWS >= AMG_DENSE_MAX, "dense rows exceed the tail kernel's row span");
 
 template <int NR>
-__device__ __forceinline__ void tail_smooth(const Slices& sl, const float* __restrict__ A, int q0, int q1, float th,
-                                            const float* __restrict__ src, const float* __restrict__ b,
-                                            float* __restrict__ dst, bool post) {
+__device__ __forceinline__ void tail_smooth(const Slices& sl, const float* __restrict__ A, int q0, int q1,
+                                            const SmoothCoef& k, const float* __restrict__ b,
+                                            const float* __restrict__ src, const float* __restrict__ w,
+                                            float* __restrict__ dst) {
     using S = TailShape<NR>;
     const int lane = threadIdx.x & 63, grp = (threadIdx.x >> 6) / S::TW, sub = (threadIdx.x >> 6) % S::TW;
     auto xs = [&](int64_t node, int c, int h) { return src[(6 * node + c) * NR + h]; };
@@ -1521,7 +1547,10 @@ __device__ __forceinline__ void tail_smooth(const Slices& sl, const float* __res
 #pragma unroll
                 for (int h = 0; h < NR; ++h) {
                     const int64_t o = (6 * i + c) * NR + h;
-                    dst[o] = post ? src[o] + (b[o] - acc[c][h]) / th : b[o] - acc[c][h] / th;
+                    float y = k.cv * src[o] + k.ca * acc[c][h];
+                    if (k.use_b) y += k.cb * b[o];
+                    if (k.use_w) y += w[o];
+                    dst[o] = y;
                 }
         }
         __syncthreads();  // the cross-wave buffer is reused by the next round
@@ -1531,10 +1560,10 @@ __device__ __forceinline__ void tail_smooth(const Slices& sl, const float* __res
 template <int NR>
 __global__ __launch_bounds__(TailShape<NR>::THREADS) void k_mg_tail(
     const int32_t* __restrict__ list, Slices sl, const float* __restrict__ A, const float* __restrict__ theta,
-    const int32_t* __restrict__ node_off, const int32_t* __restrict__ pt_ptr, const int32_t* __restrict__ pt_row,
+    const float* __restrict__ cheb, const int32_t* __restrict__ node_off, const int32_t* __restrict__ pt_ptr, const int32_t* __restrict__ pt_row,
     const _Float16* __restrict__ PT, const int32_t* __restrict__ p_ptr, const int32_t* __restrict__ p_col,
     const _Float16* __restrict__ P, const int32_t* __restrict__ term, const float* __restrict__ inv, SysState st,
-    const float* __restrict__ b, float* __restrict__ t, float* __restrict__ x) {
+    const float* __restrict__ b, float* __restrict__ t, float* __restrict__ r, float* __restrict__ x) {
     using S = TailShape<NR>;
     __shared__ float bc[AMG_DENSE_MAX * NR], xc[AMG_DENSE_MAX * NR];
     __shared__ float part[S::CHUNKS][S::ROWS * NR];
@@ -1542,14 +1571,15 @@ __global__ __launch_bounds__(TailShape<NR>::THREADS) void k_mg_tail(
     if (structure_done<NR>(st, s)) return;
     const int tid = threadIdx.x;
     const int q0 = sl.sys_off[s], q1 = sl.sys_off[s + 1];
-    const float th = theta[s];
+    const float th = theta[s], ch = cheb[s];
     const int64_t n0 = node_off[s], n1 = node_off[s + 1];
     const int64_t c0 = term[4 * s + 1];
     const int N = 6 * term[4 * s + 2];
     const float* Ai = inv + term[4 * s + 3];
-    // 1. pre-smoothing into t
-    tail_smooth<NR>(sl, A, q0, q1, th, b, b, t, false);
-    // 2. restriction into LDS: lanes 4u..4u+3 take every fourth block of coarse row u
+    // 1. pre-smoothing: t, then the residual r of x = c (b + t)
+    tail_smooth<NR>(sl, A, q0, q1, smooth_coef(SM_PRE1, th, ch), b, b, nullptr, t);
+    tail_smooth<NR>(sl, A, q0, q1, smooth_coef(SM_PRE2, th, ch), b, t, nullptr, r);
+    // 2. restriction of r into LDS: lanes 4u..4u+3 take every fourth block of coarse row u
     for (int u0 = 0; u0 < 4 * N; u0 += S::THREADS) {
         const int v = u0 + tid;
         const bool valid = v < 4 * N;
@@ -1567,11 +1597,11 @@ __global__ __launch_bounds__(TailShape<NR>::THREADS) void k_mg_tail(
                 const _Float16* pr = PT + (int64_t)q * 36 + c * 6;
                 float pv[6];
 #pragma unroll
-                for (int r = 0; r < 6; ++r) pv[r] = (float)pr[r];
+                for (int m = 0; m < 6; ++m) pv[m] = (float)pr[m];
 #pragma unroll
-                for (int r = 0; r < 6; ++r)
+                for (int m = 0; m < 6; ++m)
 #pragma unroll
-                    for (int h = 0; h < NR; ++h) acc[h] += pv[r] * t[(6 * i + r) * NR + h];
+                    for (int h = 0; h < NR; ++h) acc[h] += pv[m] * r[(6 * i + m) * NR + h];
             }
         }
 #pragma unroll
@@ -1616,28 +1646,29 @@ __global__ __launch_bounds__(TailShape<NR>::THREADS) void k_mg_tail(
         xc[u] = v;
     }
     __syncthreads();
-    // 4. prolongation with the smoothing step before it, into t
+    // 4. prolongation with the pre-smoothing result, z1 = c (b + t) + P x_c, into r
     for (int64_t u = 6 * n0 + tid; u < 6 * n1; u += S::THREADS) {
         const int64_t i = u / 6;
-        const int r = (int)(u % 6);
+        const int row = (int)(u % 6);
         float acc[NR];
 #pragma unroll
-        for (int h = 0; h < NR; ++h) acc[h] = b[u * NR + h] / th;
+        for (int h = 0; h < NR; ++h) acc[h] = ch * (b[u * NR + h] + t[u * NR + h]);
 #pragma unroll 4
         for (int q = p_ptr[i]; q < p_ptr[i + 1]; ++q) {
             const int J = p_col[q] - (int)c0;
-            const _Float16* pb = P + (int64_t)q * 36 + r * 6;
+            const _Float16* pb = P + (int64_t)q * 36 + row * 6;
 #pragma unroll
             for (int c = 0; c < 6; ++c)
 #pragma unroll
                 for (int h = 0; h < NR; ++h) acc[h] += (float)pb[c] * xc[(6 * J + c) * NR + h];
         }
 #pragma unroll
-        for (int h = 0; h < NR; ++h) t[u * NR + h] = acc[h];
+        for (int h = 0; h < NR; ++h) r[u * NR + h] = acc[h];
     }
     __syncthreads();
-    // 5. post-smoothing into x
-    tail_smooth<NR>(sl, A, q0, q1, th, t, b, x, true);
+    // 5. post-smoothing: r0 = b - A z1 into t, then x
+    tail_smooth<NR>(sl, A, q0, q1, smooth_coef(SM_POST1, th, ch), b, r, nullptr, t);
+    tail_smooth<NR>(sl, A, q0, q1, smooth_coef(SM_POST2, th, ch), b, t, r, x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1752,6 +1783,7 @@ static int amg_build_device(Plan* pl, AmgState& A) {
         AMG_TRY(amg_alloc(A, &D.d_rho, (size_t)ns));
         AMG_TRY(amg_alloc(A, &D.d_omega, (size_t)ns));
         AMG_TRY(amg_alloc(A, &D.d_theta, (size_t)ns));
+        AMG_TRY(amg_alloc(A, &D.d_cheb, (size_t)ns));
         AMG_TRY(amg_alloc(A, &D.d_pv[0], 6 * (size_t)H.n));
         AMG_TRY(amg_alloc(A, &D.d_pv[1], 6 * (size_t)H.n));
         AMG_TRY(amg_alloc(A, &D.d_ppart, 2 * (size_t)D.n_wg));
@@ -1854,7 +1886,7 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
                                (const uint8_t*)D.d_cont, D.n_wg);
         hipLaunchKernelGGL(k_amg_theta, dim3(nblk(ns, 64)), dim3(64), 0, s, ns, sys_wg_off,
                            (const double*)D.d_ppart, AMG_POWER_STEPS - 1, (const uint8_t*)D.d_cont, D.n_wg, D.d_rho,
-                           D.d_omega, D.d_theta);
+                           D.d_omega, D.d_theta, l == 0 ? nullptr : D.d_cheb);
         if (D.nP > 0) {
             hipLaunchKernelGGL(k_amg_psmooth, dim3(nblk(D.nP * 6, 256)), dim3(256), 0, s, D.nP, D.d_p_row,
                                D.d_p_cptr, D.d_p_cj, D.d_p_caref, D.d_node_sys, (const double*)D.d_omega, A64,
@@ -1909,7 +1941,7 @@ static int amg_vectors(Plan* pl, AmgState& A, int nrhs) {
     if (A.vec_nrhs < nrhs) {
         for (int l = 1; l < A.nl; ++l) {
             AmgLevelDev& D = A.dev[l];
-            for (float** v : {&D.d_b, &D.d_t, &D.d_x}) {
+            for (float** v : {&D.d_b, &D.d_t, &D.d_r, &D.d_x}) {
                 if (*v) JSSO_HIP(hipFree(*v));
                 *v = nullptr;
                 const size_t bytes = 6 * (size_t)std::max(D.n, 1) * nrhs * sizeof(float);
@@ -1956,24 +1988,31 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
             else
                 hipLaunchKernelGGL((k_mg_restrict<float, NR>), gr, dim3(256), 0, s, C.n, C.d_node_sys,
                                    (const uint8_t*)A.d_rskip[l], D.d_pt_ptr, D.d_pt_row, (const _Float16*)D.d_PT16,
-                                   w.st, (const float*)D.d_t, C.d_b);
+                                   w.st, (const float*)D.d_r, C.d_b);
         }
         if (l + 2 < nl && A.tail_list[l + 1].size() < (size_t)std::count(A.host[l + 1].cont.begin(),
                                                                           A.host[l + 1].cont.end(), 1))
             wps_dispatch<float, NR>(4, [&](auto W) {
                 constexpr int WP = decltype(W)::value;
-                hipLaunchKernelGGL((k_mg_pre<NR, WP>), slice_grid(C.n_wg), dim3(64 * WP), 0, s,
-                                   level_slices(pl, A, l + 1), (const float*)C.d_A32, (const float*)C.d_theta,
-                                   (const uint8_t*)A.d_smooth[l + 1], w.st, (const float*)C.d_b, C.d_t);
+                const Slices cs = level_slices(pl, A, l + 1);
+                // t = b - A b / theta, then r = b - c (theta (b - t) + A t)
+                hipLaunchKernelGGL((k_mg_smooth<NR, WP>), slice_grid(C.n_wg), dim3(64 * WP), 0, s, cs,
+                                   (const float*)C.d_A32, (const float*)C.d_theta, (const float*)C.d_cheb,
+                                   (const uint8_t*)A.d_smooth[l + 1], w.st, (int)SM_PRE1, (const float*)C.d_b,
+                                   (const float*)C.d_b, (const float*)nullptr, C.d_t);
+                hipLaunchKernelGGL((k_mg_smooth<NR, WP>), slice_grid(C.n_wg), dim3(64 * WP), 0, s, cs,
+                                   (const float*)C.d_A32, (const float*)C.d_theta, (const float*)C.d_cheb,
+                                   (const uint8_t*)A.d_smooth[l + 1], w.st, (int)SM_PRE2, (const float*)C.d_b,
+                                   (const float*)C.d_t, (const float*)nullptr, C.d_r);
             });
         if (!A.tail_list[l + 1].empty()) {
             const AmgLevelDev& T = A.dev[l + 1];
             hipLaunchKernelGGL((k_mg_tail<NR>), dim3((unsigned)A.tail_list[l + 1].size()),
                                dim3(TailShape<NR>::THREADS), 0, s,
                                A.d_tail_list[l + 1], level_slices(pl, A, l + 1), (const float*)T.d_A32,
-                               (const float*)T.d_theta, A.d_node_off[l + 1], T.d_pt_ptr, T.d_pt_row,
-                               (const _Float16*)T.d_PT16, T.d_p_ptr, T.d_p_col, (const _Float16*)T.d_P16, A.d_term,
-                               (const float*)A.d_inv, w.st, (const float*)T.d_b, T.d_t, T.d_x);
+                               (const float*)T.d_theta, (const float*)T.d_cheb, A.d_node_off[l + 1], T.d_pt_ptr,
+                               T.d_pt_row, (const _Float16*)T.d_PT16, T.d_p_ptr, T.d_p_col, (const _Float16*)T.d_P16,
+                               A.d_term, (const float*)A.d_inv, w.st, (const float*)T.d_b, T.d_t, T.d_r, T.d_x);
         }
     }
     for (int l = 1; l < nl; ++l) {
@@ -1985,10 +2024,12 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
     for (int l = nl - 2; l >= 0; --l) {
         const AmgLevelDev& D = A.dev[l];
         const AmgLevelDev& C = A.dev[l + 1];
+        if (C.n == 0) continue;
         if (l == 0) {
             hipLaunchKernelGGL((k_mg_prolong<VT, NR>), dim3(nblk(6 * (int64_t)D.n, 256)), dim3(256), 0, s, D.n,
                                D.d_node_sys, (const uint8_t*)A.d_smooth[0], D.d_p_ptr, D.d_p_col, (const _Float16*)D.d_P16,
-                               (const float*)D.d_theta, w.st, (const VT*)bf.r[k & 1], (const float*)C.d_x, bf.t0);
+                               (const float*)D.d_theta, (const float*)nullptr, w.st, (const VT*)bf.r[k & 1],
+                               (const float*)nullptr, (const float*)C.d_x, bf.t0);
             wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
                 constexpr int WP = decltype(W)::value;
                 hipLaunchKernelGGL((k_pcg_post<VT, NR, WP>), slice_grid(pl->n_wg), dim3(64 * WP), 0, s,
@@ -1997,16 +2038,22 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
                                    (const VT*)bf.r[k & 1], (const VT*)bf.r[(k - 1) & 1], (const VT*)bf.t0, bf.z);
             });
         } else if (A.tail_list[l].size() < (size_t)std::count(A.host[l].cont.begin(), A.host[l].cont.end(), 1)) {
+            // z1 = c (b + t) + P x_c into r, r0 = b - A z1 into t, x = z1 + c (2 r0 - A r0 / theta)
             hipLaunchKernelGGL((k_mg_prolong<float, NR>), dim3(nblk(6 * (int64_t)D.n, 256)), dim3(256), 0, s, D.n,
                                D.d_node_sys, (const uint8_t*)A.d_smooth[l], D.d_p_ptr, D.d_p_col,
-                               (const _Float16*)D.d_P16, (const float*)D.d_theta, w.st, (const float*)D.d_b,
-                               (const float*)C.d_x, D.d_t);
+                               (const _Float16*)D.d_P16, (const float*)D.d_theta, (const float*)D.d_cheb, w.st,
+                               (const float*)D.d_b, (const float*)D.d_t, (const float*)C.d_x, D.d_r);
             wps_dispatch<float, NR>(4, [&](auto W) {
                 constexpr int WP = decltype(W)::value;
-                hipLaunchKernelGGL((k_mg_post<NR, WP>), slice_grid(D.n_wg), dim3(64 * WP), 0, s,
-                                   level_slices(pl, A, l), (const float*)D.d_A32, (const float*)D.d_theta,
-                                   (const uint8_t*)A.d_smooth[l], w.st, (const float*)D.d_b, (const float*)D.d_t,
-                                   D.d_x);
+                const Slices fs = level_slices(pl, A, l);
+                hipLaunchKernelGGL((k_mg_smooth<NR, WP>), slice_grid(D.n_wg), dim3(64 * WP), 0, s, fs,
+                                   (const float*)D.d_A32, (const float*)D.d_theta, (const float*)D.d_cheb,
+                                   (const uint8_t*)A.d_smooth[l], w.st, (int)SM_POST1, (const float*)D.d_b,
+                                   (const float*)D.d_r, (const float*)nullptr, D.d_t);
+                hipLaunchKernelGGL((k_mg_smooth<NR, WP>), slice_grid(D.n_wg), dim3(64 * WP), 0, s, fs,
+                                   (const float*)D.d_A32, (const float*)D.d_theta, (const float*)D.d_cheb,
+                                   (const uint8_t*)A.d_smooth[l], w.st, (int)SM_POST2, (const float*)nullptr,
+                                   (const float*)D.d_t, (const float*)D.d_r, D.d_x);
             });
         }
     }
@@ -2282,7 +2329,7 @@ void amg_destroy(Plan* pl) {
     for (void* v : {A->d_z, A->d_t0, A->d_p1})
         if (v) (void)hipFree(v);
     for (auto& D : A->dev)
-        for (float* v : {D.d_b, D.d_t, D.d_x})
+        for (float* v : {D.d_b, D.d_t, D.d_r, D.d_x})
             if (v) (void)hipFree(v);
     delete A;
     pl->amg = nullptr;
