@@ -590,12 +590,13 @@ __device__ __forceinline__ double amg_hash(int64_t dof) {
 }
 
 // One power-iteration step y = A v on the slices of a level (lane = block row;
-// the FP32 copy of the matrix: only the largest eigenvalue is wanted): step 0
+// level 0 reads the FP16 copy that its smoother applies, coarse levels the FP32
+// one: only the largest eigenvalue is wanted): step 0
 // takes the hashed start vector, later steps v = y_prev / |y_prev| of their
 // structure (from the per-slice partials of the previous step).  Writes y and
 // this slice's |y|^2.  Structures without a coarser level are skipped.
-template <int WPS>
-__global__ __launch_bounds__(64 * WPS) void k_amg_power(Slices sl, const float* __restrict__ A, int step,
+template <typename MT, int WPS>
+__global__ __launch_bounds__(64 * WPS) void k_amg_power(Slices sl, const MT* __restrict__ A, int step,
                                                         const double* __restrict__ vin, double* __restrict__ vout,
                                                         double* __restrict__ part, const uint8_t* __restrict__ cont,
                                                         int32_t n_wg) {
@@ -619,7 +620,7 @@ __global__ __launch_bounds__(64 * WPS) void k_amg_power(Slices sl, const float* 
     double acc[6][1];
 #pragma unroll
     for (int c = 0; c < 6; ++c) acc[c][0] = wave == 0 ? v(ii, c, 0) : 0.0;
-    ell_apply<float, double, 1, WPS>(sl, A, w, ma, mb, acc, v);
+    ell_apply<MT, double, 1, WPS>(sl, A, w, ma, mb, acc, v);
     if (wave != 0) return;
     double yy = 0.0;
     if (on) {
@@ -1879,11 +1880,16 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
             hipLaunchKernelGGL(k_amg_qr, dim3((unsigned)D.n_agg), dim3(64), 0, s, D.n_agg, D.d_agg_ptr,
                                D.d_agg_nodes, (const double*)D.d_B, l == 0 ? (const double*)pl->d_crds : nullptr,
                                D.d_node_sys, A.d_node_off0, D.d_Q, D.d_R);
-        const float* A32 = l == 0 ? pl->d_Ke32 : D.d_A32;
-        for (int step = 0; step < AMG_POWER_STEPS; ++step)
-            hipLaunchKernelGGL(k_amg_power<4>, slice_grid(D.n_wg), dim3(256), 0, s, sl, A32, step,
-                               (const double*)D.d_pv[(step + 1) & 1], D.d_pv[step & 1], D.d_ppart,
-                               (const uint8_t*)D.d_cont, D.n_wg);
+        for (int step = 0; step < AMG_POWER_STEPS; ++step) {
+            if (l == 0)
+                hipLaunchKernelGGL((k_amg_power<_Float16, 4>), slice_grid(D.n_wg), dim3(256), 0, s, sl,
+                                   (const _Float16*)pl->d_Ke16, step, (const double*)D.d_pv[(step + 1) & 1],
+                                   D.d_pv[step & 1], D.d_ppart, (const uint8_t*)D.d_cont, D.n_wg);
+            else
+                hipLaunchKernelGGL((k_amg_power<float, 4>), slice_grid(D.n_wg), dim3(256), 0, s, sl,
+                                   (const float*)D.d_A32, step, (const double*)D.d_pv[(step + 1) & 1],
+                                   D.d_pv[step & 1], D.d_ppart, (const uint8_t*)D.d_cont, D.n_wg);
+        }
         hipLaunchKernelGGL(k_amg_theta, dim3(nblk(ns, 64)), dim3(64), 0, s, ns, sys_wg_off,
                            (const double*)D.d_ppart, AMG_POWER_STEPS - 1, (const uint8_t*)D.d_cont, D.n_wg, D.d_rho,
                            D.d_omega, D.d_theta, l == 0 ? nullptr : D.d_cheb);
