@@ -132,6 +132,7 @@ struct AmgState {
     std::vector<int32_t*> d_tail_list, d_direct_list, d_node_off;
     std::vector<uint8_t*> d_smooth, d_rskip;
     float* d_inv = nullptr;           // dense inverses, column-major, FP32
+    double* d_Anat = nullptr;         // set-up scratch: a level's FP64 blocks in natural layout [position][36]
     int64_t inv_total = 0;
     int vec_nrhs = 0;                 // V-cycle vectors sized for this many load cases
     int wps0 = 4;                     // waves per slice of the level-0 kernels
@@ -427,6 +428,26 @@ __device__ __forceinline__ double ablk(const double* __restrict__ A, int32_t are
     return (aref & 1) ? ell64(A, vp, c * 6 + r) : ell64(A, vp, r * 6 + c);
 }
 
+// The same entry from the natural-layout copy (k_amg_unpack): a block row is 48
+// consecutive bytes there, three 16-byte pieces 1 KB apart in the ELL.
+__device__ __forceinline__ double ablk_nat(const double* __restrict__ An, int32_t aref, int r, int c) {
+    if (aref < 0) return r == c ? 1.0 : 0.0;
+    const int64_t vp = aref >> 1;
+    return (aref & 1) ? An[vp * 36 + c * 6 + r] : An[vp * 36 + r * 6 + c];
+}
+
+// FP64 ELL [slot][18][64 positions][2] -> [position][36], thread per (slot, pair
+// group, position): reads cover consecutive addresses, each position's 18 writes
+// fill its 288 bytes.
+__global__ __launch_bounds__(256) void k_amg_unpack(int64_t npos, const double2* __restrict__ ell,
+                                                    double2* __restrict__ nat) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npos * 18) return;
+    const int64_t lane = t & (ROWS_PER_WG - 1), sg = t / ROWS_PER_WG;
+    const int64_t slot = sg / 18, g = sg % 18;
+    nat[(slot * ROWS_PER_WG + lane) * 18 + g] = ell[t];
+}
+
 // Rigid-body modes of every aggregated fine node relative to its aggregate's
 // root, constrained rows zeroed, in the scaled variables: B_i = L_i^T M_i with
 // L_i^T = (L_i^-1)^T solved for (upper triangular).  Columns: translations x, y,
@@ -686,7 +707,7 @@ __global__ __launch_bounds__(256) void k_amg_psmooth(int64_t nP, const int32_t* 
         const int32_t aref = p_caref[q];
         double a[6];
 #pragma unroll
-        for (int m = 0; m < 6; ++m) a[m] = aref < 0 ? (m == r ? 1.0 - om : 0.0) : -om * ablk(A, aref, r, m);
+        for (int m = 0; m < 6; ++m) a[m] = aref < 0 ? (m == r ? 1.0 - om : 0.0) : -om * ablk_nat(A, aref, r, m);
 #pragma unroll
         for (int m = 0; m < 6; ++m)
 #pragma unroll
@@ -711,7 +732,7 @@ __global__ __launch_bounds__(256) void k_amg_w(int64_t nW, const int32_t* __rest
         const int32_t aref = w_caref[q];
         double a[6];
 #pragma unroll
-        for (int m = 0; m < 6; ++m) a[m] = ablk(A, aref, r, m);
+        for (int m = 0; m < 6; ++m) a[m] = ablk_nat(A, aref, r, m);
 #pragma unroll
         for (int m = 0; m < 6; ++m)
 #pragma unroll
@@ -1811,6 +1832,9 @@ static int amg_build_device(Plan* pl, AmgState& A) {
     A.d_term_list.assign(nl, nullptr);
     for (int l = 0; l < nl; ++l) AMG_TRY(amg_upload(A, &A.d_term_list[l], A.term_list[l]));
     AMG_TRY(amg_alloc(A, &A.d_inv, (size_t)off));
+    int64_t max_pos = pl->ell_slots;
+    for (int l = 1; l < nl; ++l) max_pos = std::max<int64_t>(max_pos, A.host[l].ell.ell_slots);
+    AMG_TRY(amg_alloc(A, &A.d_Anat, (size_t)max_pos * 36));
     // fused last sparse levels (JSSO_AMG_FUSE=0 keeps every level on the per-level kernels)
     const char* fe = getenv("JSSO_AMG_FUSE");
     const bool fuse = !(fe && fe[0] == '0');
@@ -1894,11 +1918,14 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
                            (const double*)D.d_ppart, AMG_POWER_STEPS - 1, (const uint8_t*)D.d_cont, D.n_wg, D.d_rho,
                            D.d_omega, D.d_theta, l == 0 ? nullptr : D.d_cheb);
         if (D.nP > 0) {
+            const int64_t npos = l == 0 ? pl->ell_slots : D.ell_slots;
+            hipLaunchKernelGGL(k_amg_unpack, dim3(nblk(npos * 18, 256)), dim3(256), 0, s, npos, (const double2*)A64,
+                               (double2*)A.d_Anat);
             hipLaunchKernelGGL(k_amg_psmooth, dim3(nblk(D.nP * 6, 256)), dim3(256), 0, s, D.nP, D.d_p_row,
-                               D.d_p_cptr, D.d_p_cj, D.d_p_caref, D.d_node_sys, (const double*)D.d_omega, A64,
-                               (const double*)D.d_Q, D.d_P64);
+                               D.d_p_cptr, D.d_p_cj, D.d_p_caref, D.d_node_sys, (const double*)D.d_omega,
+                               (const double*)A.d_Anat, (const double*)D.d_Q, D.d_P64);
             hipLaunchKernelGGL(k_amg_w, dim3(nblk(D.nW * 6, 256)), dim3(256), 0, s, D.nW, D.d_w_cptr, D.d_w_caref,
-                               D.d_w_cp, A64, (const double*)D.d_P64, D.d_W64);
+                               D.d_w_cp, (const double*)A.d_Anat, (const double*)D.d_P64, D.d_W64);
         }
         const int64_t nnzc = A.host[l + 1].colind.size();
         if (nnzc > 0) {
