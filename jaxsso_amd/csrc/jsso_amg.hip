@@ -130,6 +130,10 @@ struct AmgState {
     // the fused kernel)
     std::vector<std::vector<int32_t>> tail_list, direct_list;
     std::vector<int32_t*> d_tail_list, d_direct_list, d_node_off;
+    // per right-hand-side count (TAIL_NR) and level: the fused structures whose
+    // vectors fit in LDS [0] and the others [1]
+    std::vector<int32_t> tail_n[5][2];
+    std::vector<int32_t*> d_tail_split[5][2];
     std::vector<uint8_t*> d_smooth, d_rskip;
     float* d_inv = nullptr;           // dense inverses, column-major, FP32
     double* d_Anat = nullptr;         // set-up scratch: a level's FP64 blocks in natural layout [position][36]
@@ -1537,17 +1541,25 @@ struct TailShape {
     static constexpr int THREADS = 64 * TW * TG;
     static constexpr int ROWS = 160;                             // >= AMG_DENSE_MAX
     static constexpr int CHUNKS = THREADS / ROWS;                // column chunks of the dense product
+    // entries of the inverse each thread loads at the start (its whole column
+    // chunk of its row) when that fits the registers
+    static constexpr int PREFETCH = CHUNKS >= 6 ? (AMG_DENSE_MAX + CHUNKS - 1) / CHUNKS : 0;
+    // structures of at most VEC_NODES nodes keep the level's four vectors in LDS
+    // (48 KB); larger ones pass them through global memory
+    static constexpr int VEC_NODES = 512 / NR;
 };
+constexpr int TAIL_NR[5] = {1, 2, 3, 4, 8};   // right-hand-side counts of the solver
 static_assert(TailShape<1>::ROWS >= AMG_DENSE_MAX, "dense rows exceed the tail kernel's row span");
 
+// (vectors indexed from node nb: the structure's first node when they live in LDS, 0 in global memory)
 template <int NR>
 __device__ __forceinline__ void tail_smooth(const Slices& sl, const float* __restrict__ A, int q0, int q1,
-                                            const SmoothCoef& k, const float* __restrict__ b,
+                                            const SmoothCoef& k, int64_t nb, const float* __restrict__ b,
                                             const float* __restrict__ src, const float* __restrict__ w,
                                             float* __restrict__ dst) {
     using S = TailShape<NR>;
     const int lane = threadIdx.x & 63, grp = (threadIdx.x >> 6) / S::TW, sub = (threadIdx.x >> 6) % S::TW;
-    auto xs = [&](int64_t node, int c, int h) { return src[(6 * node + c) * NR + h]; };
+    auto xs = [&](int64_t node, int c, int h) { return src[(6 * (node - nb) + c) * NR + h]; };
     for (int base = q0; base < q1; base += S::TG) {
         // every group runs a slice (groups past the end repeat the last one) so that
         // all waves meet the same barriers; only owners store
@@ -1568,7 +1580,7 @@ __device__ __forceinline__ void tail_smooth(const Slices& sl, const float* __res
             for (int c = 0; c < 6; ++c)
 #pragma unroll
                 for (int h = 0; h < NR; ++h) {
-                    const int64_t o = (6 * i + c) * NR + h;
+                    const int64_t o = (6 * (i - nb) + c) * NR + h;
                     float y = k.cv * src[o] + k.ca * acc[c][h];
                     if (k.use_b) y += k.cb * b[o];
                     if (k.use_w) y += w[o];
@@ -1579,28 +1591,49 @@ __device__ __forceinline__ void tail_smooth(const Slices& sl, const float* __res
     }
 }
 
-template <int NR>
+template <int NR, bool LV>
 __global__ __launch_bounds__(TailShape<NR>::THREADS) void k_mg_tail(
     const int32_t* __restrict__ list, Slices sl, const float* __restrict__ A, const float* __restrict__ theta,
     const float* __restrict__ cheb, const int32_t* __restrict__ node_off, const int32_t* __restrict__ pt_ptr, const int32_t* __restrict__ pt_row,
     const _Float16* __restrict__ PT, const int32_t* __restrict__ p_ptr, const int32_t* __restrict__ p_col,
     const _Float16* __restrict__ P, const int32_t* __restrict__ term, const float* __restrict__ inv, SysState st,
-    const float* __restrict__ b, float* __restrict__ t, float* __restrict__ r, float* __restrict__ x) {
+    const float* __restrict__ bg, float* __restrict__ tg, float* __restrict__ rg, float* __restrict__ xg) {
     using S = TailShape<NR>;
     __shared__ float bc[AMG_DENSE_MAX * NR], xc[AMG_DENSE_MAX * NR];
     __shared__ float part[S::CHUNKS][S::ROWS * NR];
+    __shared__ float vec[LV ? 4 : 1][LV ? S::VEC_NODES * 6 * NR : 1];
     const int s = list[blockIdx.x];
     if (structure_done<NR>(st, s)) return;
     const int tid = threadIdx.x;
     const int q0 = sl.sys_off[s], q1 = sl.sys_off[s + 1];
     const float th = theta[s], ch = cheb[s];
     const int64_t n0 = node_off[s], n1 = node_off[s + 1];
+    // the level's vectors b, t, r, x: in LDS from node n0 (LV), else in global memory
+    const int64_t nb = LV ? n0 : 0;
+    const float* b = LV ? vec[0] : bg;
+    float* t = LV ? vec[1] : tg;
+    float* r = LV ? vec[2] : rg;
+    float* x = LV ? vec[3] : xg;
+    if constexpr (LV) {
+        for (int64_t u = tid; u < 6 * (n1 - n0) * NR; u += S::THREADS) vec[0][u] = bg[6 * n0 * NR + u];
+        __syncthreads();
+    }
     const int64_t c0 = term[4 * s + 1];
     const int N = 6 * term[4 * s + 2];
     const float* Ai = inv + term[4 * s + 3];
+    // the dense inverse does not depend on anything computed here: its loads are
+    // issued now and arrive while the smoothing runs
+    const int dr = tid % S::ROWS, dk = tid / S::ROWS;
+    const bool dense_lane = dk < S::CHUNKS && dr < N;
+    float ainv[S::PREFETCH > 0 ? S::PREFETCH : 1];
+#pragma unroll
+    for (int m = 0; m < S::PREFETCH; ++m) {
+        const int c = dk + m * S::CHUNKS;
+        ainv[m] = dense_lane && c < N ? Ai[(int64_t)c * N + dr] : 0.f;
+    }
     // 1. pre-smoothing: t, then the residual r of x = c (b + t)
-    tail_smooth<NR>(sl, A, q0, q1, smooth_coef(SM_PRE1, th, ch), b, b, nullptr, t);
-    tail_smooth<NR>(sl, A, q0, q1, smooth_coef(SM_PRE2, th, ch), b, t, nullptr, r);
+    tail_smooth<NR>(sl, A, q0, q1, smooth_coef(SM_PRE1, th, ch), nb, b, b, nullptr, t);
+    tail_smooth<NR>(sl, A, q0, q1, smooth_coef(SM_PRE2, th, ch), nb, b, t, nullptr, r);
     // 2. restriction of r into LDS: lanes 4u..4u+3 take every fourth block of coarse row u
     for (int u0 = 0; u0 < 4 * N; u0 += S::THREADS) {
         const int v = u0 + tid;
@@ -1623,7 +1656,7 @@ __global__ __launch_bounds__(TailShape<NR>::THREADS) void k_mg_tail(
 #pragma unroll
                 for (int m = 0; m < 6; ++m)
 #pragma unroll
-                    for (int h = 0; h < NR; ++h) acc[h] += pv[m] * r[(6 * i + m) * NR + h];
+                    for (int h = 0; h < NR; ++h) acc[h] += pv[m] * r[(6 * (i - nb) + m) * NR + h];
             }
         }
 #pragma unroll
@@ -1636,7 +1669,24 @@ __global__ __launch_bounds__(TailShape<NR>::THREADS) void k_mg_tail(
     __syncthreads();
     // 3. dense solve: thread (chunk k, row r) sums columns k, k + CHUNKS, ... of
     // row r (the inverse is symmetric: read by columns, coalesced over r)
-    {
+    if constexpr (S::PREFETCH > 0) {
+        if (dense_lane) {
+            float a0[NR], a1[NR];
+#pragma unroll
+            for (int h = 0; h < NR; ++h) a0[h] = a1[h] = 0.f;
+#pragma unroll
+            for (int m = 0; m < S::PREFETCH; m += 2) {
+                const int c0 = dk + m * S::CHUNKS, c1 = c0 + S::CHUNKS;
+#pragma unroll
+                for (int h = 0; h < NR; ++h) {
+                    if (c0 < N) a0[h] += ainv[m] * bc[c0 * NR + h];
+                    if (m + 1 < S::PREFETCH && c1 < N) a1[h] += ainv[m + 1] * bc[c1 * NR + h];
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < NR; ++h) part[dk][dr * NR + h] = a0[h] + a1[h];
+        }
+    } else {
         const int r = tid % S::ROWS, k = tid / S::ROWS;
         if (k < S::CHUNKS && r < N) {
             float a0[NR], a1[NR];
@@ -1673,8 +1723,9 @@ __global__ __launch_bounds__(TailShape<NR>::THREADS) void k_mg_tail(
         const int64_t i = u / 6;
         const int row = (int)(u % 6);
         float acc[NR];
+        const int64_t ul = u - 6 * nb;
 #pragma unroll
-        for (int h = 0; h < NR; ++h) acc[h] = ch * (b[u * NR + h] + t[u * NR + h]);
+        for (int h = 0; h < NR; ++h) acc[h] = ch * (b[ul * NR + h] + t[ul * NR + h]);
 #pragma unroll 4
         for (int q = p_ptr[i]; q < p_ptr[i + 1]; ++q) {
             const int J = p_col[q] - (int)c0;
@@ -1685,12 +1736,15 @@ __global__ __launch_bounds__(TailShape<NR>::THREADS) void k_mg_tail(
                 for (int h = 0; h < NR; ++h) acc[h] += (float)pb[c] * xc[(6 * J + c) * NR + h];
         }
 #pragma unroll
-        for (int h = 0; h < NR; ++h) r[u * NR + h] = acc[h];
+        for (int h = 0; h < NR; ++h) r[ul * NR + h] = acc[h];
     }
     __syncthreads();
     // 5. post-smoothing: r0 = b - A z1 into t, then x
-    tail_smooth<NR>(sl, A, q0, q1, smooth_coef(SM_POST1, th, ch), b, r, nullptr, t);
-    tail_smooth<NR>(sl, A, q0, q1, smooth_coef(SM_POST2, th, ch), b, t, r, x);
+    tail_smooth<NR>(sl, A, q0, q1, smooth_coef(SM_POST1, th, ch), nb, b, r, nullptr, t);
+    tail_smooth<NR>(sl, A, q0, q1, smooth_coef(SM_POST2, th, ch), nb, b, t, r, x);
+    if constexpr (LV) {   // tail_smooth ended at a barrier
+        for (int64_t u = tid; u < 6 * (n1 - n0) * NR; u += S::THREADS) xg[6 * n0 * NR + u] = vec[3][u];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1859,6 +1913,21 @@ static int amg_build_device(Plan* pl, AmgState& A) {
         AMG_TRY(amg_upload(A, &A.d_smooth[l], smooth));
         AMG_TRY(amg_upload(A, &A.d_rskip[l], rskip));
         AMG_TRY(amg_upload(A, &A.d_tail_list[l], A.tail_list[l]));
+        for (int v = 0; v < 5; ++v) {
+            const int cap = 512 / TAIL_NR[v];   // TailShape<NR>::VEC_NODES
+            std::vector<int32_t> fit, big;
+            for (int32_t st : A.tail_list[l])
+                (A.host[l].node_off[st + 1] - A.host[l].node_off[st] <= cap ? fit : big).push_back(st);
+            if (l == 0)
+                for (int f = 0; f < 2; ++f) {
+                    A.tail_n[v][f].assign(nl, 0);
+                    A.d_tail_split[v][f].assign(nl, nullptr);
+                }
+            A.tail_n[v][0][l] = (int32_t)fit.size();
+            A.tail_n[v][1][l] = (int32_t)big.size();
+            AMG_TRY(amg_upload(A, &A.d_tail_split[v][0][l], fit));
+            AMG_TRY(amg_upload(A, &A.d_tail_split[v][1][l], big));
+        }
         AMG_TRY(amg_upload(A, &A.d_direct_list[l], A.direct_list[l]));
         AMG_TRY(amg_upload(A, &A.d_node_off[l], A.host[l].node_off));
     }
@@ -2040,12 +2109,21 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
             });
         if (!A.tail_list[l + 1].empty()) {
             const AmgLevelDev& T = A.dev[l + 1];
-            hipLaunchKernelGGL((k_mg_tail<NR>), dim3((unsigned)A.tail_list[l + 1].size()),
-                               dim3(TailShape<NR>::THREADS), 0, s,
-                               A.d_tail_list[l + 1], level_slices(pl, A, l + 1), (const float*)T.d_A32,
-                               (const float*)T.d_theta, (const float*)T.d_cheb, A.d_node_off[l + 1], T.d_pt_ptr,
-                               T.d_pt_row, (const _Float16*)T.d_PT16, T.d_p_ptr, T.d_p_col, (const _Float16*)T.d_P16,
-                               A.d_term, (const float*)A.d_inv, w.st, (const float*)T.d_b, T.d_t, T.d_r, T.d_x);
+            constexpr int v = NR == 1 ? 0 : NR == 2 ? 1 : NR == 3 ? 2 : NR == 4 ? 3 : 4;
+            static_assert(TAIL_NR[v] == NR, "right-hand-side count without tail lists");
+            auto launch = [&](auto LVc) {
+                constexpr bool LV = decltype(LVc)::value;
+                const int cnt = A.tail_n[v][LV ? 0 : 1][l + 1];
+                if (cnt == 0) return;
+                hipLaunchKernelGGL((k_mg_tail<NR, LV>), dim3((unsigned)cnt), dim3(TailShape<NR>::THREADS), 0, s,
+                                   A.d_tail_split[v][LV ? 0 : 1][l + 1], level_slices(pl, A, l + 1),
+                                   (const float*)T.d_A32, (const float*)T.d_theta, (const float*)T.d_cheb,
+                                   A.d_node_off[l + 1], T.d_pt_ptr, T.d_pt_row, (const _Float16*)T.d_PT16, T.d_p_ptr,
+                                   T.d_p_col, (const _Float16*)T.d_P16, A.d_term, (const float*)A.d_inv, w.st,
+                                   (const float*)T.d_b, T.d_t, T.d_r, T.d_x);
+            };
+            launch(std::true_type{});
+            launch(std::false_type{});
         }
     }
     for (int l = 1; l < nl; ++l) {
