@@ -99,3 +99,30 @@ def test_amg_compliance_gradient_matches_oracle():
     assert abs(C.cpu().numpy()[0, 0] - ref["C"]) / ref["C"] < 1e-10
     assert rel(dc.cpu().numpy().reshape(ref["dcrds"].shape), ref["dcrds"]) < 1e-8
     assert rel(dq.cpu().numpy().reshape(ref["dprop_quad"].shape), ref["dprop_quad"]) < 1e-8
+
+
+@pytest.mark.parametrize("nrhs", [1, 4])
+def test_amg_unfused_coarse_levels_match_oracle(monkeypatch, nrhs):
+    """JSSO_AMG_FUSE=0 (read when a plan builds its hierarchy) runs each
+    structure's last sparse level with the per-level smoothing, restriction and
+    prolongation kernels and the dense solve with k_mg_direct instead of the fused
+    k_mg_tail: the same preconditioner up to FP32 rounding, so the same solution
+    to the solver's tolerance and nearly the same iteration count.  Two cfg1 shells
+    (three levels: the fused level is level 1) and a 60x60 pinned-edge shell (four
+    levels: level 1 smoothed by the per-level kernels, level 2 fused)."""
+    designs = [wl.cfg1(0), wl.cfg1(1), wl.shell_pinned_edges(60, 6.0, 1.0)]
+    print(f"amg unfused nrhs {nrhs}", flush=True)
+    its = {}
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("JSSO_AMG_FUSE", fuse)
+        b = StructureBatch(designs, n_cases=nrhs)
+        u = b.solve(precision="mixed", tol=1e-10, preconditioner="amg").cpu().numpy().reshape(-1, nrhs)
+        info = b.plan.last_info
+        assert info["converged"] == info["n_systems"] == 3 * nrhs, (fuse, info)
+        its[fuse] = info["iterations"]
+        for i, st in enumerate(designs):
+            o = 6 * int(b.node_off[i])
+            cases = st.load_cases()
+            for c in range(min(nrhs, cases.shape[0])):
+                assert rel(u[o:o + st.ndof, c], oracle_u(st, cases[c])) < 1e-8, (fuse, i, c, info)
+    assert abs(its["1"] - its["0"]) <= 8, its
