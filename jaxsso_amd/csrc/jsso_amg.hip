@@ -896,13 +896,17 @@ __global__ __launch_bounds__(256) void k_amg_nullc(int32_t nc, const double* __r
 // triangle packed row by row (entry (r, c), c <= r, at r (r + 1) / 2 + c; 98 KB
 // for N = 156):
 //  1. the matrix (unit diagonal blocks plus the stored off-diagonal blocks) is
-//     factored by a right-looking Cholesky A = L L^T, two barriers per column
-//     (scale the column into `col`, then the rank-1 update of the trailing
+//     factored by a right-looking Cholesky A = L L^T in blocks of four columns,
+//     two barriers per block (the panel, then the rank-4 update of the trailing
 //     triangle by a 32 x 32 thread tile);
 //  2. X = L^-1 replaces L in place by right-looking forward substitution on
-//     L X = I: at step k the row k of X is finished (divided by L_kk) and every
-//     later row i takes S_ij -= L_ik X_kj for j <= k, where S_ij (held at (i, j)
-//     since step j) becomes X_ij when its row is finished;
+//     L X = I in blocks of four rows, three barriers per block: the block rows of
+//     X are finished (thread per column), then every later row i takes
+//     S_ij -= L_i,block X_block,j, where S_ij (held at (i, j) since the block of
+//     column j) becomes X_ij when its row is finished.
+//     (the column-by-column form, one column per barrier pair, took 330 us for
+//     128 structures of 138 DOF, this one 302 us: the FP64 LDS traffic of the
+//     trailing updates, not the barriers, sets the cost);
 //  3. A^-1 = X^T X by 4 x 4 register blocks of the lower block triangle, each
 //     written to both triangles (the result is exactly symmetric), in FP32,
 //     row-major.
@@ -914,8 +918,9 @@ __global__ __launch_bounds__(1024) void k_amg_dense(const int32_t* __restrict__ 
                                                    const int32_t* __restrict__ browof,
                                                    const int32_t* __restrict__ kref, const double* __restrict__ A,
                                                    float* __restrict__ inv) {
+    constexpr int DB = 4;   // columns (rows) per block step
     __shared__ double M[AMG_DENSE_MAX * (AMG_DENSE_MAX + 1) / 2];
-    __shared__ double col[AMG_DENSE_MAX];
+    __shared__ double pan[AMG_DENSE_MAX][DB];
     const int s = list[blockIdx.x];
     const int node0 = term[4 * s + 1], nt = term[4 * s + 2];
     const int64_t off = term[4 * s + 3];
@@ -933,35 +938,105 @@ __global__ __launch_bounds__(1024) void k_amg_dense(const int32_t* __restrict__ 
         M[tri(R, C)] = kref[k] < 0 ? (r == c ? 1.0 : 0.0) : ablk(A, kref[k], r, c);
     }
     __syncthreads();
-    // 1. Cholesky
-    for (int j = 0; j < N; ++j) {
-        const double d = M[tri(j, j)];
-        const double ljj = d > 0.0 ? sqrt(d) : 1.0;
-        for (int r = j + 1 + tid; r < N; r += blockDim.x) {
-            const double v = M[tri(r, j)] / ljj;
-            col[r] = v;
-            M[tri(r, j)] = v;
+    // 1. Cholesky, blocks of DB columns: every thread factors the diagonal block
+    // in registers and the row threads below it form their panel row
+    // (L21 = A21 L11^-T) into `pan`; after the barrier the block and the panel
+    // are written back and the trailing triangle takes the rank-DB update.
+    for (int k0 = 0; k0 < N; k0 += DB) {
+        const int kb = N - k0 < DB ? N - k0 : DB;
+        double L11[DB][DB];
+#pragma unroll
+        for (int a = 0; a < DB; ++a)
+#pragma unroll
+            for (int c = 0; c < DB; ++c) L11[a][c] = 0.0;
+#pragma unroll
+        for (int c = 0; c < DB; ++c) {
+            if (c < kb) {
+                double d = M[tri(k0 + c, k0 + c)];
+#pragma unroll
+                for (int m = 0; m < c; ++m) d -= L11[c][m] * L11[c][m];
+                const double lcc = d > 0.0 ? sqrt(d) : 1.0;
+                L11[c][c] = lcc;
+#pragma unroll
+                for (int a = c + 1; a < DB; ++a) {
+                    if (a < kb) {
+                        double v = M[tri(k0 + a, k0 + c)];
+#pragma unroll
+                        for (int m = 0; m < c; ++m) v -= L11[a][m] * L11[c][m];
+                        L11[a][c] = v / lcc;
+                    }
+                }
+            }
+        }
+        for (int i = k0 + kb + tid; i < N; i += blockDim.x) {
+            double pv[DB];
+#pragma unroll
+            for (int c = 0; c < DB; ++c) {
+                pv[c] = 0.0;
+                if (c < kb) {
+                    double v = M[tri(i, k0 + c)];
+#pragma unroll
+                    for (int m = 0; m < c; ++m) v -= pv[m] * L11[c][m];
+                    pv[c] = v / L11[c][c];
+                    pan[i][c] = pv[c];
+                }
+            }
         }
         __syncthreads();
-        if (tid == 0) M[tri(j, j)] = ljj;
-        for (int r = j + 1 + ty; r < N; r += 32) {
-            const double cr = col[r];
-            for (int c = j + 1 + tx; c <= r; c += 32) M[tri(r, c)] -= cr * col[c];
-        }
+#pragma unroll
+        for (int a = 0; a < DB; ++a)
+#pragma unroll
+            for (int c = 0; c <= a; ++c)
+                if (tid == a * DB + c && a < kb) M[tri(k0 + a, k0 + c)] = L11[a][c];
+        for (int i = k0 + kb + tid; i < N; i += blockDim.x)
+            for (int c = 0; c < kb; ++c) M[tri(i, k0 + c)] = pan[i][c];
+        for (int r = k0 + kb + ty; r < N; r += 32)
+            for (int c = k0 + kb + tx; c <= r; c += 32) {
+                double v = 0.0;
+                for (int m = 0; m < kb; ++m) v += pan[r][m] * pan[c][m];
+                M[tri(r, c)] -= v;
+            }
         __syncthreads();
     }
-    // 2. X = L^-1 in place
-    for (int k = 0; k < N; ++k) {
-        const double lkk = M[tri(k, k)];
-        for (int j = tid; j < k; j += blockDim.x) M[tri(k, j)] /= lkk;
-        for (int i = k + 1 + tid; i < N; i += blockDim.x) col[i] = M[tri(i, k)];
+    // 2. X = L^-1 in place by forward substitution on L X = I, blocks of DB rows:
+    // every thread reads the block's L into registers and the row threads below
+    // copy their panel of L into `pan`; then thread j finishes column j of the
+    // block rows (S_kj, held at (k, j) since earlier blocks, minus the block's
+    // own terms, over L_kk); then every later row i takes S_ij -= L_i,block X_block,j
+    // (for j inside the block S_ij starts at 0: (i, j) still held L_ij).
+    for (int k0 = 0; k0 < N; k0 += DB) {
+        const int kb = N - k0 < DB ? N - k0 : DB;
+        double Lb[DB][DB];
+#pragma unroll
+        for (int a = 0; a < DB; ++a)
+#pragma unroll
+            for (int c = 0; c < DB; ++c) Lb[a][c] = (a < kb && c <= a) ? M[tri(k0 + a, k0 + c)] : 0.0;
+        for (int i = k0 + kb + tid; i < N; i += blockDim.x)
+            for (int c = 0; c < kb; ++c) pan[i][c] = M[tri(i, k0 + c)];
         __syncthreads();
-        const double xkk = 1.0 / lkk;
-        if (tid == 0) M[tri(k, k)] = xkk;
-        for (int i = k + 1 + ty; i < N; i += 32) {
-            const double li = col[i];
-            for (int j = tx; j <= k; j += 32) M[tri(i, j)] = j < k ? M[tri(i, j)] - li * M[tri(k, j)] : -li * xkk;
+        for (int j = tid; j < k0 + kb; j += blockDim.x) {
+            double x[DB];
+#pragma unroll
+            for (int a = 0; a < DB; ++a) {
+                x[a] = 0.0;
+                const int k = k0 + a;
+                if (a < kb && j <= k) {
+                    double v = j < k0 ? M[tri(k, j)] : (j == k ? 1.0 : 0.0);
+#pragma unroll
+                    for (int m = 0; m < a; ++m) v -= Lb[a][m] * x[m];   // x[m] = 0 where j > k0 + m
+                    x[a] = v / Lb[a][a];
+                    M[tri(k, j)] = x[a];
+                }
+            }
         }
+        __syncthreads();
+        for (int i = k0 + kb + ty; i < N; i += 32)
+            for (int j = tx; j < k0 + kb; j += 32) {
+                double v = j < k0 ? M[tri(i, j)] : 0.0;
+                for (int m = 0; m < kb; ++m)
+                    if (j <= k0 + m) v -= pan[i][m] * M[tri(k0 + m, j)];
+                M[tri(i, j)] = v;
+            }
         __syncthreads();
     }
     // 3. A^-1 = X^T X: (A^-1)_rc = sum_{k >= max(r, c)} X_kr X_kc
