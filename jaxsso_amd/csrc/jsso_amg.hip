@@ -691,85 +691,121 @@ __global__ void k_amg_theta(int32_t ns, const int32_t* __restrict__ sys_wg_off, 
     }
 }
 
-// P(i, J) = sum over its products of ((1 - omega) delta_ij I - omega A_off(i, j)) Q_j;
-// thread per (block, row r)
+// Both products below run one thread per output block (36 accumulators): the
+// thread reads each operand block as 18 16-byte pieces of consecutive memory
+// and forms all 36 outputs from it.  (A thread per block row issued 6 x 7
+// 8-byte loads per product for the same bytes and ran at half the occupancy.)
+__device__ __forceinline__ void load_blk(const double* __restrict__ p, double (&v)[36]) {
+    const double2* q = reinterpret_cast<const double2*>(p);
+#pragma unroll
+    for (int e = 0; e < 18; ++e) {
+        const double2 t = q[e];
+        v[2 * e] = t.x;
+        v[2 * e + 1] = t.y;
+    }
+}
+
+// P(i, J) = sum over its products of ((1 - omega) delta_ij I - omega A_off(i, j)) Q_j
 __global__ __launch_bounds__(256) void k_amg_psmooth(int64_t nP, const int32_t* __restrict__ p_row,
                                                      const int32_t* __restrict__ p_cptr,
                                                      const int32_t* __restrict__ p_cj,
                                                      const int32_t* __restrict__ p_caref,
                                                      const int32_t* __restrict__ node_sys,
-                                                     const double* __restrict__ omega, const double* __restrict__ A,
+                                                     const double* __restrict__ omega, const double* __restrict__ An,
                                                      const double* __restrict__ Q, double* __restrict__ P) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nP * 6) return;
-    const int64_t b = t / 6;
-    const int r = (int)(t % 6);
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nP) return;
     const double om = omega[node_sys[p_row[b]]];
-    double out[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double out[36];
+#pragma unroll
+    for (int e = 0; e < 36; ++e) out[e] = 0.0;
     for (int q = p_cptr[b]; q < p_cptr[b + 1]; ++q) {
-        const double* Qj = Q + (int64_t)p_cj[q] * 36;
+        double qv[36];
+        load_blk(Q + (int64_t)p_cj[q] * 36, qv);
         const int32_t aref = p_caref[q];
-        double a[6];
+        if (aref < 0) {
 #pragma unroll
-        for (int m = 0; m < 6; ++m) a[m] = aref < 0 ? (m == r ? 1.0 - om : 0.0) : -om * ablk_nat(A, aref, r, m);
+            for (int e = 0; e < 36; ++e) out[e] += (1.0 - om) * qv[e];
+        } else {
+            double av[36];
+            load_blk(An + (int64_t)(aref >> 1) * 36, av);
+            const bool tr = aref & 1;
 #pragma unroll
-        for (int m = 0; m < 6; ++m)
+            for (int r = 0; r < 6; ++r)
 #pragma unroll
-            for (int c = 0; c < 6; ++c) out[c] += a[m] * Qj[m * 6 + c];
+                for (int m = 0; m < 6; ++m) {
+                    const double a = -om * (tr ? av[m * 6 + r] : av[r * 6 + m]);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) out[r * 6 + c] += a * qv[m * 6 + c];
+                }
+        }
     }
+    double2* o = reinterpret_cast<double2*>(P + b * 36);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) P[b * 36 + r * 6 + c] = out[c];
+    for (int e = 0; e < 18; ++e) o[e] = make_double2(out[2 * e], out[2 * e + 1]);
 }
 
-// W(i, J) = sum over its products of A(i, j) P(j, J); thread per (block, row r)
+// W(i, J) = sum over its products of A(i, j) P(j, J)
 __global__ __launch_bounds__(256) void k_amg_w(int64_t nW, const int32_t* __restrict__ w_cptr,
                                                const int32_t* __restrict__ w_caref, const int32_t* __restrict__ w_cp,
-                                               const double* __restrict__ A, const double* __restrict__ P,
+                                               const double* __restrict__ An, const double* __restrict__ P,
                                                double* __restrict__ W) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nW * 6) return;
-    const int64_t b = t / 6;
-    const int r = (int)(t % 6);
-    double out[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nW) return;
+    double out[36];
+#pragma unroll
+    for (int e = 0; e < 36; ++e) out[e] = 0.0;
     for (int q = w_cptr[b]; q < w_cptr[b + 1]; ++q) {
-        const double* Pb = P + (int64_t)w_cp[q] * 36;
+        double pv[36];
+        load_blk(P + (int64_t)w_cp[q] * 36, pv);
         const int32_t aref = w_caref[q];
-        double a[6];
+        if (aref < 0) {
 #pragma unroll
-        for (int m = 0; m < 6; ++m) a[m] = ablk_nat(A, aref, r, m);
+            for (int e = 0; e < 36; ++e) out[e] += pv[e];
+        } else {
+            double av[36];
+            load_blk(An + (int64_t)(aref >> 1) * 36, av);
+            const bool tr = aref & 1;
 #pragma unroll
-        for (int m = 0; m < 6; ++m)
+            for (int r = 0; r < 6; ++r)
 #pragma unroll
-            for (int c = 0; c < 6; ++c) out[c] += a[m] * Pb[m * 6 + c];
+                for (int m = 0; m < 6; ++m) {
+                    const double a = tr ? av[m * 6 + r] : av[r * 6 + m];
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) out[r * 6 + c] += a * pv[m * 6 + c];
+                }
+        }
     }
+    double2* o = reinterpret_cast<double2*>(W + b * 36);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) W[b * 36 + r * 6 + c] = out[c];
+    for (int e = 0; e < 18; ++e) o[e] = make_double2(out[2 * e], out[2 * e + 1]);
 }
 
 // A_c(I, J) = sum over its products of P(i, I)^T W(i, J) (blocks with I <= J);
-// thread per (block, row r)
+// thread per output block, like the two products above
 __global__ __launch_bounds__(256) void k_amg_galerkin(int64_t nnzc, const int32_t* __restrict__ c_cptr,
                                                       const int32_t* __restrict__ c_p, const int32_t* __restrict__ c_w,
                                                       const double* __restrict__ P, const double* __restrict__ W,
                                                       double* __restrict__ Ac) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nnzc * 6) return;
-    const int64_t k = t / 6;
-    const int r = (int)(t % 6);
-    double out[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nnzc) return;
+    double out[36];
+#pragma unroll
+    for (int e = 0; e < 36; ++e) out[e] = 0.0;
     for (int q = c_cptr[k]; q < c_cptr[k + 1]; ++q) {
-        const double* Pb = P + (int64_t)c_p[q] * 36;
-        const double* Wb = W + (int64_t)c_w[q] * 36;
-        double pc[6];
+        double pv[36], wv[36];
+        load_blk(P + (int64_t)c_p[q] * 36, pv);
+        load_blk(W + (int64_t)c_w[q] * 36, wv);
 #pragma unroll
-        for (int m = 0; m < 6; ++m) pc[m] = Pb[m * 6 + r];
+        for (int r = 0; r < 6; ++r)
 #pragma unroll
-        for (int m = 0; m < 6; ++m)
+            for (int m = 0; m < 6; ++m)
 #pragma unroll
-            for (int c = 0; c < 6; ++c) out[c] += pc[m] * Wb[m * 6 + c];
+                for (int c = 0; c < 6; ++c) out[r * 6 + c] += pv[m * 6 + r] * wv[m * 6 + c];
     }
+    double2* o = reinterpret_cast<double2*>(Ac + k * 36);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) Ac[k * 36 + r * 6 + c] = out[c];
+    for (int e = 0; e < 18; ++e) o[e] = make_double2(out[2 * e], out[2 * e + 1]);
 }
 
 // Cholesky factor of each coarse diagonal block and its inverse.  A pivot that
@@ -2065,15 +2101,15 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
             const int64_t npos = l == 0 ? pl->ell_slots : D.ell_slots;
             hipLaunchKernelGGL(k_amg_unpack, dim3(nblk(npos * 18, 256)), dim3(256), 0, s, npos, (const double2*)A64,
                                (double2*)A.d_Anat);
-            hipLaunchKernelGGL(k_amg_psmooth, dim3(nblk(D.nP * 6, 256)), dim3(256), 0, s, D.nP, D.d_p_row,
+            hipLaunchKernelGGL(k_amg_psmooth, dim3(nblk(D.nP, 256)), dim3(256), 0, s, D.nP, D.d_p_row,
                                D.d_p_cptr, D.d_p_cj, D.d_p_caref, D.d_node_sys, (const double*)D.d_omega,
                                (const double*)A.d_Anat, (const double*)D.d_Q, D.d_P64);
-            hipLaunchKernelGGL(k_amg_w, dim3(nblk(D.nW * 6, 256)), dim3(256), 0, s, D.nW, D.d_w_cptr, D.d_w_caref,
+            hipLaunchKernelGGL(k_amg_w, dim3(nblk(D.nW, 256)), dim3(256), 0, s, D.nW, D.d_w_cptr, D.d_w_caref,
                                D.d_w_cp, (const double*)A.d_Anat, (const double*)D.d_P64, D.d_W64);
         }
         const int64_t nnzc = A.host[l + 1].colind.size();
         if (nnzc > 0) {
-            hipLaunchKernelGGL(k_amg_galerkin, dim3(nblk(nnzc * 6, 256)), dim3(256), 0, s, nnzc, D.d_c_cptr, D.d_c_p,
+            hipLaunchKernelGGL(k_amg_galerkin, dim3(nblk(nnzc, 256)), dim3(256), 0, s, nnzc, D.d_c_cptr, D.d_c_p,
                                D.d_c_w, (const double*)D.d_P64, (const double*)D.d_W64, A.d_Ac[l + 1]);
             hipLaunchKernelGGL(k_amg_cfactor, dim3(nblk(C.n, 128)), dim3(128), 0, s, C.n, A.d_cdiag[l + 1],
                                (const double*)A.d_Ac[l + 1], A.d_Lc[l + 1], A.d_Lci[l + 1]);
