@@ -2434,17 +2434,39 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
         return JSSO_OK;
     };
 
-    // convergence flags read one chunk behind the GPU (as in the block-Jacobi solver)
+    // Convergence flags (and the FP64 relative residuals) are read one chunk
+    // behind the GPU, as in the block-Jacobi solver.  The residual is only tested
+    // at a chunk's end, so a chunk that would straddle the predicted crossing of
+    // tol is cut to end just after it: the slowest running system's residuals at
+    // the last two known chunk ends give its convergence factor per iteration,
+    // extrapolated over the chunk in flight.  A cut chunk has an even length
+    // (the chunk graphs need even starts) and runs with eager launches; a wrong
+    // prediction costs one more replacement.
     std::vector<int> chunk_end;
+    std::vector<std::pair<int, double>> known;   // (iteration, slowest running system's residual)
     int it = 0, it_done = 0, chunks_done = 0, c = 0;
     for (;;) {
         const bool more = it < maxit;
         if (more) {
-            const int steps = std::min(E, maxit - it);
+            int steps = std::min(E, maxit - it);
+            const size_t nk = known.size();
+            if (nk >= 2 && known[nk - 1].first > known[nk - 2].first && known[nk - 1].second > tol &&
+                known[nk - 1].second < known[nk - 2].second) {
+                const double per_it = std::log(known[nk - 1].second / known[nk - 2].second) /
+                                      (known[nk - 1].first - known[nk - 2].first);
+                const double kstar = known[nk - 1].first + std::log(tol / known[nk - 1].second) / per_it;
+                if (kstar > it && kstar < it + steps - 1) {
+                    int cut = (int)std::ceil(kstar - it) + 1;
+                    cut += cut & 1;
+                    steps = std::max(2, std::min(steps, cut));
+                }
+            }
             if ((rc = enqueue_chunk(it, steps))) return rc;
             it += steps;
             chunk_end.push_back(it);
             JSSO_HIP(hipMemcpyAsync(pl->h_flags + (size_t)(c & 1) * nsys, w.st.active, nsys * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, s));
+            JSSO_HIP(hipMemcpyAsync(pl->h_rel + (size_t)(c & 1) * nsys, w.st.relres, nsys * sizeof(double),
                                     hipMemcpyDeviceToHost, s));
             JSSO_HIP(hipEventRecord(pl->flag_ev[c & 1], s));
             ++c;
@@ -2454,13 +2476,20 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
             JSSO_HIP(hipEventSynchronize(pl->flag_ev[chk & 1]));
             timing_collect();
             const int32_t* fl = pl->h_flags + (size_t)(chk & 1) * nsys;
+            const double* rl = pl->h_rel + (size_t)(chk & 1) * nsys;
             bool any = false;
-            for (int k = 0; k < nsys && !any; ++k) any = fl[k] != 0;
+            double worst = 0.0;
+            for (int k = 0; k < nsys; ++k)
+                if (fl[k] != 0) {
+                    any = true;
+                    worst = std::max(worst, rl[k]);
+                }
             if (!any) {
                 it_done = chunk_end[chk];
                 chunks_done = chk + 1;
                 break;
             }
+            known.push_back({chunk_end[chk], worst});
         }
         if (!more) {
             it_done = it;

@@ -443,6 +443,7 @@ int jsso_plan_destroy(jsso_plan* plan) {
     for (auto& e : q->flag_ev)
         if (e) (void)hipEventDestroy(e);
     if (q->h_flags) (void)hipHostFree(q->h_flags);
+    if (q->h_rel) (void)hipHostFree(q->h_rel);
     for (void* a : q->allocs) (void)hipFree(a);
     for (auto& b : q->ws)
         if (b.p) (void)hipFree(b.p);

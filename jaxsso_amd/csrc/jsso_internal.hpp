@@ -108,6 +108,7 @@ struct Plan {
     hipStream_t capture_stream = nullptr;
     bool graphs_ok = true;           // false after a failed capture: launch eagerly
     int32_t* h_flags = nullptr;      // pinned host copies of the convergence flags, two chunks' worth
+    double* h_rel = nullptr;         // and of the systems' relative FP64 residuals (AMG chunk schedule)
     size_t h_flags_count = 0;
     hipEvent_t flag_ev[2] = {nullptr, nullptr};
     double* d_Ke64 = nullptr;        // scaled off-diagonal blocks, sliced ELL, FP64

@@ -700,9 +700,12 @@ bool graphs_enabled() {
 int ensure_flag_buffers(Plan* pl, size_t nsys) {
     if (pl->h_flags_count < 2 * nsys) {
         if (pl->h_flags) JSSO_HIP(hipHostFree(pl->h_flags));
+        if (pl->h_rel) JSSO_HIP(hipHostFree(pl->h_rel));
         pl->h_flags = nullptr;
+        pl->h_rel = nullptr;
         pl->h_flags_count = 0;
         JSSO_HIP(hipHostMalloc((void**)&pl->h_flags, 2 * nsys * sizeof(int32_t), hipHostMallocDefault));
+        JSSO_HIP(hipHostMalloc((void**)&pl->h_rel, 2 * nsys * sizeof(double), hipHostMallocDefault));
         pl->h_flags_count = 2 * nsys;
     }
     for (auto& e : pl->flag_ev)
