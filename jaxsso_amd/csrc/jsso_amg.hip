@@ -1151,9 +1151,9 @@ __device__ __forceinline__ void wave_totals(const double* __restrict__ cg, int s
 // (the caller puts the own-row term into wave 0's acc beforehand).  The next
 // slot's column and value position are requested before the current slot's
 // products; a slot past the end reads the slice's metadata word instead.
-template <typename MT, typename VT, int NR, int WPS, int G = 1, typename F>
-__device__ __forceinline__ void ell_apply(const Slices& sl, const MT* __restrict__ A, int w, const int4& ma,
-                                          const int4& mb, VT (&acc)[6][NR], F&& xj) {
+template <typename MT, typename VT, int NR, int WPS, int G = 1, typename F, typename Pre>
+__device__ __forceinline__ void ell_apply_pre(const Slices& sl, const MT* __restrict__ A, int w, const int4& ma,
+                                              const int4& mb, VT (&acc)[6][NR], F&& xj, Pre&& pre) {
     // G groups of WPS waves, each group on a slice of its own
     __shared__ VT red[WPS > 1 ? G : 1][WPS > 1 ? WPS - 1 : 1][WPS > 1 ? 6 * NR : 1][64];
     const int lane = threadIdx.x & 63, wave = WPS > 1 ? (threadIdx.x >> 6) % WPS : 0;
@@ -1168,6 +1168,7 @@ __device__ __forceinline__ void ell_apply(const Slices& sl, const MT* __restrict
     int j = wave;
     int64_t col = *(j < nslot ? sl.ell_col + cpos(j) : dummy);
     int vp = *(j < nslot ? sl.ell_vpos + cpos(j) : dummy);
+    pre();   // (work that xj needs, run while the first slot's indices arrive)
     for (; j < nslot; j += WPS) {
         VT x[6][NR];
 #pragma unroll
@@ -1216,6 +1217,12 @@ __device__ __forceinline__ void ell_apply(const Slices& sl, const MT* __restrict
     }
 }
 
+template <typename MT, typename VT, int NR, int WPS, int G = 1, typename F>
+__device__ __forceinline__ void ell_apply(const Slices& sl, const MT* __restrict__ A, int w, const int4& ma,
+                                          const int4& mb, VT (&acc)[6][NR], F&& xj) {
+    ell_apply_pre<MT, VT, NR, WPS, G>(sl, A, w, ma, mb, acc, xj, [] {});
+}
+
 // The rows of a slice form one contiguous run of (row1 - row0) * 6 * NR values
 // per vector: wave 0 puts value(c, h) of its lane's row into LDS, then every
 // thread of the workgroup copies the run, so each store instruction writes
@@ -1253,26 +1260,38 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_dir(Slices sl, Partials P, Sys
     const int64_t i = (int64_t)ma.y + lane;
     const bool on = i < ma.z;
     const int64_t ii = on ? i : (int64_t)ma.y;
-    double t1[NR], t2[NR], t0[NR];
-    wave_totals<NR>(P.cg[(k - 1) & 1], CG_RR, mc.x, mc.y, t1);
-    wave_totals<NR>(P.cg[(k - 1) & 1], CG_RQ, mc.x, mc.y, t2);
-    wave_totals<NR>(P.cg[k & 1], CG_RR, mc.x, mc.y, t0);
-    VT beta[NR];
+    // own rows requested first; beta (from the partial sums) is formed while
+    // they and the first slot's indices arrive
+    VT zo[6][NR], po[6][NR];
 #pragma unroll
-    for (int h = 0; h < NR; ++h) {
-        const double b = t0[h] > 0.0 ? (t1[h] - t2[h]) / t0[h] : 0.0;
-        beta[h] = (isfinite(b) && b > 0.0) ? VT(b) : VT(0);
-    }
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int h = 0; h < NR; ++h) {
+            const int64_t o = (6 * ii + c) * NR + h;
+            zo[c][h] = z[o];
+            po[c][h] = pprev[o];
+        }
+    VT beta[NR];
     auto pk = [&](int64_t node, int c, int h) {
         const int64_t o = (6 * node + c) * NR + h;
         return z[o] + beta[h] * pprev[o];
     };
     VT acc[6][NR];
+    ell_apply_pre<MT, VT, NR, WPS>(sl, A, w, ma, mb, acc, pk, [&] {
+        double t1[NR], t2[NR], t0[NR];
+        wave_totals<NR>(P.cg[(k - 1) & 1], CG_RR, mc.x, mc.y, t1);
+        wave_totals<NR>(P.cg[(k - 1) & 1], CG_RQ, mc.x, mc.y, t2);
+        wave_totals<NR>(P.cg[k & 1], CG_RR, mc.x, mc.y, t0);
 #pragma unroll
-    for (int c = 0; c < 6; ++c)
+        for (int h = 0; h < NR; ++h) {
+            const double b = t0[h] > 0.0 ? (t1[h] - t2[h]) / t0[h] : 0.0;
+            beta[h] = (isfinite(b) && b > 0.0) ? VT(b) : VT(0);
+        }
 #pragma unroll
-        for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? pk(ii, c, h) : VT(0);
-    ell_apply<MT, VT, NR, WPS>(sl, A, w, ma, mb, acc, pk);
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? zo[c][h] + beta[h] * po[c][h] : VT(0);
+    });
     __shared__ VT stage[64 * 6 * NR];
     double v[NR];
 #pragma unroll
@@ -1313,28 +1332,41 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_pre(Slices sl, Partials P, Sys
     const int64_t i = (int64_t)ma.y + lane;
     const bool on = i < ma.z;
     const int64_t ii = on ? i : (int64_t)ma.y;
+    // own rows requested first; alpha (from the partial sums) is formed while
+    // they and the first slot's indices arrive
+    VT ro[6][NR], qo[6][NR];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int h = 0; h < NR; ++h) {
+            const int64_t o = (6 * ii + c) * NR + h;
+            ro[c][h] = FRESH ? rcur[o] : rprev[o];
+            qo[c][h] = FRESH ? VT(0) : q[o];
+        }
     VT alpha[NR];
-#pragma unroll
-    for (int h = 0; h < NR; ++h) alpha[h] = VT(0);
-    if (!FRESH) {
-        double rz[NR], pq[NR];
-        wave_totals<NR>(P.cg[(k - 1) & 1], CG_RR, mc.x, mc.y, rz);
-        wave_totals<NR>(P.cg[k & 1], CG_PQ, mc.x, mc.y, pq);
-#pragma unroll
-        for (int h = 0; h < NR; ++h)
-            alpha[h] = (pq[h] > 0.0 && isfinite(pq[h]) && rz[h] > 0.0) ? VT(rz[h] / pq[h]) : VT(0);
-    }
     auto rk = [&](int64_t node, int c, int h) {
         const int64_t o = (6 * node + c) * NR + h;
         return FRESH ? rcur[o] : rprev[o] - alpha[h] * q[o];
     };
     const VT th = (VT)theta[ma.x];
     VT acc[6][NR];
+    ell_apply_pre<_Float16, VT, NR, WPS>(sl, A, w, ma, mb, acc, rk, [&] {
 #pragma unroll
-    for (int c = 0; c < 6; ++c)
+        for (int h = 0; h < NR; ++h) alpha[h] = VT(0);
+        if (!FRESH) {
+            double rz[NR], pq[NR];
+            wave_totals<NR>(P.cg[(k - 1) & 1], CG_RR, mc.x, mc.y, rz);
+            wave_totals<NR>(P.cg[k & 1], CG_PQ, mc.x, mc.y, pq);
 #pragma unroll
-        for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? rk(ii, c, h) : VT(0);
-    ell_apply<_Float16, VT, NR, WPS>(sl, A, w, ma, mb, acc, rk);
+            for (int h = 0; h < NR; ++h)
+                alpha[h] = (pq[h] > 0.0 && isfinite(pq[h]) && rz[h] > 0.0) ? VT(rz[h] / pq[h]) : VT(0);
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int h = 0; h < NR; ++h)
+                acc[c][h] = wave == 0 ? (FRESH ? ro[c][h] : ro[c][h] - alpha[h] * qo[c][h]) : VT(0);
+    });
     __shared__ VT stage[64 * 6 * NR];
     flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) { return rk(ii, c, h) - acc[c][h] / th; }, t0);
     if (!FRESH) {
