@@ -1253,9 +1253,10 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_dir(Slices sl, Partials P, Sys
                                                       VT* __restrict__ q) {
     const int w = slice_of_block(sl);
     if (w < 0) return;
+    // (metadata requested together with the slice's flags, not after them)
+    const int4 ma = sl.meta[3 * w], mb = sl.meta[3 * w + 1], mc = sl.meta[3 * w + 2];
     const int amask = st.slice_act[w];
     if (!amask) return;
-    const int4 ma = sl.meta[3 * w], mb = sl.meta[3 * w + 1], mc = sl.meta[3 * w + 2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t i = (int64_t)ma.y + lane;
     const bool on = i < ma.z;
@@ -1325,9 +1326,10 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_pre(Slices sl, Partials P, Sys
                                                       VT* __restrict__ xlo, VT* __restrict__ t0) {
     const int w = slice_of_block(sl);
     if (w < 0) return;
+    // (metadata requested together with the slice's flags, not after them)
+    const int4 ma = sl.meta[3 * w], mb = sl.meta[3 * w + 1], mc = sl.meta[3 * w + 2];
     const int amask = st.slice_act[w];
     if (!amask) return;
-    const int4 ma = sl.meta[3 * w], mb = sl.meta[3 * w + 1], mc = sl.meta[3 * w + 2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t i = (int64_t)ma.y + lane;
     const bool on = i < ma.z;
@@ -1391,9 +1393,9 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_post(Slices sl, Partials P, Sy
                                                        const VT* __restrict__ z1, VT* __restrict__ z) {
     const int w = slice_of_block(sl);
     if (w < 0) return;
+    const int4 ma = sl.meta[3 * w], mb = sl.meta[3 * w + 1];
     const int amask = st.slice_act[w];
     if (!amask) return;
-    const int4 ma = sl.meta[3 * w], mb = sl.meta[3 * w + 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t i = (int64_t)ma.y + lane;
     const bool on = i < ma.z;
