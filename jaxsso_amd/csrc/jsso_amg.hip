@@ -1345,6 +1345,21 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_pre(Slices sl, Partials P, Sys
             ro[c][h] = FRESH ? rcur[o] : rprev[o];
             qo[c][h] = FRESH ? VT(0) : q[o];
         }
+    // own rows of x and p for the update at the end: requested now when the
+    // registers allow it (one FP32 right-hand side)
+    constexpr bool PF = !FRESH && NR == 1 && sizeof(VT) == 4;
+    constexpr int NP = PF ? NR : 1;
+    VT xo[6][NP], pko[6][NP];
+    if constexpr (PF) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int h = 0; h < NP; ++h) {
+                const int64_t o = (6 * ii + c) * NR + h;
+                xo[c][h] = xlo[o];
+                pko[c][h] = p[o];
+            }
+    }
     VT alpha[NR];
     auto rk = [&](int64_t node, int c, int h) {
         const int64_t o = (6 * node + c) * NR + h;
@@ -1378,7 +1393,8 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_pre(Slices sl, Partials P, Sys
         }, rcur);
         flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) {
             const int64_t o = (6 * ii + c) * NR + h;
-            return ((amask >> h) & 1) ? xlo[o] + alpha[h] * p[o] : xlo[o];
+            const VT xv = PF ? xo[c][PF ? h : 0] : xlo[o];
+            return ((amask >> h) & 1) ? xv + alpha[h] * (PF ? pko[c][PF ? h : 0] : p[o]) : xv;
         }, xlo);
     }
 }
