@@ -1571,14 +1571,15 @@ __global__ __launch_bounds__(256) void k_mg_restrict(int32_t nc, const int32_t* 
     const int64_t jc = u >> 2;
     const bool valid = jc < 6 * (int64_t)nc;
     const int J = valid ? (int)(jc / 6) : 0, c = (int)(jc % 6);
+    // (the column's range is requested together with the structure's flags)
+    const int qa = pt_ptr[J] + g, q1 = pt_ptr[J + 1];
     const bool run = valid && !skip[node_sys_c[J]] && !structure_done<NR>(st, node_sys_c[J]);
     float acc[NR];
 #pragma unroll
     for (int h = 0; h < NR; ++h) acc[h] = 0.f;
     if (run) {
-        const int q1 = pt_ptr[J + 1];
 #pragma unroll 4
-        for (int q = pt_ptr[J] + g; q < q1; q += 4) {
+        for (int q = qa; q < q1; q += 4) {
             const int64_t i = pt_row[q];
             const _Float16* pr = PT + (int64_t)q * 36 + c * 6;
             float pv[6];
@@ -1613,19 +1614,27 @@ __global__ __launch_bounds__(256) void k_mg_prolong(int32_t n, const int32_t* __
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= 6 * (int64_t)n) return;
     const int i = (int)(u / 6), r = (int)(u % 6);
+    // (the row's range and vector entries are requested together with the structure's flags)
+    const int qa = p_ptr[i], qb = p_ptr[i + 1];
+    float bv[NR], tv[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+        bv[h] = (float)b[u * NR + h];
+        tv[h] = cheb ? t[u * NR + h] : 0.f;
+    }
     const int s = node_sys[i];
     if (!cont[s] || structure_done<NR>(st, s)) return;
     float acc[NR];
     if (cheb) {
         const float c = cheb[s];
 #pragma unroll
-        for (int h = 0; h < NR; ++h) acc[h] = c * ((float)b[u * NR + h] + t[u * NR + h]);
+        for (int h = 0; h < NR; ++h) acc[h] = c * (bv[h] + tv[h]);
     } else {
         const float th = theta[s];
 #pragma unroll
-        for (int h = 0; h < NR; ++h) acc[h] = (float)b[u * NR + h] / th;
+        for (int h = 0; h < NR; ++h) acc[h] = bv[h] / th;
     }
-    for (int q = p_ptr[i]; q < p_ptr[i + 1]; ++q) {
+    for (int q = qa; q < qb; ++q) {
         const int64_t J = p_col[q];
         const _Float16* pb = P + (int64_t)q * 36 + r * 6;
 #pragma unroll
