@@ -170,12 +170,17 @@ __global__ __launch_bounds__(64 * WPS) void k_replace(Slices sl, Partials P, Sys
     for (int c = 0; c < 6; ++c)
 #pragma unroll
         for (int h = 0; h < NR; ++h) acc[c][h] = 0.0;
+    // the next slot's column and value position are requested before the
+    // current slot's products (a slot past the end reads the metadata word)
+    const int32_t* dummy = (const int32_t*)&sl.meta[3 * w];
+    auto cpos = [&](int jj) -> int64_t {
+        return jj < nup ? (int64_t)pup + (int64_t)jj * ROWS_PER_WG + lane
+                        : (int64_t)plo + (int64_t)(jj - nup) * ROWS_PER_WG + lane;
+    };
+    int64_t col = *(wave < nslot ? sl.ell_col + cpos(wave) : dummy);
+    int vp = *(wave < nslot ? sl.ell_vpos + cpos(wave) : dummy);
     for (int j = wave; j < nslot; j += WPS) {
         const bool upper = j < nup;
-        const int64_t pos = upper ? (int64_t)pup + (int64_t)j * ROWS_PER_WG + lane
-                                  : (int64_t)plo + (int64_t)(j - nup) * ROWS_PER_WG + lane;
-        const int64_t col = sl.ell_col[pos];
-        const int vp = sl.ell_vpos[pos];
         double xj[6][NR];
 #pragma unroll
         for (int c = 0; c < 6; ++c)
@@ -186,6 +191,9 @@ __global__ __launch_bounds__(64 * WPS) void k_replace(Slices sl, Partials P, Sys
             load_block(Aw + (int64_t)j * 36 * ROWS_PER_WG, lane, av);
         else
             load_block(A + (int64_t)(vp >> 6) * 36 * ROWS_PER_WG, vp & (ROWS_PER_WG - 1), av);
+        const int jn = j + WPS;
+        const int64_t coln = *(jn < nslot ? sl.ell_col + cpos(jn) : dummy);
+        const int vpn = *(jn < nslot ? sl.ell_vpos + cpos(jn) : dummy);
         if (upper) {
 #pragma unroll
             for (int e = 0; e < 36; ++e)
@@ -197,6 +205,8 @@ __global__ __launch_bounds__(64 * WPS) void k_replace(Slices sl, Partials P, Sys
 #pragma unroll
                 for (int h = 0; h < NR; ++h) acc[e % 6][h] += av[e] * xj[e / 6][h];
         }
+        col = coln;
+        vp = vpn;
     }
     if (WPS > 1) {
         if (wave > 0) {
