@@ -631,21 +631,27 @@ __global__ __launch_bounds__(64 * WPS) void k_amg_power(Slices sl, const MT* __r
     if (!cont[ma.x]) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double sc = 1.0;
-    if (step > 0) {
-        const double* pp = part + (int64_t)((step - 1) & 1) * n_wg;
-        double t = 0.0;
-        for (int q = mc.x + lane; q < mc.y; q += 64) t += pp[q];
-        t = wave_sum(t);
-        sc = t > 0.0 ? 1.0 / sqrt(t) : 0.0;
-    }
     auto v = [&](int64_t node, int c, int) { return step == 0 ? amg_hash(6 * node + c) : vin[6 * node + c] * sc; };
     const int64_t i = (int64_t)ma.y + lane;
     const bool on = i < ma.z;
     const int64_t ii = on ? i : (int64_t)ma.y;
-    double acc[6][1];
+    // own rows requested first; the scale (from the partial sums) is formed while
+    // they and the first slot's indices arrive
+    double vo[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) acc[c][0] = wave == 0 ? v(ii, c, 0) : 0.0;
-    ell_apply<MT, double, 1, WPS>(sl, A, w, ma, mb, acc, v);
+    for (int c = 0; c < 6; ++c) vo[c] = step == 0 ? amg_hash(6 * ii + c) : vin[6 * ii + c];
+    double acc[6][1];
+    ell_apply_pre<MT, double, 1, WPS>(sl, A, w, ma, mb, acc, v, [&] {
+        if (step > 0) {
+            const double* pp = part + (int64_t)((step - 1) & 1) * n_wg;
+            double t = 0.0;
+            for (int q = mc.x + lane; q < mc.y; q += 64) t += pp[q];
+            t = wave_sum(t);
+            sc = t > 0.0 ? 1.0 / sqrt(t) : 0.0;
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c][0] = wave == 0 ? (step == 0 ? vo[c] : vo[c] * sc) : 0.0;
+    });
     if (wave != 0) return;
     double yy = 0.0;
     if (on) {
@@ -1468,6 +1474,19 @@ __global__ __launch_bounds__(SOLVER_WG) void k_pcg_fold(Slices sl, Partials P, S
     const int s0 = sl.sys_off[sys], s1 = sl.sys_off[sys + 1];
     const double* rzp = P.cg[(k - 1) & 1];
     const double* pqp = P.cg[k & 1];
+    // this thread's row, requested before the reduction
+    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
+    const bool row = R < 6 * (int64_t)sl.row1[w];
+    const int64_t Rr = row ? R : 6 * (int64_t)sl.row0[w];
+    VT xl[NR], pv[NR];
+    double xv[NR];
+#pragma unroll
+    for (int h = 0; h < NR; ++h) {
+        const int64_t o = Rr * NR + h;
+        xl[h] = xlo[o];
+        pv[h] = p[o];
+        xv[h] = x64[o];
+    }
     double v[2 * NR];
 #pragma unroll
     for (int a = 0; a < 2 * NR; ++a) v[a] = 0.0;
@@ -1479,15 +1498,14 @@ __global__ __launch_bounds__(SOLVER_WG) void k_pcg_fold(Slices sl, Partials P, S
         }
     }
     group_sum<2 * NR, NWAVES>(v, lds);
-    const int64_t R = 6 * (int64_t)sl.row0[w] + threadIdx.x;
-    if (R >= 6 * (int64_t)sl.row1[w]) return;
+    if (!row) return;
 #pragma unroll
     for (int h = 0; h < NR; ++h) {
         if (!st.active[sys * NR + h]) continue;
         const double rz = v[h], pq = v[NR + h];
         const VT alpha = (pq > 0.0 && isfinite(pq) && rz > 0.0) ? VT(rz / pq) : VT(0);
         const int64_t o = R * NR + h;
-        x64[o] += (double)(xlo[o] + alpha * p[o]);
+        x64[o] = xv[h] + (double)(xl[h] + alpha * pv[h]);
         xlo[o] = VT(0);
     }
 }
