@@ -1907,24 +1907,46 @@ __global__ __launch_bounds__(TailShape<NR>::THREADS) void k_mg_tail(
     }
     __syncthreads();
     // 4. prolongation with the pre-smoothing result, z1 = c (b + t) + P x_c, into r
-    for (int64_t u = 6 * n0 + tid; u < 6 * n1; u += S::THREADS) {
-        const int64_t i = u / 6;
-        const int row = (int)(u % 6);
-        float acc[NR];
-        const int64_t ul = u - 6 * nb;
+    // (rows u and u + THREADS per thread, block m of both in the same step, so that
+    // their index and value loads are in flight together)
+    for (int64_t ub = 6 * n0 + tid; ub < 6 * n1; ub += 2 * S::THREADS) {
+        bool ok[2];
+        int qs[2], qe[2], row[2];
+        int64_t ul[2];
+        float acc[2][NR];
 #pragma unroll
-        for (int h = 0; h < NR; ++h) acc[h] = ch * (b[ul * NR + h] + t[ul * NR + h]);
-#pragma unroll 4
-        for (int q = p_ptr[i]; q < p_ptr[i + 1]; ++q) {
-            const int J = p_col[q] - (int)c0;
-            const _Float16* pb = P + (int64_t)q * 36 + row * 6;
+        for (int a = 0; a < 2; ++a) {
+            const int64_t u = ub + a * S::THREADS;
+            ok[a] = u < 6 * n1;
+            const int64_t uc = ok[a] ? u : ub;
+            const int64_t i = uc / 6;
+            row[a] = (int)(uc % 6);
+            ul[a] = uc - 6 * nb;
+            qs[a] = p_ptr[i];
+            qe[a] = ok[a] ? p_ptr[i + 1] : qs[a];
 #pragma unroll
-            for (int c = 0; c < 6; ++c)
+            for (int h = 0; h < NR; ++h) acc[a][h] = ch * (b[ul[a] * NR + h] + t[ul[a] * NR + h]);
+        }
+        const int len = qe[0] - qs[0] > qe[1] - qs[1] ? qe[0] - qs[0] : qe[1] - qs[1];
+        for (int m = 0; m < len; ++m) {
 #pragma unroll
-                for (int h = 0; h < NR; ++h) acc[h] += (float)pb[c] * xc[(6 * J + c) * NR + h];
+            for (int a = 0; a < 2; ++a) {
+                if (qs[a] + m >= qe[a]) continue;
+                const int q = qs[a] + m;
+                const int J = p_col[q] - (int)c0;
+                const _Float16* pb = P + (int64_t)q * 36 + row[a] * 6;
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+#pragma unroll
+                    for (int h = 0; h < NR; ++h) acc[a][h] += (float)pb[c] * xc[(6 * J + c) * NR + h];
+            }
         }
 #pragma unroll
-        for (int h = 0; h < NR; ++h) r[ul * NR + h] = acc[h];
+        for (int a = 0; a < 2; ++a)
+            if (ok[a]) {
+#pragma unroll
+                for (int h = 0; h < NR; ++h) r[ul[a] * NR + h] = acc[a][h];
+            }
     }
     __syncthreads();
     // 5. post-smoothing: r0 = b - A z1 into t, then x
