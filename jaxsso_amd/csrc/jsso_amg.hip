@@ -1250,6 +1250,29 @@ __device__ __forceinline__ void flush_rows(VT* stage, const int4& ma, F&& value,
     __syncthreads();
 }
 
+// M such runs (one per vector) through one staging round: the stores of M
+// flush_rows calls with one pair of barriers.  `stage` holds M * 64 * 6 * NR values.
+template <typename VT, int NR, int WPS, int M, typename F>
+__device__ __forceinline__ void flush_rows_m(VT* stage, const int4& ma, F&& value, VT* const (&dst)[M]) {
+    constexpr int S = 64 * 6 * NR;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (wave == 0) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int h = 0; h < NR; ++h)
+#pragma unroll
+                for (int m = 0; m < M; ++m) stage[m * S + lane * 6 * NR + c * NR + h] = value(m, c, h);
+    }
+    __syncthreads();
+    const int64_t base = (int64_t)ma.y * 6 * NR;
+    const int n = (ma.z - ma.y) * 6 * NR;
+    for (int u = threadIdx.x; u < n; u += 64 * WPS)
+#pragma unroll
+        for (int m = 0; m < M; ++m) dst[m][base + u] = stage[m * S + u];
+    __syncthreads();
+}
+
 // Direction and product of iteration k: beta from the partials of k - 1 and
 // k - 2; p_k = z + beta p_{k-1} (own and gathered rows); q = A p_k; partials <p_k, q>.
 template <typename MT, typename VT, int NR, int WPS>
@@ -1303,7 +1326,8 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_dir(Slices sl, Partials P, Sys
     double v[NR];
 #pragma unroll
     for (int h = 0; h < NR; ++h) v[h] = 0.0;
-    // (inactive load cases keep p; their q is never used)
+    // (inactive load cases keep p; their q is never used; one staging round for
+    // both vectors measured slower here)
     flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) {
         const VT pn = pk(ii, c, h);
         if (on && ((amask >> h) & 1)) v[h] += (double)pn * (double)acc[c][h];
@@ -1390,18 +1414,26 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_pre(Slices sl, Partials P, Sys
             for (int h = 0; h < NR; ++h)
                 acc[c][h] = wave == 0 ? (FRESH ? ro[c][h] : ro[c][h] - alpha[h] * qo[c][h]) : VT(0);
     });
-    __shared__ VT stage[64 * 6 * NR];
-    flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) { return rk(ii, c, h) - acc[c][h] / th; }, t0);
-    if (!FRESH) {
-        // (inactive load cases keep r and x)
-        flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) {
-            return ((amask >> h) & 1) ? rk(ii, c, h) : rprev[(6 * ii + c) * NR + h];
-        }, rcur);
-        flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) {
-            const int64_t o = (6 * ii + c) * NR + h;
-            const VT xv = PF ? xo[c][PF ? h : 0] : xlo[o];
-            return ((amask >> h) & 1) ? xv + alpha[h] * (PF ? pko[c][PF ? h : 0] : p[o]) : xv;
-        }, xlo);
+    // t0, r and x through one staging round when that stays small (<= 12 KB)
+    constexpr bool JOIN = !FRESH && 3 * 64 * 6 * NR * sizeof(VT) <= 12288;
+    __shared__ VT stage[(JOIN ? 3 : 1) * 64 * 6 * NR];
+    // (inactive load cases keep r and x)
+    auto val = [&](int m, int c, int h) {
+        const int64_t o = (6 * ii + c) * NR + h;
+        if (m == 0) return rk(ii, c, h) - acc[c][h] / th;
+        if (m == 1) return ((amask >> h) & 1) ? rk(ii, c, h) : rprev[o];
+        const VT xv = PF ? xo[c][PF ? h : 0] : xlo[o];
+        return ((amask >> h) & 1) ? xv + alpha[h] * (PF ? pko[c][PF ? h : 0] : p[o]) : xv;
+    };
+    if constexpr (JOIN) {
+        VT* const dst[3] = {t0, rcur, xlo};
+        flush_rows_m<VT, NR, WPS, 3>(stage, ma, val, dst);
+    } else {
+        flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) { return val(0, c, h); }, t0);
+        if (!FRESH) {
+            flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) { return val(1, c, h); }, rcur);
+            flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) { return val(2, c, h); }, xlo);
+        }
     }
 }
 
