@@ -1834,15 +1834,11 @@ __global__ __launch_bounds__(TailShape<NR>::THREADS) void k_mg_tail(
     float* t = LV ? vec[1] : tg;
     float* r = LV ? vec[2] : rg;
     float* x = LV ? vec[3] : xg;
-    if constexpr (LV) {
-        for (int64_t u = tid; u < 6 * (n1 - n0) * NR; u += S::THREADS) vec[0][u] = bg[6 * n0 * NR + u];
-        __syncthreads();
-    }
     const int64_t c0 = term[4 * s + 1];
     const int N = 6 * term[4 * s + 2];
     const float* Ai = inv + term[4 * s + 3];
     // the dense inverse does not depend on anything computed here: its loads are
-    // issued now and arrive while the smoothing runs
+    // issued first and arrive while b is staged and the smoothing runs
     const int dr = tid % S::ROWS, dk = tid / S::ROWS;
     const bool dense_lane = dk < S::CHUNKS && dr < N;
     float ainv[S::PREFETCH > 0 ? S::PREFETCH : 1];
@@ -1850,6 +1846,10 @@ __global__ __launch_bounds__(TailShape<NR>::THREADS) void k_mg_tail(
     for (int m = 0; m < S::PREFETCH; ++m) {
         const int c = dk + m * S::CHUNKS;
         ainv[m] = dense_lane && c < N ? Ai[(int64_t)c * N + dr] : 0.f;
+    }
+    if constexpr (LV) {
+        for (int64_t u = tid; u < 6 * (n1 - n0) * NR; u += S::THREADS) vec[0][u] = bg[6 * n0 * NR + u];
+        __syncthreads();
     }
     // 1. pre-smoothing: t, then the residual r of x = c (b + t)
     tail_smooth<NR>(sl, A, q0, q1, smooth_coef(SM_PRE1, th, ch), nb, b, b, nullptr, t);
