@@ -61,7 +61,9 @@ enum {
 enum {
     JSSO_PREC_FP64 = 0,  /* FP64 matrix and vectors, batched, reliable FP64 residual */
     JSSO_PREC_MIXED = 1, /* FP32 matrix + FP32 CG recurrences, FP64 reliable updates  */
-    JSSO_PREC_PLAIN = 2  /* baseline: FP64 block-Jacobi PCG, one structure per launch */
+    JSSO_PREC_PLAIN = 2, /* baseline: FP64 block-Jacobi PCG, one structure per launch */
+    JSSO_PREC_MIXED_V64 = 3 /* FP32 matrix in the CG product, FP64 vectors and CG
+                               recurrences, FP64 reliable updates                     */
 };
 
 /* preconditioner of the batched precisions (jsso_plan_set_preconditioner) */
@@ -80,6 +82,12 @@ typedef struct {
     int32_t waves_per_slice; /* batched CG kernel variant used (1, 2 or 4 wavefronts
                                 per 64-block-row slice); 0 on the plain path      */
     int32_t levels;          /* AMG levels (JSSO_PC_AMG), 0 otherwise                 */
+    int32_t dir_launches;    /* AMG: launches of the direction/product kernel, incl.
+                                the chunk queued ahead of the convergence test       */
+    int32_t full_iterations; /* AMG: iterations run while every system was running   */
+    int32_t at_floor;        /* AMG: systems (counted in `converged`) that stopped at
+                                the attainable accuracy of the FP64 residual,
+                                ~eps (||A|| ||x|| + ||b||) / ||b||, above tol      */
 } jsso_info;
 
 /* plan size indices for jsso_plan_sizes */
@@ -114,6 +122,29 @@ int jsso_element_ke_beam(int32_t n, const double* d_crds, const double* d_props,
 int jsso_element_ke_quad(int32_t n, const double* d_crds, const double* d_props,
                          double* d_ke, void* stream);
 
+/* Element-level parts of the reference's element API, one element per thread,
+ * row-major outputs; every output may be NULL.
+ * Beams (d_crds n*6, d_props n*6 as above): d_T n*144 <- BeamCol.T (element.py:
+ * 66-105; 4 diagonal copies of the direction cosines), d_klocal n*144 <-
+ * BeamCol.K_local (element.py:107-128). */
+int jsso_element_beam_local(int32_t n, const double* d_crds, const double* d_props,
+                            double* d_T, double* d_klocal, void* stream);
+/* Quads (d_crds n*12, d_props n*5 as above): d_xy n*8 local in-plane coordinates
+ * [x1,y1,..,x4,y4] (node 3 the origin) and d_normals n*9 (the element normal and
+ * the normals of triangles 1-2-3 and 1-3-4) <- Quad.loc_crds (element.py:488-539);
+ * d_T n*576 <- Quad.T (element.py:643-696); d_klocal n*576 (k_m + k_b before the
+ * rotation) <- Quad.element_K_quad_local (element.py:1086-1095). */
+int jsso_element_quad_local(int32_t n, const double* d_crds, const double* d_props,
+                            double* d_xy, double* d_normals, double* d_T, double* d_klocal,
+                            void* stream);
+/* Quads at natural coordinates (r, s), from d_xy (n*8, as jsso_element_quad_local
+ * writes): d_J n*4 <- Quad.J, d_Bk n*36 <- Quad.B_kappa, d_Bg n*24 <-
+ * Quad.B_gamma_MITC4, d_Bm n*24 <- Quad.B_m (element.py:698-818); from d_props
+ * (n*5): d_Cb n*9, d_Cs n*4, d_Cm n*9 <- Quad.Cb / Cs / Cm (element.py:820-875). */
+int jsso_element_quad_gauss(int32_t n, const double* d_xy, const double* d_props, double r,
+                            double s, double* d_J, double* d_Bk, double* d_Bg, double* d_Bm,
+                            double* d_Cb, double* d_Cs, double* d_Cm, void* stream);
+
 /* Numeric assembly of K from nodal coordinates d_crds (n_nodes*3) and element
  * properties (d_prop_beam n_beam*6, d_prop_quad n_quad*5; NULL when absent).
  * Builds the plan's FP64 K, and the constrained, block-Jacobi-scaled FP64 and
@@ -143,6 +174,33 @@ int jsso_adjoint_grad(jsso_plan* plan, const double* d_crds, const double* d_pro
 /* y = K x with the assembled, unconstrained K (nrhs = 1). */
 int jsso_spmv(jsso_plan* plan, const double* d_x, double* d_y, void* stream);
 
+/* y = K x with K given by its block values d_K (36*nnzb, block-CSR order of the
+ * plan's pattern, unconstrained, as jsso_assemble's d_K_out), so that a K kept
+ * by the caller is applied as itself whatever the plan assembled since; d_K NULL
+ * uses the plan's current assembly (nrhs = 1).
+ *   <- BCOO K @ x / K_aug @ u_aug             JaxSSO/assemblemodel.py:111-163,196-213 */
+int jsso_spmv_k(jsso_plan* plan, const double* d_K, const double* d_x, double* d_y, void* stream);
+
+/* Dense direct solve: the constrained K (rows/columns of constrained DOFs replaced
+ * by identity, u = 0 there) factorised in FP64 on the device -- by a blocked
+ * Cholesky, or, when the plan's last assembly holds members parallel to global Y
+ * (K not symmetric, element.py:92-94), by a blocked LU with partial pivoting --
+ * then triangular solves.  transpose != 0 solves Kc^T u = f (the adjoint system).
+ * d_K as for jsso_spmv_k (NULL = the plan's current assembly); d_f, d_u:
+ * (6*n_nodes) x nrhs, nrhs <= 64.  Needs 8 N^2 bytes of device memory (N =
+ * 6*n_nodes rounded up to a multiple of 64).  Returns JSSO_ESINGULAR for a
+ * stiffness that is not positive definite (Cholesky) or singular (LU).
+ *   <- jax_dense_solve (dense LU of K_aug)      JaxSSO/solver.py:75-93
+ *      and, for non-symmetric K, SuperLU on K_aug / K_aug^T  JaxSSO/solver.py:195-209,236
+ * jsso_solve routes a plan with such members to the LU path itself, whatever
+ * the precision asked for (info->iterations = 0). */
+int jsso_dense_solve(jsso_plan* plan, const double* d_K, const double* d_f, double* d_u, int32_t nrhs,
+                     int32_t transpose, void* stream);
+/* *h_symmetric = 0 when the plan's last assembly holds members parallel to global
+ * Y (its K is not symmetric, so an adjoint needs the transposed solve), else 1.
+ * Synchronises the stream. */
+int jsso_plan_symmetric(jsso_plan* plan, int32_t* h_symmetric, void* stream);
+
 /* Compliance-type reduction per structure and load case:
  *     d_C[s][h] = scale * sum over the DOFs of structure s of f[dof][h] u[dof][h]
  * (d_C: n_sys x nrhs; scale 1/2 gives the strain-energy objective C = 1/2 f.u of
@@ -150,6 +208,15 @@ int jsso_spmv(jsso_plan* plan, const double* d_x, double* d_y, void* stream);
  * partial sums added in a fixed order, no atomics. */
 int jsso_compliance(jsso_plan* plan, const double* d_f, const double* d_u, int32_t nrhs,
                     double scale, double* d_C, void* stream);
+
+/* y = B x (+ c) for a sparse B in CSR on the device (n_rows rows, int32 row
+ * pointers and columns, FP64 values; d_c may be NULL): the design-parameter maps
+ * of the reference's examples, e.g. the hat filter z = B_ij @ p and, with B^T
+ * passed in CSR, its gradient map sens @ B_ij
+ *   <- Examples/Shells_Mannheim_Multihalle_Shape.ipynb:375-378,454-461 (the
+ *      reference builds and applies B_ij as a dense numpy matrix) */
+int jsso_csr_spmv(int32_t n_rows, const int32_t* d_rowptr, const int32_t* d_colind, const double* d_vals,
+                  const double* d_x, const double* d_c, double* d_y, void* stream);
 
 /* Tuning of the batched CG kernel (no counterpart in the reference: the
  * reference hands the solve to SuperLU).  The first batched solve on a plan for a
@@ -174,11 +241,26 @@ int jsso_plan_amg_levels(jsso_plan* plan, int32_t* h_nodes, int32_t cap, int32_t
  * of the batched solver's dominant kernel: with block-Jacobi the fused CG
  * iteration that applies the scaled stiffness (a captured chunk of iterations
  * counts as that many launches); with AMG the direction and CG-product kernel
- * (k_pcg_dir), sampled only where it is launched eagerly (JSSO_CG_GRAPH=0).  every > 0
- * brackets one such launch in `every` with events on the solver's stream; 0 disables.  jsso_timing_read returns the mean duration
+ * (k_pcg_dir) of one iteration chunk in `every` (that chunk is launched eagerly).
+ * With block-Jacobi, every > 0 brackets one eager launch in `every` (a captured
+ * chunk counts as its launches) with events on the solver's stream; 0 disables.  jsso_timing_read returns the mean duration
  * (ms) and the number of sampled launches, optionally resetting the counters. */
 int jsso_timing_enable(int32_t every);
 int jsso_timing_read(double* h_avg_ms, int64_t* h_samples, int32_t reset);
+
+/* Timing slots of jsso_timing_read_slot (no counterpart in the reference).
+ * SOLVE is what jsso_timing_read returns: the block-Jacobi k_cgf launches as
+ * above, or with AMG the k_pcg_dir launches of one iteration chunk in `every`,
+ * which then runs with eager launches (the other chunks keep their graphs).
+ * The phase slots bracket every call while timing is on: ELEMENT the element
+ * stiffness kernels of jsso_assemble, GATHER its gather/scale kernels (diagonal
+ * factors and the solver's ELL copies), ADJOINT the kernels of
+ * jsso_adjoint_grad, SETUP the AMG numeric set-up after an assembly. */
+enum {
+    JSSO_TIMING_SOLVE = 0, JSSO_TIMING_ELEMENT = 1, JSSO_TIMING_GATHER = 2,
+    JSSO_TIMING_ADJOINT = 3, JSSO_TIMING_SETUP = 4, JSSO_TIMING_SLOTS = 5
+};
+int jsso_timing_read_slot(int32_t slot, double* h_avg_ms, int64_t* h_samples, int32_t reset);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,7 @@ class SSO_model:
         self.objective_args = None
         self._strain_energy = False
         self.last_u = None
+        self._node_filter = None
 
     # -- parameters (SSO_model.py:125-204) --------------------------------
     def add_nodeparameter(self, nodeparameter):
@@ -145,10 +146,41 @@ class SSO_model:
             pq = pq.index_put((ix[6], ix[7]), v[ix[8]])
         return pb, pq
 
+    def set_node_filter(self, B):
+        """Filter the node parameters on the GPU before they reach the model, as
+        the reference's shape-optimisation example does outside the library
+        (``z = (B_ij @ p)[design_nodes]``, gradient ``(sens @ B_ij[design_nodes])[design_nodes]``,
+        Examples/Shells_Mannheim_Multihalle_Shape.ipynb:454-461).
+
+        ``B`` (n_node x n_node, scipy sparse or dense) acts on the full vector of
+        node coordinates along the parameters' axis, with the parameters'
+        current values at their nodes; every node parameter must use the same
+        axis.  Parameter values then mean the unfiltered design variables x, and
+        ``value_grad_params`` returns dObjective/dx.  ``None`` removes the filter."""
+        if B is None:
+            self._node_filter = None
+            return
+        from .parameterization import DeviceLinearMap
+        import scipy.sparse as sp
+        axes = set(int(a) for a in self.nodeparameters_xyzs)
+        if len(axes) != 1:
+            raise ValueError("a node filter needs all node parameters on one axis")
+        axis = axes.pop()
+        tags = np.asarray(self.nodeparameters_tags, dtype=np.int64)
+        Bs = sp.csr_matrix(B, dtype=np.float64)
+        n = Bs.shape[0]
+        others = np.setdiff1d(np.arange(n), tags)
+        base = np.asarray(self.model.crds, dtype=float)[:, axis]
+        Bt = Bs[tags]
+        self._node_filter = DeviceLinearMap(Bt[:, tags], Bt[:, others] @ base[others])
+
     def _inputs(self, parameter_values):
         p = _to_dev(parameter_values)
         if self.n_node_params > 0:
-            crds = self.node_params_crds(p[:self.n_node_params])
+            v = p[:self.n_node_params]
+            if self._node_filter is not None:
+                v = self._node_filter(v)
+            crds = self.node_params_crds(v)
         else:
             crds = torch.as_tensor(self.model.crds, dtype=torch.float64, device=p.device)
         if self.n_ele_params > 0:
@@ -160,9 +192,15 @@ class SSO_model:
         return crds, pb, pq, f
 
     def _opts(self, which_solver, enforce_scipy_sparse):
+        """Solver choice as ``Model.select_solver`` makes it (model.py:340-356):
+        'dense' -> the dense direct solve; 'sparse' -> the GPU CG solver whatever
+        ``enforce_scipy_sparse`` says (both of the reference's sparse backends map
+        to it)."""
         m = self.model
+        if which_solver not in ("dense", "sparse"):
+            raise ValueError("Please select the right solver: dense or sparse")
         return ag.SolveOptions(precision=getattr(m, "precision", "fp64"), tol=getattr(m, "tol", 1e-10),
-                               maxit=getattr(m, "maxit", 200000))
+                               maxit=getattr(m, "maxit", 200000), method=which_solver)
 
     # -- parameters -> results (SSO_model.py:224-270) ----------------------
     def params_u(self, parameter_values, which_solver="sparse", enforce_scipy_sparse=True):

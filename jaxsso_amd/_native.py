@@ -15,13 +15,16 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libjsso_hip.so")
 
 JSSO_OK = 0
 JSSO_NOT_CONVERGED = 1
-PREC = {"fp64": 0, "mixed": 1, "plain": 2}
+PREC = {"fp64": 0, "mixed": 1, "plain": 2, "mixed64": 3}
 
 EXPORTS = ("jsso_version", "jsso_last_error", "jsso_plan_create", "jsso_plan_destroy", "jsso_plan_sizes",
            "jsso_plan_pattern", "jsso_element_ke_beam", "jsso_element_ke_quad", "jsso_assemble",
            "jsso_solve", "jsso_adjoint_grad", "jsso_spmv", "jsso_compliance", "jsso_plan_cg_tuning",
            "jsso_timing_enable", "jsso_timing_read", "jsso_plan_set_preconditioner",
-           "jsso_plan_amg_levels")
+           "jsso_plan_amg_levels", "jsso_timing_read_slot", "jsso_spmv_k", "jsso_dense_solve",
+           "jsso_element_beam_local", "jsso_element_quad_local", "jsso_element_quad_gauss",
+           "jsso_plan_symmetric", "jsso_csr_spmv")
+TIMING_SLOTS = {"solve": 0, "element": 1, "gather": 2, "adjoint": 3, "setup": 4}
 
 
 class NativeUnavailable(RuntimeError):
@@ -38,7 +41,8 @@ class Info(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("replacements", ctypes.c_int32),
                 ("converged", ctypes.c_int32), ("n_systems", ctypes.c_int32),
                 ("rel_residual", ctypes.c_double), ("waves_per_slice", ctypes.c_int32),
-                ("levels", ctypes.c_int32)]
+                ("levels", ctypes.c_int32), ("dir_launches", ctypes.c_int32),
+                ("full_iterations", ctypes.c_int32), ("at_floor", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -73,6 +77,14 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.jsso_plan_cg_tuning.argtypes = [P, I32, I32, I32, P]
     lib.jsso_timing_enable.argtypes = [I32]
     lib.jsso_timing_read.argtypes = [P, P, I32]
+    lib.jsso_timing_read_slot.argtypes = [I32, P, P, I32]
+    lib.jsso_spmv_k.argtypes = [P, P, P, P, V]
+    lib.jsso_dense_solve.argtypes = [P, P, P, P, I32, I32, V]
+    lib.jsso_plan_symmetric.argtypes = [P, P, V]
+    lib.jsso_csr_spmv.argtypes = [I32, P, P, P, P, P, P, V]
+    lib.jsso_element_beam_local.argtypes = [I32, P, P, P, P, V]
+    lib.jsso_element_quad_local.argtypes = [I32, P, P, P, P, P, P, V]
+    lib.jsso_element_quad_gauss.argtypes = [I32, P, P, D, D, P, P, P, P, P, P, P, V]
     lib.jsso_plan_set_preconditioner.argtypes = [P, I32]
     lib.jsso_plan_amg_levels.argtypes = [P, P, I32, P]
     for name in EXPORTS:

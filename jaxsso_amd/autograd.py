@@ -17,10 +17,25 @@ from .plan import DEFAULT_MAXIT, DEFAULT_TOL, FEPlan
 
 
 class SolveOptions:
-    def __init__(self, precision="fp64", tol=DEFAULT_TOL, maxit=DEFAULT_MAXIT):
+    """How the state solves run: ``method`` "sparse" (the GPU CG solver, with
+    ``precision`` / ``tol`` / ``maxit``) or "dense" (the dense FP64 Cholesky of
+    ``jsso_dense_solve``)."""
+
+    def __init__(self, precision="fp64", tol=DEFAULT_TOL, maxit=DEFAULT_MAXIT, method="sparse"):
+        if method not in ("sparse", "dense"):
+            raise ValueError(f"unknown solve method {method!r} (sparse or dense)")
         self.precision = precision
         self.tol = tol
         self.maxit = maxit
+        self.method = method
+
+
+def _solve(plan, f, opts, transpose=False):
+    if opts.method == "dense":
+        return plan.dense_solve(f, transpose=transpose)
+    if transpose:
+        return plan.solve_transpose(f, precision=opts.precision, tol=opts.tol, maxit=opts.maxit)
+    return plan.solve(f, precision=opts.precision, tol=opts.tol, maxit=opts.maxit)
 
 
 def _grads(plan, crds, pb, pq, u, lam, scale, need, weight=None):
@@ -42,7 +57,7 @@ class _Displacement(torch.autograd.Function):
     @staticmethod
     def forward(ctx, crds, prop_beam, prop_quad, f, plan: FEPlan, opts: SolveOptions):
         plan.assemble(crds, prop_beam, prop_quad)
-        u = plan.solve(f, precision=opts.precision, tol=opts.tol, maxit=opts.maxit)
+        u = _solve(plan, f, opts)
         ctx.plan, ctx.opts = plan, opts
         ctx.save_for_backward(crds, prop_beam, prop_quad, u)
         return u
@@ -51,8 +66,8 @@ class _Displacement(torch.autograd.Function):
     def backward(ctx, u_bar):
         crds, pb, pq, u = ctx.saved_tensors
         plan, opts = ctx.plan, ctx.opts
-        plan.assemble(crds, pb, pq)  # the plan may have been re-assembled since forward
-        lam = plan.solve(u_bar.contiguous(), precision=opts.precision, tol=opts.tol, maxit=opts.maxit)
+        plan.ensure_assembled(crds, pb, pq)  # the plan may have been re-assembled since forward
+        lam = _solve(plan, u_bar.contiguous(), opts, transpose=True)
         need = ctx.needs_input_grad
         dc, db, dq = _grads(plan, crds, pb, pq, u, lam, 0.0, need)
         f_bar = lam if need[3] else None
@@ -63,20 +78,28 @@ class _Compliance(torch.autograd.Function):
     @staticmethod
     def forward(ctx, crds, prop_beam, prop_quad, f, plan: FEPlan, opts: SolveOptions):
         plan.assemble(crds, prop_beam, prop_quad)
-        u = plan.solve(f, precision=opts.precision, tol=opts.tol, maxit=opts.maxit)
-        ctx.plan = plan
-        ctx.save_for_backward(crds, prop_beam, prop_quad, u)
+        u = _solve(plan, f, opts)
+        ctx.plan, ctx.opts = plan, opts
+        ctx.save_for_backward(crds, prop_beam, prop_quad, u, f)
         C = 0.5 * (f * u).sum(0) if f.dim() == 2 else 0.5 * torch.dot(f, u)
         ctx.mark_non_differentiable(u)
         return C, u
 
     @staticmethod
     def backward(ctx, C_bar, _u_bar):
-        crds, pb, pq, u = ctx.saved_tensors
+        crds, pb, pq, u, f = ctx.saved_tensors
         plan = ctx.plan
         need = ctx.needs_input_grad
-        dc, db, dq = _grads(plan, crds, pb, pq, u, None, 0.5, need, weight=C_bar)
-        f_bar = (u * C_bar) if need[3] else None   # dC/df = K^-1 f = u
+        plan.ensure_assembled(crds, pb, pq)  # the plan may have been re-assembled since forward
+        if plan.symmetric():
+            dc, db, dq = _grads(plan, crds, pb, pq, u, None, 0.5, need, weight=C_bar)
+            f_bar = (u * C_bar) if need[3] else None   # dC/df = K^-1 f = u
+        else:
+            # K not symmetric (members parallel to global Y): lam = K^-T f/2, as the
+            # reference's custom VJP solves with K_aug^T (solver.py:236)
+            lam = _solve(plan, 0.5 * f, ctx.opts, transpose=True)
+            dc, db, dq = _grads(plan, crds, pb, pq, u, lam, 0.0, need, weight=C_bar)
+            f_bar = ((0.5 * u + lam) * C_bar) if need[3] else None   # dC/df = (K^-1 + K^-T) f / 2
         return dc, db, dq, f_bar, None, None
 
 

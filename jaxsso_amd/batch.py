@@ -46,7 +46,8 @@ class StructureBatch:
         m = n_cases or max(c.shape[0] for c in cases)
         F = np.zeros((6 * int(self.node_off[-1]), m))
         for c, o in zip(cases, self.node_off[:-1]):
-            F[6 * o:6 * o + c.shape[1], :c.shape[0]] = c[:m].T
+            k = min(c.shape[0], m)
+            F[6 * o:6 * o + c.shape[1], :k] = c[:k].T
         self.n_cases = m
         self.f = torch.as_tensor(F, device=dev).contiguous()
 
@@ -74,6 +75,10 @@ class StructureBatch:
         """Compliance gradients per structure and case: (dcrds, dprop_beam, dprop_quad),
         packed like the batch ([nodes|beams|quads] x [3|6|5] x cases)."""
         u = self.u if u is None else u
+        if not self.plan.symmetric():
+            # members parallel to global Y: K is not symmetric, lam = K^-T f/2
+            lam = self.plan.solve_transpose(0.5 * self.f.reshape(u.shape))
+            return self.plan.adjoint(self.crds, self.prop_beam, self.prop_quad, u, lam=lam, lam_scale=0.0)
         return self.plan.adjoint(self.crds, self.prop_beam, self.prop_quad, u, lam=None, lam_scale=0.5)
 
     def compliance_and_gradient(self, precision="fp64", tol=DEFAULT_TOL, maxit=DEFAULT_MAXIT, preconditioner=None):

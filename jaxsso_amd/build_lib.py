@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT, "libjsso_hip.so")
-SOURCES = ["jsso_api.hip", "jsso_element.hip", "jsso_assemble.hip", "jsso_solve.hip", "jsso_amg.hip"]
+SOURCES = ["jsso_api.hip", "jsso_element.hip", "jsso_assemble.hip", "jsso_solve.hip", "jsso_amg.hip",
+           "jsso_dense.hip", "jsso_param.hip"]
 ARCH = os.environ.get("JSSO_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -45,7 +46,7 @@ def build(verbose: bool = False) -> str:
             print(r.stderr, file=sys.stderr)
         return obj
 
-    with ThreadPoolExecutor(max_workers=min(5, len(SOURCES))) as ex:
+    with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-o", tmp, *objs]

@@ -185,8 +185,17 @@ struct QuadGP {
 };
 
 template <typename T>
+JDEV void quad_gp_rs(const T xy[8], double r, double s, QuadGP<T>& G, T Jm[2][2] = nullptr);
+
+template <typename T>
 JDEV void quad_gp(const T xy[8], int g, QuadGP<T>& G) {
-    const double r = gauss_r(g), s = gauss_s(g);
+    quad_gp_rs<T>(xy, gauss_r(g), gauss_s(g), G);
+}
+
+// The same at any natural coordinates (r, s); Jm, if given, receives the 2x2
+// Jacobian of Quad.J (element.py:698-709).
+template <typename T>
+JDEV void quad_gp_rs(const T xy[8], double r, double s, QuadGP<T>& G, T Jm[2][2]) {
     G.r = r;
     G.s = s;
     const T x1 = xy[0], y1 = xy[1], x2 = xy[2], y2 = xy[3], x3 = xy[4], y3 = xy[5],
@@ -197,6 +206,12 @@ JDEV void quad_gp(const T xy[8], int g, QuadGP<T>& G) {
     T J11 = 0.25 * (y1 * (r + 1) - y2 * (r - 1) + y3 * (r - 1) - y4 * (r + 1));
     T det = J00 * J11 - J01 * J10;
     G.detJ = det;
+    if (Jm) {
+        Jm[0][0] = J00;
+        Jm[0][1] = J01;
+        Jm[1][0] = J10;
+        Jm[1][1] = J11;
+    }
     const double dN0[4] = {0.25 * (1 + s), 0.25 * (-1 - s), 0.25 * (-1 + s), 0.25 * (1 - s)};
     const double dN1[4] = {0.25 * (1 + r), 0.25 * (1 - r), 0.25 * (-1 + r), 0.25 * (-1 - r)};
 #pragma unroll

@@ -51,6 +51,10 @@ constexpr double AMG_THETA = 0.62;        // level-0 smoothing step z = r / (AMG
 constexpr double AMG_CHEB_HI = 1.1;
 constexpr double AMG_CHEB_LO = 0.1;
 constexpr int AMG_STALL_WINDOW = 256;     // iterations without a 10% gain that end a solve (floor)
+// attainable-accuracy floor (k_decide): within 4 x eps (rho ||x|| + ||b||) / ||b||
+// and no 10% gain for 24 iterations (three replacement chunks)
+constexpr double AMG_FLOOR_FACTOR = 4.0;
+constexpr int AMG_FLOOR_WINDOW = 24;
 constexpr double AMG_RHO_SAFETY = 1.12;   // the power iteration approaches rho from below
 constexpr int AMG_MAXLEVELS = 12;
 
@@ -145,7 +149,7 @@ struct AmgState {
     size_t z_bytes = 0;
     uint64_t gen = 1;                 // bumped whenever a buffer the graphs point at changes
     std::vector<void*> allocs;
-    CgGraph graph[2][9][2];
+    CgGraph graph[3][9][2];       // per precision slot (prec_slot) x load cases x [iterations | replacement]
 };
 
 // Greedy aggregation of one structure's nodes [n0, n1) of a level.  `active`
@@ -2412,22 +2416,28 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
     }
 }
 
-// one ordinary PCG iteration k; `timed` brackets the direction and CG-product
-// kernel (the iteration's largest) with the sampled events of
-// jsso_timing_enable (eager launches only)
+// one ordinary PCG iteration k; `timed` (eager launches only) brackets the
+// direction and CG-product kernel (the iteration's largest) with an event pair
+// appended to `timed`, settled by the caller (jsso_timing_enable)
 template <typename MT, typename VT, int NR>
 static void amg_step(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs<VT>& bf, const MT* A0, int k,
-                     hipStream_t s, bool timed = false) {
+                     hipStream_t s, std::vector<std::pair<hipEvent_t, hipEvent_t>>* timed = nullptr) {
     const Slices sl = level_slices(pl, A, 0);
     const dim3 g = slice_grid(pl->n_wg);
     const _Float16* A16 = (const _Float16*)pl->d_Ke16;
     wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
         constexpr int WP = decltype(W)::value;
-        std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
-        if (timed) timing_begin(s, ev);
+        std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+        if (timed) {
+            ev = timing_pair();
+            if (ev.first) (void)hipEventRecord(ev.first, s);
+        }
         hipLaunchKernelGGL((k_pcg_dir<MT, VT, NR, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A0,
                            (const VT*)bf.z, (const VT*)bf.p[(k - 1) & 1], bf.p[k & 1], bf.q);
-        if (timed) timing_end(s, ev);
+        if (ev.first) {
+            (void)hipEventRecord(ev.second, s);
+            timed->push_back(ev);
+        }
         hipLaunchKernelGGL((k_pcg_pre<VT, NR, false, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A16,
                            (const float*)A.dev[0].d_theta, (const VT*)bf.r[(k - 1) & 1], bf.r[k & 1],
                            (const VT*)bf.q, (const VT*)bf.p[k & 1], bf.xlo, bf.t0);
@@ -2454,7 +2464,8 @@ static void amg_replace(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs
     constexpr int RW = WpsFits<double, NR, 4>::value ? 4 : 1;
     hipLaunchKernelGGL((k_replace<VT, NR, RW>), g, dim3(64 * RW), 0, s, sl, w.P, w.st, (const double*)pl->d_Ke64,
                        (const double*)w.b64, (const double*)w.x64, bf.r[k & 1]);
-    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, steps, AMG_STALL_WINDOW);
+    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, steps,
+                       AMG_STALL_WINDOW, (const double*)A.dev[0].d_rho, AMG_FLOOR_FACTOR, AMG_FLOOR_WINDOW);
     hipLaunchKernelGGL((k_save<NR>), g, dim3(SOLVER_WG), 0, s, sl, (const int32_t*)w.st.save, (const double*)w.x64,
                        w.xbest);
     wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
@@ -2493,7 +2504,10 @@ static int amg_prepare(Plan* pl, hipStream_t s) {
     }
     if (!pl->amg_valid) {
         JSSO_REQUIRE(pl->d_crds, JSSO_EINVAL, "AMG needs the node coordinates of an assembly");
+        std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
+        timing_begin_slot(s, ev, JSSO_TIMING_SETUP);
         if ((rc = amg_setup_numeric(pl, A, s))) return rc;
+        timing_end(s, ev);
         pl->amg_valid = true;
     }
     return JSSO_OK;
@@ -2527,7 +2541,8 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
     JSSO_HIP(hipMemsetAsync(w.P.cg[0], 0, 2 * 4 * (size_t)pl->n_wg * NR * sizeof(double), s));
     hipLaunchKernelGGL((k_init<VT, NR>), gridS, dim3(SOLVER_WG), 0, s, sl, w.P, pl->d_fixed, pl->d_Linv, f, w.b64,
                        w.x64, w.xbest, bf.xlo, bf.r[0], bf.p[0]);
-    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 0, tol, 0, 0);
+    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 0, tol, 0, 0,
+                       (const double*)nullptr, 0.0, 0);
     wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
         constexpr int WP = decltype(W)::value;
         hipLaunchKernelGGL((k_pcg_pre<VT, NR, true, WP>), gridS, dim3(64 * WP), 0, s, sl, w.P, w.st, 0,
@@ -2542,10 +2557,20 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
     const void* key[CgGraph::NKEY] = {A0,        w.r[0],   w.r[1],   w.q[0],         w.p,         A.d_p1,
                                       A.d_z,     A.d_t0,   w.xlo,    w.P.cg[0],      w.st.slice_act, w.b64,
                                       w.x64,     w.xbest,  w.st.active, (const void*)(uintptr_t)A.gen};
-    CgGraph(&gc)[2] = A.graph[sizeof(MT) == 4 ? 0 : 1][NR];
+    CgGraph(&gc)[2] = A.graph[prec_slot<MT, VT>()][NR];
+    // timing samples: the k_pcg_dir launches of sampled chunks, kept per chunk and
+    // counted only for chunks that started with every system still running (all
+    // slices active, so the launch processes the whole plan)
+    std::vector<std::vector<std::pair<hipEvent_t, hipEvent_t>>> samples;
+    int chunk_no = 0;
     auto enqueue_chunk = [&](int it0, int steps) -> int {
         hipGraphExec_t ga = nullptr, gb = nullptr;
-        if (steps == E && E % 2 == 0 && it0 % 2 == 0 && pl->graphs_ok && graphs_enabled()) {
+        // a chunk sampled for timing runs eagerly with its k_pcg_dir launches bracketed
+        const bool sampled = timing_sample_chunk();
+        samples.emplace_back();
+        auto* tv = sampled ? &samples[chunk_no] : nullptr;
+        ++chunk_no;
+        if (!sampled && steps == E && E % 2 == 0 && it0 % 2 == 0 && pl->graphs_ok && graphs_enabled()) {
             ga = capture_graph(pl, gc[0], key, 0, E, tol, [&](hipStream_t cs) {
                 for (int it = 1; it < E; ++it) amg_step<MT, VT, NR>(pl, A, w, bf, A0, it, cs);
             });
@@ -2558,7 +2583,7 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
             JSSO_HIP(hipGraphLaunch(ga, s));
             JSSO_HIP(hipGraphLaunch(gb, s));
         } else {
-            for (int it = it0 + 1; it < it0 + steps; ++it) amg_step<MT, VT, NR>(pl, A, w, bf, A0, it, s, true);
+            for (int it = it0 + 1; it < it0 + steps; ++it) amg_step<MT, VT, NR>(pl, A, w, bf, A0, it, s, tv);
             amg_replace<MT, VT, NR>(pl, A, w, bf, A0, it0 + steps, steps, tol, s);
         }
         JSSO_HIP(hipGetLastError());
@@ -2575,6 +2600,7 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
     // prediction costs one more replacement.
     std::vector<int> chunk_end;
     std::vector<std::pair<int, double>> known;   // (iteration, slowest running system's residual)
+    std::vector<char> all_running_after;         // per chunk read: every system still running after it
     int it = 0, it_done = 0, chunks_done = 0, c = 0;
     for (;;) {
         const bool more = it < maxit;
@@ -2609,12 +2635,15 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
             const int32_t* fl = pl->h_flags + (size_t)(chk & 1) * nsys;
             const double* rl = pl->h_rel + (size_t)(chk & 1) * nsys;
             bool any = false;
+            int running = 0;
             double worst = 0.0;
             for (int k = 0; k < nsys; ++k)
                 if (fl[k] != 0) {
                     any = true;
+                    ++running;
                     worst = std::max(worst, rl[k]);
                 }
+            if ((int)all_running_after.size() == chk) all_running_after.push_back(running == nsys);
             if (!any) {
                 it_done = chunk_end[chk];
                 chunks_done = chk + 1;
@@ -2638,10 +2667,15 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
     JSSO_HIP(hipMemcpyAsync(status.data(), w.st.status, nsys * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     JSSO_HIP(hipStreamSynchronize(s));
     timing_collect();
-    int conv = 0;
+    for (size_t c = 0; c < samples.size(); ++c) {
+        const bool full = c == 0 || (c - 1 < all_running_after.size() && all_running_after[c - 1]);
+        for (auto& e : samples[c]) timing_settle(e, JSSO_TIMING_SOLVE, full);
+    }
+    int conv = 0, floor = 0;
     double mx = 0.0;
     for (int k = 0; k < nsys; ++k) {
-        conv += status[k] == SYS_CONVERGED ? 1 : 0;
+        conv += (status[k] == SYS_CONVERGED || status[k] == SYS_FLOOR) ? 1 : 0;
+        floor += status[k] == SYS_FLOOR ? 1 : 0;
         mx = std::max(mx, rel[k]);
     }
     if (info) {
@@ -2652,6 +2686,11 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
         info->rel_residual = mx;
         info->waves_per_slice = 1;
         info->levels = A.nl;
+        info->dir_launches = it;
+        info->at_floor = floor;
+        int full = 0;
+        while (full < (int)all_running_after.size() && all_running_after[full]) ++full;
+        info->full_iterations = full < (int)chunk_end.size() ? chunk_end[full] : it_done;
     }
     return conv == nsys ? JSSO_OK : JSSO_NOT_CONVERGED;
 }
@@ -2683,11 +2722,12 @@ static int amg_dispatch(Plan* pl, const double* f, double* u, int nrhs, double t
     }
 }
 
-int amg_solve(Plan* pl, const double* f, double* u, int nrhs, bool mixed, double tol, int maxit, jsso_info* info,
+int amg_solve(Plan* pl, const double* f, double* u, int nrhs, int slot, double tol, int maxit, jsso_info* info,
               hipStream_t s) {
     int rc = amg_prepare(pl, s);
     if (rc) return rc;
-    if (mixed) return amg_dispatch<float, float>(pl, f, u, nrhs, tol, maxit, info, s);
+    if (slot == PS_F32) return amg_dispatch<float, float>(pl, f, u, nrhs, tol, maxit, info, s);
+    if (slot == PS_F32_V64) return amg_dispatch<float, double>(pl, f, u, nrhs, tol, maxit, info, s);
     return amg_dispatch<double, double>(pl, f, u, nrhs, tol, maxit, info, s);
 }
 

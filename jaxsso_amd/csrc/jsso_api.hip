@@ -447,6 +447,7 @@ int jsso_plan_destroy(jsso_plan* plan) {
     for (void* a : q->allocs) (void)hipFree(a);
     for (auto& b : q->ws)
         if (b.p) (void)hipFree(b.p);
+    if (q->d_dense) (void)hipFree(q->d_dense);
     delete plan;
     return JSSO_OK;
 }
@@ -494,6 +495,36 @@ int jsso_element_ke_quad(int32_t n, const double* d_crds, const double* d_props,
     return launch_ke_quad(n, d_crds, nullptr, nullptr, d_props, d_ke, (hipStream_t)stream);
 }
 
+int jsso_element_beam_local(int32_t n, const double* d_crds, const double* d_props, double* d_T, double* d_klocal,
+                            void* stream) {
+    if (n < 0 || (n > 0 && (!d_crds || (d_klocal && !d_props)))) {
+        set_error("invalid arguments");
+        return JSSO_EINVAL;
+    }
+    return launch_beam_local(n, d_crds, d_props, d_T, d_klocal, (hipStream_t)stream);
+}
+
+int jsso_element_quad_local(int32_t n, const double* d_crds, const double* d_props, double* d_xy, double* d_normals,
+                            double* d_T, double* d_klocal, void* stream) {
+    if (n < 0 || (n > 0 && (!d_crds || (d_klocal && !d_props)))) {
+        set_error("invalid arguments");
+        return JSSO_EINVAL;
+    }
+    return launch_quad_local(n, d_crds, d_props, d_xy, d_normals, d_T, d_klocal, (hipStream_t)stream);
+}
+
+int jsso_element_quad_gauss(int32_t n, const double* d_xy, const double* d_props, double r, double s, double* d_J,
+                            double* d_Bk, double* d_Bg, double* d_Bm, double* d_Cb, double* d_Cs, double* d_Cm,
+                            void* stream) {
+    const bool geo = d_J || d_Bk || d_Bg || d_Bm, mat = d_Cb || d_Cs || d_Cm;
+    if (n < 0 || (n > 0 && ((geo && !d_xy) || (mat && !d_props)))) {
+        set_error("invalid arguments");
+        return JSSO_EINVAL;
+    }
+    return launch_quad_gauss(n, geo ? d_xy : nullptr, mat ? d_props : nullptr, r, s, d_J, d_Bk, d_Bg, d_Bm, d_Cb,
+                             d_Cs, d_Cm, (hipStream_t)stream);
+}
+
 int jsso_assemble(jsso_plan* plan, const double* d_crds, const double* d_prop_beam, const double* d_prop_quad,
                   double* d_K_out, void* stream) {
     if (!plan || !d_crds) {
@@ -509,12 +540,18 @@ int jsso_assemble(jsso_plan* plan, const double* d_crds, const double* d_prop_be
     int rc0 = ensure_device(p);
     if (rc0) return rc0;
     JSSO_HIP(hipMemsetAsync(p->d_status, 0, sizeof(int32_t), s));
+    p->status_cache = -1;
+    std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
+    timing_begin_slot(s, ev, JSSO_TIMING_ELEMENT);
     int rc = launch_ke_beam(p->n_beam, nullptr, p->d_cnct_beam, d_crds, d_prop_beam, p->d_ke, nullptr, p->d_status, s);
     if (rc) return rc;
     rc = launch_ke_quad(p->n_quad, nullptr, p->d_cnct_quad, d_crds, d_prop_quad, p->d_ke + (size_t)p->n_beam * 4 * 36, s);
     if (rc) return rc;
+    timing_end(s, ev);
+    timing_begin_slot(s, ev, JSSO_TIMING_GATHER);
     rc = launch_assemble(p, s);
     if (rc) return rc;
+    timing_end(s, ev);
     JSSO_HIP(hipMemcpyAsync(p->d_crds, d_crds, (size_t)p->n_nodes * 3 * sizeof(double), hipMemcpyDeviceToDevice, s));
     p->amg_valid = false;
     if (d_K_out && p->nnzb) {
@@ -537,9 +574,30 @@ int jsso_solve(jsso_plan* plan, const double* d_f, double* d_u, int32_t nrhs, in
         return JSSO_EINVAL;
     }
     hipStream_t s = (hipStream_t)stream;
+    if (precision < JSSO_PREC_FP64 || precision > JSSO_PREC_MIXED_V64) {
+        set_error("unknown precision %d", precision);
+        return JSSO_EINVAL;
+    }
+    int rc = ensure_device(p);
+    if (rc) return rc;
+    int32_t status = 0;
+    if ((rc = assembly_status(p, s, &status))) return rc;
+    if (status & 2) {
+        // members parallel to global Y: K is not symmetric (element.py:92-94); the
+        // reference factorises it directly (SuperLU), so does this path (dense LU)
+        if ((rc = launch_full_blocks(p, s))) return rc;
+        if ((rc = dense_solve(p, p->d_K, d_f, d_u, nrhs, true, false, s))) return rc;
+        if (h_info) {
+            *h_info = jsso_info{};
+            h_info->converged = p->n_sys * nrhs;
+            h_info->n_systems = p->n_sys * nrhs;
+        }
+        return JSSO_OK;
+    }
     switch (precision) {
-        case JSSO_PREC_FP64: return solve_batched(p, d_f, d_u, nrhs, false, tol, maxit, h_info, s);
-        case JSSO_PREC_MIXED: return solve_batched(p, d_f, d_u, nrhs, true, tol, maxit, h_info, s);
+        case JSSO_PREC_FP64: return solve_batched(p, d_f, d_u, nrhs, PS_F64, tol, maxit, h_info, s);
+        case JSSO_PREC_MIXED: return solve_batched(p, d_f, d_u, nrhs, PS_F32, tol, maxit, h_info, s);
+        case JSSO_PREC_MIXED_V64: return solve_batched(p, d_f, d_u, nrhs, PS_F32_V64, tol, maxit, h_info, s);
         case JSSO_PREC_PLAIN: return solve_plain(p, d_f, d_u, nrhs, tol, maxit, h_info, s);
         default: set_error("unknown precision %d", precision); return JSSO_EINVAL;
     }
@@ -573,10 +631,14 @@ int jsso_adjoint_grad(jsso_plan* plan, const double* d_crds, const double* d_pro
         JSSO_REQUIRE(e == hipSuccess, JSSO_ENOMEM, "gradient scratch hipMalloc(%zu): %s", need, hipGetErrorString(e));
         g.bytes = need;
     }
+    std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
+    timing_begin_slot(s, ev, JSSO_TIMING_ADJOINT);
     int rc = launch_adjoint(p, d_crds, d_prop_beam, d_prop_quad, d_u, d_lam, lam_scale, nrhs, (double*)g.p,
                             d_dprop_beam, d_dprop_quad, s);
     if (rc) return rc;
-    return launch_gather_grad(p, (const double*)g.p, nrhs, d_dcrds, s);
+    rc = launch_gather_grad(p, (const double*)g.p, nrhs, d_dcrds, s);
+    timing_end(s, ev);
+    return rc;
 }
 
 int jsso_spmv(jsso_plan* plan, const double* d_x, double* d_y, void* stream) {
@@ -594,10 +656,82 @@ int jsso_spmv(jsso_plan* plan, const double* d_x, double* d_y, void* stream) {
     return launch_spmv_full(p, p->d_K, d_x, d_y, (hipStream_t)stream);
 }
 
+int jsso_spmv_k(jsso_plan* plan, const double* d_K, const double* d_x, double* d_y, void* stream) {
+    if (!plan || !d_x || !d_y) {
+        set_error("NULL argument");
+        return JSSO_EINVAL;
+    }
+    Plan* p = P(plan);
+    int rc = ensure_device(p);
+    if (rc) return rc;
+    if (!d_K) {
+        if (!p->assembled) {
+            set_error("jsso_spmv_k without K values before jsso_assemble");
+            return JSSO_EINVAL;
+        }
+        if ((rc = launch_full_blocks(p, (hipStream_t)stream))) return rc;
+        d_K = p->d_K;
+    }
+    return launch_spmv_full(p, d_K, d_x, d_y, (hipStream_t)stream);
+}
+
+int jsso_plan_symmetric(jsso_plan* plan, int32_t* h_symmetric, void* stream) {
+    if (!plan || !h_symmetric) {
+        set_error("NULL argument");
+        return JSSO_EINVAL;
+    }
+    Plan* p = P(plan);
+    if (!p->assembled) {
+        set_error("jsso_plan_symmetric before jsso_assemble");
+        return JSSO_EINVAL;
+    }
+    int32_t status = 0;
+    int rc = assembly_status(p, (hipStream_t)stream, &status);
+    if (rc) return rc;
+    *h_symmetric = (status & 2) ? 0 : 1;
+    return JSSO_OK;
+}
+
+int jsso_dense_solve(jsso_plan* plan, const double* d_K, const double* d_f, double* d_u, int32_t nrhs,
+                     int32_t transpose, void* stream) {
+    if (!plan || !d_f || !d_u || nrhs < 1 || nrhs > 64) {
+        set_error("invalid dense-solve arguments (nrhs 1..64)");
+        return JSSO_EINVAL;
+    }
+    Plan* p = P(plan);
+    int rc = ensure_device(p);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if (!d_K) {
+        if (!p->assembled) {
+            set_error("jsso_dense_solve without K values before jsso_assemble");
+            return JSSO_EINVAL;
+        }
+        if ((rc = launch_full_blocks(p, s))) return rc;
+        d_K = p->d_K;
+    }
+    int32_t status = 0;
+    if (p->assembled && (rc = assembly_status(p, s, &status))) return rc;
+    return dense_solve(p, d_K, d_f, d_u, nrhs, (status & 2) != 0, transpose != 0, s);
+}
+
+int jsso_csr_spmv(int32_t n_rows, const int32_t* d_rowptr, const int32_t* d_colind, const double* d_vals,
+                  const double* d_x, const double* d_c, double* d_y, void* stream) {
+    if (n_rows < 0 || (n_rows > 0 && (!d_rowptr || !d_colind || !d_vals || !d_x || !d_y))) {
+        set_error("invalid CSR product arguments");
+        return JSSO_EINVAL;
+    }
+    return launch_csr_spmv(n_rows, d_rowptr, d_colind, d_vals, d_x, d_c, d_y, (hipStream_t)stream);
+}
+
 int jsso_timing_enable(int32_t every) { return timing_enable(every); }
 
 int jsso_timing_read(double* h_avg_ms, int64_t* h_samples, int32_t reset) {
-    return timing_read(h_avg_ms, h_samples, reset);
+    return timing_read(JSSO_TIMING_SOLVE, h_avg_ms, h_samples, reset);
+}
+
+int jsso_timing_read_slot(int32_t slot, double* h_avg_ms, int64_t* h_samples, int32_t reset) {
+    return timing_read(slot, h_avg_ms, h_samples, reset);
 }
 
 int jsso_compliance(jsso_plan* plan, const double* d_f, const double* d_u, int32_t nrhs, double scale, double* d_C,
@@ -628,12 +762,14 @@ int jsso_plan_amg_levels(jsso_plan* plan, int32_t* h_nodes, int32_t cap, int32_t
     return amg_levels(P(plan), h_nodes, cap, h_levels);
 }
 int jsso_plan_cg_tuning(jsso_plan* plan, int32_t precision, int32_t nrhs, int32_t set_wps, int32_t* h_wps) {
-    if (!plan || (precision != JSSO_PREC_FP64 && precision != JSSO_PREC_MIXED) || nrhs < 1 || nrhs > 8 ||
+    if (!plan || (precision != JSSO_PREC_FP64 && precision != JSSO_PREC_MIXED && precision != JSSO_PREC_MIXED_V64) ||
+        nrhs < 1 || nrhs > 8 ||
         !(set_wps < 0 || set_wps == 0 || set_wps == 1 || set_wps == 2 || set_wps == 4)) {
         set_error("invalid tuning arguments (precision FP64 or MIXED, nrhs 1..8, wps -1, 0, 1, 2 or 4)");
         return JSSO_EINVAL;
     }
-    int32_t& slot = P(plan)->cg_wps[precision == JSSO_PREC_MIXED ? 0 : 1][nrhs];
+    const int ps = precision == JSSO_PREC_MIXED ? PS_F32 : precision == JSSO_PREC_FP64 ? PS_F64 : PS_F32_V64;
+    int32_t& slot = P(plan)->cg_wps[ps][nrhs];
     if (set_wps >= 0) slot = set_wps;
     if (h_wps) *h_wps = slot;
     return JSSO_OK;

@@ -85,7 +85,12 @@ __global__ __launch_bounds__(256) void ke_beam_kernel(int32_t n, const double* _
 // 256-thread workgroup.  Lanes 0-3 stage the per-Gauss-point tiles (det J, dH,
 // shear B) in LDS, lanes 0-7 the drilling candidates; then every lane forms
 // its 6x6 block of T^T (k_m + k_b) T (element.py:1073-1084).
+// MODE: QK_BLOCKS output [e][a][b][6x6] (assembly), QK_GLOBAL row-major 24x24
+// per element, QK_LOCAL row-major 24x24 of k_m + k_b before the rotation
+// (Quad.element_K_quad_local, element.py:1086-1095).
 // ---------------------------------------------------------------------------
+enum { QK_BLOCKS = 0, QK_GLOBAL = 1, QK_LOCAL = 2 };
+
 struct QuadTile {
     double R[9];
     double t, E, nu, kx, ky;
@@ -95,7 +100,7 @@ struct QuadTile {
     double diag[8];
 };
 
-template <bool BLOCKS>
+template <int MODE>
 __global__ __launch_bounds__(256) void ke_quad_kernel(int32_t n, const double* __restrict__ crds_e,
                                                       const int32_t* __restrict__ cnct,
                                                       const double* __restrict__ crds_nodes,
@@ -215,6 +220,13 @@ __global__ __launch_bounds__(256) void ke_quad_kernel(int32_t n, const double* _
         for (int i = 1; i < 8; ++i) m = fmin(m, fabs(T.diag[i]));
         Lb[5][5] = m / 1000.0;
     }
+    if (MODE == QK_LOCAL) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i)
+#pragma unroll
+            for (int j = 0; j < 6; ++j) ke[(int64_t)e * 576 + (6 * a + i) * 24 + 6 * b + j] = Lb[i][j];
+        return;
+    }
     double R[3][3];
 #pragma unroll
     for (int i = 0; i < 9; ++i) R[i / 3][i % 3] = T.R[i];
@@ -239,7 +251,7 @@ __global__ __launch_bounds__(256) void ke_quad_kernel(int32_t n, const double* _
     for (int i = 0; i < 6; ++i)
 #pragma unroll
         for (int j = 0; j < 6; ++j) {
-            if (BLOCKS)
+            if (MODE == QK_BLOCKS)
                 ke[((int64_t)e * 16 + lane) * 36 + i * 6 + j] = out[i][j];
             else
                 ke[(int64_t)e * 576 + (6 * a + i) * 24 + 6 * b + j] = out[i][j];
@@ -267,11 +279,177 @@ int launch_ke_quad(int32_t n, const double* crds_e, const int32_t* cnct, const d
     if (n <= 0) return JSSO_OK;
     const int grid = (n + 15) / 16;
     if (cnct)
-        hipLaunchKernelGGL(ke_quad_kernel<true>, dim3(grid), dim3(256), 0, s, n, crds_e, cnct,
+        hipLaunchKernelGGL(ke_quad_kernel<QK_BLOCKS>, dim3(grid), dim3(256), 0, s, n, crds_e, cnct,
                            crds_nodes, props, ke);
     else
-        hipLaunchKernelGGL(ke_quad_kernel<false>, dim3(grid), dim3(256), 0, s, n, crds_e, cnct,
+        hipLaunchKernelGGL(ke_quad_kernel<QK_GLOBAL>, dim3(grid), dim3(256), 0, s, n, crds_e, cnct,
                            crds_nodes, props, ke);
+    JSSO_HIP(hipGetLastError());
+    return JSSO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Element-level parts of the reference's element API, one thread per element.
+// ---------------------------------------------------------------------------
+
+// BeamCol.T (12x12 = blockdiag of the direction cosines, element.py:66-105) and
+// BeamCol.K_local (element.py:107-128), row-major; either output may be NULL.
+__global__ __launch_bounds__(256) void beam_local_kernel(int32_t n, const double* __restrict__ crds,
+                                                         const double* __restrict__ props, double* __restrict__ T,
+                                                         double* __restrict__ kl) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double c[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) c[k] = crds[6 * e + k];
+    double D[3][3], L;
+    beam_dircos<double>(c, D, L);
+    if (T) {
+        double* t = T + (int64_t)e * 144;
+        for (int i = 0; i < 12; ++i)
+            for (int j = 0; j < 12; ++j) t[i * 12 + j] = (i / 3 == j / 3) ? D[i % 3][j % 3] : 0.0;
+    }
+    if (kl) {
+        const double E = props[6 * e + 0], G = props[6 * e + 1], Iy = props[6 * e + 2], Iz = props[6 * e + 3],
+                     Jt = props[6 * e + 4], A = props[6 * e + 5];
+        double* k = kl + (int64_t)e * 144;
+        for (int i = 0; i < 12; ++i)
+            for (int j = 0; j < 12; ++j) k[i * 12 + j] = beam_kloc<double>(i, j, L, E, G, Iy, Iz, Jt, A);
+    }
+}
+
+// Quad.loc_crds (in-plane coordinates, the normal of the element's plane and of
+// the triangles 1-2-3 and 1-3-4, element.py:488-539) and Quad.T (24x24 =
+// blockdiag of the frame, element.py:643-696); each output may be NULL.
+__global__ __launch_bounds__(256) void quad_frame_kernel(int32_t n, const double* __restrict__ crds,
+                                                         double* __restrict__ xy_out, double* __restrict__ normals,
+                                                         double* __restrict__ T) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double c[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) c[k] = crds[12 * e + k];
+    V3<double> R[3];
+    double xy[8];
+    quad_frame<double>(c, R, xy);
+    if (xy_out)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) xy_out[8 * e + k] = xy[k];
+    if (normals) {
+        const V3<double> X0{c[0], c[1], c[2]}, X1{c[3], c[4], c[5]}, X2{c[6], c[7], c[8]}, X3{c[9], c[10], c[11]};
+        const V3<double> v31 = sub(X0, X2), v32 = sub(X1, X2), v34 = sub(X3, X2);
+        V3<double> t1 = cross(v31, v32), t2 = cross(v34, v31);
+        t1 = scale(t1, norm(t1));
+        t2 = scale(t2, norm(t2));
+        const double nv[9] = {R[2].x, R[2].y, R[2].z, t1.x, t1.y, t1.z, t2.x, t2.y, t2.z};
+#pragma unroll
+        for (int k = 0; k < 9; ++k) normals[9 * e + k] = nv[k];
+    }
+    if (T) {
+        const double D[3][3] = {{R[0].x, R[0].y, R[0].z}, {R[1].x, R[1].y, R[1].z}, {R[2].x, R[2].y, R[2].z}};
+        double* t = T + (int64_t)e * 576;
+        for (int i = 0; i < 24; ++i)
+            for (int j = 0; j < 24; ++j) t[i * 24 + j] = (i / 3 == j / 3) ? D[i % 3][j % 3] : 0.0;
+    }
+}
+
+// At natural coordinates (r, s): Quad.J (2x2), B_kappa (3x12), B_gamma_MITC4
+// (2x12), B_m (3x8) from the in-plane coordinates (element.py:698-818), and the
+// constitutive Cb (3x3), Cs (2x2), Cm (3x3) from [t, E, nu, kx_mod, ky_mod]
+// (element.py:820-875); row-major, each output may be NULL.
+__global__ __launch_bounds__(256) void quad_gauss_kernel(int32_t n, const double* __restrict__ xy_in,
+                                                         const double* __restrict__ props, double r, double s,
+                                                         double* __restrict__ Jo, double* __restrict__ Bk,
+                                                         double* __restrict__ Bg, double* __restrict__ Bm,
+                                                         double* __restrict__ Cb, double* __restrict__ Cs,
+                                                         double* __restrict__ Cm) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    if (xy_in) {
+        double xy[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) xy[k] = xy_in[8 * e + k];
+        QuadGP<double> G;
+        double Jm[2][2];
+        quad_gp_rs<double>(xy, r, s, G, Jm);
+        if (Jo)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Jo[4 * e + k] = Jm[k / 2][k % 2];
+        if (Bk) {
+            double* b = Bk + (int64_t)e * 36;
+            for (int a = 0; a < 4; ++a)
+                for (int c = 0; c < 3; ++c) {
+                    double k0, k1, k2;
+                    quad_bkappa_col<double>(G, a, c, k0, k1, k2);
+                    b[0 * 12 + 3 * a + c] = k0;
+                    b[1 * 12 + 3 * a + c] = k1;
+                    b[2 * 12 + 3 * a + c] = k2;
+                }
+        }
+        if (Bg) {
+            double* b = Bg + (int64_t)e * 24;
+            for (int k = 0; k < 12; ++k) quad_bgamma_col<double>(xy, G, k, b[k], b[12 + k]);
+        }
+        if (Bm) {
+            double* b = Bm + (int64_t)e * 24;
+            for (int a = 0; a < 4; ++a) {
+                b[0 * 8 + 2 * a] = G.dH[0][a];
+                b[0 * 8 + 2 * a + 1] = 0.0;
+                b[1 * 8 + 2 * a] = 0.0;
+                b[1 * 8 + 2 * a + 1] = G.dH[1][a];
+                b[2 * 8 + 2 * a] = G.dH[1][a];
+                b[2 * 8 + 2 * a + 1] = G.dH[0][a];
+            }
+        }
+    }
+    if (props) {
+        const double t = props[5 * e + 0], E = props[5 * e + 1], nu = props[5 * e + 2], kx = props[5 * e + 3],
+                     ky = props[5 * e + 4];
+        double cb, cs, M[3][3];
+        quad_plate_consts<double>(t, E, nu, cb, cs);
+        if (Cb) {
+            const double v[9] = {cb, cb * nu, 0.0, cb * nu, cb, 0.0, 0.0, 0.0, cb * (1.0 - nu) / 2.0};
+#pragma unroll
+            for (int k = 0; k < 9; ++k) Cb[9 * e + k] = v[k];
+        }
+        if (Cs) {
+            Cs[4 * e + 0] = cs;
+            Cs[4 * e + 1] = 0.0;
+            Cs[4 * e + 2] = 0.0;
+            Cs[4 * e + 3] = cs;
+        }
+        if (Cm) {
+            quad_cm<double>(E, nu, kx, ky, M);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) Cm[9 * e + k] = M[k / 3][k % 3];
+        }
+    }
+}
+
+int launch_beam_local(int32_t n, const double* crds, const double* props, double* T, double* kl, hipStream_t s) {
+    if (n <= 0) return JSSO_OK;
+    hipLaunchKernelGGL(beam_local_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, crds, props, T, kl);
+    JSSO_HIP(hipGetLastError());
+    return JSSO_OK;
+}
+
+int launch_quad_local(int32_t n, const double* crds, const double* props, double* xy, double* normals, double* T,
+                      double* kl, hipStream_t s) {
+    if (n <= 0) return JSSO_OK;
+    if (xy || normals || T)
+        hipLaunchKernelGGL(quad_frame_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, crds, xy, normals, T);
+    if (kl)
+        hipLaunchKernelGGL(ke_quad_kernel<QK_LOCAL>, dim3((n + 15) / 16), dim3(256), 0, s, n, crds,
+                           (const int32_t*)nullptr, (const double*)nullptr, props, kl);
+    JSSO_HIP(hipGetLastError());
+    return JSSO_OK;
+}
+
+int launch_quad_gauss(int32_t n, const double* xy, const double* props, double r, double s_, double* J, double* Bk,
+                      double* Bg, double* Bm, double* Cb, double* Cs, double* Cm, hipStream_t s) {
+    if (n <= 0) return JSSO_OK;
+    hipLaunchKernelGGL(quad_gauss_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, xy, props, r, s_, J, Bk, Bg,
+                       Bm, Cb, Cs, Cm);
     JSSO_HIP(hipGetLastError());
     return JSSO_OK;
 }

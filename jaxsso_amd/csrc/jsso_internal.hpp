@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/jsso.h"
@@ -36,6 +37,10 @@ constexpr int QDIR = 17;   // quad adjoint directions: 12 coordinates + 5 proper
 constexpr int WS_ADJ_SLOT = 12;  // Plan::ws slot of the adjoint's gradient scratch (solver slots are below it)
 constexpr int WS_FU_SLOT = 13;   // Plan::ws slot of the f.u reduction's per-slice partials
 constexpr int ROWS_PER_WG = 64;   // block rows per solver slice (one wavefront in the SpMV)
+// batched-solver variants: matrix type MT x vector type VT
+enum { PS_F32 = 0, PS_F64 = 1, PS_F32_V64 = 2 };
+template <typename MT, typename VT>
+constexpr int prec_slot() { return sizeof(MT) == 8 ? PS_F64 : (sizeof(VT) == 4 ? PS_F32 : PS_F32_V64); }
 constexpr int SOLVER_WG = 6 * ROWS_PER_WG;   // threads of the scalar-row solver kernels
 
 // Symmetric sliced ELL of a block pattern (built on the host, see build_ell).
@@ -103,8 +108,9 @@ struct Plan {
     int64_t ell_slots = 0;           // value positions (value slots x 64), incl. one all-zero slot
     // waves per slice of the batched CG kernel, per [FP32 | FP64][load cases]:
     // 0 = not tuned yet (the next solve measures the variants and stores the fastest)
-    int32_t cg_wps[2][9] = {};
-    CgGraph cg_graph[2][9][2];       // captured CG chunks, per [FP32 | FP64][load cases][iterations | replacement]
+    int32_t cg_wps[3][9] = {};
+    CgGraph cg_graph[3][9][2];       // captured CG chunks, per precision slot (prec_slot) x load cases x
+                                     // [iterations | replacement]
     hipStream_t capture_stream = nullptr;
     bool graphs_ok = true;           // false after a failed capture: launch eagerly
     int32_t* h_flags = nullptr;      // pinned host copies of the convergence flags, two chunks' worth
@@ -121,6 +127,7 @@ struct Plan {
     double* d_Minv = nullptr;     // block-Jacobi inverse (plain path)
     double* d_Kc = nullptr;       // constrained K incl. diagonal (plain path)
     int32_t* d_status = nullptr;  // device error flags
+    int32_t status_cache = -1;    // host copy of *d_status once read after an assembly (-1: not read)
     bool assembled = false;
     bool full_valid = false;      // d_K / d_Kc hold the current assembly
     double* d_crds = nullptr;     // node coordinates of the current assembly (for the AMG null space)
@@ -128,6 +135,9 @@ struct Plan {
     int32_t precond = JSSO_PC_AMG;
     AmgState* amg = nullptr;
     bool amg_valid = false;       // the AMG numeric set-up matches the current assembly
+    // dense direct solve (jsso_dense_solve): N x N factor + right-hand sides, grown on demand
+    double* d_dense = nullptr;
+    size_t dense_bytes = 0;
     // solver workspace, grown on demand
     std::vector<DBuf> ws;
     int ws_nrhs = 0;
@@ -154,22 +164,40 @@ int launch_ke_beam(int32_t n, const double* crds_e, const int32_t* cnct, const d
                    const double* props, double* ke, int32_t* flags, int32_t* status, hipStream_t s);
 int launch_ke_quad(int32_t n, const double* crds_e, const int32_t* cnct, const double* crds_nodes,
                    const double* props, double* ke, hipStream_t s);
+int launch_beam_local(int32_t n, const double* crds, const double* props, double* T, double* kl, hipStream_t s);
+int launch_quad_local(int32_t n, const double* crds, const double* props, double* xy, double* normals, double* T,
+                      double* kl, hipStream_t s);
+int launch_quad_gauss(int32_t n, const double* xy, const double* props, double r, double s_, double* J, double* Bk,
+                      double* Bg, double* Bm, double* Cb, double* Cs, double* Cm, hipStream_t s);
 int launch_assemble(Plan* p, hipStream_t s);
+int launch_csr_spmv(int32_t n_rows, const int32_t* rowptr, const int32_t* colind, const double* vals,
+                    const double* x, const double* c, double* y, hipStream_t s);
 int launch_full_blocks(Plan* p, hipStream_t s);
 int launch_adjoint(Plan* p, const double* crds, const double* prop_beam, const double* prop_quad,
                    const double* u, const double* lam, double lam_scale, int nrhs, double* gbuf,
                    double* dprop_beam, double* dprop_quad, hipStream_t s);
 int launch_gather_grad(Plan* p, const double* gbuf, int nrhs, double* dcrds, hipStream_t s);
 int launch_spmv_full(Plan* p, const double* K, const double* x, double* y, hipStream_t s);
-int solve_batched(Plan* p, const double* f, double* u, int nrhs, bool mixed, double tol,
+int solve_batched(Plan* p, const double* f, double* u, int nrhs, int slot, double tol,
                   int maxit, jsso_info* info, hipStream_t s);
 int timing_enable(int every);
-int timing_read(double* avg_ms, int64_t* samples, int reset);
+int timing_read(int slot, double* avg_ms, int64_t* samples, int reset);
+void timing_begin(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev);
+void timing_begin_batch(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev, int launches);
+void timing_begin_slot(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev, int slot);
+bool timing_sample_chunk();
+std::pair<hipEvent_t, hipEvent_t> timing_pair();
+void timing_settle(std::pair<hipEvent_t, hipEvent_t> e, int slot, bool count);
+void timing_end(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>* ev);
+void timing_collect();
 int launch_compliance(Plan* p, const double* f, const double* u, int nrhs, double scale, double* C, hipStream_t s);
-int amg_solve(Plan* p, const double* f, double* u, int nrhs, bool mixed, double tol, int maxit, jsso_info* info,
+int amg_solve(Plan* p, const double* f, double* u, int nrhs, int slot, double tol, int maxit, jsso_info* info,
               hipStream_t s);
 int amg_levels(Plan* p, int32_t* h_nodes, int32_t cap, int32_t* h_levels);
 void amg_destroy(Plan* p);
+int dense_solve(Plan* p, const double* K, const double* f, double* u, int nrhs, bool lu, bool transpose,
+                hipStream_t s);
+int assembly_status(Plan* p, hipStream_t s, int32_t* status);
 int solve_plain(Plan* p, const double* f, double* u, int nrhs, double tol, int maxit,
                 jsso_info* info, hipStream_t s);
 

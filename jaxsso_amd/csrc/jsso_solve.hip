@@ -43,18 +43,25 @@ namespace jsso {
 
 
 // ---------------------------------------------------------------------------
-// sampled event timing of k_cgf (jsso_timing_enable / jsso_timing_read)
+// sampled event timing (jsso_timing_enable / jsso_timing_read_slot)
 // ---------------------------------------------------------------------------
+// Slot JSSO_TIMING_SOLVE holds the batched solver's dominant kernel: k_cgf
+// (block-Jacobi; one eager launch in `every`, or a captured chunk of launches
+// as that many) or k_pcg_dir (AMG; one chunk in `every` is launched eagerly and
+// each of its k_pcg_dir launches is bracketed).  The other slots bracket whole
+// phases on every call while timing is on.
 struct KernelTiming {
     int every = 0;
-    int64_t counter = 0, samples = 0;
-    double total_ms = 0.0;
+    int64_t counter = 0, chunk_counter = 0;
+    int64_t samples[JSSO_TIMING_SLOTS] = {};
+    double total_ms[JSSO_TIMING_SLOTS] = {};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pool, pending;
     std::vector<int> pending_launches;   // launches each pending event pair brackets
+    std::vector<int> pending_slot;
 };
 static KernelTiming g_timing;
 
-static void timing_start(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev, int launches) {
+static void timing_start(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev, int launches, int slot) {
     std::pair<hipEvent_t, hipEvent_t> e;
     if (!g_timing.pool.empty()) {
         e = g_timing.pool.back();
@@ -64,6 +71,7 @@ static void timing_start(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev, 
     }
     g_timing.pending.push_back(e);
     g_timing.pending_launches.push_back(launches);
+    g_timing.pending_slot.push_back(slot);
     ev = &g_timing.pending.back();
     (void)hipEventRecord(ev->first, s);
 }
@@ -72,14 +80,56 @@ static void timing_start(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev, 
 void timing_begin(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev) {
     ev = nullptr;
     if (g_timing.every <= 0 || (g_timing.counter++ % g_timing.every) != 0) return;
-    timing_start(s, ev, 1);
+    timing_start(s, ev, 1, JSSO_TIMING_SOLVE);
 }
 
 // a graph of `launches` kernels back to back: bracketed whenever timing is on
 void timing_begin_batch(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev, int launches) {
     ev = nullptr;
     if (g_timing.every <= 0) return;
-    timing_start(s, ev, launches);
+    timing_start(s, ev, launches, JSSO_TIMING_SOLVE);
+}
+
+// every call while timing is on (phases: element kernels, gathers, adjoint,
+// AMG set-up; the launches of a sampled AMG chunk)
+void timing_begin_slot(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev, int slot) {
+    ev = nullptr;
+    if (g_timing.every <= 0) return;
+    timing_start(s, ev, 1, slot);
+}
+
+// true for one AMG chunk in `every` while timing is on: that chunk is launched
+// eagerly so that its k_pcg_dir launches can be bracketed
+bool timing_sample_chunk() {
+    if (g_timing.every <= 0) return false;
+    return (g_timing.chunk_counter++ % g_timing.every) == 0;
+}
+
+// An event pair owned by the caller (a deferred sample): recorded around a
+// launch now, added to a slot by timing_settle once the caller knows whether the
+// sample counts.
+std::pair<hipEvent_t, hipEvent_t> timing_pair() {
+    std::pair<hipEvent_t, hipEvent_t> e{nullptr, nullptr};
+    if (!g_timing.pool.empty()) {
+        e = g_timing.pool.back();
+        g_timing.pool.pop_back();
+    } else if (hipEventCreate(&e.first) != hipSuccess || hipEventCreate(&e.second) != hipSuccess) {
+        e = {nullptr, nullptr};
+    }
+    return e;
+}
+
+// the events have completed (the caller synchronised): add to `slot` if
+// `count`, then return the pair to the pool
+void timing_settle(std::pair<hipEvent_t, hipEvent_t> e, int slot, bool count) {
+    if (!e.first) return;
+    float ms = 0.f;
+    if (count && hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
+        g_timing.total_ms[slot] += ms;
+        g_timing.samples[slot] += 1;
+    }
+    (void)hipGetLastError();
+    g_timing.pool.push_back(e);
 }
 
 void timing_end(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>* ev) {
@@ -92,36 +142,46 @@ void timing_collect() {
     for (size_t i = 0; i < g_timing.pending.size(); ++i) {
         auto e = g_timing.pending[i];
         const int n = g_timing.pending_launches[i];
+        const int slot = g_timing.pending_slot[i];
         if (hipEventQuery(e.second) == hipErrorNotReady) {
             g_timing.pending[keep] = e;
             g_timing.pending_launches[keep] = n;
+            g_timing.pending_slot[keep] = slot;
             ++keep;
             continue;
         }
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
-            g_timing.total_ms += ms;
-            g_timing.samples += n;
+            g_timing.total_ms[slot] += ms;
+            g_timing.samples[slot] += n;
         }
         g_timing.pool.push_back(e);
     }
     g_timing.pending.resize(keep);
     g_timing.pending_launches.resize(keep);
+    g_timing.pending_slot.resize(keep);
     (void)hipGetLastError();
 }
 
 int timing_enable(int every) {
     g_timing.every = every > 0 ? every : 0;
     g_timing.counter = 0;
+    g_timing.chunk_counter = 0;
     return JSSO_OK;
 }
 
-int timing_read(double* avg_ms, int64_t* samples, int reset) {
-    if (avg_ms) *avg_ms = g_timing.samples ? g_timing.total_ms / g_timing.samples : 0.0;
-    if (samples) *samples = g_timing.samples;
+int timing_read(int slot, double* avg_ms, int64_t* samples, int reset) {
+    JSSO_REQUIRE(slot >= 0 && slot < JSSO_TIMING_SLOTS, JSSO_EINVAL, "timing slot %d out of range", slot);
     if (reset) {
-        g_timing.total_ms = 0.0;
-        g_timing.samples = 0;
+        // wait for what is still in flight, so that a reset leaves nothing behind
+        for (auto& e : g_timing.pending) (void)hipEventSynchronize(e.second);
+    }
+    timing_collect();
+    if (avg_ms) *avg_ms = g_timing.samples[slot] ? g_timing.total_ms[slot] / g_timing.samples[slot] : 0.0;
+    if (samples) *samples = g_timing.samples[slot];
+    if (reset) {
+        g_timing.total_ms[slot] = 0.0;
+        g_timing.samples[slot] = 0;
     }
     return JSSO_OK;
 }
@@ -483,7 +543,7 @@ int get_workspace(Plan* pl, int nrhs, Workspace& w) {
     const size_t npart = (size_t)pl->n_wg * nrhs;
     size_t need[WS_N] = {};
     for (int i = WS_B; i <= WS_P; ++i) need[i] = n * 8;
-    need[WS_PART] = npart * 8 * 14;
+    need[WS_PART] = npart * 8 * 15;
     need[WS_SD] = nsys * 8 * 3;
     need[WS_SI] = (nsys * 5 + (size_t)pl->n_wg) * 4;
     if (pl->ws.size() < WS_N) pl->ws.resize(WS_N);
@@ -513,6 +573,7 @@ int get_workspace(Plan* pl, int nrhs, Workspace& w) {
     w.P.b = pp + 8 * npart;
     w.P.t = pp + 9 * npart;
     w.P.rtrue = pp + 10 * npart;
+    w.P.xx = pp + 14 * npart;
     w.P.rr0 = pp + 11 * npart;
     w.P.rr1 = pp + 12 * npart;
     w.P.pqp = pp + 13 * npart;
@@ -530,17 +591,31 @@ int get_workspace(Plan* pl, int nrhs, Workspace& w) {
     return JSSO_OK;
 }
 
-int check_status(Plan* p, hipStream_t s) {
+// the assembly's flags: 1 = a constrained diagonal block is not positive
+// definite, 2 = a beam-column is parallel to global Y (K not symmetric)
+int assembly_status(Plan* p, hipStream_t s, int32_t* status) {
     JSSO_REQUIRE(p->on_device, JSSO_EINVAL, "plan not uploaded to the device");
+    if (p->status_cache < 0) {
+        int32_t v = 0;
+        JSSO_HIP(hipMemcpyAsync(&v, p->d_status, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        JSSO_HIP(hipStreamSynchronize(s));
+        p->status_cache = v;
+    }
+    *status = p->status_cache;
+    return JSSO_OK;
+}
+
+int check_status(Plan* p, hipStream_t s) {
     int32_t status = 0;
-    JSSO_HIP(hipMemcpyAsync(&status, p->d_status, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    JSSO_HIP(hipStreamSynchronize(s));
+    int rc = assembly_status(p, s, &status);
+    if (rc) return rc;
     JSSO_REQUIRE(!(status & 1), JSSO_ESINGULAR,
                  "a constrained diagonal block of K is not positive definite (unsupported or "
                  "isolated node)");
     JSSO_REQUIRE(!(status & 2), JSSO_EUNSUPPORTED,
                  "a beam-column is parallel to global Y: the reference's transformation "
-                 "(element.py:92-94) is not orthogonal and K is not symmetric; CG cannot solve it");
+                 "(element.py:92-94) is not orthogonal and K is not symmetric; CG cannot solve it "
+                 "(jsso_solve routes such plans to the dense LU solve)");
     return JSSO_OK;
 }
 
@@ -580,10 +655,10 @@ template <typename MT, typename VT, int NR>
 static void launch_init(Plan* pl, const Slices& sl, const Workspace& w, const double* f, const MT* A,
                         VT* const (&rb)[2], VT* const (&qb)[2], VT* p, VT* xlo, double tol, hipStream_t s) {
     const dim3 gridS = slice_grid(sl.count);
-    const int nsys = pl->n_sys * NR;
     hipLaunchKernelGGL((k_init<VT, NR>), gridS, dim3(SOLVER_WG), 0, s, sl, w.P, pl->d_fixed, pl->d_Linv, f, w.b64,
                        w.x64, w.xbest, xlo, rb[0], p);
-    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 0, tol, 0, 0);
+    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 0, tol, 0, 0,
+                       (const double*)nullptr, 0.0, 0);
     hipLaunchKernelGGL((k_cgf<MT, VT, NR, CG_START, 1>), gridS, dim3(64), 0, s, sl, w.P, w.st, 0, A,
                        (const VT*)rb[1], (const VT*)qb[1], rb[0], qb[0], p, xlo);
 }
@@ -658,7 +733,8 @@ static void launch_replace(Plan* pl, int wps, const Slices& sl, const Workspace&
     }
     if (!done)
         hipLaunchKernelGGL((k_replace<VT, NR, 1>), gridS, dim3(64), 0, s, sl, w.P, w.st, A64, b64, x64, rb[it & 1]);
-    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, steps, 8192);
+    hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 1, tol, steps, 8192,
+                       (const double*)nullptr, 0.0, 0);
     hipLaunchKernelGGL((k_save<NR>), gridS, dim3(SOLVER_WG), 0, s, sl, (const int32_t*)w.st.save,
                        (const double*)w.x64, w.xbest);
     const VT* rp = rb[(it - 1) & 1];
@@ -727,7 +803,7 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
     VT *xlo = (VT*)w.xlo, *p = (VT*)w.p;
     VT* const rb[2] = {(VT*)w.r[0], (VT*)w.r[1]};
     VT* const qb[2] = {(VT*)w.q[0], (VT*)w.q[1]};
-    int& wps = pl->cg_wps[sizeof(MT) == 4 ? 0 : 1][NR];
+    int& wps = pl->cg_wps[prec_slot<MT, VT>()][NR];
     launch_init<MT, VT, NR>(pl, sl, w, f, A, rb, qb, p, xlo, tol, s);
     JSSO_HIP(hipGetLastError());
     if (wps <= 0) {
@@ -741,7 +817,7 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
     const void* key[CgGraph::NKEY] = {A,         rb[0],       rb[1],       qb[0],         qb[1],   p,
                                       xlo,       w.P.cg[0],   w.P.cg[1],   w.st.slice_act, w.b64,  w.x64,
                                       w.xbest,   w.st.best,   w.st.active, pl->d_Ke64};
-    CgGraph(&gc)[2] = pl->cg_graph[sizeof(MT) == 4 ? 0 : 1][NR];
+    CgGraph(&gc)[2] = pl->cg_graph[prec_slot<MT, VT>()][NR];
 
     // One chunk: iterations it0+1 .. it0+steps-1, then the replacement step at it0+steps.
     auto enqueue_chunk = [&](int it0, int steps) -> int {
@@ -824,7 +900,7 @@ static int run_batched(Plan* pl, const double* f, double* u, double tol, int max
     int conv = 0;
     double mx = 0.0;
     for (int k = 0; k < nsys; ++k) {
-        conv += status[k] == SYS_CONVERGED ? 1 : 0;
+        conv += (status[k] == SYS_CONVERGED || status[k] == SYS_FLOOR) ? 1 : 0;
         mx = std::max(mx, rel[k]);
     }
     if (info) {
@@ -897,12 +973,14 @@ int launch_compliance(Plan* p, const double* f, const double* u, int nrhs, doubl
     }
 }
 
-int solve_batched(Plan* p, const double* f, double* u, int nrhs, bool mixed, double tol, int maxit,
+int solve_batched(Plan* p, const double* f, double* u, int nrhs, int slot, double tol, int maxit,
                   jsso_info* info, hipStream_t s) {
     int rc = check_status(p, s);
     if (rc) return rc;
-    if (p->precond == JSSO_PC_AMG) return amg_solve(p, f, u, nrhs, mixed, tol, maxit, info, s);
-    if (mixed) return dispatch_nr<float, float>(p, f, u, nrhs, tol, maxit, replace_interval(true), info, s);
+    if (p->precond == JSSO_PC_AMG) return amg_solve(p, f, u, nrhs, slot, tol, maxit, info, s);
+    if (slot == PS_F32) return dispatch_nr<float, float>(p, f, u, nrhs, tol, maxit, replace_interval(true), info, s);
+    if (slot == PS_F32_V64)
+        return dispatch_nr<float, double>(p, f, u, nrhs, tol, maxit, replace_interval(true), info, s);
     return dispatch_nr<double, double>(p, f, u, nrhs, tol, maxit, replace_interval(false), info, s);
 }
 

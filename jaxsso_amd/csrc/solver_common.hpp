@@ -10,7 +10,9 @@
 namespace jsso {
 
 constexpr int NWAVES = SOLVER_WG / 64;
-enum { SYS_RUNNING = 0, SYS_CONVERGED = 1, SYS_STALLED = 2 };
+// SYS_FLOOR: stopped at the attainable accuracy of the FP64 residual (counted as
+// converged, see k_decide)
+enum { SYS_RUNNING = 0, SYS_CONVERGED = 1, SYS_STALLED = 2, SYS_FLOOR = 3 };
 
 // ---------------------------------------------------------------------------
 // slices and reductions
@@ -53,6 +55,7 @@ enum { CG_RR = 0, CG_PQ = 1, CG_RQ = 2, CG_QQ = 3 };
 struct Partials {
     double* cg[2];           // fused CG: <r,r>, <p,q>, <r,q>, <q,q>
     double *b, *t, *rtrue;   // <b,b>; true FP64 <r,r> and low-precision <r,r> after a replacement
+    double* xx;              // <x,x> of the FP64 solution at a replacement
     double *rr0, *rr1;       // plain path: alternating <r,z>
     double* pqp;             // plain path: <p,q>
     __host__ __device__ double* rr(int it) const { return (it & 1) ? rr1 : rr0; }
@@ -198,10 +201,6 @@ static_assert(WS_ADJ == WS_ADJ_SLOT, "adjoint scratch slot");
 int get_workspace(Plan* pl, int nrhs, Workspace& w);
 int check_status(Plan* p, hipStream_t s);
 Slices make_slices(Plan* pl, int base, int count);
-void timing_begin(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev);
-void timing_begin_batch(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>*& ev, int launches);
-void timing_end(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>* ev);
-void timing_collect();
 bool graphs_enabled();
 int ensure_flag_buffers(Plan* pl, size_t nsys);
 

@@ -53,17 +53,24 @@ __global__ __launch_bounds__(SOLVER_WG) void k_init(Slices sl, Partials P, const
 // iterations (counted per system, so the kernel can sit in a captured graph).
 // A system whose true residual has not improved by 10% for max(min_window,
 // it/2) iterations has reached its attainable accuracy and stops.
+// Attainable-accuracy floor (rho given: an estimate of ||A||_2 per structure):
+// the FP64 residual of any iterate x carries rounding errors of about
+// eps (||A|| ||x|| + ||b||), so a system whose relative residual is within
+// floor_factor of eps (rho ||x|| + ||b||) / ||b|| and has not improved by 10%
+// for floor_window iterations stops there as SYS_FLOOR (converged as far as
+// FP64 allows) instead of waiting out the long window.
 // Every lane computes the same (butterfly) totals; lane 0 writes the state.
 template <int NR>
 __global__ __launch_bounds__(64) void k_decide(Slices sl, Partials P, SysState st, int nsys_struct, int mode,
-                                               double tol, int steps, int min_window) {
+                                               double tol, int steps, int min_window, const double* __restrict__ rho,
+                                               double floor_factor, int floor_window) {
     const int sys = blockIdx.x;
     if (sys >= nsys_struct) return;
     const int lane = threadIdx.x;
     const int s0 = sl.sys_off[sys], s1 = sl.sys_off[sys + 1];
-    double ta[NR], tb[NR];
+    double ta[NR], tb[NR], tx[NR];
 #pragma unroll
-    for (int h = 0; h < NR; ++h) ta[h] = tb[h] = 0.0;
+    for (int h = 0; h < NR; ++h) ta[h] = tb[h] = tx[h] = 0.0;
     for (int w = s0 + lane; w < s1; w += 64) {
 #pragma unroll
         for (int h = 0; h < NR; ++h) {
@@ -72,6 +79,7 @@ __global__ __launch_bounds__(64) void k_decide(Slices sl, Partials P, SysState s
             } else {
                 ta[h] += P.t[(int64_t)w * NR + h];
                 tb[h] += P.rtrue[(int64_t)w * NR + h];
+                if (rho) tx[h] += P.xx[(int64_t)w * NR + h];
             }
         }
     }
@@ -79,6 +87,7 @@ __global__ __launch_bounds__(64) void k_decide(Slices sl, Partials P, SysState s
     for (int h = 0; h < NR; ++h) {
         ta[h] = wave_sum(ta[h]);
         tb[h] = wave_sum(tb[h]);
+        tx[h] = wave_sum(tx[h]);
     }
     int mask = 0;
 #pragma unroll
@@ -113,10 +122,15 @@ __global__ __launch_bounds__(64) void k_decide(Slices sl, Partials P, SysState s
         // plateaus for a few thousand iterations before it moves on: only a
         // longer flat stretch is taken as the attainable-accuracy floor there
         const int window = it / 2 > min_window ? it / 2 : min_window;
+        const double bn = sqrt(st.bnorm2[k]);
+        const double floor_rel = rho ? 2.220446049250313e-16 * (rho[sys] * sqrt(tx[h]) + bn) / bn : 0.0;
         int active = 1, status = SYS_RUNNING;
         if (rel <= tol || tb[h] == 0.0) {
             active = 0;
             status = SYS_CONVERGED;
+        } else if (rho && rel <= floor_factor * floor_rel && it - stall > floor_window) {
+            active = 0;
+            status = SYS_FLOOR;
         } else if (!isfinite(rel) || rel > 1e3 * nbest || it - stall > window) {
             active = 0;
             status = SYS_STALLED;
@@ -224,9 +238,9 @@ __global__ __launch_bounds__(64 * WPS) void k_replace(Slices sl, Partials P, Sys
 #pragma unroll
                 for (int h = 0; h < NR; ++h) acc[c][h] += red[u][c * NR + h][lane];
     }
-    double vt[NR], vr[NR];
+    double vt[NR], vr[NR], vx[NR];
 #pragma unroll
-    for (int h = 0; h < NR; ++h) vt[h] = vr[h] = 0.0;
+    for (int h = 0; h < NR; ++h) vt[h] = vr[h] = vx[h] = 0.0;
     if (on) {
 #pragma unroll
         for (int c = 0; c < 6; ++c)
@@ -239,18 +253,21 @@ __global__ __launch_bounds__(64 * WPS) void k_replace(Slices sl, Partials P, Sys
                 rcur[o] = rl;
                 vt[h] += rv * rv;
                 vr[h] += (double)rl * (double)rl;
+                vx[h] += xown[c][h] * xown[c][h];
             }
     }
 #pragma unroll
     for (int h = 0; h < NR; ++h) {
         vt[h] = wave_sum(vt[h]);
         vr[h] = wave_sum(vr[h]);
+        vx[h] = wave_sum(vx[h]);
     }
     if (lane == 0) {
 #pragma unroll
         for (int h = 0; h < NR; ++h) {
             P.t[(int64_t)w * NR + h] = vt[h];
             P.rtrue[(int64_t)w * NR + h] = vr[h];
+            P.xx[(int64_t)w * NR + h] = vx[h];
         }
     }
 }

@@ -80,6 +80,27 @@ class FEPlan:
         self._keep = (crds, pb, pq)
         return Kout
 
+    def ensure_assembled(self, crds, prop_beam, prop_quad):
+        """Assemble unless the plan's current assembly was made from these very tensors."""
+        k = getattr(self, "_keep", None)
+        if k is None or k[0] is not crds or k[1] is not prop_beam or k[2] is not prop_quad:
+            self.assemble(crds, prop_beam, prop_quad)
+
+    def symmetric(self) -> bool:
+        """False when the last assembly holds beam-columns parallel to global Y: the
+        reference's transformation for them is not orthogonal (element.py:92-94), K
+        is not symmetric, solves run on the dense LU path and adjoints need K^T."""
+        out = ctypes.c_int32(1)
+        nat.check(nat.lib().jsso_plan_symmetric(self._h, ctypes.byref(out), nat.stream_ptr()))
+        return bool(out.value)
+
+    def solve_transpose(self, f: torch.Tensor, **kw) -> torch.Tensor:
+        """K^T u = f (u = 0 at supports): the adjoint solve.  K is symmetric unless
+        the assembly holds members parallel to global Y (then the dense LU of K^T)."""
+        if self.symmetric():
+            return self.solve(f, **kw)
+        return self.dense_solve(f, transpose=True)
+
     def set_preconditioner(self, name: str) -> None:
         """Preconditioner of the batched precisions: 'amg' (smoothed-aggregation
         multigrid V-cycle, the default) or 'block_jacobi' (split block-Jacobi scaling)."""
@@ -164,13 +185,36 @@ class FEPlan:
             dq = None if dq is None else dq[..., 0]
         return dc, db, dq
 
-    def spmv(self, x: torch.Tensor) -> torch.Tensor:
-        """y = K x with the assembled (unconstrained) K."""
+    def spmv(self, x: torch.Tensor, K: torch.Tensor | None = None) -> torch.Tensor:
+        """y = K x with the unconstrained K: the block values ``K`` ((nnzb,6,6) in
+        this plan's pattern) if given, else the plan's current assembly."""
         L = nat.lib()
         x = _dev64(x)
         y = torch.empty_like(x)
-        nat.check(L.jsso_spmv(self._h, nat.ptr(x), nat.ptr(y), nat.stream_ptr()))
+        Kv = None if K is None else _dev64(K)
+        if Kv is not None and Kv.numel() != self.nnzb * 36:
+            raise ValueError(f"K has {Kv.numel()} values, the pattern {self.nnzb * 36}")
+        nat.check(L.jsso_spmv_k(self._h, nat.ptr(Kv), nat.ptr(x), nat.ptr(y), nat.stream_ptr()))
         return y
+
+    def dense_solve(self, f: torch.Tensor, K: torch.Tensor | None = None, transpose: bool = False) -> torch.Tensor:
+        """u with u = 0 at supports and K_ff u_f = f_f (K_ff^T with ``transpose``), by
+        the dense FP64 factorisation of the constrained K on the device
+        (``jsso_dense_solve``: Cholesky, or pivoted LU when the assembly holds
+        members parallel to global Y).  ``K``: block values ((nnzb,6,6)) or None for
+        the plan's current assembly; f: (ndof,) or (ndof, m)."""
+        L = nat.lib()
+        f = _dev64(f)
+        vec = f.dim() == 1
+        f2 = f.reshape(self.ndof, -1).contiguous()
+        u = torch.empty_like(f2)
+        Kv = None if K is None else _dev64(K)
+        if Kv is not None and Kv.numel() != self.nnzb * 36:
+            raise ValueError(f"K has {Kv.numel()} values, the pattern {self.nnzb * 36}")
+        nat.check(L.jsso_dense_solve(self._h, nat.ptr(Kv), nat.ptr(f2), nat.ptr(u), f2.shape[1], int(bool(transpose)),
+                                     nat.stream_ptr()))
+        self.last_info = {"iterations": 0, "converged": self.n_sys * f2.shape[1], "method": "dense"}
+        return u.reshape(f.shape) if vec else u
 
     def K_scalar_csr(self, Kvals: torch.Tensor):
         """Expand block values (nnzb,6,6) to a host scipy CSR matrix."""

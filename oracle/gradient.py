@@ -23,18 +23,21 @@ from . import solve as osolve
 H = 1e-30
 
 
-def _cs_contract(fn, crds_e, props_e, ue, le, which, chunk=1024):
-    """d/dp of -le^T K_e(p) ue for every input column of ``crds_e`` or ``props_e``."""
+def _cs_contract(fn, crds_e, props_e, ue, le, which, chunk=1024, cols=None):
+    """d/dp of -le^T K_e(p) ue for every input column of ``crds_e`` or ``props_e``
+    (or only for the columns listed in ``cols``)."""
     n = crds_e.shape[0]
     base = crds_e if which == "crds" else props_e
-    ndir = base.shape[1]
+    width = base.shape[1]
+    cols = np.arange(width) if cols is None else np.asarray(cols)
+    ndir = len(cols)
     out = np.zeros((n, ndir))
     for i0 in range(0, n, chunk):
         i1 = min(n, i0 + chunk)
         m = i1 - i0
         c = np.repeat(crds_e[i0:i1], ndir, axis=0).astype(complex)
         p = np.repeat(props_e[i0:i1], ndir, axis=0).astype(complex)
-        pert = np.tile(np.eye(ndir), (m, 1)) * (1j * H)
+        pert = np.tile(np.eye(width)[cols], (m, 1)) * (1j * H)
         if which == "crds":
             c = c + pert
         else:

@@ -213,12 +213,17 @@ def test_precisions_agree_cfg1():
     assert rel(us["plain"], us["fp64"]) < 1e-9
 
 
+@pytest.mark.filterwarnings("error::RuntimeWarning")
 def test_mannheim_filtered_shape_optimisation():
     """The reference's filtered shape-optimisation example on its own quad mesh
     (Examples/Shells_Mannheim_Multihalle_Shape.ipynb, data Examples/Data/Mannheim_Quad;
     the notebook's random start z ~ U(0.5, 0.51) replaced by z = 0.5): the
     filtered objective and gradient at the start against the oracle, then
-    normalised gradient descent lowers the compliance at every step."""
+    normalised gradient descent lowers the compliance at every step.  The thin,
+    nearly flat shell has an FP64 residual floor near the 1e-10 tolerance: the
+    solve must end as converged (at the floor if need be), with no warning.
+    The same filter inside the parameter map (SSO_model.set_node_filter, HIP
+    CSR products) gives the same objective and gradient."""
     import os
     from jaxsso_amd.optimization import Optimization
     from jaxsso_amd.parameterization import linear_filter, load_quad_mesh_csv
@@ -256,6 +261,19 @@ def test_mannheim_filtered_shape_optimisation():
         return C, (Bd.T @ sens)[design]      # the notebook's (sens @ B_ij[design_nodes])[design_nodes]
 
     C0, g0 = objective_filter(zs[design])
+    info = m.plan().last_info
+    print(f"Mannheim shell solve: {info}")
+    assert info["converged"] == 1 and info["iterations"] < 600, info
+    sso_f = SSO_model(m)
+    for nd in design:
+        sso_f.add_nodeparameter(NodeParameter(int(nd), 2))
+    sso_f.initialize_parameters_values()
+    sso_f.set_objective("strain energy")
+    sso_f.set_node_filter(B)
+    sso_f.update_nodeparameter(zs[design])
+    Cf, gf = sso_f.value_grad_params()
+    assert abs(Cf - C0) / C0 < 1e-12
+    assert rel(gf, g0) < 1e-10
     crds = np.stack([xs, ys, zf], 1)
     props = np.tile([0.1, 1e10, 0.3, 1.0, 1.0], (len(cnct), 1))
     known = np.sort(np.concatenate([6 * bc + k for k in range(6)]))

@@ -1,0 +1,135 @@
+"""Structures with beam-columns exactly parallel to global Y.
+
+The reference transforms such a member with a non-orthogonal matrix
+(JaxSSO/element.py:92-94), so its K is not symmetric, and solves it directly
+(SuperLU on K_aug, solver.py:195-209; the adjoint with K_aug^T, solver.py:236).
+Here such plans go to the dense LU solve on the GPU.  Checked against the
+oracle's restatement of exactly that sequence: u and C to 1e-8, property
+gradients to 1e-7 (the reference's own augmented solves spread by ~1e-7 on
+these grids), coordinate gradients NaN as JAX gives them."""
+import numpy as np
+import pytest
+import torch
+
+from jaxsso_amd import workloads as wl
+from jaxsso_amd.batch import StructureBatch
+from jaxsso_amd.SSO_model import ElementParameter, NodeParameter, SSO_model
+from oracle import elements as el
+from oracle import gradient as og
+from oracle import solve as osolve
+from oracle import system
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def flat_grid(n=6):
+    """A flat beam grid: every y-direction member is exactly parallel to global Y."""
+    st = wl.gridshell(n, rise=0.0, noise=0.0, cfg=0, design=0)
+    _, yflag = el.beam_dircos(el.gather_crds(st.crds, st.cnct_beamcols))
+    assert yflag.sum() == n * (n + 1)
+    return st
+
+
+def build_model(st):
+    import jaxsso_amd.model as Model
+    m = Model.Model()
+    for i, (x, y, z) in enumerate(st.crds):
+        m.add_node(i, x, y, z)
+    for e, (c, p) in enumerate(zip(st.cnct_beamcols, st.prop_beamcols)):
+        m.add_beamcol(e, int(c[0]), int(c[1]), *p)
+    m.known_indices = [int(k) for k in st.known_id]
+    for n in range(st.n_node):
+        f = st.nodal_loads[6 * n:6 * n + 6]
+        if np.any(f):
+            m.add_nodal_load(n, list(f))
+    m.model_ready()
+    return m
+
+
+def test_flat_grid_solve_matches_superlu():
+    st = flat_grid()
+    Ko = system.assemble_K(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads)
+    assert abs(Ko - Ko.T).max() > 1e-3 * abs(Ko).max()           # K really is not symmetric
+    Kao, fao = system.augment(Ko, st.known_id, st.nodal_loads)
+    u_ref = osolve.solve_augmented(Kao, fao, st.ndof)              # the reference's SuperLU
+    u_acc = osolve.solve_reduced(Ko, st.known_id, st.nodal_loads)   # refined
+    for prec in ("fp64", "mixed", "plain"):
+        m = build_model(st)
+        m.precision = prec
+        m.solve()
+        u = m.u.cpu().numpy()
+        assert rel(u, u_acc) < 1e-10, prec
+        assert rel(u, u_ref) < 1e-8, prec
+        C = float(m.strain_energy())
+        assert abs(C - 0.5 * st.nodal_loads @ u_ref) / abs(0.5 * st.nodal_loads @ u_ref) < 1e-8
+    m = build_model(st)
+    m.solve(which_solver="dense")
+    assert rel(m.u.cpu().numpy(), u_acc) < 1e-10
+
+
+def test_flat_grid_gradients():
+    """Compliance gradient: property (E of every beam) derivatives finite and equal
+    to the oracle's faithful sequence (lambda from the transposed solve, lambda !=
+    u/2 here); coordinate derivatives NaN wherever a Y-parallel member touches."""
+    st = flat_grid()
+    ref = og.compliance_and_gradient(st, faithful=True)
+    m = build_model(st)
+    sso = SSO_model(m)
+    for e in range(st.n_beamcol):
+        sso.add_eleparameter(ElementParameter(e, 0, 0))
+    for n in st.design_nodes[:5]:
+        sso.add_nodeparameter(NodeParameter(int(n), 2))
+    sso.initialize_parameters_values()
+    sso.set_objective("strain energy")
+    C, g = sso.value_grad_params()
+    assert abs(C - ref["C"]) / ref["C"] < 1e-8
+    gz, gE = g[:5], g[5:]
+    assert np.all(np.isnan(gz))
+    assert np.all(np.isfinite(gE))
+    assert rel(gE, ref["dprop_beam"][:, 0]) < 1e-7
+    # lambda = K^-T f/2 really differs from u/2 on this grid
+    lam_sym = og.element_gradients(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads,
+                                   ref["u"], 0.5 * ref["u"])["beam_props"][:, 0]
+    assert rel(lam_sym, ref["dprop_beam"][:, 0]) > 1e-4
+    # the batched driver takes the same route
+    b = StructureBatch([st])
+    Cb, (_, db, _) = b.compliance_and_gradient(precision="mixed")
+    assert abs(float(Cb[0, 0]) - ref["C"]) / ref["C"] < 1e-8
+    assert rel(db.cpu().numpy()[:, 0], ref["dprop_beam"][:, 0]) < 1e-7
+
+
+def test_user_objective_transposed_adjoint():
+    """A user objective on u: lambda = K^-T dObj/du through the transposed dense
+    solve; the E gradient against central finite differences of the oracle."""
+    st = flat_grid(4)
+    m = build_model(st)
+    sso = SSO_model(m)
+    for e in range(3):
+        sso.add_eleparameter(ElementParameter(e, 0, 0))
+    sso.initialize_parameters_values()
+    w = torch.zeros(st.ndof, dtype=torch.float64, device="cuda")
+    w[2::6] = 1.0
+    sso.set_objective("user", func=lambda s, u: (w * u).sum() ** 2)
+    v, g = sso.value_grad_params()
+
+    def obj(E0):
+        pb = st.prop_beamcols.copy()
+        pb[:3, 0] = E0
+        K = system.assemble_K(st.crds, st.cnct_beamcols, pb, st.cnct_quads, st.prop_quads)
+        u = osolve.solve_reduced(K, st.known_id, st.nodal_loads)
+        return u[2::6].sum() ** 2
+
+    E = st.prop_beamcols[:3, 0]
+    assert abs(v - obj(E)) / abs(v) < 1e-9
+    fd = []
+    for k in range(3):
+        h = 1e-6 * E[k]
+        Ep, Em = E.copy(), E.copy()
+        Ep[k] += h
+        Em[k] -= h
+        fd.append((obj(Ep) - obj(Em)) / (2 * h))
+    assert rel(g, fd) < 1e-6
