@@ -225,10 +225,11 @@ def simp_headline(args, L, nat):
     return out
 
 
-def shell200(args, L, nat):
+def shell200(args, L, nat, profile=False):
     """The 200x200 MITC-4 shell (242,406 DOF) as one structure: evals/s on the
     AMG path (mixed and FP64) and the plain FP64 path, errors against the
-    committed FP64 oracle fixture, and the mixed path's k_pcg_dir roofline."""
+    committed FP64 oracle fixture, and the mixed path's k_pcg_dir roofline
+    (profile: the mixed path only, with its solve log)."""
     import torch
     from jaxsso_amd import workloads as wl
     from jaxsso_amd.batch import StructureBatch
@@ -238,9 +239,13 @@ def shell200(args, L, nat):
     b = StructureBatch([st])
     res = {"workload": "MITC-4 shell 200x200 over 20 m (40,401 nodes, 40,000 quads, 242,406 DOF), edges pinned, "
                        "smooth Fourier thickness; one structure per evaluation"}
-    for key, prec, pc, reps in (("mixed", "mixed", "amg", 10), ("fp64", "fp64", "amg", 5),
-                                ("plain", "plain", None, 1)):
+    solve_log = []
+    paths = (("mixed", "mixed", "amg", 10), ("fp64", "fp64", "amg", 5), ("plain", "plain", None, 1))
+    for key, prec, pc, reps in paths[:1] if profile else paths:
         b.compliance_and_gradient(precision=prec, tol=args.tol, preconditioner=pc)     # warm-up
+        if key == "mixed":
+            i = b.plan.last_info
+            solve_log.append([i["iterations"], i["dir_launches"], i["full_iterations"]])
         torch.cuda.synchronize()
         if key == "mixed":
             L.jsso_timing_enable(TIMING_EVERY)
@@ -249,6 +254,9 @@ def shell200(args, L, nat):
         t0 = time.perf_counter()
         for _ in range(reps):
             C, (dc, _, dq) = b.compliance_and_gradient(precision=prec, tol=args.tol, preconditioner=pc)
+            if key == "mixed":
+                i = b.plan.last_info
+                solve_log.append([i["iterations"], i["dir_launches"], i["full_iterations"]])
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / reps
         dcn = dc.cpu().numpy().reshape(st.n_node, 3)
@@ -279,7 +287,66 @@ def shell200(args, L, nat):
                                "assembly_GBps": (eb + gb) / ((te + tg) * 1e-3) / 1e9 if te + tg > 0 else None}
             res["phases_ms_per_eval"] = {"element_ms": te, "gather_ms": tg, "amg_setup_ms": t["setup"][0],
                                          "adjoint_ms": t["adjoint"][0]}
-    res["speedup_mixed_vs_plain_fp64"] = res["mixed"]["evals_per_s"] / res["plain"]["evals_per_s"]
+    res["solve_log"] = {"fields": ["iterations", "dir_launches", "full_iterations"], "solves": solve_log}
+    if not profile:
+        res["speedup_mixed_vs_plain_fp64"] = res["mixed"]["evals_per_s"] / res["plain"]["evals_per_s"]
+    return res
+
+
+def simp_loop(args, L, nat, n_steps=8, step=0.05):
+    """The SIMP batch inside an optimisation loop: every step evaluates the 64
+    designs, then moves every element density by a normalised gradient step
+    (rho <- clip(rho - step * g / max|g|, 0.01, 1) per design), so each step
+    assembles and solves new designs.  The same sequence runs twice: with the AMG
+    hierarchy rebuilt after every assembly, and with it kept across steps
+    (AmgKeepPolicy: only the level-0 smoothing bound follows each design; a
+    rebuild when the iterations grow by 25%).  Reported: evals/s, iterations per
+    step, AMG set-up ms per step, rebuilds, and the largest relative difference
+    of the compliances of the two runs."""
+    import torch
+    from jaxsso_amd import workloads as wl
+    from jaxsso_amd.batch import StructureBatch
+    from jaxsso_amd.plan import AmgKeepPolicy
+    designs = wl.cfg4(args.designs)
+    b = StructureBatch(designs)
+    nd, nq = args.designs, designs[0].n_quad
+    rho0 = torch.as_tensor(np.concatenate([d.rho for d in designs]), device=b.f.device)
+    res = {"workload": f"SIMP batch ({nd} x 100x100 shells), {n_steps} normalised-gradient steps on rho "
+                       f"(step {step}), FP64"}
+    Cs = {}
+    for keep in (False, True):
+        rho = rho0.clone()
+        b.plan.amg_reuse(0)
+        b.prop_quad[:, 1] = rho ** 3 * 30e9
+        b.compliance_and_gradient(precision="fp64", tol=args.tol)        # warm-up on the start design
+        policy = AmgKeepPolicy(b.plan) if keep else None
+        torch.cuda.synchronize()
+        L.jsso_timing_enable(TIMING_EVERY)
+        _slot(L, nat, "setup")
+        iters, Cl = [], []
+        t0 = time.perf_counter()
+        for _ in range(n_steps):
+            b.prop_quad[:, 1] = rho ** 3 * 30e9
+            C, (_, _, dq) = b.compliance_and_gradient(precision="fp64", tol=args.tol)
+            g = (dq[:, 1] * 3.0 * rho ** 2 * 30e9).reshape(nd, nq)
+            gmax = g.abs().amax(1, keepdim=True).clamp_min(1e-300)
+            rho = (rho.reshape(nd, nq) - step * g / gmax).clamp(0.01, 1.0).reshape(-1)
+            iters.append(b.plan.last_info["iterations"])
+            Cl.append(C[:, 0])
+            if policy:
+                policy.after_solve()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        L.jsso_timing_enable(0)
+        setup_ms, _ = _slot(L, nat, "setup")
+        Cs[keep] = torch.stack(Cl)
+        key = "keep_hierarchy" if keep else "rebuild_every_step"
+        res[key] = {"evals_per_s": nd * n_steps / dt, "ms_per_step": dt * 1e3 / n_steps, "cg_iterations": iters,
+                    "amg_setup_ms_per_step": setup_ms, "amg": b.plan.amg_reuse(-1),
+                    "rebuilds_triggered": policy.rebuilds if policy else None}
+    b.plan.amg_reuse(0)
+    res["max_rel_diff_C"] = float(((Cs[True] - Cs[False]).abs() / Cs[False].abs()).max())
+    res["speedup_keep_vs_rebuild"] = res["keep_hierarchy"]["evals_per_s"] / res["rebuild_every_step"]["evals_per_s"]
     return res
 
 
@@ -456,9 +523,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle-check", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the headline batch")
-    ap.add_argument("--profile-run", action="store_true",
-                    help="only the headline's warm-up and timed steps (for rocprofv3); implies --no-extras "
-                         "--no-cpu-baseline --no-oracle-check")
+    ap.add_argument("--profile-run", nargs="?", const="simp", choices=["simp", "shell200"],
+                    help="only the headline's warm-up and timed steps, or the 200x200 shell's mixed path "
+                         "(for rocprofv3); implies --no-extras --no-cpu-baseline --no-oracle-check")
     ap.add_argument("--trace-summary", metavar="DIR",
                     help="summarise a rocprofv3 CSV kernel trace (with --bench-log) and exit")
     ap.add_argument("--bench-log", metavar="FILE", help="output of the profiled bench.py --profile-run")
@@ -491,12 +558,19 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.profile_run == "shell200":
+        r = shell200(args, L, nat, profile=True)
+        print(json.dumps({"workload": r["workload"], "roofline": r["roofline"], "solve_log": r["solve_log"],
+                          "mixed": r["mixed"]}), flush=True)
+        return
     h = simp_headline(args, L, nat)
     b = h.pop("b")
     extras = {}
     if not args.no_extras:
         print("# 200x200 shell", flush=True)
         extras["shell200"] = shell200(args, L, nat)
+        print("# SIMP optimisation loop", flush=True)
+        extras["simp_loop"] = simp_loop(args, L, nat)
         print("# gridshell batch", flush=True)
         extras["gridshell_batch"] = gridshell_batch(args)
         print("# 40x40 shell", flush=True)

@@ -183,8 +183,9 @@ int jsso_spmv_k(jsso_plan* plan, const double* d_K, const double* d_x, double* d
 
 /* Dense direct solve: the constrained K (rows/columns of constrained DOFs replaced
  * by identity, u = 0 there) factorised in FP64 on the device -- by a blocked
- * Cholesky, or, when the plan's last assembly holds members parallel to global Y
- * (K not symmetric, element.py:92-94), by a blocked LU with partial pivoting --
+ * Cholesky, or, when the plan's last assembly makes K non-symmetric (members
+ * parallel to global Y, element.py:92-94; quads with kx_mod != ky_mod and nu != 0,
+ * whose Cm is not symmetric, element.py:852-875), by a blocked LU with partial pivoting --
  * then triangular solves.  transpose != 0 solves Kc^T u = f (the adjoint system).
  * d_K as for jsso_spmv_k (NULL = the plan's current assembly); d_f, d_u:
  * (6*n_nodes) x nrhs, nrhs <= 64.  Needs 8 N^2 bytes of device memory (N =
@@ -197,7 +198,8 @@ int jsso_spmv_k(jsso_plan* plan, const double* d_K, const double* d_x, double* d
 int jsso_dense_solve(jsso_plan* plan, const double* d_K, const double* d_f, double* d_u, int32_t nrhs,
                      int32_t transpose, void* stream);
 /* *h_symmetric = 0 when the plan's last assembly holds members parallel to global
- * Y (its K is not symmetric, so an adjoint needs the transposed solve), else 1.
+ * Y or quads with kx_mod != ky_mod (its K is not symmetric, so an adjoint needs
+ * the transposed solve), else 1.
  * Synchronises the stream. */
 int jsso_plan_symmetric(jsso_plan* plan, int32_t* h_symmetric, void* stream);
 
@@ -218,14 +220,26 @@ int jsso_compliance(jsso_plan* plan, const double* d_f, const double* d_u, int32
 int jsso_csr_spmv(int32_t n_rows, const int32_t* d_rowptr, const int32_t* d_colind, const double* d_vals,
                   const double* d_x, const double* d_c, double* d_y, void* stream);
 
-/* Tuning of the batched CG kernel (no counterpart in the reference: the
+/* Tuning of the batched CG kernels (no counterpart in the reference: the
  * reference hands the solve to SuperLU).  The first batched solve on a plan for a
  * (precision, nrhs) pair times the 1-, 2- and 4-wavefront-per-slice variants on
- * the live problem and keeps the fastest for the plan's later solves.
- * set_wps >= 0 overrides the stored choice (0 = tune again on the next solve,
- * 1/2/4 = force); h_wps, if not NULL, receives the stored choice (0 = untuned). */
+ * the live problem and keeps the fastest for the plan's later solves: with the
+ * block-Jacobi preconditioner the fused iteration kernel k_cgf, with AMG the
+ * level-0 kernels (timed on k_pcg_dir, the direction and CG product).  The
+ * choice is kept per preconditioner; this call reads or sets the one of the
+ * plan's current preconditioner.  set_wps >= 0 overrides the stored choice
+ * (0 = tune again on the next solve, 1/2/4 = force); h_wps, if not NULL,
+ * receives the stored choice (0 = untuned). */
 int jsso_plan_cg_tuning(jsso_plan* plan, int32_t precision, int32_t nrhs, int32_t set_wps,
                         int32_t* h_wps);
+
+/* Tuning of the assembly's MITC-4 element kernel (no counterpart in the
+ * reference): elements per workgroup (4, 8 or 16; 16 lanes per element), timed
+ * on the live plan at its first jsso_assemble (every variant writes the same
+ * blocks) and kept for the plan's later assemblies.  set_epw >= 0 overrides
+ * (0 = tune again at the next assembly, 4/8/16 = force); h_epw, if not NULL,
+ * receives the stored choice (0 = untuned). */
+int jsso_plan_element_tuning(jsso_plan* plan, int32_t set_epw, int32_t* h_epw);
 
 /* Preconditioner of the batched solves (JSSO_PREC_FP64 / JSSO_PREC_MIXED); no
  * counterpart in the reference, which factorises with SuperLU (JaxSSO/solver.py:
@@ -233,6 +247,18 @@ int jsso_plan_cg_tuning(jsso_plan* plan, int32_t precision, int32_t nrhs, int32_
  * pattern (host, once) and its numeric operators after each jsso_assemble (device,
  * at the next solve); the plain path always uses block-Jacobi. */
 int jsso_plan_set_preconditioner(jsso_plan* plan, int32_t pc);
+/* Reuse of the AMG hierarchy across assemblies, for optimisation loops (no
+ * counterpart in the reference, which refactorises with SuperLU every time).
+ * mode 0 (default): every solve after a jsso_assemble rebuilds the numeric
+ * hierarchy.  mode 1: keep the hierarchy built for an earlier design; after an
+ * assembly only the level-0 smoothing bound follows the new matrix (a few
+ * warm-started power-iteration steps), the coarse operators and prolongators
+ * stay (the V-cycle is still a fixed SPD preconditioner, iterations grow as the
+ * design moves away).  mode 2: rebuild fully at the next solve, then keep (mode
+ * 1).  mode -1 only queries.  h_counts (3 entries, may be NULL) receives the
+ * mode, the full set-ups and the refreshes run so far. */
+int jsso_plan_amg_reuse(jsso_plan* plan, int32_t mode, int64_t* h_counts);
+
 /* Symbolic AMG hierarchy of a plan (host only, no device work): *h_levels receives
  * the number of levels, h_nodes[l] (up to cap entries) the block rows of level l. */
 int jsso_plan_amg_levels(jsso_plan* plan, int32_t* h_nodes, int32_t cap, int32_t* h_levels);

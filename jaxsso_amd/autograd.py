@@ -95,7 +95,7 @@ class _Compliance(torch.autograd.Function):
             dc, db, dq = _grads(plan, crds, pb, pq, u, None, 0.5, need, weight=C_bar)
             f_bar = (u * C_bar) if need[3] else None   # dC/df = K^-1 f = u
         else:
-            # K not symmetric (members parallel to global Y): lam = K^-T f/2, as the
+            # K not symmetric (members parallel to global Y, orthotropic quads): lam = K^-T f/2, as the
             # reference's custom VJP solves with K_aug^T (solver.py:236)
             lam = _solve(plan, 0.5 * f, ctx.opts, transpose=True)
             dc, db, dq = _grads(plan, crds, pb, pq, u, lam, 0.0, need, weight=C_bar)

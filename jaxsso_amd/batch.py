@@ -76,7 +76,7 @@ class StructureBatch:
         packed like the batch ([nodes|beams|quads] x [3|6|5] x cases)."""
         u = self.u if u is None else u
         if not self.plan.symmetric():
-            # members parallel to global Y: K is not symmetric, lam = K^-T f/2
+            # members parallel to global Y or quads with kx_mod != ky_mod: K is not symmetric, lam = K^-T f/2
             lam = self.plan.solve_transpose(0.5 * self.f.reshape(u.shape))
             return self.plan.adjoint(self.crds, self.prop_beam, self.prop_quad, u, lam=lam, lam_scale=0.0)
         return self.plan.adjoint(self.crds, self.prop_beam, self.prop_quad, u, lam=None, lam_scale=0.5)

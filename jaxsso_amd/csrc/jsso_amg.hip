@@ -46,14 +46,18 @@ constexpr int AMG_MAXC = 26;       // nodes of a structure's directly solved coa
 constexpr int AMG_QR_RPL = 5;      // null-space rows per lane in the per-aggregate QR
 constexpr int AMG_MAX_AGG = AMG_QR_RPL * 64 / 6;   // nodes per aggregate (53)
 constexpr int AMG_POWER_STEPS = 12;
+constexpr int AMG_POWER_STEPS_WARM = 4;   // level-0 steps of a kept hierarchy's refresh (warm start)
 constexpr double AMG_THETA = 0.62;        // level-0 smoothing step z = r / (AMG_THETA rho)
 // coarse levels: degree-2 Chebyshev smoothing on [AMG_CHEB_LO hi, hi], hi = AMG_CHEB_HI rho
 constexpr double AMG_CHEB_HI = 1.1;
 constexpr double AMG_CHEB_LO = 0.1;
 constexpr int AMG_STALL_WINDOW = 256;     // iterations without a 10% gain that end a solve (floor)
-// attainable-accuracy floor (k_decide): within 4 x eps (rho ||x|| + ||b||) / ||b||
-// and no 10% gain for 24 iterations (three replacement chunks)
-constexpr double AMG_FLOOR_FACTOR = 4.0;
+// attainable-accuracy floor (k_decide): within 16 x eps (rho ||x|| + ||b||) / ||b||
+// and no 10% gain for 24 iterations (three replacement chunks).  The rounding
+// error of a computed FP64 residual is bounded by ~n eps (|b| + |A||x|) per row
+// (n ~ 54 terms per row of a shell), typically ~sqrt(n) eps, and || |A||x| || can
+// exceed ||A|| ||x||: 16 covers that spread; the 10% window still has to pass.
+constexpr double AMG_FLOOR_FACTOR = 16.0;
 constexpr int AMG_FLOOR_WINDOW = 24;
 constexpr double AMG_RHO_SAFETY = 1.12;   // the power iteration approaches rho from below
 constexpr int AMG_MAXLEVELS = 12;
@@ -150,6 +154,9 @@ struct AmgState {
     uint64_t gen = 1;                 // bumped whenever a buffer the graphs point at changes
     std::vector<void*> allocs;
     CgGraph graph[3][9][2];       // per precision slot (prec_slot) x load cases x [iterations | replacement]
+    bool built = false;           // a full numeric set-up has run (a kept hierarchy can be reused)
+    int power_next = AMG_POWER_STEPS;   // level 0: index of the next power-iteration step (parities of
+                                        // the vectors and partials continue across refreshes)
 };
 
 // Greedy aggregation of one structure's nodes [n0, n1) of a level.  `active`
@@ -1139,19 +1146,40 @@ struct AmgWpsFits {
     static constexpr bool value = WPS == 1 || (size_t)(WPS - 1) * 6 * NR * 64 * sizeof(VT) <= 24576;
 };
 
-// Totals over the slices [s0, s1) of one structure of slot `slot` of cg, in one
-// wavefront (every lane holds the same butterfly total).
-template <int NR>
-__device__ __forceinline__ void wave_totals(const double* __restrict__ cg, int slot, int s0, int s1,
-                                            double (&t)[NR]) {
+// Totals over the slices [s0, s1) of one structure of M (buffer, slot) pairs, in
+// one wavefront (every lane holds the same butterfly totals).  Each lane issues
+// the loads of 4 slices at once, then adds them in slice order.
+template <int NR, int M>
+__device__ __forceinline__ void wave_totals_m(const double* const (&cg)[M], const int (&slot)[M], int s0, int s1,
+                                              double (&t)[M][NR]) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int h = 0; h < NR; ++h) t[h] = 0.0;
-    for (int q = s0 + lane; q < s1; q += 64)
+    for (int m = 0; m < M; ++m)
 #pragma unroll
-        for (int h = 0; h < NR; ++h) t[h] += cg[((int64_t)q * 4 + slot) * NR + h];
+        for (int h = 0; h < NR; ++h) t[m][h] = 0.0;
+    for (int q0 = s0 + lane; q0 < s1; q0 += 256) {
+        double v[4][M][NR];
 #pragma unroll
-    for (int h = 0; h < NR; ++h) t[h] = wave_sum(t[h]);
+        for (int u = 0; u < 4; ++u) {
+            const int q = min(q0 + 64 * u, s1 - 1);
+#pragma unroll
+            for (int m = 0; m < M; ++m)
+#pragma unroll
+                for (int h = 0; h < NR; ++h) v[u][m][h] = cg[m][((int64_t)q * 4 + slot[m]) * NR + h];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (q0 + 64 * u < s1) {
+#pragma unroll
+                for (int m = 0; m < M; ++m)
+#pragma unroll
+                    for (int h = 0; h < NR; ++h) t[m][h] += v[u][m][h];
+            }
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m)
+#pragma unroll
+        for (int h = 0; h < NR; ++h) t[m][h] = wave_sum(t[m][h]);
 }
 
 // acc[c][h] += sum over the slice's ELL slots of A_off(i, j) x(j), x(j) given
@@ -1312,13 +1340,15 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_dir(Slices sl, Partials P, Sys
     };
     VT acc[6][NR];
     ell_apply_pre<MT, VT, NR, WPS>(sl, A, w, ma, mb, acc, pk, [&] {
-        double t1[NR], t2[NR], t0[NR];
-        wave_totals<NR>(P.cg[(k - 1) & 1], CG_RR, mc.x, mc.y, t1);
-        wave_totals<NR>(P.cg[(k - 1) & 1], CG_RQ, mc.x, mc.y, t2);
-        wave_totals<NR>(P.cg[k & 1], CG_RR, mc.x, mc.y, t0);
+        const double* const cgp = (k & 1) ? P.cg[0] : P.cg[1];   // P.cg[(k - 1) & 1]
+        const double* const cgc = (k & 1) ? P.cg[1] : P.cg[0];   // P.cg[k & 1]
+        const double* const bufs[3] = {cgp, cgp, cgc};
+        const int slots[3] = {CG_RR, CG_RQ, CG_RR};
+        double tt[3][NR];
+        wave_totals_m<NR, 3>(bufs, slots, mc.x, mc.y, tt);
 #pragma unroll
         for (int h = 0; h < NR; ++h) {
-            const double b = t0[h] > 0.0 ? (t1[h] - t2[h]) / t0[h] : 0.0;
+            const double b = tt[2][h] > 0.0 ? (tt[0][h] - tt[1][h]) / tt[2][h] : 0.0;
             beta[h] = (isfinite(b) && b > 0.0) ? VT(b) : VT(0);
         }
 #pragma unroll
@@ -1333,9 +1363,9 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_dir(Slices sl, Partials P, Sys
     // (inactive load cases keep p; their q is never used; one staging round for
     // both vectors measured slower here)
     flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) {
-        const VT pn = pk(ii, c, h);
+        const VT pn = zo[c][h] + beta[h] * po[c][h];
         if (on && ((amask >> h) & 1)) v[h] += (double)pn * (double)acc[c][h];
-        return ((amask >> h) & 1) ? pn : pprev[(6 * ii + c) * NR + h];
+        return ((amask >> h) & 1) ? pn : po[c][h];
     }, pcur);
     flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) { return acc[c][h]; }, q);
     if (wave != 0) return;
@@ -1405,12 +1435,13 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_pre(Slices sl, Partials P, Sys
 #pragma unroll
         for (int h = 0; h < NR; ++h) alpha[h] = VT(0);
         if (!FRESH) {
-            double rz[NR], pq[NR];
-            wave_totals<NR>(P.cg[(k - 1) & 1], CG_RR, mc.x, mc.y, rz);
-            wave_totals<NR>(P.cg[k & 1], CG_PQ, mc.x, mc.y, pq);
+            const double* const bufs[2] = {(k & 1) ? P.cg[0] : P.cg[1], (k & 1) ? P.cg[1] : P.cg[0]};
+            const int slots[2] = {CG_RR, CG_PQ};
+            double tt[2][NR];
+            wave_totals_m<NR, 2>(bufs, slots, mc.x, mc.y, tt);
 #pragma unroll
             for (int h = 0; h < NR; ++h)
-                alpha[h] = (pq[h] > 0.0 && isfinite(pq[h]) && rz[h] > 0.0) ? VT(rz[h] / pq[h]) : VT(0);
+                alpha[h] = (tt[1][h] > 0.0 && isfinite(tt[1][h]) && tt[0][h] > 0.0) ? VT(tt[0][h] / tt[1][h]) : VT(0);
         }
 #pragma unroll
         for (int c = 0; c < 6; ++c)
@@ -1424,8 +1455,9 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_pre(Slices sl, Partials P, Sys
     // (inactive load cases keep r and x)
     auto val = [&](int m, int c, int h) {
         const int64_t o = (6 * ii + c) * NR + h;
-        if (m == 0) return rk(ii, c, h) - acc[c][h] / th;
-        if (m == 1) return ((amask >> h) & 1) ? rk(ii, c, h) : rprev[o];
+        const VT rn = FRESH ? ro[c][h] : ro[c][h] - alpha[h] * qo[c][h];
+        if (m == 0) return rn - acc[c][h] / th;
+        if (m == 1) return ((amask >> h) & 1) ? rn : ro[c][h];
         const VT xv = PF ? xo[c][PF ? h : 0] : xlo[o];
         return ((amask >> h) & 1) ? xv + alpha[h] * (PF ? pko[c][PF ? h : 0] : p[o]) : xv;
     };
@@ -2270,6 +2302,32 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
                            A.d_term, D.d_rowptr, D.d_colind, D.d_browof, D.d_kref, (const double*)D.d_A64, A.d_inv);
     }
     JSSO_HIP(hipGetLastError());
+    A.built = true;
+    A.power_next = AMG_POWER_STEPS;
+    return JSSO_OK;
+}
+
+// Refresh of a kept hierarchy after an assembly (jsso_plan_amg_reuse): the
+// coarse levels and the prolongators stay as built for an earlier design (the
+// V-cycle remains a fixed symmetric positive definite preconditioner, only a
+// less exact one as the design moves away); the level-0 smoothing follows the
+// new matrix through its spectral bound, a few power-iteration steps started
+// from the last eigenvector estimate.
+static int amg_setup_light(Plan* pl, AmgState& A, hipStream_t s) {
+    AmgLevelDev& D = A.dev[0];
+    const Slices sl = level_slices(pl, A, 0);
+    const int ns = pl->n_sys;
+    const int s0 = A.power_next;
+    for (int step = s0; step < s0 + AMG_POWER_STEPS_WARM; ++step)
+        hipLaunchKernelGGL((k_amg_power<_Float16, 4>), slice_grid(D.n_wg), dim3(256), 0, s, sl,
+                           (const _Float16*)pl->d_Ke16, step, (const double*)D.d_pv[(step + 1) & 1],
+                           D.d_pv[step & 1], D.d_ppart, (const uint8_t*)D.d_cont, D.n_wg);
+    const int last = s0 + AMG_POWER_STEPS_WARM - 1;
+    hipLaunchKernelGGL(k_amg_theta, dim3(nblk(ns, 64)), dim3(64), 0, s, ns, (const int32_t*)pl->d_sys_wg_off,
+                       (const double*)D.d_ppart, last, (const uint8_t*)D.d_cont, D.n_wg, D.d_rho, D.d_omega,
+                       D.d_theta, (float*)nullptr);
+    JSSO_HIP(hipGetLastError());
+    A.power_next = last + 1;
     return JSSO_OK;
 }
 
@@ -2506,10 +2564,62 @@ static int amg_prepare(Plan* pl, hipStream_t s) {
         JSSO_REQUIRE(pl->d_crds, JSSO_EINVAL, "AMG needs the node coordinates of an assembly");
         std::pair<hipEvent_t, hipEvent_t>* ev = nullptr;
         timing_begin_slot(s, ev, JSSO_TIMING_SETUP);
-        if ((rc = amg_setup_numeric(pl, A, s))) return rc;
+        if (pl->amg_reuse == 1 && A.built) {
+            if ((rc = amg_setup_light(pl, A, s))) return rc;
+            ++pl->amg_light;
+        } else {
+            if ((rc = amg_setup_numeric(pl, A, s))) return rc;
+            ++pl->amg_full;
+            if (pl->amg_reuse == 2) pl->amg_reuse = 1;
+        }
         timing_end(s, ev);
         pl->amg_valid = true;
     }
+    return JSSO_OK;
+}
+
+// Autotuning of the level-0 kernels' waves per slice (1, 2 or 4 wavefronts share
+// a 64-row slice's ELL slots): the direction and CG-product kernel, the
+// iteration's longest, is timed for every variant that fits the register budget
+// on the live plan, right after the start-up kernels have set the slices'
+// activity (its outputs are rewritten by the first iteration, so the timing
+// leaves no trace), and the fastest variant serves every level-0 kernel of this
+// (precision, load cases) from then on.
+template <typename MT, typename VT, int NR>
+static int tune_amg_wps(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs<VT>& bf, const MT* A0,
+                        hipStream_t s, int32_t& out) {
+    const Slices sl = level_slices(pl, A, 0);
+    const dim3 g = slice_grid(pl->n_wg);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    JSSO_HIP(hipEventCreate(&e0));
+    JSSO_HIP(hipEventCreate(&e1));
+    float best = 0.f;
+    int pick = 0;
+    for (int cand : {4, 2, 1}) {
+        int used = 0;
+        wps_dispatch<VT, NR>(cand, [&](auto W) { used = decltype(W)::value; });
+        if (used != cand) continue;   // does not fit the register budget
+        float ms = 0.f;
+        for (int rep = 0; rep < 4; ++rep) {
+            wps_dispatch<VT, NR>(cand, [&](auto W) {
+                constexpr int WP = decltype(W)::value;
+                if (rep == 1) (void)hipEventRecord(e0, s);
+                hipLaunchKernelGGL((k_pcg_dir<MT, VT, NR, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, 1, A0,
+                                   (const VT*)bf.z, (const VT*)bf.p[0], bf.p[1], bf.q);
+            });
+        }
+        (void)hipEventRecord(e1, s);
+        JSSO_HIP(hipEventSynchronize(e1));
+        JSSO_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (pick == 0 || ms < best) {
+            best = ms;
+            pick = cand;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    JSSO_HIP(hipGetLastError());
+    out = pick > 0 ? pick : 1;
     return JSSO_OK;
 }
 
@@ -2543,6 +2653,9 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
                        w.x64, w.xbest, bf.xlo, bf.r[0], bf.p[0]);
     hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 0, tol, 0, 0,
                        (const double*)nullptr, 0.0, 0);
+    int32_t& wsel = pl->amg_wps[prec_slot<MT, VT>()][NR];
+    if (wsel <= 0 && (rc = tune_amg_wps<MT, VT, NR>(pl, A, w, bf, A0, s, wsel))) return rc;
+    A.wps0 = wsel;
     wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
         constexpr int WP = decltype(W)::value;
         hipLaunchKernelGGL((k_pcg_pre<VT, NR, true, WP>), gridS, dim3(64 * WP), 0, s, sl, w.P, w.st, 0,
@@ -2571,11 +2684,11 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
         auto* tv = sampled ? &samples[chunk_no] : nullptr;
         ++chunk_no;
         if (!sampled && steps == E && E % 2 == 0 && it0 % 2 == 0 && pl->graphs_ok && graphs_enabled()) {
-            ga = capture_graph(pl, gc[0], key, 0, E, tol, [&](hipStream_t cs) {
+            ga = capture_graph(pl, gc[0], key, A.wps0, E, tol, [&](hipStream_t cs) {
                 for (int it = 1; it < E; ++it) amg_step<MT, VT, NR>(pl, A, w, bf, A0, it, cs);
             });
             if (ga)
-                gb = capture_graph(pl, gc[1], key, 0, E, tol, [&](hipStream_t cs) {
+                gb = capture_graph(pl, gc[1], key, A.wps0, E, tol, [&](hipStream_t cs) {
                     amg_replace<MT, VT, NR>(pl, A, w, bf, A0, E, E, tol, cs);
                 });
         }
@@ -2684,7 +2797,7 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
         info->converged = conv;
         info->n_systems = nsys;
         info->rel_residual = mx;
-        info->waves_per_slice = 1;
+        info->waves_per_slice = A.wps0;
         info->levels = A.nl;
         info->dir_launches = it;
         info->at_floor = floor;

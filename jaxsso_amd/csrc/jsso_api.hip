@@ -545,7 +545,9 @@ int jsso_assemble(jsso_plan* plan, const double* d_crds, const double* d_prop_be
     timing_begin_slot(s, ev, JSSO_TIMING_ELEMENT);
     int rc = launch_ke_beam(p->n_beam, nullptr, p->d_cnct_beam, d_crds, d_prop_beam, p->d_ke, nullptr, p->d_status, s);
     if (rc) return rc;
-    rc = launch_ke_quad(p->n_quad, nullptr, p->d_cnct_quad, d_crds, d_prop_quad, p->d_ke + (size_t)p->n_beam * 4 * 36, s);
+    double* ke_q = p->d_ke + (size_t)p->n_beam * 4 * 36;
+    if (p->n_quad > 0 && p->ke_epw <= 0 && (rc = tune_ke_quad(p, d_crds, d_prop_quad, ke_q, s, p->ke_epw))) return rc;
+    rc = launch_ke_quad(p->n_quad, nullptr, p->d_cnct_quad, d_crds, d_prop_quad, ke_q, s, p->ke_epw, p->d_status);
     if (rc) return rc;
     timing_end(s, ev);
     timing_begin_slot(s, ev, JSSO_TIMING_GATHER);
@@ -754,6 +756,24 @@ int jsso_plan_set_preconditioner(jsso_plan* plan, int32_t pc) {
     P(plan)->precond = pc;
     return JSSO_OK;
 }
+int jsso_plan_amg_reuse(jsso_plan* plan, int32_t mode, int64_t* h_counts) {
+    if (!plan || mode < -1 || mode > 2) {
+        set_error("invalid AMG reuse mode %d (-1 query, 0 rebuild, 1 keep, 2 rebuild once then keep)", mode);
+        return JSSO_EINVAL;
+    }
+    Plan* p = P(plan);
+    if (mode >= 0) {
+        p->amg_reuse = mode;
+        if (mode == 2) p->amg_valid = false;   // rebuild at the next solve even without a new assembly
+    }
+    if (h_counts) {
+        h_counts[0] = p->amg_reuse;
+        h_counts[1] = p->amg_full;
+        h_counts[2] = p->amg_light;
+    }
+    return JSSO_OK;
+}
+
 int jsso_plan_amg_levels(jsso_plan* plan, int32_t* h_nodes, int32_t cap, int32_t* h_levels) {
     if (!plan || cap < 0 || (cap > 0 && !h_nodes)) {
         set_error("invalid arguments");
@@ -761,6 +781,17 @@ int jsso_plan_amg_levels(jsso_plan* plan, int32_t* h_nodes, int32_t cap, int32_t
     }
     return amg_levels(P(plan), h_nodes, cap, h_levels);
 }
+int jsso_plan_element_tuning(jsso_plan* plan, int32_t set_epw, int32_t* h_epw) {
+    if (!plan || !(set_epw < 0 || set_epw == 0 || set_epw == 4 || set_epw == 8 || set_epw == 16)) {
+        set_error("invalid element tuning arguments (epw -1, 0, 4, 8 or 16)");
+        return JSSO_EINVAL;
+    }
+    Plan* p = P(plan);
+    if (set_epw >= 0) p->ke_epw = set_epw;
+    if (h_epw) *h_epw = p->ke_epw;
+    return JSSO_OK;
+}
+
 int jsso_plan_cg_tuning(jsso_plan* plan, int32_t precision, int32_t nrhs, int32_t set_wps, int32_t* h_wps) {
     if (!plan || (precision != JSSO_PREC_FP64 && precision != JSSO_PREC_MIXED && precision != JSSO_PREC_MIXED_V64) ||
         nrhs < 1 || nrhs > 8 ||
@@ -769,7 +800,8 @@ int jsso_plan_cg_tuning(jsso_plan* plan, int32_t precision, int32_t nrhs, int32_
         return JSSO_EINVAL;
     }
     const int ps = precision == JSSO_PREC_MIXED ? PS_F32 : precision == JSSO_PREC_FP64 ? PS_F64 : PS_F32_V64;
-    int32_t& slot = P(plan)->cg_wps[ps][nrhs];
+    Plan* p = P(plan);
+    int32_t& slot = p->precond == JSSO_PC_AMG ? p->amg_wps[ps][nrhs] : p->cg_wps[ps][nrhs];
     if (set_wps >= 0) slot = set_wps;
     if (h_wps) *h_wps = slot;
     return JSSO_OK;

@@ -1,5 +1,7 @@
 // Element stiffness kernels (beam-column, MITC-4) and the forward-mode adjoint
 // kernels.  Element math lives in element_math.hpp.
+#include <mutex>
+
 #include "element_math.hpp"
 #include "jsso_internal.hpp"
 
@@ -87,7 +89,9 @@ __global__ __launch_bounds__(256) void ke_beam_kernel(int32_t n, const double* _
 // its 6x6 block of T^T (k_m + k_b) T (element.py:1073-1084).
 // MODE: QK_BLOCKS output [e][a][b][6x6] (assembly), QK_GLOBAL row-major 24x24
 // per element, QK_LOCAL row-major 24x24 of k_m + k_b before the rotation
-// (Quad.element_K_quad_local, element.py:1086-1095).
+// (Quad.element_K_quad_local, element.py:1086-1095).  EPW elements per
+// workgroup (4, 8 or 16: 64, 128 or 256 threads), chosen per plan by timing on
+// the live assembly (tune_ke_quad).
 // ---------------------------------------------------------------------------
 enum { QK_BLOCKS = 0, QK_GLOBAL = 1, QK_LOCAL = 2 };
 
@@ -100,15 +104,15 @@ struct QuadTile {
     double diag[8];
 };
 
-template <int MODE>
-__global__ __launch_bounds__(256) void ke_quad_kernel(int32_t n, const double* __restrict__ crds_e,
-                                                      const int32_t* __restrict__ cnct,
-                                                      const double* __restrict__ crds_nodes,
-                                                      const double* __restrict__ props,
-                                                      double* __restrict__ ke) {
-    __shared__ QuadTile sh[16];
+template <int MODE, int EPW = 16>
+__global__ __launch_bounds__(16 * EPW) void ke_quad_kernel(int32_t n, const double* __restrict__ crds_e,
+                                                           const int32_t* __restrict__ cnct,
+                                                           const double* __restrict__ crds_nodes,
+                                                           const double* __restrict__ props,
+                                                           double* __restrict__ ke, int32_t* status = nullptr) {
+    __shared__ QuadTile sh[EPW];
     const int le = threadIdx.x >> 4, lane = threadIdx.x & 15;
-    const int e = blockIdx.x * 16 + le;
+    const int e = blockIdx.x * EPW + le;
     const bool valid = e < n;
     QuadTile& T = sh[le];
     if (valid && lane < 4) {
@@ -146,6 +150,9 @@ __global__ __launch_bounds__(256) void ke_quad_kernel(int32_t n, const double* _
             T.nu = props[5 * e + 2];
             T.kx = props[5 * e + 3];
             T.ky = props[5 * e + 4];
+            // Cm of element.py:852-875 is [[Ex, nu Ex, 0], [nu Ey, Ey, 0], ...]: not
+            // symmetric when kx_mod != ky_mod (and nu != 0), so neither is this Ke
+            if (status && T.nu != 0.0 && T.kx != T.ky) atomicOr(status, 2);
         }
     }
     __syncthreads();
@@ -275,15 +282,26 @@ int launch_ke_beam(int32_t n, const double* crds_e, const int32_t* cnct, const d
 }
 
 int launch_ke_quad(int32_t n, const double* crds_e, const int32_t* cnct, const double* crds_nodes,
-                   const double* props, double* ke, hipStream_t s) {
+                   const double* props, double* ke, hipStream_t s, int epw, int32_t* status) {
     if (n <= 0) return JSSO_OK;
     const int grid = (n + 15) / 16;
-    if (cnct)
-        hipLaunchKernelGGL(ke_quad_kernel<QK_BLOCKS>, dim3(grid), dim3(256), 0, s, n, crds_e, cnct,
-                           crds_nodes, props, ke);
-    else
-        hipLaunchKernelGGL(ke_quad_kernel<QK_GLOBAL>, dim3(grid), dim3(256), 0, s, n, crds_e, cnct,
-                           crds_nodes, props, ke);
+    if (cnct) {
+        switch (epw) {
+            case 4:
+                hipLaunchKernelGGL((ke_quad_kernel<QK_BLOCKS, 4>), dim3((n + 3) / 4), dim3(64), 0, s, n, crds_e, cnct,
+                                   crds_nodes, props, ke, status);
+                break;
+            case 8:
+                hipLaunchKernelGGL((ke_quad_kernel<QK_BLOCKS, 8>), dim3((n + 7) / 8), dim3(128), 0, s, n, crds_e,
+                                   cnct, crds_nodes, props, ke, status);
+                break;
+            default:
+                hipLaunchKernelGGL((ke_quad_kernel<QK_BLOCKS, 16>), dim3(grid), dim3(256), 0, s, n, crds_e, cnct,
+                                   crds_nodes, props, ke, status);
+        }
+    } else
+        hipLaunchKernelGGL((ke_quad_kernel<QK_GLOBAL, 16>), dim3(grid), dim3(256), 0, s, n, crds_e, cnct,
+                           crds_nodes, props, ke, status);
     JSSO_HIP(hipGetLastError());
     return JSSO_OK;
 }
@@ -426,6 +444,37 @@ __global__ __launch_bounds__(256) void quad_gauss_kernel(int32_t n, const double
     }
 }
 
+// Elements per workgroup of the assembly's MITC-4 kernel, timed on the live
+// plan at its first assembly (every variant writes the same element blocks,
+// bit for bit): 4, 8 and 16 elements (64, 128, 256 threads) per workgroup.
+int tune_ke_quad(Plan* p, const double* crds, const double* props, double* ke, hipStream_t s, int32_t& out) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    JSSO_HIP(hipEventCreate(&e0));
+    JSSO_HIP(hipEventCreate(&e1));
+    float best = 0.f;
+    int pick = 16;
+    for (int epw : {16, 8, 4}) {
+        int rc = launch_ke_quad(p->n_quad, nullptr, p->d_cnct_quad, crds, props, ke, s, epw, p->d_status);   // warm
+        if (rc) return rc;
+        JSSO_HIP(hipEventRecord(e0, s));
+        for (int rep = 0; rep < 2; ++rep)
+            if ((rc = launch_ke_quad(p->n_quad, nullptr, p->d_cnct_quad, crds, props, ke, s, epw, p->d_status)))
+                return rc;
+        JSSO_HIP(hipEventRecord(e1, s));
+        JSSO_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        JSSO_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (epw == 16 || ms < best) {
+            best = ms;
+            pick = epw;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    out = pick;
+    return JSSO_OK;
+}
+
 int launch_beam_local(int32_t n, const double* crds, const double* props, double* T, double* kl, hipStream_t s) {
     if (n <= 0) return JSSO_OK;
     hipLaunchKernelGGL(beam_local_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, crds, props, T, kl);
@@ -439,8 +488,8 @@ int launch_quad_local(int32_t n, const double* crds, const double* props, double
     if (xy || normals || T)
         hipLaunchKernelGGL(quad_frame_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, crds, xy, normals, T);
     if (kl)
-        hipLaunchKernelGGL(ke_quad_kernel<QK_LOCAL>, dim3((n + 15) / 16), dim3(256), 0, s, n, crds,
-                           (const int32_t*)nullptr, (const double*)nullptr, props, kl);
+        hipLaunchKernelGGL((ke_quad_kernel<QK_LOCAL, 16>), dim3((n + 15) / 16), dim3(256), 0, s, n, crds,
+                           (const int32_t*)nullptr, (const double*)nullptr, props, kl, (int32_t*)nullptr);
     JSSO_HIP(hipGetLastError());
     return JSSO_OK;
 }

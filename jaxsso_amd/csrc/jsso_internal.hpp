@@ -109,6 +109,10 @@ struct Plan {
     // waves per slice of the batched CG kernel, per [FP32 | FP64][load cases]:
     // 0 = not tuned yet (the next solve measures the variants and stores the fastest)
     int32_t cg_wps[3][9] = {};
+    // waves per slice of the AMG level-0 kernels, per precision slot x load cases
+    // (0 = not tuned yet: the next AMG solve times the variants on the live plan)
+    int32_t amg_wps[3][9] = {};
+    int32_t ke_epw = 0;            // MITC-4 elements per workgroup of the assembly (0 = tune at the next one)
     CgGraph cg_graph[3][9][2];       // captured CG chunks, per precision slot (prec_slot) x load cases x
                                      // [iterations | replacement]
     hipStream_t capture_stream = nullptr;
@@ -135,6 +139,8 @@ struct Plan {
     int32_t precond = JSSO_PC_AMG;
     AmgState* amg = nullptr;
     bool amg_valid = false;       // the AMG numeric set-up matches the current assembly
+    int32_t amg_reuse = 0;        // jsso_plan_amg_reuse mode
+    int64_t amg_full = 0, amg_light = 0;   // numeric set-ups run: full, and refreshes of a kept hierarchy
     // dense direct solve (jsso_dense_solve): N x N factor + right-hand sides, grown on demand
     double* d_dense = nullptr;
     size_t dense_bytes = 0;
@@ -163,7 +169,8 @@ int dalloc(Plan* p, T** ptr, size_t count) {
 int launch_ke_beam(int32_t n, const double* crds_e, const int32_t* cnct, const double* crds_nodes,
                    const double* props, double* ke, int32_t* flags, int32_t* status, hipStream_t s);
 int launch_ke_quad(int32_t n, const double* crds_e, const int32_t* cnct, const double* crds_nodes,
-                   const double* props, double* ke, hipStream_t s);
+                   const double* props, double* ke, hipStream_t s, int epw = 16, int32_t* status = nullptr);
+int tune_ke_quad(Plan* p, const double* crds, const double* props, double* ke, hipStream_t s, int32_t& out);
 int launch_beam_local(int32_t n, const double* crds, const double* props, double* T, double* kl, hipStream_t s);
 int launch_quad_local(int32_t n, const double* crds, const double* props, double* xy, double* normals, double* T,
                       double* kl, hipStream_t s);

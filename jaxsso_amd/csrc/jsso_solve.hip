@@ -613,9 +613,9 @@ int check_status(Plan* p, hipStream_t s) {
                  "a constrained diagonal block of K is not positive definite (unsupported or "
                  "isolated node)");
     JSSO_REQUIRE(!(status & 2), JSSO_EUNSUPPORTED,
-                 "a beam-column is parallel to global Y: the reference's transformation "
-                 "(element.py:92-94) is not orthogonal and K is not symmetric; CG cannot solve it "
-                 "(jsso_solve routes such plans to the dense LU solve)");
+                 "K is not symmetric (a beam-column parallel to global Y, element.py:92-94, or a quad with "
+                 "kx_mod != ky_mod, element.py:852-875); CG cannot solve it (jsso_solve routes such plans "
+                 "to the dense LU solve)");
     return JSSO_OK;
 }
 

@@ -58,7 +58,8 @@ __global__ __launch_bounds__(SOLVER_WG) void k_init(Slices sl, Partials P, const
 // eps (||A|| ||x|| + ||b||), so a system whose relative residual is within
 // floor_factor of eps (rho ||x|| + ||b||) / ||b|| and has not improved by 10%
 // for floor_window iterations stops there as SYS_FLOOR (converged as far as
-// FP64 allows) instead of waiting out the long window.
+// FP64 allows) instead of waiting out the long window; so does one whose best
+// residual is in that band when the iteration diverges.
 // Every lane computes the same (butterfly) totals; lane 0 writes the state.
 template <int NR>
 __global__ __launch_bounds__(64) void k_decide(Slices sl, Partials P, SysState st, int nsys_struct, int mode,
@@ -124,16 +125,22 @@ __global__ __launch_bounds__(64) void k_decide(Slices sl, Partials P, SysState s
         const int window = it / 2 > min_window ? it / 2 : min_window;
         const double bn = sqrt(st.bnorm2[k]);
         const double floor_rel = rho ? 2.220446049250313e-16 * (rho[sys] * sqrt(tx[h]) + bn) / bn : 0.0;
+        // the best iterate lies within the floor band (x of a diverging iterate is
+        // still close to the best one while rel stays small, so ||x|| holds)
+        const bool near_floor = rho && isfinite(rel) && rel < 1e-3 && nbest <= floor_factor * floor_rel;
         int active = 1, status = SYS_RUNNING;
         if (rel <= tol || tb[h] == 0.0) {
             active = 0;
             status = SYS_CONVERGED;
-        } else if (rho && rel <= floor_factor * floor_rel && it - stall > floor_window) {
+        } else if (near_floor && it - stall > floor_window) {
             active = 0;
             status = SYS_FLOOR;
         } else if (!isfinite(rel) || rel > 1e3 * nbest || it - stall > window) {
+            // past the floor, the replaced residual (rounding noise) no longer fits
+            // the recurrences and the iteration can jump away: the best iterate
+            // is then as accurate as FP64 allows
             active = 0;
-            status = SYS_STALLED;
+            status = near_floor ? SYS_FLOOR : SYS_STALLED;
         }
         if (lane == 0) {
             st.relres[k] = rel;

@@ -87,16 +87,18 @@ class FEPlan:
             self.assemble(crds, prop_beam, prop_quad)
 
     def symmetric(self) -> bool:
-        """False when the last assembly holds beam-columns parallel to global Y: the
-        reference's transformation for them is not orthogonal (element.py:92-94), K
-        is not symmetric, solves run on the dense LU path and adjoints need K^T."""
+        """False when the last assembly holds beam-columns parallel to global Y (the
+        reference's transformation for them is not orthogonal, element.py:92-94) or
+        quads with kx_mod != ky_mod (non-symmetric Cm, element.py:852-875): K is not
+        symmetric, solves run on the dense LU path and adjoints need K^T."""
         out = ctypes.c_int32(1)
         nat.check(nat.lib().jsso_plan_symmetric(self._h, ctypes.byref(out), nat.stream_ptr()))
         return bool(out.value)
 
     def solve_transpose(self, f: torch.Tensor, **kw) -> torch.Tensor:
         """K^T u = f (u = 0 at supports): the adjoint solve.  K is symmetric unless
-        the assembly holds members parallel to global Y (then the dense LU of K^T)."""
+        the assembly holds members parallel to global Y or orthotropic quads (then
+        the dense LU of K^T)."""
         if self.symmetric():
             return self.solve(f, **kw)
         return self.dense_solve(f, transpose=True)
@@ -108,6 +110,24 @@ class FEPlan:
             raise ValueError(f"unknown preconditioner {name!r} ({sorted(PRECONDITIONERS)})")
         nat.check(nat.load_library().jsso_plan_set_preconditioner(self._h, PRECONDITIONERS[name]))
         self.preconditioner = name
+
+    def element_tuning(self, set_epw: int = -1) -> int:
+        """MITC-4 elements per workgroup of the assembly kernel (4, 8 or 16): 0 until
+        the first assembly has timed the variants on this plan; set_epw 4/8/16
+        forces one, 0 re-tunes at the next assembly."""
+        out = ctypes.c_int32(0)
+        nat.check(nat.load_library().jsso_plan_element_tuning(self._h, int(set_epw), ctypes.byref(out)))
+        return int(out.value)
+
+    def amg_reuse(self, mode: int = -1) -> dict:
+        """Reuse of the AMG hierarchy across assemblies (optimisation loops): mode 0
+        rebuilds it after every assembly (default), 1 keeps the hierarchy of an
+        earlier design and refreshes only the level-0 smoothing bound, 2 rebuilds
+        once at the next solve and then keeps it; -1 only reads.  Returns the mode
+        and the counts of full set-ups and refreshes run so far."""
+        out = np.zeros(3, np.int64)
+        nat.check(nat.load_library().jsso_plan_amg_reuse(self._h, int(mode), nat.ptr(out)))
+        return {"mode": int(out[0]), "full_setups": int(out[1]), "refreshes": int(out[2])}
 
     def amg_levels(self) -> list:
         """Block rows of every level of the plan's AMG hierarchy (built on the host)."""
@@ -220,6 +240,34 @@ class FEPlan:
         """Expand block values (nnzb,6,6) to a host scipy CSR matrix."""
         rowptr, colind = self.pattern()
         return blocks_to_csr(rowptr, colind, Kvals.detach().cpu().numpy())
+
+
+class AmgKeepPolicy:
+    """Keeps a plan's AMG hierarchy across the assemblies of an optimisation loop
+    (``FEPlan.amg_reuse`` mode 1: after each assembly only the level-0 smoothing
+    bound follows the new design) and rebuilds it once the CG iteration count has
+    grown by more than ``growth`` over the count of the first solve after the
+    last rebuild.  Call :meth:`after_solve` after every solve."""
+
+    def __init__(self, plan: "FEPlan", growth: float = 1.25):
+        self.plan = plan
+        self.growth = float(growth)
+        self.base = None
+        self.rebuilds = 0
+        plan.amg_reuse(2)          # build at the next solve, then keep
+
+    def after_solve(self) -> bool:
+        """True when this solve's iteration count triggers a rebuild at the next solve."""
+        it = self.plan.last_info["iterations"]
+        if self.base is None:
+            self.base = it
+            return False
+        if it > self.growth * self.base:
+            self.plan.amg_reuse(2)
+            self.base = None
+            self.rebuilds += 1
+            return True
+        return False
 
 
 def blocks_to_csr(rowptr, colind, vals):

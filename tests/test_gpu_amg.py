@@ -126,3 +126,78 @@ def test_amg_unfused_coarse_levels_match_oracle(monkeypatch, nrhs):
             for c in range(min(nrhs, cases.shape[0])):
                 assert rel(u[o:o + st.ndof, c], oracle_u(st, cases[c])) < 1e-8, (fuse, i, c, info)
     assert abs(its["1"] - its["0"]) <= 8, its
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_amg_level0_wave_variants_agree(precision):
+    """Every waves-per-slice variant of the AMG level-0 kernels (1, 2, 4) solves a
+    batch of uneven shells to the oracle; the variants differ only in summation
+    order (1e-9); an untuned plan times them on the live problem, records its
+    choice, reports it in the solve info and repeats itself bit for bit."""
+    designs = [wl.cfg1(0), wl.shell_pinned_edges(12, 8.0, 1.0, design=2)]
+    b = StructureBatch(designs)
+    assert b.plan.cg_tuning(precision, 1) == 0
+    u0 = b.solve(precision=precision, tol=1e-10).cpu().numpy()
+    chosen = b.plan.cg_tuning(precision, 1)
+    assert chosen in (1, 2, 4) and b.plan.last_info["waves_per_slice"] == chosen
+    u0b = b.solve(precision=precision, tol=1e-10).cpu().numpy()
+    assert np.array_equal(u0, u0b)
+    refs = []
+    for st in designs:
+        K = system.assemble_K(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads)
+        refs.append(osolve.solve_reduced(K, st.known_id, st.nodal_loads))
+    sols = {}
+    for wps in (1, 2, 4):
+        b.plan.cg_tuning(precision, 1, set_wps=wps)
+        u = b.solve(precision=precision, tol=1e-10).cpu().numpy()
+        assert b.plan.last_info["converged"] == 2
+        for i, st in enumerate(designs):
+            o = 6 * int(b.node_off[i])
+            assert rel(u[o:o + st.ndof], refs[i]) < 1e-8, (wps, i)
+        sols[wps] = u
+    assert rel(sols[2], sols[1]) < 1e-9 and rel(sols[4], sols[1]) < 1e-9
+
+
+def test_element_kernel_tile_variants_agree():
+    """The assembly's MITC-4 kernel at 4, 8 and 16 elements per workgroup writes
+    the same K bit for bit; the first assembly records a timed choice."""
+    st = wl.cfg1(3)
+    from jaxsso_amd.plan import FEPlan
+    plan = FEPlan([0, st.n_node], st.cnct_beamcols, st.cnct_quads, st.known_id)
+    assert plan.element_tuning() == 0
+    crds, pq = torch.tensor(st.crds), torch.tensor(st.prop_quads)
+    K0 = plan.assemble(crds, None, pq, export_K=True).clone()
+    assert plan.element_tuning() in (4, 8, 16)
+    for epw in (4, 8, 16):
+        plan.element_tuning(epw)
+        K = plan.assemble(crds, None, pq, export_K=True)
+        assert torch.equal(K, K0), epw
+
+
+def test_amg_hierarchy_kept_across_assemblies():
+    """jsso_plan_amg_reuse: a hierarchy built for one thickness field serves the
+    next designs (only the level-0 smoothing bound is refreshed): the solution
+    still matches the oracle, the counts show one full set-up and the refreshes,
+    and a forced rebuild (mode 2) runs one more full set-up."""
+    st = wl.cfg1(0)
+    b = StructureBatch([st])
+    info0 = None
+    b.plan.amg_reuse(2)
+    rng = np.random.default_rng(5)
+    t0 = b.prop_quad[:, 0].clone()
+    for k in range(3):
+        if k:
+            b.prop_quad[:, 0] = t0 * torch.tensor(rng.uniform(0.9, 1.1, st.n_quad), device="cuda")
+        u = b.solve(precision="fp64", tol=1e-10).cpu().numpy()
+        info0 = info0 or dict(b.plan.last_info)
+        pq = b.prop_quad.cpu().numpy()
+        K = system.assemble_K(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, pq)
+        ref = osolve.solve_reduced(K, st.known_id, st.nodal_loads)
+        assert rel(u, ref) < 1e-8, k
+        assert b.plan.last_info["converged"] == 1
+    c = b.plan.amg_reuse(-1)
+    assert c["mode"] == 1 and c["full_setups"] == 1 and c["refreshes"] == 2, c
+    b.plan.amg_reuse(2)
+    b.solve(precision="fp64", tol=1e-10)
+    assert b.plan.amg_reuse(-1)["full_setups"] == 2
+    b.plan.amg_reuse(0)

@@ -1,4 +1,5 @@
-"""Structures with beam-columns exactly parallel to global Y.
+"""Structures whose K is not symmetric: beam-columns exactly parallel to global
+Y, and MITC-4 quads with kx_mod != ky_mod (the last test).
 
 The reference transforms such a member with a non-orthogonal matrix
 (JaxSSO/element.py:92-94), so its K is not symmetric, and solves it directly
@@ -133,3 +134,41 @@ def test_user_objective_transposed_adjoint():
         Em[k] -= h
         fd.append((obj(Ep) - obj(Em)) / (2 * h))
     assert rel(g, fd) < 1e-6
+
+
+def orthotropic_shell(n=6):
+    """A shell with kx_mod != ky_mod: Cm of element.py:852-875 is
+    [[Ex, nu Ex, 0], [nu Ey, Ey, 0], [0, 0, G]] with Ex != Ey, so Ke and K are
+    not symmetric either."""
+    st = wl.shell_pinned_edges(n, 6.0, 1.0, design=3)
+    st.prop_quads[:, 3] = 1.2
+    st.prop_quads[:, 4] = 0.8
+    return st
+
+
+def test_orthotropic_shell_solve_and_gradients():
+    st = orthotropic_shell()
+    Ko = system.assemble_K(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads)
+    assert abs(Ko - Ko.T).max() > 1e-3 * abs(Ko).max()           # K really is not symmetric
+    ref = og.compliance_and_gradient(st, faithful=True)
+    u_acc = osolve.solve_reduced(Ko, st.known_id, st.nodal_loads)
+    b = StructureBatch([st])
+    b.assemble()
+    assert not b.plan.symmetric()
+    for prec in ("fp64", "mixed"):
+        u = b.solve(precision=prec).cpu().numpy()
+        assert rel(u, u_acc) < 1e-10, prec
+    assert not b.plan.symmetric()
+    C, (dc, _, dq) = b.compliance_and_gradient(precision="fp64")
+    assert abs(float(C[0, 0]) - ref["C"]) / ref["C"] < 1e-8
+    assert rel(dq.cpu().numpy(), ref["dprop_quad"]) < 1e-7
+    assert rel(dc.cpu().numpy(), ref["dcrds"]) < 1e-7
+    # lambda = K^-T f/2 differs from u/2: the symmetric shortcut would be wrong here
+    lam_sym = og.element_gradients(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads,
+                                   ref["u"], 0.5 * ref["u"])["quad_props"]
+    assert rel(lam_sym, ref["dprop_quad"]) > 1e-4
+    # kx_mod == ky_mod keeps the symmetric CG route
+    st2 = wl.shell_pinned_edges(6, 6.0, 1.0, design=3)
+    b2 = StructureBatch([st2])
+    b2.solve(precision="fp64")
+    assert b2.plan.symmetric()
