@@ -171,6 +171,20 @@ int jsso_adjoint_grad(jsso_plan* plan, const double* d_crds, const double* d_pro
                       double lam_scale, int32_t nrhs, double* d_dcrds,
                       double* d_dprop_beam, double* d_dprop_quad, void* stream);
 
+/* jsso_adjoint_grad without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as jsso_adjoint_grad; the per-element gradient scratch is a fixed
+ * 64 MiB device array per GPU instead of plan workspace, so (12 per beam + 17 per
+ * quad) x nrhs doubles must fit in 8 Mi doubles (JSSO_EINVAL otherwise, with the
+ * sizes in the message).  The plan must already be on the device (any
+ * jsso_assemble puts it there).  Calls on one GPU share the array: calls that
+ * overlap on one GPU race. */
+int jsso_adjoint_grad_noalloc(jsso_plan* plan, const double* d_crds, const double* d_prop_beam,
+                              const double* d_prop_quad, const double* d_u, const double* d_lam,
+                              double lam_scale, int32_t nrhs, double* d_dcrds,
+                              double* d_dprop_beam, double* d_dprop_quad, void* stream);
+
 /* y = K x with the assembled, unconstrained K (nrhs = 1). */
 int jsso_spmv(jsso_plan* plan, const double* d_x, double* d_y, void* stream);
 

@@ -23,7 +23,7 @@ EXPORTS = ("jsso_version", "jsso_last_error", "jsso_plan_create", "jsso_plan_des
            "jsso_timing_enable", "jsso_timing_read", "jsso_plan_set_preconditioner",
            "jsso_plan_amg_levels", "jsso_timing_read_slot", "jsso_spmv_k", "jsso_dense_solve",
            "jsso_element_beam_local", "jsso_element_quad_local", "jsso_element_quad_gauss",
-           "jsso_plan_symmetric", "jsso_csr_spmv", "jsso_plan_element_tuning",
+           "jsso_plan_symmetric", "jsso_csr_spmv", "jsso_adjoint_grad_noalloc", "jsso_plan_element_tuning",
            "jsso_plan_amg_reuse")
 TIMING_SLOTS = {"solve": 0, "element": 1, "gather": 2, "adjoint": 3, "setup": 4}
 
@@ -83,6 +83,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.jsso_dense_solve.argtypes = [P, P, P, P, I32, I32, V]
     lib.jsso_plan_symmetric.argtypes = [P, P, V]
     lib.jsso_csr_spmv.argtypes = [I32, P, P, P, P, P, P, V]
+    lib.jsso_adjoint_grad_noalloc.argtypes = [P, P, P, P, P, P, D, I32, P, P, P, V]
     lib.jsso_plan_element_tuning.argtypes = [P, I32, P]
     lib.jsso_plan_amg_reuse.argtypes = [P, I32, P]
     lib.jsso_element_beam_local.argtypes = [I32, P, P, P, P, V]

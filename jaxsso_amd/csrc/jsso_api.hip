@@ -643,6 +643,38 @@ int jsso_adjoint_grad(jsso_plan* plan, const double* d_crds, const double* d_pro
     return rc;
 }
 
+int jsso_adjoint_grad_noalloc(jsso_plan* plan, const double* d_crds, const double* d_prop_beam,
+                              const double* d_prop_quad, const double* d_u, const double* d_lam, double lam_scale,
+                              int32_t nrhs, double* d_dcrds, double* d_dprop_beam, double* d_dprop_quad,
+                              void* stream) {
+    if (!plan || !d_crds || !d_u || nrhs < 1) {
+        set_error("invalid adjoint arguments");
+        return JSSO_EINVAL;
+    }
+    Plan* p = P(plan);
+    if ((p->n_beam > 0 && !d_prop_beam) || (p->n_quad > 0 && !d_prop_quad)) {
+        set_error("missing element properties");
+        return JSSO_EINVAL;
+    }
+    // uploading the plan allocates: it must have happened before (jsso_assemble does it)
+    JSSO_REQUIRE(p->on_device, JSSO_EINVAL,
+                 "jsso_adjoint_grad_noalloc: the plan is not on the device yet (call jsso_assemble first)");
+    double* gbuf = nullptr;
+    size_t cap = 0;
+    int rc = noalloc_scratch(&gbuf, &cap);
+    if (rc) return rc;
+    const size_t need = ((size_t)p->n_beam * BDIR + (size_t)p->n_quad * QDIR) * nrhs * sizeof(double);
+    JSSO_REQUIRE(need <= cap, JSSO_EINVAL,
+                 "jsso_adjoint_grad_noalloc: needs %zu bytes of scratch ((12 per beam + 17 per quad) x %d load "
+                 "cases x 8), the fixed scratch holds %zu; use jsso_adjoint_grad",
+                 need, (int)nrhs, cap);
+    hipStream_t s = (hipStream_t)stream;
+    rc = launch_adjoint(p, d_crds, d_prop_beam, d_prop_quad, d_u, d_lam, lam_scale, nrhs, gbuf, d_dprop_beam,
+                        d_dprop_quad, s);
+    if (rc) return rc;
+    return launch_gather_grad(p, gbuf, nrhs, d_dcrds, s);
+}
+
 int jsso_spmv(jsso_plan* plan, const double* d_x, double* d_y, void* stream) {
     if (!plan || !d_x || !d_y) {
         set_error("NULL argument");

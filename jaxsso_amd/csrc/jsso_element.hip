@@ -753,6 +753,35 @@ int launch_adjoint(Plan* p, const double* crds, const double* prop_beam, const d
     return JSSO_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Fixed gradient scratch for jsso_adjoint_grad_noalloc: a file-scope device
+// array (one copy per GPU, held until the process ends), so that callers that
+// cannot allocate (graph capture, serving loops) get the adjoint without a
+// hipMalloc.  Its device address is looked up once per device with
+// hipGetSymbolAddress; host code never passes the array's name itself (that is
+// a host-side stand-in, not a device address).  Every call on a GPU shares
+// that GPU's array: calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+int noalloc_scratch(double** ptr, size_t* bytes) {
+    static std::mutex mu;
+    static void* dev_ptr[64] = {};
+    int dev = -1;
+    JSSO_HIP(hipGetDevice(&dev));
+    JSSO_REQUIRE(dev >= 0 && dev < 64, JSSO_EINVAL, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!dev_ptr[dev]) {
+        void* p = nullptr;
+        JSSO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        JSSO_REQUIRE(p, JSSO_EHIP, "hipGetSymbolAddress gave no address for the adjoint scratch");
+        dev_ptr[dev] = p;
+    }
+    *ptr = (double*)dev_ptr[dev];
+    *bytes = sizeof(double) * (size_t)NOALLOC_SCRATCH_DOUBLES;
+    return JSSO_OK;
+}
+
 int launch_gather_grad(Plan* p, const double* gbuf, int nrhs, double* dcrds, hipStream_t s) {
     JSSO_REQUIRE(p->on_device, JSSO_EINVAL, "plan not uploaded to the device");
     if (!dcrds || p->n_nodes == 0) return JSSO_OK;

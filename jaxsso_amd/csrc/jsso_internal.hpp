@@ -36,6 +36,9 @@ constexpr int BDIR = 12;   // beam adjoint directions: 6 coordinates + 6 propert
 constexpr int QDIR = 17;   // quad adjoint directions: 12 coordinates + 5 properties
 constexpr int WS_ADJ_SLOT = 12;  // Plan::ws slot of the adjoint's gradient scratch (solver slots are below it)
 constexpr int WS_FU_SLOT = 13;   // Plan::ws slot of the f.u reduction's per-slice partials
+// fixed device scratch of jsso_adjoint_grad_noalloc: 8 Mi doubles = 64 MiB per GPU
+// ((12 per beam + 17 per quad) x load cases: e.g. 490k quads with one load case)
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = size_t(1) << 23;
 constexpr int ROWS_PER_WG = 64;   // block rows per solver slice (one wavefront in the SpMV)
 // batched-solver variants: matrix type MT x vector type VT
 enum { PS_F32 = 0, PS_F64 = 1, PS_F32_V64 = 2 };
@@ -184,6 +187,7 @@ int launch_adjoint(Plan* p, const double* crds, const double* prop_beam, const d
                    const double* u, const double* lam, double lam_scale, int nrhs, double* gbuf,
                    double* dprop_beam, double* dprop_quad, hipStream_t s);
 int launch_gather_grad(Plan* p, const double* gbuf, int nrhs, double* dcrds, hipStream_t s);
+int noalloc_scratch(double** ptr, size_t* bytes);
 int launch_spmv_full(Plan* p, const double* K, const double* x, double* y, hipStream_t s);
 int solve_batched(Plan* p, const double* f, double* u, int nrhs, int slot, double tol,
                   int maxit, jsso_info* info, hipStream_t s);
