@@ -121,6 +121,8 @@ def simp_headline(args, L, nat):
         return C, dc, drho
 
     print(f"# SIMP batch: {args.designs} shells, {b.plan.ndof} DOF in one plan", flush=True)
+    if args.wps:
+        b.plan.cg_tuning(args.precision, 1, args.wps)
     for _ in range(args.warmup):
         step(args.precision)
     torch.cuda.synchronize()
@@ -143,7 +145,7 @@ def simp_headline(args, L, nat):
     print(f"# timed: {ms_per_step:.1f} ms/step, {value:.1f} evals/s, CG iterations {iters}, info {info}", flush=True)
 
     s_m, s_v, tname, _ = SIZES[args.precision]
-    kname = f"k_pcg_dir<{tname}, 1, 4>"
+    kname = f"k_pcg_dir<{tname}, 1, {info['waves_per_slice']}>"
     alg = _dir_bytes(b.plan, s_m, s_v)
     avg_ms, nsamp = t["solve"]
     achieved = alg / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
@@ -242,6 +244,9 @@ def shell200(args, L, nat, profile=False):
     solve_log = []
     paths = (("mixed", "mixed", "amg", 10), ("fp64", "fp64", "amg", 5), ("plain", "plain", None, 1))
     for key, prec, pc, reps in paths[:1] if profile else paths:
+        if args.wps and key == "mixed":
+            b.plan.set_preconditioner("amg")
+            b.plan.cg_tuning(prec, 1, args.wps)
         b.compliance_and_gradient(precision=prec, tol=args.tol, preconditioner=pc)     # warm-up
         if key == "mixed":
             i = b.plan.last_info
@@ -271,7 +276,7 @@ def shell200(args, L, nat, profile=False):
         if key == "mixed":
             L.jsso_timing_enable(0)
             t = {name: _slot(L, nat, name) for name in nat.TIMING_SLOTS}
-            kname = "k_pcg_dir<float, float, 1, 4>"
+            kname = f"k_pcg_dir<float, float, 1, {b.plan.last_info['waves_per_slice']}>"
             alg = _dir_bytes(b.plan, 4, 4)
             avg_ms, nsamp = t["solve"]
             ach = alg / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
@@ -438,11 +443,17 @@ def trace_summary(trace_dir, bench_log):
         print(f"{us / 1e3:10.2f} {n:7d} {us / n:9.2f}  {short}")
     d = [(s1 - s0) / 1e3 for s0, s1, name in rows if name.startswith(f"void jsso::{kname}")]
     log = out["solve_log"]["solves"]
+    # the live tuner's timing launches (first solve on the plan, before its own
+    # launches) are not part of any solve
+    extra = len(d) - sum(l for _, l, _ in log)
+    print(f"# {kname}: {len(d)} launches in the trace, {sum(l for _, l, _ in log)} expected from solve_log"
+          f"{f' (the first {extra}: the tuner)' if extra > 0 else ''}")
+    if extra > 0:
+        d = d[extra:]
     full, pos = [], 0
     for iters, launches, full_it in log:
         full.extend(d[pos:pos + full_it])
         pos += launches
-    print(f"# {kname}: {len(d)} launches in the trace, {sum(l for _, l, _ in log)} expected from solve_log")
     if full:
         alg = out["roofline"]["alg_bytes_per_launch"]
         avg = sum(full) / len(full)
@@ -450,6 +461,20 @@ def trace_summary(trace_dir, bench_log):
               f"avg {avg:.2f} us, min {min(full):.2f}, max {max(full):.2f}; "
               f"{alg} algorithmic bytes -> {alg / (avg * 1e-6) / 1e9:.1f} GB/s "
               f"(bench.py events: {out['roofline']['avg_launch_us']:.2f} us)")
+    # one iteration of the last solve with every design running (the kernels
+    # from one roofline-kernel launch to the next) and the whole solve's GPU time
+    starts = [(s0, s1) for s0, s1, name in rows if name.startswith(f"void jsso::{kname}")][extra if extra > 0 else 0:]
+    iters, launches, full_it = log[-1]
+    last = starts[len(starts) - launches:]
+    if full_it >= 2 and len(last) == launches:
+        a, b = last[full_it // 2][0], last[full_it // 2 + 1][0]
+        print(f"# one full iteration (iteration {full_it // 2 + 1} of the last solve): {(b - a) / 1e3:.1f} us "
+              f"from one {kname} start to the next")
+        for s0, s1, name in rows:
+            if a <= s0 < b:
+                print(f"#   +{(s0 - a) / 1e3:7.1f} us  {(s1 - s0) / 1e3:7.1f} us  {re.sub(chr(32) + '+', ' ', name)[:90]}")
+        print(f"# last solve: {iters} iterations ({full_it} with every design running), "
+              f"{(last[-1][0] - last[0][0]) / 1e6:.2f} ms from its first to its last {kname} launch")
 
 
 def traffic_summary(fetch_dir, write_dir, bench_log):
@@ -481,6 +506,9 @@ def traffic_summary(fetch_dir, write_dir, bench_log):
                         copies.append(v)
         vals.sort()
         v = [x for _, x in vals]
+        extra = len(v) - sum(l for _, l, _ in log)    # the live tuner's launches come first
+        if extra > 0:
+            v = v[extra:]
         full, pos = [], 0
         for iters, launches, full_it in log:
             full.extend(v[pos:pos + full_it])
@@ -523,6 +551,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle-check", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the headline batch")
+    ap.add_argument("--wps", type=int, choices=[1, 2, 4], default=None,
+                    help="pin the level-0 waves per slice instead of the live tuner's choice (profiled runs pin "
+                         "the unprofiled run's choice, so that every run times the same kernel)")
     ap.add_argument("--profile-run", nargs="?", const="simp", choices=["simp", "shell200"],
                     help="only the headline's warm-up and timed steps, or the 200x200 shell's mixed path "
                          "(for rocprofv3); implies --no-extras --no-cpu-baseline --no-oracle-check")

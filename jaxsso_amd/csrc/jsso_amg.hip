@@ -38,6 +38,8 @@
 #include <numeric>
 #include <type_traits>
 
+#include <hip/hip_ext.h>
+
 #include "solver_kernels.hpp"
 
 namespace jsso {
@@ -1266,8 +1268,8 @@ __device__ __forceinline__ void ell_apply(const Slices& sl, const MT* __restrict
 // thread of the workgroup copies the run, so each store instruction writes
 // consecutive addresses (whole cache lines rather than 4-byte pieces 24 bytes
 // apart).  Called by all threads.
-template <typename VT, int NR, int WPS, typename F>
-__device__ __forceinline__ void flush_rows(VT* stage, const int4& ma, F&& value, VT* __restrict__ dst) {
+template <typename VT, int NR, int WPS, typename F, typename DT>
+__device__ __forceinline__ void flush_rows(VT* stage, const int4& ma, F&& value, DT* __restrict__ dst) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (wave == 0) {
 #pragma unroll
@@ -1278,14 +1280,16 @@ __device__ __forceinline__ void flush_rows(VT* stage, const int4& ma, F&& value,
     __syncthreads();
     const int64_t base = (int64_t)ma.y * 6 * NR;
     const int n = (ma.z - ma.y) * 6 * NR;
-    for (int u = threadIdx.x; u < n; u += 64 * WPS) dst[base + u] = stage[u];
+    for (int u = threadIdx.x; u < n; u += 64 * WPS) dst[base + u] = (DT)stage[u];
     __syncthreads();
 }
 
 // M such runs (one per vector) through one staging round: the stores of M
-// flush_rows calls with one pair of barriers.  `stage` holds M * 64 * 6 * NR values.
+// flush_rows calls with one pair of barriers.  `stage` holds M * 64 * 6 * NR
+// values; run 0 goes to the FP32 vector dst0, runs 1.. to dst[1..].
 template <typename VT, int NR, int WPS, int M, typename F>
-__device__ __forceinline__ void flush_rows_m(VT* stage, const int4& ma, F&& value, VT* const (&dst)[M]) {
+__device__ __forceinline__ void flush_rows_m(VT* stage, const int4& ma, F&& value, float* __restrict__ dst0,
+                                             VT* const (&dst)[M]) {
     constexpr int S = 64 * 6 * NR;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (wave == 0) {
@@ -1299,9 +1303,11 @@ __device__ __forceinline__ void flush_rows_m(VT* stage, const int4& ma, F&& valu
     __syncthreads();
     const int64_t base = (int64_t)ma.y * 6 * NR;
     const int n = (ma.z - ma.y) * 6 * NR;
-    for (int u = threadIdx.x; u < n; u += 64 * WPS)
+    for (int u = threadIdx.x; u < n; u += 64 * WPS) {
+        dst0[base + u] = (float)stage[u];
 #pragma unroll
-        for (int m = 0; m < M; ++m) dst[m][base + u] = stage[m * S + u];
+        for (int m = 1; m < M; ++m) dst[m][base + u] = stage[m * S + u];
+    }
     __syncthreads();
 }
 
@@ -1387,7 +1393,7 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_pre(Slices sl, Partials P, Sys
                                                       const _Float16* __restrict__ A, const float* __restrict__ theta,
                                                       const VT* __restrict__ rprev, VT* __restrict__ rcur,
                                                       const VT* __restrict__ q, const VT* __restrict__ p,
-                                                      VT* __restrict__ xlo, VT* __restrict__ t0) {
+                                                      VT* __restrict__ xlo, float* __restrict__ t0) {
     const int w = slice_of_block(sl);
     if (w < 0) return;
     // (metadata requested together with the slice's flags, not after them)
@@ -1462,8 +1468,8 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_pre(Slices sl, Partials P, Sys
         return ((amask >> h) & 1) ? xv + alpha[h] * (PF ? pko[c][PF ? h : 0] : p[o]) : xv;
     };
     if constexpr (JOIN) {
-        VT* const dst[3] = {t0, rcur, xlo};
-        flush_rows_m<VT, NR, WPS, 3>(stage, ma, val, dst);
+        VT* const dst[3] = {nullptr, rcur, xlo};
+        flush_rows_m<VT, NR, WPS, 3>(stage, ma, val, t0, dst);
     } else {
         flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) { return val(0, c, h); }, t0);
         if (!FRESH) {
@@ -1480,7 +1486,7 @@ template <typename VT, int NR, int WPS>
 __global__ __launch_bounds__(64 * WPS) void k_pcg_post(Slices sl, Partials P, SysState st, int k,
                                                        const _Float16* __restrict__ A, const float* __restrict__ theta,
                                                        const VT* __restrict__ r, const VT* __restrict__ rprev,
-                                                       const VT* __restrict__ z1, VT* __restrict__ z) {
+                                                       const float* __restrict__ z1, VT* __restrict__ z) {
     const int w = slice_of_block(sl);
     if (w < 0) return;
     const int4 ma = sl.meta[3 * w], mb = sl.meta[3 * w + 1];
@@ -1490,7 +1496,7 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_post(Slices sl, Partials P, Sy
     const int64_t i = (int64_t)ma.y + lane;
     const bool on = i < ma.z;
     const int64_t ii = on ? i : (int64_t)ma.y;
-    auto zz = [&](int64_t node, int c, int h) { return z1[(6 * node + c) * NR + h]; };
+    auto zz = [&](int64_t node, int c, int h) { return (VT)z1[(6 * node + c) * NR + h]; };
     const VT th = (VT)theta[ma.x];
     VT acc[6][NR];
 #pragma unroll
@@ -1689,14 +1695,14 @@ __global__ __launch_bounds__(256) void k_mg_restrict(int32_t nc, const int32_t* 
 // fine DOF row (i, r) of the structures that have a coarser level:
 //   level 0 (cheb null):  z1 = b / theta + P x_c
 //   coarse levels:        z1 = c (b + t) + P x_c   (t = b - A b / theta from SM_PRE1)
-template <typename VT, int NR>
+template <typename VT, typename ZT, int NR>
 __global__ __launch_bounds__(256) void k_mg_prolong(int32_t n, const int32_t* __restrict__ node_sys,
                                                     const uint8_t* __restrict__ cont, const int32_t* __restrict__ p_ptr,
                                                     const int32_t* __restrict__ p_col, const _Float16* __restrict__ P,
                                                     const float* __restrict__ theta, const float* __restrict__ cheb,
                                                     SysState st, const VT* __restrict__ b,
                                                     const float* __restrict__ t, const float* __restrict__ xc,
-                                                    VT* __restrict__ z1) {
+                                                    ZT* __restrict__ z1) {
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= 6 * (int64_t)n) return;
     const int i = (int)(u / 6), r = (int)(u % 6);
@@ -1729,7 +1735,7 @@ __global__ __launch_bounds__(256) void k_mg_prolong(int32_t n, const int32_t* __
             for (int h = 0; h < NR; ++h) acc[h] += (float)pb[c] * xc[(6 * J + c) * NR + h];
     }
 #pragma unroll
-    for (int h = 0; h < NR; ++h) z1[u * NR + h] = (VT)acc[h];
+    for (int h = 0; h < NR; ++h) z1[u * NR + h] = (ZT)acc[h];
 }
 
 // Coarsest levels: x = A^-1 b with the dense (symmetric) inverse of each
@@ -2368,7 +2374,8 @@ template <typename VT>
 struct AmgBufs {
     VT* r[2];
     VT* p[2];
-    VT *q, *z, *t0, *xlo;
+    VT *q, *z, *xlo;
+    float* t0;   // level-0 preconditioner vector (pre-smoothing residual, then z1): FP32 in every precision
 };
 
 // The V-cycle after the level-0 pre-smoothing of iteration k (t0 holds
@@ -2388,9 +2395,9 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
                                                                                A.host[l].cont.end(), 1);
         if (any_restrict) {
             if (l == 0)
-                hipLaunchKernelGGL((k_mg_restrict<VT, NR>), gr, dim3(256), 0, s, C.n, C.d_node_sys,
+                hipLaunchKernelGGL((k_mg_restrict<float, NR>), gr, dim3(256), 0, s, C.n, C.d_node_sys,
                                    (const uint8_t*)A.d_rskip[l], D.d_pt_ptr, D.d_pt_row, (const _Float16*)D.d_PT16,
-                                   w.st, (const VT*)bf.t0, C.d_b);
+                                   w.st, (const float*)bf.t0, C.d_b);
             else
                 hipLaunchKernelGGL((k_mg_restrict<float, NR>), gr, dim3(256), 0, s, C.n, C.d_node_sys,
                                    (const uint8_t*)A.d_rskip[l], D.d_pt_ptr, D.d_pt_row, (const _Float16*)D.d_PT16,
@@ -2441,7 +2448,7 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
         const AmgLevelDev& C = A.dev[l + 1];
         if (C.n == 0) continue;
         if (l == 0) {
-            hipLaunchKernelGGL((k_mg_prolong<VT, NR>), dim3(nblk(6 * (int64_t)D.n, 256)), dim3(256), 0, s, D.n,
+            hipLaunchKernelGGL((k_mg_prolong<VT, float, NR>), dim3(nblk(6 * (int64_t)D.n, 256)), dim3(256), 0, s, D.n,
                                D.d_node_sys, (const uint8_t*)A.d_smooth[0], D.d_p_ptr, D.d_p_col, (const _Float16*)D.d_P16,
                                (const float*)D.d_theta, (const float*)nullptr, w.st, (const VT*)bf.r[k & 1],
                                (const float*)nullptr, (const float*)C.d_x, bf.t0);
@@ -2450,11 +2457,11 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
                 hipLaunchKernelGGL((k_pcg_post<VT, NR, WP>), slice_grid(pl->n_wg), dim3(64 * WP), 0, s,
                                    level_slices(pl, A, 0), w.P, w.st, k, (const _Float16*)pl->d_Ke16,
                                    (const float*)D.d_theta,
-                                   (const VT*)bf.r[k & 1], (const VT*)bf.r[(k - 1) & 1], (const VT*)bf.t0, bf.z);
+                                   (const VT*)bf.r[k & 1], (const VT*)bf.r[(k - 1) & 1], (const float*)bf.t0, bf.z);
             });
         } else if (A.tail_list[l].size() < (size_t)std::count(A.host[l].cont.begin(), A.host[l].cont.end(), 1)) {
             // z1 = c (b + t) + P x_c into r, r0 = b - A z1 into t, x = z1 + c (2 r0 - A r0 / theta)
-            hipLaunchKernelGGL((k_mg_prolong<float, NR>), dim3(nblk(6 * (int64_t)D.n, 256)), dim3(256), 0, s, D.n,
+            hipLaunchKernelGGL((k_mg_prolong<float, float, NR>), dim3(nblk(6 * (int64_t)D.n, 256)), dim3(256), 0, s, D.n,
                                D.d_node_sys, (const uint8_t*)A.d_smooth[l], D.d_p_ptr, D.d_p_col,
                                (const _Float16*)D.d_P16, (const float*)D.d_theta, (const float*)D.d_cheb, w.st,
                                (const float*)D.d_b, (const float*)D.d_t, (const float*)C.d_x, D.d_r);
@@ -2486,15 +2493,17 @@ static void amg_step(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs<VT
     wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
         constexpr int WP = decltype(W)::value;
         std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
-        if (timed) {
-            ev = timing_pair();
-            if (ev.first) (void)hipEventRecord(ev.first, s);
-        }
-        hipLaunchKernelGGL((k_pcg_dir<MT, VT, NR, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A0,
-                           (const VT*)bf.z, (const VT*)bf.p[(k - 1) & 1], bf.p[k & 1], bf.q);
+        if (timed) ev = timing_pair();
         if (ev.first) {
-            (void)hipEventRecord(ev.second, s);
+            // the dispatch itself stamps the events (kernel start and end, as a
+            // profiler sees them), not the stream around it
+            hipExtLaunchKernelGGL((k_pcg_dir<MT, VT, NR, WP>), g, dim3(64 * WP), 0, s, ev.first, ev.second, 0, sl,
+                                  w.P, w.st, k, A0, (const VT*)bf.z, (const VT*)bf.p[(k - 1) & 1], bf.p[k & 1],
+                                  bf.q);
             timed->push_back(ev);
+        } else {
+            hipLaunchKernelGGL((k_pcg_dir<MT, VT, NR, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A0,
+                               (const VT*)bf.z, (const VT*)bf.p[(k - 1) & 1], bf.p[k & 1], bf.q);
         }
         hipLaunchKernelGGL((k_pcg_pre<VT, NR, false, WP>), g, dim3(64 * WP), 0, s, sl, w.P, w.st, k, A16,
                            (const float*)A.dev[0].d_theta, (const VT*)bf.r[(k - 1) & 1], bf.r[k & 1],
@@ -2643,7 +2652,7 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
     bf.p[1] = (VT*)A.d_p1;
     bf.q = (VT*)w.q[0];
     bf.z = (VT*)A.d_z;
-    bf.t0 = (VT*)A.d_t0;
+    bf.t0 = (float*)A.d_t0;
     bf.xlo = (VT*)w.xlo;
     const dim3 gridS = slice_grid(pl->n_wg);
     // start: b, x = 0, r_0 = b, p = 0; both partial buffers cleared (so the
