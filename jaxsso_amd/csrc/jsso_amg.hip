@@ -474,7 +474,7 @@ __global__ __launch_bounds__(256) void k_amg_null0(int32_t n, const int32_t* __r
                                                    const int32_t* __restrict__ agg_nodes,
                                                    const double* __restrict__ crds, const double* __restrict__ Linv,
                                                    const uint8_t* __restrict__ fixed, double* __restrict__ B) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)(xcd_wg() * blockDim.x + threadIdx.x);
     if (i >= n) return;
     double* out = B + (int64_t)i * 36;
     const int a = agg[i];
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(64) void k_amg_qr(int32_t na, const int32_t* __rest
                                                const double* __restrict__ crds, const int32_t* __restrict__ node_sys,
                                                const int32_t* __restrict__ node_off, double* __restrict__ Q,
                                                double* __restrict__ R) {
-    const int a = blockIdx.x;
+    const int a = (int)xcd_wg();
     if (a >= na) return;
     const int lane = threadIdx.x;
     const int m0 = agg_ptr[a], nrow = 6 * (agg_ptr[a + 1] - m0);
@@ -732,7 +732,7 @@ __global__ __launch_bounds__(256) void k_amg_psmooth(int64_t nP, const int32_t* 
                                                      const int32_t* __restrict__ node_sys,
                                                      const double* __restrict__ omega, const double* __restrict__ An,
                                                      const double* __restrict__ Q, double* __restrict__ P) {
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t b = xcd_wg() * blockDim.x + threadIdx.x;
     if (b >= nP) return;
     const double om = omega[node_sys[p_row[b]]];
     double out[36];
@@ -769,7 +769,7 @@ __global__ __launch_bounds__(256) void k_amg_w(int64_t nW, const int32_t* __rest
                                                const int32_t* __restrict__ w_caref, const int32_t* __restrict__ w_cp,
                                                const double* __restrict__ An, const double* __restrict__ P,
                                                double* __restrict__ W) {
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t b = xcd_wg() * blockDim.x + threadIdx.x;
     if (b >= nW) return;
     double out[36];
 #pragma unroll
@@ -806,7 +806,7 @@ __global__ __launch_bounds__(256) void k_amg_galerkin(int64_t nnzc, const int32_
                                                       const int32_t* __restrict__ c_p, const int32_t* __restrict__ c_w,
                                                       const double* __restrict__ P, const double* __restrict__ W,
                                                       double* __restrict__ Ac) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = xcd_wg() * blockDim.x + threadIdx.x;
     if (k >= nnzc) return;
     double out[36];
 #pragma unroll
@@ -891,7 +891,7 @@ __global__ __launch_bounds__(256) void k_amg_cscale(int64_t nval, const int32_t*
                                                     const int32_t* __restrict__ colind, const double* __restrict__ Ac,
                                                     const double* __restrict__ Lci, double* __restrict__ A64,
                                                     float* __restrict__ A32) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = xcd_wg() * blockDim.x + threadIdx.x;
     if (t >= nval * 36) return;
     const int64_t vp = t / 36;
     const int e = (int)(t % 36), r = e / 6, c = e % 6;
@@ -920,7 +920,7 @@ __global__ __launch_bounds__(256) void k_amg_pscale(int64_t nP, const int32_t* _
                                                     const int32_t* __restrict__ p_tpos,
                                                     const double* __restrict__ P, const double* __restrict__ Lci,
                                                     _Float16* __restrict__ P16, _Float16* __restrict__ PT16) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = xcd_wg() * blockDim.x + threadIdx.x;
     if (t >= nP * 36) return;
     const int64_t b = t / 36;
     const int e = (int)(t % 36), r = e / 6, c = e % 6;
@@ -936,7 +936,7 @@ __global__ __launch_bounds__(256) void k_amg_pscale(int64_t nP, const int32_t* _
 // coarse null space in the scaled coarse variables: B_J = Lc_J^T R_J
 __global__ __launch_bounds__(256) void k_amg_nullc(int32_t nc, const double* __restrict__ Lc,
                                                    const double* __restrict__ R, double* __restrict__ B) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = xcd_wg() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nc * 36) return;
     const int64_t J = t / 36;
     const int e = (int)(t % 36), r = e / 6, c = e % 6;
@@ -2250,11 +2250,11 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
         const Slices sl = level_slices(pl, A, l);
         const int32_t* sys_wg_off = l == 0 ? pl->d_sys_wg_off : D.d_sys_wg_off;
         if (l == 0)
-            hipLaunchKernelGGL(k_amg_null0, dim3(nblk(D.n, 256)), dim3(256), 0, s, D.n, D.d_agg, D.d_agg_ptr,
+            hipLaunchKernelGGL(k_amg_null0, xcd_grid(nblk(D.n, 256)), dim3(256), 0, s, D.n, D.d_agg, D.d_agg_ptr,
                                D.d_agg_nodes, (const double*)pl->d_crds, (const double*)pl->d_Linv,
                                (const uint8_t*)pl->d_fixed, D.d_B);
         if (D.n_agg > 0)
-            hipLaunchKernelGGL(k_amg_qr, dim3((unsigned)D.n_agg), dim3(64), 0, s, D.n_agg, D.d_agg_ptr,
+            hipLaunchKernelGGL(k_amg_qr, xcd_grid(D.n_agg), dim3(64), 0, s, D.n_agg, D.d_agg_ptr,
                                D.d_agg_nodes, (const double*)D.d_B, l == 0 ? (const double*)pl->d_crds : nullptr,
                                D.d_node_sys, A.d_node_off0, D.d_Q, D.d_R);
         for (int step = 0; step < AMG_POWER_STEPS; ++step) {
@@ -2274,30 +2274,30 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
             const int64_t npos = l == 0 ? pl->ell_slots : D.ell_slots;
             hipLaunchKernelGGL(k_amg_unpack, dim3(nblk(npos * 18, 256)), dim3(256), 0, s, npos, (const double2*)A64,
                                (double2*)A.d_Anat);
-            hipLaunchKernelGGL(k_amg_psmooth, dim3(nblk(D.nP, 256)), dim3(256), 0, s, D.nP, D.d_p_row,
+            hipLaunchKernelGGL(k_amg_psmooth, xcd_grid(nblk(D.nP, 256)), dim3(256), 0, s, D.nP, D.d_p_row,
                                D.d_p_cptr, D.d_p_cj, D.d_p_caref, D.d_node_sys, (const double*)D.d_omega,
                                (const double*)A.d_Anat, (const double*)D.d_Q, D.d_P64);
-            hipLaunchKernelGGL(k_amg_w, dim3(nblk(D.nW, 256)), dim3(256), 0, s, D.nW, D.d_w_cptr, D.d_w_caref,
+            hipLaunchKernelGGL(k_amg_w, xcd_grid(nblk(D.nW, 256)), dim3(256), 0, s, D.nW, D.d_w_cptr, D.d_w_caref,
                                D.d_w_cp, (const double*)A.d_Anat, (const double*)D.d_P64, D.d_W64);
         }
         const int64_t nnzc = A.host[l + 1].colind.size();
         if (nnzc > 0) {
-            hipLaunchKernelGGL(k_amg_galerkin, dim3(nblk(nnzc, 256)), dim3(256), 0, s, nnzc, D.d_c_cptr, D.d_c_p,
+            hipLaunchKernelGGL(k_amg_galerkin, xcd_grid(nblk(nnzc, 256)), dim3(256), 0, s, nnzc, D.d_c_cptr, D.d_c_p,
                                D.d_c_w, (const double*)D.d_P64, (const double*)D.d_W64, A.d_Ac[l + 1]);
             hipLaunchKernelGGL(k_amg_cfactor, dim3(nblk(C.n, 128)), dim3(128), 0, s, C.n, A.d_cdiag[l + 1],
                                (const double*)A.d_Ac[l + 1], A.d_Lc[l + 1], A.d_Lci[l + 1]);
             const int64_t nval = C.ell_slots - ROWS_PER_WG;
             if (nval > 0)
-                hipLaunchKernelGGL(k_amg_cscale, dim3(nblk(nval * 36, 256)), dim3(256), 0, s, nval, C.d_ell_src,
+                hipLaunchKernelGGL(k_amg_cscale, xcd_grid(nblk(nval * 36, 256)), dim3(256), 0, s, nval, C.d_ell_src,
                                    C.d_browof, C.d_colind, (const double*)A.d_Ac[l + 1],
                                    (const double*)A.d_Lci[l + 1], C.d_A64, C.d_A32);
         }
         if (D.nP > 0)
-            hipLaunchKernelGGL(k_amg_pscale, dim3(nblk(D.nP * 36, 256)), dim3(256), 0, s, D.nP, D.d_p_col,
+            hipLaunchKernelGGL(k_amg_pscale, xcd_grid(nblk(D.nP * 36, 256)), dim3(256), 0, s, D.nP, D.d_p_col,
                                D.d_p_tpos, (const double*)D.d_P64, (const double*)A.d_Lci[l + 1], D.d_P16,
                                D.d_PT16);
         if (C.d_B && C.n > 0)
-            hipLaunchKernelGGL(k_amg_nullc, dim3(nblk((int64_t)C.n * 36, 256)), dim3(256), 0, s, C.n,
+            hipLaunchKernelGGL(k_amg_nullc, xcd_grid(nblk((int64_t)C.n * 36, 256)), dim3(256), 0, s, C.n,
                                (const double*)A.d_Lc[l + 1], (const double*)D.d_R, C.d_B);
         (void)H;
     }

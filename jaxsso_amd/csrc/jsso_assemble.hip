@@ -52,7 +52,7 @@ __global__ __launch_bounds__(128) void diag_factor_kernel(int32_t n_nodes, const
                                                           const double* __restrict__ ke,
                                                           double* __restrict__ Linv, double* __restrict__ Minv,
                                                           int32_t* status) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)(xcd_wg() * blockDim.x + threadIdx.x);
     if (i >= n_nodes) return;
     double A[6][6], L[6][6], Li[6][6];
     double B[36];
@@ -250,7 +250,7 @@ int launch_assemble(Plan* p, hipStream_t s) {
     JSSO_REQUIRE(p->on_device, JSSO_EINVAL, "plan not uploaded to the device");
     p->full_valid = false;
     if (p->n_nodes > 0) {
-        hipLaunchKernelGGL(diag_factor_kernel, dim3((p->n_nodes + 127) / 128), dim3(128), 0, s, p->n_nodes,
+        hipLaunchKernelGGL(diag_factor_kernel, xcd_grid((p->n_nodes + 127) / 128), dim3(128), 0, s, p->n_nodes,
                            p->d_diag, p->d_cptr, p->d_csrc, p->d_fixed, p->d_ke, p->d_Linv, p->d_Minv,
                            p->d_status);
         JSSO_HIP(hipGetLastError());

@@ -631,10 +631,12 @@ __global__ __launch_bounds__(256) void adj_quad_fwd(int32_t n, int32_t n_beam, c
                                                     const double* __restrict__ lam, double lam_scale, int nrhs,
                                                     int rhs, double* __restrict__ gbuf,
                                                     double* __restrict__ dprop) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)QA_FWD * n) return;
-    const int k = (int)(t / n);
-    const int64_t e = t % n;
+    // workgroup g: direction g % QA_FWD of the 256 elements of chunk g / QA_FWD, so
+    // that the 11 workgroups of a chunk (one XCD, in sequence) share its inputs in L2
+    const int64_t g = xcd_wg();
+    const int k = (int)(g % QA_FWD);
+    const int64_t e = (g / QA_FWD) * 256 + threadIdx.x;
+    if (e >= n) return;
     int nd[4];
     V3<double> R[3];
     double xy[8];
@@ -662,7 +664,7 @@ __global__ __launch_bounds__(256) void adj_quad_frame(int32_t n, int32_t n_beam,
                                                       const double* __restrict__ u, const double* __restrict__ lam,
                                                       double lam_scale, int nrhs, int rhs,
                                                       double* __restrict__ gbuf) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t e = xcd_wg() * blockDim.x + threadIdx.x;
     if (e >= n) return;
     int nd[4];
     V3<double> R[3];
@@ -683,7 +685,7 @@ __global__ __launch_bounds__(12 * QA_E) void adj_quad_combine(int32_t n, int32_t
                                                               double* __restrict__ gbuf) {
     __shared__ double part[QA_E][15];   // dphi/dxy (6 free entries), dphi/dR (9)
     const int tid = threadIdx.x;
-    const int64_t e0 = (int64_t)blockIdx.x * QA_E;
+    const int64_t e0 = xcd_wg() * QA_E;
     for (int v = tid; v < QA_E * 15; v += blockDim.x) {
         const int el = v / 15, q = v % 15;
         if (e0 + el < n) part[el][q] = gbuf[quad_slot(n_beam, e0 + el, q, nrhs, rhs)];
@@ -716,7 +718,7 @@ __global__ __launch_bounds__(12 * QA_E) void adj_quad_combine(int32_t n, int32_t
 __global__ void gather_grad_kernel(int32_t n_nodes, int nrhs, const int32_t* __restrict__ nptr,
                                    const int32_t* __restrict__ nsrc, const double* __restrict__ gbuf,
                                    double* __restrict__ dcrds) {
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t tid = xcd_wg() * blockDim.x + threadIdx.x;
     if (tid >= (int64_t)n_nodes * 3 * nrhs) return;
     const int rhs = (int)(tid % nrhs);
     const int64_t nc = tid / nrhs;
@@ -740,12 +742,12 @@ int launch_adjoint(Plan* p, const double* crds, const double* prop_beam, const d
     if (p->n_quad > 0) {
         const int nq = p->n_quad;
         for (int rhs = 0; rhs < nrhs; ++rhs) {
-            hipLaunchKernelGGL(adj_quad_fwd, dim3((unsigned)(((int64_t)QA_FWD * nq + 255) / 256)), dim3(256), 0, s,
+            hipLaunchKernelGGL(adj_quad_fwd, xcd_grid((int64_t)QA_FWD * ((nq + 255) / 256)), dim3(256), 0, s,
                                nq, p->n_beam, p->d_cnct_quad, crds, prop_quad, u, lam, lam_scale, nrhs, rhs, gbuf,
                                dprop_quad);
-            hipLaunchKernelGGL(adj_quad_frame, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, nq, p->n_beam,
+            hipLaunchKernelGGL(adj_quad_frame, xcd_grid((nq + 255) / 256), dim3(256), 0, s, nq, p->n_beam,
                                p->d_cnct_quad, crds, prop_quad, u, lam, lam_scale, nrhs, rhs, gbuf);
-            hipLaunchKernelGGL(adj_quad_combine, dim3((unsigned)((nq + QA_E - 1) / QA_E)), dim3(12 * QA_E), 0, s,
+            hipLaunchKernelGGL(adj_quad_combine, xcd_grid((nq + QA_E - 1) / QA_E), dim3(12 * QA_E), 0, s,
                                nq, p->n_beam, p->d_cnct_quad, crds, nrhs, rhs, gbuf);
         }
         JSSO_HIP(hipGetLastError());
@@ -786,7 +788,7 @@ int launch_gather_grad(Plan* p, const double* gbuf, int nrhs, double* dcrds, hip
     JSSO_REQUIRE(p->on_device, JSSO_EINVAL, "plan not uploaded to the device");
     if (!dcrds || p->n_nodes == 0) return JSSO_OK;
     const int64_t total = (int64_t)p->n_nodes * 3 * nrhs;
-    hipLaunchKernelGGL(gather_grad_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(gather_grad_kernel, xcd_grid((total + 255) / 256), dim3(256), 0, s,
                        p->n_nodes, nrhs, p->d_nptr, p->d_nsrc, gbuf, dcrds);
     JSSO_HIP(hipGetLastError());
     return JSSO_OK;

@@ -40,6 +40,17 @@ constexpr int WS_FU_SLOT = 13;   // Plan::ws slot of the f.u reduction's per-sli
 // ((12 per beam + 17 per quad) x load cases: e.g. 490k quads with one load case)
 constexpr size_t NOALLOC_SCRATCH_DOUBLES = size_t(1) << 23;
 constexpr int ROWS_PER_WG = 64;   // block rows per solver slice (one wavefront in the SpMV)
+
+// Grids of element- or block-wise kernels: 8 * ceil(n / 8) workgroups, index
+// remapped so that each XCD (workgroups go round-robin to the 8 XCDs) runs one
+// contiguous eighth of the work.  Outputs next to each other read the same
+// inputs (a mesh neighbourhood, a block row), which then stay in one L2 instead
+// of being fetched from HBM by every XCD.
+static inline dim3 xcd_grid(int64_t nwg) { return dim3((unsigned)(8 * ((nwg + 7) / 8))); }
+__device__ __forceinline__ int64_t xcd_wg() {
+    const int64_t b = blockIdx.x;
+    return (b & 7) * (int64_t)(gridDim.x >> 3) + (b >> 3);
+}
 // batched-solver variants: matrix type MT x vector type VT
 enum { PS_F32 = 0, PS_F64 = 1, PS_F32_V64 = 2 };
 template <typename MT, typename VT>
