@@ -165,6 +165,22 @@ def simp_headline(args, L, nat):
                 "elements_per_s": b.plan.n_quad / ((te + tg) * 1e-3) if te + tg > 0 else None,
                 "samples": ne}
     phases = {"element_ms": te, "gather_ms": tg, "amg_setup_ms": t["setup"][0], "adjoint_ms": t["adjoint"][0]}
+    # FP64 rate of the element kernel and of the quad adjoint (operation counts
+    # from the committed rocprofv3 SQ_INSTS_VALU_*_F64 summary of this workload)
+    ff = "simp64_shell100_fp64_flops.txt"
+    ke_name = f"ke_quad_kernel<0, {b.plan.element_tuning()}>"
+    fe = _measured_flops(ff, ke_name)
+    fa = [_measured_flops(ff, k) for k in ("adj_quad_fwd", "adj_quad_frame", "adj_quad_combine")]
+    fa = sum(fa) if all(x is not None for x in fa) else None
+    compute = {"fp64_vector_peak_TFLOPs": FP64_VECTOR_PEAK_TFLOPS, "source": f"profiles/{ff}" if fe else None,
+               "element_kernel": ke_name, "element_flops_per_launch": fe,
+               "element_TFLOPs": fe / (te * 1e-3) / 1e12 if fe and te > 0 else None,
+               "adjoint_flops_per_step": fa,
+               "adjoint_TFLOPs": fa / (t["adjoint"][0] * 1e-3) / 1e12 if fa and t["adjoint"][0] > 0 else None}
+    for k in ("element", "adjoint"):
+        v = compute[f"{k}_TFLOPs"]
+        compute[f"{k}_frac_of_fp64_peak"] = v / FP64_VECTOR_PEAK_TFLOPS if v else None
+    assembly["compute"] = compute
 
     out = {"b": b, "value": value, "ms_per_step": ms_per_step, "iters": iters, "info": info,
            "roofline": roofline, "assembly": assembly, "phases_ms_per_step": phases,
@@ -477,6 +493,50 @@ def trace_summary(trace_dir, bench_log):
               f"{(last[-1][0] - last[0][0]) / 1e6:.2f} ms from its first to its last {kname} launch")
 
 
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+FLOP_COUNTERS = ("SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_TRANS_F64")
+
+
+def flops_summary(pmc_dir, bench_log):
+    """FP64 operations per launch of the MITC-4 element kernel and of the quad
+    adjoint kernels from a rocprofv3 --pmc run of the four SQ_INSTS_VALU_*_F64
+    counters (wave instructions): 64 lanes per instruction, 2 per FMA (an upper
+    bound where lanes are masked off)."""
+    import csv
+    import glob
+    import re
+    per = {}
+    for path in sorted(glob.glob(os.path.join(pmc_dir, "**", "*counter_collection.csv"), recursive=True)):
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                if r["Counter_Name"] not in FLOP_COUNTERS:
+                    continue
+                d = per.setdefault(int(r["Dispatch_Id"]), {"name": re.sub(r"\(.*", "", r["Kernel_Name"])})
+                d[r["Counter_Name"]] = float(r["Counter_Value"])
+    agg = {}
+    for d in per.values():
+        fl = 64.0 * (d.get("SQ_INSTS_VALU_ADD_F64", 0) + d.get("SQ_INSTS_VALU_MUL_F64", 0) +
+                     d.get("SQ_INSTS_VALU_TRANS_F64", 0) + 2.0 * d.get("SQ_INSTS_VALU_FMA_F64", 0))
+        agg.setdefault(d["name"], []).append(fl)
+    print("# FP64 lane operations per launch (64 x (ADD + MUL + TRANS + 2 FMA) wave instructions)")
+    for name in sorted(agg):
+        if name.startswith("void jsso::ke_quad_kernel") or name.startswith("jsso::adj_quad"):
+            v = agg[name]
+            print(f"flops_per_launch {name.replace('void jsso::', '').replace('jsso::', '')}: {int(sum(v) / len(v))}"
+                  f"  # {len(v)} launches")
+
+
+def _measured_flops(name, kname):
+    path = os.path.join(HERE, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith(f"flops_per_launch {kname}:"):
+                return int(line.split(":")[1].split("#")[0])
+    return None
+
+
 def traffic_summary(fetch_dir, write_dir, bench_log):
     """HBM bytes per launch of the roofline kernel from two rocprofv3 --pmc runs of
     ``bench.py --profile-run`` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch),
@@ -551,15 +611,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-oracle-check", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the headline batch")
-    ap.add_argument("--wps", type=int, choices=[1, 2, 4], default=None,
-                    help="pin the level-0 waves per slice instead of the live tuner's choice (profiled runs pin "
-                         "the unprofiled run's choice, so that every run times the same kernel)")
+    ap.add_argument("--wps", type=int, choices=[0, 1, 2, 4], default=4,
+                    help="level-0 waves per slice of the AMG kernels: 4 (default; the live tuner's choice on the "
+                         "SIMP batch, pinned so that every run and profile times the same roofline kernel) or 0 "
+                         "for the live tuner")
     ap.add_argument("--profile-run", nargs="?", const="simp", choices=["simp", "shell200"],
                     help="only the headline's warm-up and timed steps, or the 200x200 shell's mixed path "
                          "(for rocprofv3); implies --no-extras --no-cpu-baseline --no-oracle-check")
     ap.add_argument("--trace-summary", metavar="DIR",
                     help="summarise a rocprofv3 CSV kernel trace (with --bench-log) and exit")
     ap.add_argument("--bench-log", metavar="FILE", help="output of the profiled bench.py --profile-run")
+    ap.add_argument("--flops-summary", metavar="DIR",
+                    help="summarise a rocprofv3 --pmc run of the SQ_INSTS_VALU_*_F64 counters and exit")
     ap.add_argument("--traffic-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) with --bench-log and exit")
     args = ap.parse_args()
@@ -568,6 +631,9 @@ def main():
         return
     if args.traffic_summary:
         traffic_summary(*args.traffic_summary, args.bench_log)
+        return
+    if args.flops_summary:
+        flops_summary(args.flops_summary, args.bench_log)
         return
     if args.profile_run:
         args.no_extras = args.no_cpu_baseline = args.no_oracle_check = True
