@@ -207,10 +207,31 @@ int jsso_spmv_k(jsso_plan* plan, const double* d_K, const double* d_x, double* d
  * stiffness that is not positive definite (Cholesky) or singular (LU).
  *   <- jax_dense_solve (dense LU of K_aug)      JaxSSO/solver.py:75-93
  *      and, for non-symmetric K, SuperLU on K_aug / K_aug^T  JaxSSO/solver.py:195-209,236
- * jsso_solve routes a plan with such members to the LU path itself, whatever
- * the precision asked for (info->iterations = 0). */
+ * jsso_solve routes such plans to the LU path itself up to 16 384 DOF (see
+ * jsso_solve_transpose for larger ones), whatever the precision asked for
+ * (info->iterations = 0). */
 int jsso_dense_solve(jsso_plan* plan, const double* d_K, const double* d_f, double* d_u, int32_t nrhs,
                      int32_t transpose, void* stream);
+/* K^T u = f with u = 0 at supports: the adjoint system of a non-symmetric plan
+ * (same arguments and results as jsso_solve; a symmetric plan is solved as by
+ * jsso_solve).
+ *   <- the custom VJP's solve with K_aug^T  JaxSSO/solver.py:236
+ * Non-symmetric plans (both entry points): the dense LU (jsso_dense_solve) up to
+ * 16 384 DOF or when members parallel to global Y are present; above that a
+ * defect correction: r = f - K u (K^T u), S du = r with S = (K + K^T)/2 (the
+ * assembly writes S into the solver copies of such a plan; inner batched solve
+ * at relative tolerance 1e-3 with the plan's preconditioner), u += du, until
+ * ||r|| / ||f|| <= tol for every system.  h_info->iterations counts the inner
+ * iterations, h_info->replacements the outer steps.  Returns JSSO_NOT_CONVERGED
+ * when a step reduces the residual by less than 10% before tol is reached. */
+int jsso_solve_transpose(jsso_plan* plan, const double* d_f, double* d_u, int32_t nrhs, int32_t precision,
+                         double tol, int32_t maxit, jsso_info* h_info, void* stream);
+/* Route of non-symmetric plans: mode 0 automatic (above), 1 always the dense LU,
+ * 2 always the defect correction (JSSO_EUNSUPPORTED with members parallel to
+ * global Y, whose transformation, element.py:92-94, leaves S indefinite);
+ * -1 only reads it.  *h_mode (may be NULL) receives the mode in force. */
+int jsso_plan_nonsymmetric_route(jsso_plan* plan, int32_t mode, int32_t* h_mode);
+
 /* *h_symmetric = 0 when the plan's last assembly holds members parallel to global
  * Y or quads with kx_mod != ky_mod (its K is not symmetric, so an adjoint needs
  * the transposed solve), else 1.

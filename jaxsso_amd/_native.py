@@ -24,7 +24,7 @@ EXPORTS = ("jsso_version", "jsso_last_error", "jsso_plan_create", "jsso_plan_des
            "jsso_plan_amg_levels", "jsso_timing_read_slot", "jsso_spmv_k", "jsso_dense_solve",
            "jsso_element_beam_local", "jsso_element_quad_local", "jsso_element_quad_gauss",
            "jsso_plan_symmetric", "jsso_csr_spmv", "jsso_adjoint_grad_noalloc", "jsso_plan_element_tuning",
-           "jsso_plan_amg_reuse")
+           "jsso_plan_amg_reuse", "jsso_solve_transpose", "jsso_plan_nonsymmetric_route")
 TIMING_SLOTS = {"solve": 0, "element": 1, "gather": 2, "adjoint": 3, "setup": 4}
 
 
@@ -86,6 +86,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.jsso_adjoint_grad_noalloc.argtypes = [P, P, P, P, P, P, D, I32, P, P, P, V]
     lib.jsso_plan_element_tuning.argtypes = [P, I32, P]
     lib.jsso_plan_amg_reuse.argtypes = [P, I32, P]
+    lib.jsso_solve_transpose.argtypes = [P, P, P, I32, I32, D, I32, ctypes.POINTER(Info), V]
+    lib.jsso_plan_nonsymmetric_route.argtypes = [P, I32, P]
     lib.jsso_element_beam_local.argtypes = [I32, P, P, P, P, V]
     lib.jsso_element_quad_local.argtypes = [I32, P, P, P, P, P, P, V]
     lib.jsso_element_quad_gauss.argtypes = [I32, P, P, D, D, P, P, P, P, P, P, P, V]

@@ -9,6 +9,7 @@
 // needs it; for a node outside all elements this adds zeros the reference
 // would not store, and such a node makes K singular in both).
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -335,6 +336,9 @@ static int build_plan(Plan* p, int32_t n_sys, const int32_t* node_off, int32_t n
     p->h_ell_col.swap(ell_col);
     p->h_ell_vpos.swap(ell_vpos);
     p->h_ell_src.swap(ell_src);
+    // position of the transposed block of every block (the pattern is structurally symmetric)
+    p->h_tpos.resize(nnzb);
+    for (int64_t k = 0; k < nnzb; ++k) p->h_tpos[k] = (int32_t)find_block(colind[k], browof[k]);
     p->h_browof.swap(browof);
     p->h_diag.swap(diag);
     p->h_cptr.swap(cptr);
@@ -364,7 +368,7 @@ static int ensure_device(Plan* p) {
         (rc = upload(p, &p->d_wg_row1, p->h_wg_row1)) || (rc = upload(p, &p->d_sys_wg_off, p->h_sys_wg_off)) ||
         (rc = upload(p, &p->d_ell_vpos, p->h_ell_vpos)) || (rc = upload(p, &p->d_ell_col, p->h_ell_col)) ||
         (rc = upload(p, &p->d_slice_meta, p->h_slice_meta)) ||
-        (rc = upload(p, &p->d_ell_src, p->h_ell_src)))
+        (rc = upload(p, &p->d_ell_src, p->h_ell_src)) || (rc = upload(p, &p->d_tpos, p->h_tpos)))
         return rc;
     const size_t n = p->n_nodes;
     if ((rc = dalloc(p, &p->d_ke, (size_t)p->n_contrib * 36)) || (rc = dalloc(p, &p->d_K, (size_t)p->nnzb * 36)) ||
@@ -564,6 +568,94 @@ int jsso_assemble(jsso_plan* plan, const double* d_crds, const double* d_prop_be
     return JSSO_OK;
 }
 
+// K u = f (or K^T u = f) for a plan whose K is not symmetric.  The reference
+// factorises K_aug (and K_aug^T for the adjoint) with SuperLU
+// (solver.py:195-209,236).  Here: the dense LU (jsso_dense_solve) up to
+// NONSYM_DENSE_MAX_DOF DOF or with members parallel to global Y; otherwise a
+// defect correction preconditioned by the batched solver on the symmetric part
+// S = (K + K^T)/2 (which the assembly writes into the solver copies of such a
+// plan):  r = f - op(K) u;  S du = r (inner AMG-CG, relative tolerance 1e-3);
+// u += du, until max ||r|| / ||f|| <= tol over the systems.  It converges when
+// the spectral radius of I - S^-1 K is below 1: 0.04-0.14 for MITC-4 shells with
+// kx_mod/ky_mod from 1.2/0.8 to 2/0.5; the transformation of Y-parallel members
+// leaves S indefinite, so those plans stay on the dense LU.
+static int solve_nonsymmetric(Plan* p, const double* d_f, double* d_u, int nrhs, int precision, double tol,
+                              int maxit, jsso_info* h_info, bool transpose, hipStream_t s, int32_t status) {
+    int rc = launch_full_blocks(p, s);
+    if (rc) return rc;
+    const int64_t ndof = 6 * (int64_t)p->n_nodes;
+    const bool dense = p->nonsym_route == 1 ||
+                       (p->nonsym_route == 0 && ((status & STATUS_Y_BEAM) || ndof <= NONSYM_DENSE_MAX_DOF));
+    if (dense) {
+        if ((rc = dense_solve(p, p->d_K, d_f, d_u, nrhs, true, transpose, s))) return rc;
+        if (h_info) {
+            *h_info = jsso_info{};
+            h_info->converged = p->n_sys * nrhs;
+            h_info->n_systems = p->n_sys * nrhs;
+        }
+        return JSSO_OK;
+    }
+    JSSO_REQUIRE(!(status & STATUS_Y_BEAM), JSSO_EUNSUPPORTED,
+                 "members parallel to global Y (element.py:92-94) leave the symmetric part of K indefinite: the "
+                 "defect correction cannot converge; use the dense route (jsso_plan_nonsymmetric_route 0 or 1)");
+    const int nsys = p->n_sys * nrhs;
+    const size_t vb = (size_t)ndof * nrhs * sizeof(double);
+    const size_t need = 2 * vb + 2 * (size_t)nsys * sizeof(double);
+    if (p->ws.size() <= WS_DEFECT_SLOT) p->ws.resize(WS_DEFECT_SLOT + 1);
+    DBuf& g = p->ws[WS_DEFECT_SLOT];
+    if (g.bytes < need) {
+        if (g.p) JSSO_HIP(hipFree(g.p));
+        g.p = nullptr;
+        g.bytes = 0;
+        hipError_t e = hipMalloc(&g.p, need);
+        JSSO_REQUIRE(e == hipSuccess, JSSO_ENOMEM, "defect-correction vectors hipMalloc(%zu): %s", need,
+                     hipGetErrorString(e));
+        g.bytes = need;
+    }
+    double* r = (double*)g.p;
+    double* du = r + (size_t)ndof * nrhs;
+    double* d_rr = du + (size_t)ndof * nrhs;
+    std::vector<double> rr(nsys), ff(nsys);
+    const int slot = precision == JSSO_PREC_MIXED ? PS_F32 : precision == JSSO_PREC_MIXED_V64 ? PS_F32_V64 : PS_F64;
+    JSSO_HIP(hipMemsetAsync(d_u, 0, vb, s));
+    int inner_its = 0, outer = 0, conv = 0;
+    double relmax = 0.0, prev = 0.0;
+    for (;; ++outer) {
+        if ((rc = launch_defect_residual(p, p->d_K, d_f, d_u, nrhs, transpose, r, s))) return rc;
+        if ((rc = launch_compliance(p, r, r, nrhs, 1.0, d_rr, s))) return rc;
+        JSSO_HIP(hipMemcpyAsync(rr.data(), d_rr, nsys * sizeof(double), hipMemcpyDeviceToHost, s));
+        JSSO_HIP(hipStreamSynchronize(s));
+        if (outer == 0) ff = rr;   // u = 0: r = f on the free DOFs
+        relmax = 0.0;
+        conv = 0;
+        for (int k = 0; k < nsys; ++k) {
+            const double rel = ff[k] > 0.0 ? std::sqrt(std::max(rr[k], 0.0) / ff[k]) : 0.0;
+            relmax = std::max(relmax, rel);
+            conv += rel <= tol ? 1 : 0;
+        }
+        if (conv == nsys || inner_its >= maxit || !std::isfinite(relmax)) break;
+        // (a step that gains less than 10%: the attainable accuracy, or no convergence)
+        if (outer >= 2 && relmax > 0.9 * prev) break;
+        prev = relmax;
+        jsso_info ii{};
+        p->sym_part_solve = true;
+        rc = solve_batched(p, r, du, nrhs, slot, std::max(1e-3, tol), maxit - inner_its, &ii, s);
+        p->sym_part_solve = false;
+        if (rc < 0) return rc;
+        inner_its += ii.iterations;
+        if ((rc = launch_axpy((int64_t)ndof * nrhs, du, d_u, s))) return rc;
+    }
+    if (h_info) {
+        *h_info = jsso_info{};
+        h_info->iterations = inner_its;
+        h_info->replacements = outer;
+        h_info->converged = conv;
+        h_info->n_systems = nsys;
+        h_info->rel_residual = relmax;
+    }
+    return conv == nsys ? JSSO_OK : JSSO_NOT_CONVERGED;
+}
+
 int jsso_solve(jsso_plan* plan, const double* d_f, double* d_u, int32_t nrhs, int32_t precision, double tol,
                int32_t maxit, jsso_info* h_info, void* stream) {
     if (!plan || !d_f || !d_u || nrhs < 1 || maxit < 1 || !(tol > 0.0)) {
@@ -584,18 +676,8 @@ int jsso_solve(jsso_plan* plan, const double* d_f, double* d_u, int32_t nrhs, in
     if (rc) return rc;
     int32_t status = 0;
     if ((rc = assembly_status(p, s, &status))) return rc;
-    if (status & 2) {
-        // members parallel to global Y: K is not symmetric (element.py:92-94); the
-        // reference factorises it directly (SuperLU), so does this path (dense LU)
-        if ((rc = launch_full_blocks(p, s))) return rc;
-        if ((rc = dense_solve(p, p->d_K, d_f, d_u, nrhs, true, false, s))) return rc;
-        if (h_info) {
-            *h_info = jsso_info{};
-            h_info->converged = p->n_sys * nrhs;
-            h_info->n_systems = p->n_sys * nrhs;
-        }
-        return JSSO_OK;
-    }
+    if (status & STATUS_NONSYM) return solve_nonsymmetric(p, d_f, d_u, nrhs, precision, tol, maxit, h_info, false, s,
+                                                           status);
     switch (precision) {
         case JSSO_PREC_FP64: return solve_batched(p, d_f, d_u, nrhs, PS_F64, tol, maxit, h_info, s);
         case JSSO_PREC_MIXED: return solve_batched(p, d_f, d_u, nrhs, PS_F32, tol, maxit, h_info, s);
@@ -603,6 +685,38 @@ int jsso_solve(jsso_plan* plan, const double* d_f, double* d_u, int32_t nrhs, in
         case JSSO_PREC_PLAIN: return solve_plain(p, d_f, d_u, nrhs, tol, maxit, h_info, s);
         default: set_error("unknown precision %d", precision); return JSSO_EINVAL;
     }
+}
+
+int jsso_solve_transpose(jsso_plan* plan, const double* d_f, double* d_u, int32_t nrhs, int32_t precision,
+                         double tol, int32_t maxit, jsso_info* h_info, void* stream) {
+    if (!plan || !d_f || !d_u || nrhs < 1 || maxit < 1 || !(tol > 0.0)) {
+        set_error("invalid solve arguments");
+        return JSSO_EINVAL;
+    }
+    Plan* p = P(plan);
+    if (!p->assembled) {
+        set_error("jsso_solve_transpose before jsso_assemble");
+        return JSSO_EINVAL;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    int rc = ensure_device(p);
+    if (rc) return rc;
+    int32_t status = 0;
+    if ((rc = assembly_status(p, s, &status))) return rc;
+    if (status & STATUS_NONSYM) return solve_nonsymmetric(p, d_f, d_u, nrhs, precision, tol, maxit, h_info, true, s,
+                                                           status);
+    return jsso_solve(plan, d_f, d_u, nrhs, precision, tol, maxit, h_info, stream);
+}
+
+int jsso_plan_nonsymmetric_route(jsso_plan* plan, int32_t mode, int32_t* h_mode) {
+    if (!plan || mode < -1 || mode > 2) {
+        set_error("invalid non-symmetric route %d (-1 query, 0 auto, 1 dense LU, 2 defect correction)", mode);
+        return JSSO_EINVAL;
+    }
+    Plan* p = P(plan);
+    if (mode >= 0) p->nonsym_route = mode;
+    if (h_mode) *h_mode = p->nonsym_route;
+    return JSSO_OK;
 }
 
 int jsso_adjoint_grad(jsso_plan* plan, const double* d_crds, const double* d_prop_beam, const double* d_prop_quad,

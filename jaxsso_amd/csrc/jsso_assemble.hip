@@ -145,8 +145,14 @@ __global__ __launch_bounds__(ASM_WG) void ell_assemble_kernel(
     int64_t n_slots, const int32_t* __restrict__ ell_src, const int32_t* __restrict__ browof,
     const int32_t* __restrict__ colind, const int32_t* __restrict__ cptr, const int32_t* __restrict__ csrc,
     const uint8_t* __restrict__ fixed, const double* __restrict__ ke, const double* __restrict__ Linv,
-    double* __restrict__ Ke64, float* __restrict__ Ke32, _Float16* __restrict__ Ke16) {
+    double* __restrict__ Ke64, float* __restrict__ Ke32, _Float16* __restrict__ Ke16, int32_t n_beam,
+    const int32_t* __restrict__ status) {
     constexpr int P = ROWS_PER_WG;
+    // K not symmetric (set by this assembly's element kernels): the solver copies
+    // hold the symmetric part (K_ij + K_ji^T) / 2, the preconditioning operator of
+    // the defect correction; the element blocks of (j, i) are the swapped
+    // element-local blocks of the same elements
+    const bool sym_part = (*status & STATUS_NONSYM) != 0;
     __shared__ double tile[P][37];
     __shared__ double li[P][37], lj[P][37];
     __shared__ int32_t cidx[P][MAX_CONTRIB], cstart[P], ccount[P], rowi[P], colj[P];
@@ -199,6 +205,24 @@ __global__ __launch_bounds__(ASM_WG) void ell_assemble_kernel(
         for (int c = 0; c < MAX_CONTRIB; ++c)
             if (c < n) sum += v[it][c];
         for (int c = MAX_CONTRIB; c < n; ++c) sum += ke[(int64_t)csrc[cstart[p] + c] * 36 + q];   // rare meshes
+        if (sym_part) {
+            const int qt = (q % 6) * 6 + q / 6;
+            double st = 0.0;
+            for (int c = 0; c < n; ++c) {
+                const int32_t src = c < MAX_CONTRIB ? cidx[p][c] : csrc[cstart[p] + c];
+                int64_t sw;
+                if (src < 4 * n_beam) {
+                    const int l = src & 3;
+                    sw = (src & ~3) | ((l & 1) << 1) | (l >> 1);
+                } else {
+                    const int64_t qq = src - 4 * (int64_t)n_beam;
+                    const int l = (int)(qq & 15);
+                    sw = 4 * (int64_t)n_beam + (qq & ~(int64_t)15) + ((l & 3) << 2) + (l >> 2);
+                }
+                st += ke[sw * 36 + qt];
+            }
+            sum = 0.5 * (sum + st);
+        }
         tile[p][q] = (n > 0 && !(rfix[p][q / 6] | cfix[p][q % 6])) ? sum : 0.0;
         li[p][q] = lv[it][0];
         lj[p][q] = lv[it][1];
@@ -259,7 +283,7 @@ int launch_assemble(Plan* p, hipStream_t s) {
     if (n_slots > 0) {
         hipLaunchKernelGGL(ell_assemble_kernel, dim3((unsigned)n_slots), dim3(ASM_WG), 0, s, n_slots, p->d_ell_src,
                            p->d_browof, p->d_colind, p->d_cptr, p->d_csrc, p->d_fixed, p->d_ke, p->d_Linv,
-                           p->d_Ke64, p->d_Ke32, (_Float16*)p->d_Ke16);
+                           p->d_Ke64, p->d_Ke32, (_Float16*)p->d_Ke16, p->n_beam, p->d_status);
         JSSO_HIP(hipGetLastError());
     }
     return JSSO_OK;
@@ -281,6 +305,62 @@ __global__ __launch_bounds__(256) void spmv_full_kernel(int32_t n_nodes, const i
         for (int c = 0; c < 6; ++c) acc += b[c] * xj[c];
     }
     y[R] = acc;
+}
+
+// r = f - op(K) u with op(K) = K or K^T (the unconstrained block CSR; u is zero
+// at constrained DOFs) and r zero at constrained DOFs; vectors [6 n][nrhs].
+// Thread per (scalar row, load case); a row's blocks in column order.
+__global__ __launch_bounds__(256) void defect_residual_kernel(int32_t n_nodes, int nrhs,
+                                                              const int32_t* __restrict__ rowptr,
+                                                              const int32_t* __restrict__ colind,
+                                                              const int32_t* __restrict__ tpos,
+                                                              const uint8_t* __restrict__ fixed,
+                                                              const double* __restrict__ K,
+                                                              const double* __restrict__ f,
+                                                              const double* __restrict__ u, double* __restrict__ r) {
+    const int64_t t = xcd_wg() * blockDim.x + threadIdx.x;
+    if (t >= 6 * (int64_t)n_nodes * nrhs) return;
+    const int h = (int)(t % nrhs);
+    const int64_t R = t / nrhs;
+    if (fixed[R]) {
+        r[t] = 0.0;
+        return;
+    }
+    const int i = (int)(R / 6), rr = (int)(R % 6);
+    double acc = 0.0;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+        const int64_t j = colind[k];
+        // (K^T)_ij = (K_ji)^T: row rr of it is column rr of the block at the transposed position
+        const double* b = tpos ? K + (int64_t)tpos[k] * 36 + rr : K + (int64_t)k * 36 + rr * 6;
+        const int st = tpos ? 6 : 1;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc += b[c * st] * u[(6 * j + c) * nrhs + h];
+    }
+    r[t] = f[t] - acc;
+}
+
+__global__ __launch_bounds__(256) void axpy_kernel(int64_t n, const double* __restrict__ x, double* __restrict__ y) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) y[t] += x[t];
+}
+
+int launch_defect_residual(Plan* p, const double* K, const double* f, const double* u, int nrhs, bool transpose,
+                           double* r, hipStream_t s) {
+    JSSO_REQUIRE(p->on_device, JSSO_EINVAL, "plan not uploaded to the device");
+    const int64_t n = 6 * (int64_t)p->n_nodes * nrhs;
+    if (n == 0) return JSSO_OK;
+    hipLaunchKernelGGL(defect_residual_kernel, xcd_grid((n + 255) / 256), dim3(256), 0, s, p->n_nodes, nrhs,
+                       p->d_rowptr, p->d_colind, transpose ? (const int32_t*)p->d_tpos : (const int32_t*)nullptr,
+                       p->d_fixed, K, f, u, r);
+    JSSO_HIP(hipGetLastError());
+    return JSSO_OK;
+}
+
+int launch_axpy(int64_t n, const double* x, double* y, hipStream_t s) {
+    if (n <= 0) return JSSO_OK;
+    hipLaunchKernelGGL(axpy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, x, y);
+    JSSO_HIP(hipGetLastError());
+    return JSSO_OK;
 }
 
 int launch_spmv_full(Plan* p, const double* K, const double* x, double* y, hipStream_t s) {

@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void ke_beam_kernel(int32_t n, const double* _
             for (int j = 0; j < 3; ++j) Di[i][j] = D[j][i];
     }
     if (flags && a == 0 && b == 0) flags[e] = special ? 1 : 0;
-    if (status && special && a == 0 && b == 0) atomicOr(status, 2);
+    if (status && special && a == 0 && b == 0) atomicOr(status, STATUS_NONSYM | STATUS_Y_BEAM);
     double out[6][6];
 #pragma unroll
     for (int pq = 0; pq < 4; ++pq) {
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(16 * EPW) void ke_quad_kernel(int32_t n, const doub
             T.ky = props[5 * e + 4];
             // Cm of element.py:852-875 is [[Ex, nu Ex, 0], [nu Ey, Ey, 0], ...]: not
             // symmetric when kx_mod != ky_mod (and nu != 0), so neither is this Ke
-            if (status && T.nu != 0.0 && T.kx != T.ky) atomicOr(status, 2);
+            if (status && T.nu != 0.0 && T.kx != T.ky) atomicOr(status, STATUS_NONSYM | STATUS_ORTHO_QUAD);
         }
     }
     __syncthreads();

@@ -36,6 +36,13 @@ constexpr int BDIR = 12;   // beam adjoint directions: 6 coordinates + 6 propert
 constexpr int QDIR = 17;   // quad adjoint directions: 12 coordinates + 5 properties
 constexpr int WS_ADJ_SLOT = 12;  // Plan::ws slot of the adjoint's gradient scratch (solver slots are below it)
 constexpr int WS_FU_SLOT = 13;   // Plan::ws slot of the f.u reduction's per-slice partials
+constexpr int WS_DEFECT_SLOT = 14;   // Plan::ws slot of the non-symmetric defect correction's vectors
+// status bits of an assembly (d_status): 1 a constrained diagonal block is not
+// positive definite; 2 K is not symmetric; 8 because of a quad with kx_mod !=
+// ky_mod; 16 because of a beam-column parallel to global Y
+constexpr int STATUS_NONSYM = 2, STATUS_ORTHO_QUAD = 8, STATUS_Y_BEAM = 16;
+// non-symmetric plans up to this many DOF take the dense LU by default
+constexpr int64_t NONSYM_DENSE_MAX_DOF = 16384;
 // fixed device scratch of jsso_adjoint_grad_noalloc: 8 Mi doubles = 64 MiB per GPU
 // ((12 per beam + 17 per quad) x load cases: e.g. 490k quads with one load case)
 constexpr size_t NOALLOC_SCRATCH_DOUBLES = size_t(1) << 23;
@@ -145,6 +152,10 @@ struct Plan {
     double* d_Minv = nullptr;     // block-Jacobi inverse (plain path)
     double* d_Kc = nullptr;       // constrained K incl. diagonal (plain path)
     int32_t* d_status = nullptr;  // device error flags
+    int32_t* d_tpos = nullptr;    // per block of the CSR: position of its transposed block
+    std::vector<int32_t> h_tpos;
+    int32_t nonsym_route = 0;     // non-symmetric K: 0 auto, 1 dense LU, 2 defect correction
+    bool sym_part_solve = false;  // set while the defect correction solves with the symmetric part
     int32_t status_cache = -1;    // host copy of *d_status once read after an assembly (-1: not read)
     bool assembled = false;
     bool full_valid = false;      // d_K / d_Kc hold the current assembly
@@ -213,6 +224,9 @@ void timing_settle(std::pair<hipEvent_t, hipEvent_t> e, int slot, bool count);
 void timing_end(hipStream_t s, std::pair<hipEvent_t, hipEvent_t>* ev);
 void timing_collect();
 int launch_compliance(Plan* p, const double* f, const double* u, int nrhs, double scale, double* C, hipStream_t s);
+int launch_defect_residual(Plan* p, const double* K, const double* f, const double* u, int nrhs, bool transpose,
+                           double* r, hipStream_t s);
+int launch_axpy(int64_t n, const double* x, double* y, hipStream_t s);
 int amg_solve(Plan* p, const double* f, double* u, int nrhs, int slot, double tol, int maxit, jsso_info* info,
               hipStream_t s);
 int amg_levels(Plan* p, int32_t* h_nodes, int32_t cap, int32_t* h_levels);

@@ -612,10 +612,10 @@ int check_status(Plan* p, hipStream_t s) {
     JSSO_REQUIRE(!(status & 1), JSSO_ESINGULAR,
                  "a constrained diagonal block of K is not positive definite (unsupported or "
                  "isolated node)");
-    JSSO_REQUIRE(!(status & 2), JSSO_EUNSUPPORTED,
+    JSSO_REQUIRE(!(status & STATUS_NONSYM) || p->sym_part_solve, JSSO_EUNSUPPORTED,
                  "K is not symmetric (a beam-column parallel to global Y, element.py:92-94, or a quad with "
                  "kx_mod != ky_mod, element.py:852-875); CG cannot solve it (jsso_solve routes such plans "
-                 "to the dense LU solve)");
+                 "to the dense LU or the defect correction)");
     return JSSO_OK;
 }
 

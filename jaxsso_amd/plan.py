@@ -97,11 +97,18 @@ class FEPlan:
 
     def solve_transpose(self, f: torch.Tensor, **kw) -> torch.Tensor:
         """K^T u = f (u = 0 at supports): the adjoint solve.  K is symmetric unless
-        the assembly holds members parallel to global Y or orthotropic quads (then
-        the dense LU of K^T)."""
-        if self.symmetric():
-            return self.solve(f, **kw)
-        return self.dense_solve(f, transpose=True)
+        the assembly holds members parallel to global Y or orthotropic quads; then
+        ``jsso_solve_transpose`` takes the dense LU of K^T or, for large plans
+        without Y-parallel members, the defect correction (``nonsymmetric_route``)."""
+        return self.solve(f, transpose=True, **kw)
+
+    def nonsymmetric_route(self, mode: int = -1) -> int:
+        """How non-symmetric plans are solved: 0 automatic (dense LU up to 16 384
+        DOF or with members parallel to global Y, else defect correction on the
+        symmetric part), 1 dense LU, 2 defect correction; -1 only reads it."""
+        out = ctypes.c_int32(0)
+        nat.check(nat.lib().jsso_plan_nonsymmetric_route(self._h, int(mode), ctypes.byref(out)))
+        return int(out.value)
 
     def set_preconditioner(self, name: str) -> None:
         """Preconditioner of the batched precisions: 'amg' (smoothed-aggregation
@@ -137,8 +144,10 @@ class FEPlan:
         return [int(v) for v in nodes[:nl.value]]
 
     def solve(self, f: torch.Tensor, precision: str = "fp64", tol: float = DEFAULT_TOL,
-              maxit: int = DEFAULT_MAXIT, warn: bool = True, preconditioner: str | None = None) -> torch.Tensor:
-        """Solve K u = f (u = 0 at supports).  f: (ndof,) or (ndof, nrhs) FP64 on the device."""
+              maxit: int = DEFAULT_MAXIT, warn: bool = True, preconditioner: str | None = None,
+              transpose: bool = False) -> torch.Tensor:
+        """Solve K u = f (K^T u = f with ``transpose``; u = 0 at supports).  f: (ndof,)
+        or (ndof, nrhs) FP64 on the device."""
         L = nat.lib()
         if preconditioner is not None and preconditioner != self.preconditioner:
             self.set_preconditioner(preconditioner)
@@ -148,8 +157,9 @@ class FEPlan:
         nrhs = f2.shape[1]
         u = torch.empty_like(f2)
         info = nat.Info()
-        rc = L.jsso_solve(self._h, nat.ptr(f2), nat.ptr(u), nrhs, nat.PREC[precision], float(tol), int(maxit),
-                          ctypes.byref(info), nat.stream_ptr())
+        entry = L.jsso_solve_transpose if transpose else L.jsso_solve
+        rc = entry(self._h, nat.ptr(f2), nat.ptr(u), nrhs, nat.PREC[precision], float(tol), int(maxit),
+                   ctypes.byref(info), nat.stream_ptr())
         nat.check(rc, allow_not_converged=True)
         self.last_info = info.as_dict()
         if rc == nat.JSSO_NOT_CONVERGED and warn:

@@ -13,6 +13,7 @@ import pytest
 import torch
 
 from jaxsso_amd import workloads as wl
+from jaxsso_amd._native import JssoError
 from jaxsso_amd.batch import StructureBatch
 from jaxsso_amd.SSO_model import ElementParameter, NodeParameter, SSO_model
 from oracle import elements as el
@@ -172,3 +173,67 @@ def test_orthotropic_shell_solve_and_gradients():
     b2 = StructureBatch([st2])
     b2.solve(precision="fp64")
     assert b2.plan.symmetric()
+
+
+def test_orthotropic_shell_defect_correction():
+    """A 30x30 shell with kx_mod 2.0, ky_mod 0.5 through the defect correction
+    (forced here; the automatic route takes it above 16 384 DOF): S = (K + K^T)/2
+    preconditions r = f - K u.  u, the transposed solve and the compliance
+    gradients against the oracle's direct solves and the dense route."""
+    st = wl.shell_pinned_edges(30, 9.0, 1.5, design=5)
+    st.prop_quads[:, 3] = 2.0
+    st.prop_quads[:, 4] = 0.5
+    Ko = system.assemble_K(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads)
+    u_acc = osolve.solve_reduced(Ko, st.known_id, st.nodal_loads)
+    b = StructureBatch([st])
+    b.assemble()
+    assert not b.plan.symmetric()
+    assert b.plan.nonsymmetric_route() == 0 and b.plan.nonsymmetric_route(2) == 2
+    for prec in ("fp64", "mixed"):
+        u = b.solve(precision=prec).cpu().numpy()
+        info = b.plan.last_info
+        assert info["converged"] == 1 and info["replacements"] >= 2 and info["iterations"] > 0, info
+        assert rel(u, u_acc) < 1e-9, (prec, rel(u, u_acc))
+    g = np.random.default_rng(7).normal(size=st.ndof)
+    g[st.known_id] = 0.0
+    lam = b.plan.solve_transpose(torch.tensor(g, device="cuda")).cpu().numpy()
+    lam_acc = osolve.solve_reduced(Ko.T.tocsr(), st.known_id, g)
+    assert rel(lam, lam_acc) < 1e-9
+    # accurate reference: lambda = K^-T f/2 by the refined direct solve (the
+    # reference's SuperLU on K_aug is itself only ~1e-8 accurate on this shell)
+    lam_c = osolve.solve_reduced(Ko.T.tocsr(), st.known_id, 0.5 * st.nodal_loads)
+    eg = og.element_gradients(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads,
+                              u_acc, lam_c)
+    dc_acc = og.node_gradient(st.n_node, st.cnct_beamcols, st.cnct_quads, eg)
+    C_acc = 0.5 * st.nodal_loads @ u_acc
+    C, (dc, _, dq) = b.compliance_and_gradient(precision="fp64")
+    assert abs(float(C[0, 0]) - C_acc) / C_acc < 1e-10
+    assert rel(dq.cpu().numpy(), eg["quad_props"]) < 1e-8
+    assert rel(dc.cpu().numpy(), dc_acc) < 1e-8
+    ref = og.compliance_and_gradient(st, faithful=True)      # the reference's own sequence
+    assert abs(float(C[0, 0]) - ref["C"]) / ref["C"] < 1e-6
+    assert rel(dq.cpu().numpy(), ref["dprop_quad"]) < 1e-5
+    b.plan.nonsymmetric_route(1)
+    u_dense = b.solve(precision="fp64").cpu().numpy()
+    assert b.plan.last_info["iterations"] == 0
+    assert rel(u_dense, u_acc) < 1e-9
+
+
+def test_defect_correction_refuses_y_parallel_members():
+    """The transformation of members parallel to global Y leaves (K + K^T)/2
+    indefinite (checked on the oracle's K), so the defect correction refuses
+    them; the automatic route keeps them on the dense LU."""
+    st = flat_grid()
+    Ko = system.assemble_K(st.crds, st.cnct_beamcols, st.prop_beamcols, st.cnct_quads, st.prop_quads)
+    Kd = Ko.toarray()
+    free = np.setdiff1d(np.arange(st.ndof), st.known_id)
+    H = 0.5 * (Kd + Kd.T)[np.ix_(free, free)]
+    assert np.linalg.eigvalsh(H).min() < 0
+    b = StructureBatch([st])
+    b.assemble()
+    b.plan.nonsymmetric_route(2)
+    with pytest.raises(JssoError, match="indefinite"):
+        b.solve(precision="fp64")
+    b.plan.nonsymmetric_route(0)
+    u = b.solve(precision="fp64").cpu().numpy()
+    assert rel(u, osolve.solve_reduced(Ko, st.known_id, st.nodal_loads)) < 1e-10
