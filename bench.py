@@ -100,6 +100,20 @@ def _measured_traffic(name, kname):
     return None, None
 
 
+def _assembly_traffic(name, kernels):
+    """Sum of the committed per-launch HBM traffic (FETCH + WRITE) of ``kernels``."""
+    path = os.path.join(HERE, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    found = {}
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("assembly_traffic "):
+                k, rest = line[len("assembly_traffic "):].split(":", 1)
+                found[k.strip()] = int(rest.split("total")[1].split()[0])
+    return sum(found[k] for k in kernels) if all(k in found for k in kernels) else None
+
+
 def simp_headline(args, L, nat):
     """The SIMP batch: timed steps, roofline, assembly rates, errors."""
     import torch
@@ -181,6 +195,13 @@ def simp_headline(args, L, nat):
         v = compute[f"{k}_TFLOPs"]
         compute[f"{k}_frac_of_fp64_peak"] = v / FP64_VECTOR_PEAK_TFLOPS if v else None
     assembly["compute"] = compute
+    tf = f"simp64_shell100_{args.precision}_traffic.txt"
+    et = _assembly_traffic(tf, [ke_name])
+    gt = _assembly_traffic(tf, ["diag_factor_kernel", "ell_assemble_kernel"])
+    assembly["measured_traffic"] = {
+        "source": f"profiles/{tf}" if et else None, "element_bytes": et, "gather_bytes": gt,
+        "element_GBps": et / (te * 1e-3) / 1e9 if et and te > 0 else None,
+        "gather_GBps": gt / (tg * 1e-3) / 1e9 if gt and tg > 0 else None}
 
     out = {"b": b, "value": value, "ms_per_step": ms_per_step, "iters": iters, "info": info,
            "roofline": roofline, "assembly": assembly, "phases_ms_per_step": phases,
@@ -552,6 +573,7 @@ def traffic_summary(fetch_dir, write_dir, bench_log):
     kname = out["roofline"]["kernel"]
     log = out["solve_log"]["solves"]
     res = {}
+    asm = {}   # assembly kernels: counter -> name -> values per launch
     for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         vals, copies = [], []
         for path in sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)):
@@ -560,10 +582,15 @@ def traffic_summary(fetch_dir, write_dir, bench_log):
                     if r["Counter_Name"] != counter:
                         continue
                     v = float(r["Counter_Value"]) * 1024.0
-                    if r["Kernel_Name"].startswith(f"void jsso::{kname}"):
+                    name = r["Kernel_Name"]
+                    if name.startswith(f"void jsso::{kname}"):
                         vals.append((int(r.get("Dispatch_Id", 0)), v))
-                    elif "copyBuffer" in r["Kernel_Name"] or "copy" in r["Kernel_Name"].lower():
+                    elif "copyBuffer" in name or "copy" in name.lower():
                         copies.append(v)
+                    for a in ("ke_quad_kernel<0, 4>", "ke_quad_kernel<0, 8>", "ke_quad_kernel<0, 16>",
+                              "diag_factor_kernel", "ell_assemble_kernel"):
+                        if a in name:
+                            asm.setdefault(counter, {}).setdefault(a, []).append(v)
         vals.sort()
         v = [x for _, x in vals]
         extra = len(v) - sum(l for _, l, _ in log)    # the live tuner's launches come first
@@ -586,6 +613,14 @@ def traffic_summary(fetch_dir, write_dir, bench_log):
         print(f"write_bytes_per_launch {kname}: {int(w * wf)}")
         print(f"traffic_bytes_per_launch {kname}: {int(traffic)}")
         print(f"# traffic / algorithmic bytes = {traffic / alg:.3f} over {n} launches")
+        # assembly kernels (every launch covers the whole batch; the element kernel's
+        # three variants are the autotuner's candidates)
+        for a in sorted(asm.get("FETCH_SIZE", {})):
+            fv, wv = asm["FETCH_SIZE"][a], asm.get("WRITE_SIZE", {}).get(a, [])
+            if fv and wv:
+                fb, wb = sum(fv) / len(fv) * ff, sum(wv) / len(wv) * wf
+                print(f"assembly_traffic {a}: fetch {int(fb)} write {int(wb)} total {int(fb + wb)}"
+                      f"  # {len(fv)} launches")
 
 
 def _cpu_model():
