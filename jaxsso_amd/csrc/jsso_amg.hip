@@ -1416,8 +1416,8 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_pre(Slices sl, Partials P, Sys
             qo[c][h] = FRESH ? VT(0) : q[o];
         }
     // own rows of x and p for the update at the end: requested now when the
-    // registers allow it (one FP32 right-hand side)
-    constexpr bool PF = !FRESH && NR == 1 && sizeof(VT) == 4;
+    // registers allow it (one right-hand side), not after the product
+    constexpr bool PF = !FRESH && NR == 1;
     constexpr int NP = PF ? NR : 1;
     VT xo[6][NP], pko[6][NP];
     if constexpr (PF) {
@@ -1498,11 +1498,19 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_post(Slices sl, Partials P, Sy
     const int64_t ii = on ? i : (int64_t)ma.y;
     auto zz = [&](int64_t node, int c, int h) { return (VT)z1[(6 * node + c) * NR + h]; };
     const VT th = (VT)theta[ma.x];
-    VT acc[6][NR];
+    // own rows of z1, r and r_{k-1} requested before the product (the row update
+    // after it then waits for no load)
+    VT acc[6][NR], ro[6][NR], rpo[6][NR], zo[6][NR];
 #pragma unroll
     for (int c = 0; c < 6; ++c)
 #pragma unroll
-        for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? zz(ii, c, h) : VT(0);
+        for (int h = 0; h < NR; ++h) {
+            const int64_t o = (6 * ii + c) * NR + h;
+            zo[c][h] = zz(ii, c, h);
+            ro[c][h] = r[o];
+            rpo[c][h] = k > 0 ? rprev[o] : VT(0);
+            acc[c][h] = wave == 0 ? zo[c][h] : VT(0);
+        }
     ell_apply<_Float16, VT, NR, WPS>(sl, A, w, ma, mb, acc, zz);
     __shared__ VT stage[64 * 6 * NR];
     double v[NR], vp[NR];
@@ -1510,12 +1518,11 @@ __global__ __launch_bounds__(64 * WPS) void k_pcg_post(Slices sl, Partials P, Sy
     for (int h = 0; h < NR; ++h) v[h] = vp[h] = 0.0;
     // (z of inactive load cases is never used)
     flush_rows<VT, NR, WPS>(stage, ma, [&](int c, int h) {
-        const int64_t o = (6 * ii + c) * NR + h;
-        const VT rv = r[o];
-        const VT zn = zz(ii, c, h) + (rv - acc[c][h]) / th;
+        const VT rv = ro[c][h];
+        const VT zn = zo[c][h] + (rv - acc[c][h]) / th;
         if (on && ((amask >> h) & 1)) {
             v[h] += (double)rv * (double)zn;
-            if (k > 0) vp[h] += (double)rprev[o] * (double)zn;   // (no r_{-1} at the start)
+            if (k > 0) vp[h] += (double)rpo[c][h] * (double)zn;   // (no r_{-1} at the start)
         }
         return zn;
     }, z);
