@@ -1733,13 +1733,31 @@ __global__ __launch_bounds__(256) void k_mg_prolong(int32_t n, const int32_t* __
 #pragma unroll
         for (int h = 0; h < NR; ++h) acc[h] = bv[h] / th;
     }
-    for (int q = qa; q < qb; ++q) {
-        const int64_t J = p_col[q];
-        const _Float16* pb = P + (int64_t)q * 36 + r * 6;
+    // four blocks per round, their loads issued together (a block past the end
+    // re-reads the row's first one and is not added)
+    for (int q0 = qa; q0 < qb; q0 += 4) {
+        int64_t J[4];
 #pragma unroll
-        for (int c = 0; c < 6; ++c)
+        for (int v = 0; v < 4; ++v) J[v] = p_col[q0 + v < qb ? q0 + v : qa];
+        float pv[4][6], xv[4][6][NR];
 #pragma unroll
-            for (int h = 0; h < NR; ++h) acc[h] += (float)pb[c] * xc[(6 * J + c) * NR + h];
+        for (int v = 0; v < 4; ++v) {
+            const _Float16* pb = P + (int64_t)(q0 + v < qb ? q0 + v : qa) * 36 + r * 6;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                pv[v][c] = (float)pb[c];
+#pragma unroll
+                for (int h = 0; h < NR; ++h) xv[v][c][h] = xc[(6 * J[v] + c) * NR + h];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            if (q0 + v < qb) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+#pragma unroll
+                    for (int h = 0; h < NR; ++h) acc[h] += pv[v][c] * xv[v][c][h];
+            }
     }
 #pragma unroll
     for (int h = 0; h < NR; ++h) z1[u * NR + h] = (ZT)acc[h];
