@@ -114,7 +114,7 @@ struct AmgLevelDev {
     int32_t *d_pt_ptr = nullptr, *d_pt_row = nullptr, *d_p_tpos = nullptr;
     int32_t *d_w_row = nullptr, *d_w_cptr = nullptr, *d_w_caref = nullptr, *d_w_cp = nullptr;
     int32_t *d_c_cptr = nullptr, *d_c_p = nullptr, *d_c_w = nullptr;
-    double *d_P64 = nullptr, *d_W64 = nullptr;
+    float *d_Pf = nullptr, *d_Wf = nullptr;
     _Float16 *d_P16 = nullptr, *d_PT16 = nullptr;   // FP16, by fine row / transposed by coarse node
     // V-cycle vectors (level >= 1), 6 n x nrhs FP32
     float *d_b = nullptr, *d_t = nullptr, *d_r = nullptr, *d_x = nullptr;
@@ -146,7 +146,7 @@ struct AmgState {
     std::vector<int32_t*> d_tail_split[5][2];
     std::vector<uint8_t*> d_smooth, d_rskip;
     float* d_inv = nullptr;           // dense inverses, column-major, FP32
-    double* d_Anat = nullptr;         // set-up scratch: a level's FP64 blocks in natural layout [position][36]
+    float* d_Anat = nullptr;         // set-up scratch: a level's blocks in natural layout [position][36], FP32
     int64_t inv_total = 0;
     int vec_nrhs = 0;                 // V-cycle vectors sized for this many load cases
     int wps0 = 4;                     // waves per slice of the level-0 kernels
@@ -457,12 +457,13 @@ __device__ __forceinline__ double ablk_nat(const double* __restrict__ An, int32_
 // group, position): reads cover consecutive addresses, each position's 18 writes
 // fill its 288 bytes.
 __global__ __launch_bounds__(256) void k_amg_unpack(int64_t npos, const double2* __restrict__ ell,
-                                                    double2* __restrict__ nat) {
+                                                    float2* __restrict__ nat) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= npos * 18) return;
     const int64_t lane = t & (ROWS_PER_WG - 1), sg = t / ROWS_PER_WG;
     const int64_t slot = sg / 18, g = sg % 18;
-    nat[(slot * ROWS_PER_WG + lane) * 18 + g] = ell[t];
+    const double2 v = ell[t];
+    nat[(slot * ROWS_PER_WG + lane) * 18 + g] = make_float2((float)v.x, (float)v.y);
 }
 
 // Rigid-body modes of every aggregated fine node relative to its aggregate's
@@ -710,10 +711,30 @@ __global__ void k_amg_theta(int32_t ns, const int32_t* __restrict__ sys_wg_off, 
     }
 }
 
-// Both products below run one thread per output block (36 accumulators): the
-// thread reads each operand block as 18 16-byte pieces of consecutive memory
-// and forms all 36 outputs from it.  (A thread per block row issued 6 x 7
-// 8-byte loads per product for the same bytes and ran at half the occupancy.)
+// The set-up products (P, W = A P, A_c = P^T W) run one thread per output block
+// (36 accumulators): the thread reads each operand block as 16-byte pieces of
+// consecutive memory and forms all 36 outputs from it.  (A thread per block row
+// issued 6 x 7 8-byte loads per product for the same bytes and ran at half the
+// occupancy.)  Their operands -- the natural-layout copy of A, P and W -- are
+// stored in FP32 and the products accumulate in FP64: the loads are what bounds
+// these kernels, and the V-cycle uses P in FP16 and the coarse operators in
+// FP32 anyway (6.8 -> 5.5 ms of set-up per step on the 64-design SIMP batch).
+__device__ __forceinline__ void load_blk(const float* __restrict__ p, double (&v)[36]) {
+    const float4* q = reinterpret_cast<const float4*>(p);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        const float4 t = q[e];
+        v[4 * e] = t.x;
+        v[4 * e + 1] = t.y;
+        v[4 * e + 2] = t.z;
+        v[4 * e + 3] = t.w;
+    }
+}
+__device__ __forceinline__ void store_blk(float* __restrict__ p, const double (&v)[36]) {
+    float4* o = reinterpret_cast<float4*>(p);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) o[e] = make_float4((float)v[4 * e], (float)v[4 * e + 1], (float)v[4 * e + 2], (float)v[4 * e + 3]);
+}
 __device__ __forceinline__ void load_blk(const double* __restrict__ p, double (&v)[36]) {
     const double2* q = reinterpret_cast<const double2*>(p);
 #pragma unroll
@@ -730,8 +751,8 @@ __global__ __launch_bounds__(256) void k_amg_psmooth(int64_t nP, const int32_t* 
                                                      const int32_t* __restrict__ p_cj,
                                                      const int32_t* __restrict__ p_caref,
                                                      const int32_t* __restrict__ node_sys,
-                                                     const double* __restrict__ omega, const double* __restrict__ An,
-                                                     const double* __restrict__ Q, double* __restrict__ P) {
+                                                     const double* __restrict__ omega, const float* __restrict__ An,
+                                                     const double* __restrict__ Q, float* __restrict__ P) {
     const int64_t b = xcd_wg() * blockDim.x + threadIdx.x;
     if (b >= nP) return;
     const double om = omega[node_sys[p_row[b]]];
@@ -759,16 +780,14 @@ __global__ __launch_bounds__(256) void k_amg_psmooth(int64_t nP, const int32_t* 
                 }
         }
     }
-    double2* o = reinterpret_cast<double2*>(P + b * 36);
-#pragma unroll
-    for (int e = 0; e < 18; ++e) o[e] = make_double2(out[2 * e], out[2 * e + 1]);
+    store_blk(P + b * 36, out);
 }
 
 // W(i, J) = sum over its products of A(i, j) P(j, J)
 __global__ __launch_bounds__(256) void k_amg_w(int64_t nW, const int32_t* __restrict__ w_cptr,
                                                const int32_t* __restrict__ w_caref, const int32_t* __restrict__ w_cp,
-                                               const double* __restrict__ An, const double* __restrict__ P,
-                                               double* __restrict__ W) {
+                                               const float* __restrict__ An, const float* __restrict__ P,
+                                               float* __restrict__ W) {
     const int64_t b = xcd_wg() * blockDim.x + threadIdx.x;
     if (b >= nW) return;
     double out[36];
@@ -795,16 +814,14 @@ __global__ __launch_bounds__(256) void k_amg_w(int64_t nW, const int32_t* __rest
                 }
         }
     }
-    double2* o = reinterpret_cast<double2*>(W + b * 36);
-#pragma unroll
-    for (int e = 0; e < 18; ++e) o[e] = make_double2(out[2 * e], out[2 * e + 1]);
+    store_blk(W + b * 36, out);
 }
 
 // A_c(I, J) = sum over its products of P(i, I)^T W(i, J) (blocks with I <= J);
 // thread per output block, like the two products above
 __global__ __launch_bounds__(256) void k_amg_galerkin(int64_t nnzc, const int32_t* __restrict__ c_cptr,
                                                       const int32_t* __restrict__ c_p, const int32_t* __restrict__ c_w,
-                                                      const double* __restrict__ P, const double* __restrict__ W,
+                                                      const float* __restrict__ P, const float* __restrict__ W,
                                                       double* __restrict__ Ac) {
     const int64_t k = xcd_wg() * blockDim.x + threadIdx.x;
     if (k >= nnzc) return;
@@ -918,13 +935,13 @@ __global__ __launch_bounds__(256) void k_amg_cscale(int64_t nval, const int32_t*
 // by fine row (prolongation) and transposed by coarse node (restriction)
 __global__ __launch_bounds__(256) void k_amg_pscale(int64_t nP, const int32_t* __restrict__ p_col,
                                                     const int32_t* __restrict__ p_tpos,
-                                                    const double* __restrict__ P, const double* __restrict__ Lci,
+                                                    const float* __restrict__ P, const double* __restrict__ Lci,
                                                     _Float16* __restrict__ P16, _Float16* __restrict__ PT16) {
     const int64_t t = xcd_wg() * blockDim.x + threadIdx.x;
     if (t >= nP * 36) return;
     const int64_t b = t / 36;
     const int e = (int)(t % 36), r = e / 6, c = e % 6;
-    const double* p = P + b * 36;
+    const float* p = P + b * 36;
     const double* l = Lci + (int64_t)p_col[b] * 36;
     double s = 0.0;
 #pragma unroll
@@ -2161,10 +2178,10 @@ static int amg_build_device(Plan* pl, AmgState& A) {
         AMG_TRY(amg_alloc(A, &D.d_B, (size_t)H.n * 36));
         AMG_TRY(amg_alloc(A, &D.d_Q, (size_t)H.n * 36));
         AMG_TRY(amg_alloc(A, &D.d_R, (size_t)D.n_agg * 36));
-        AMG_TRY(amg_alloc(A, &D.d_P64, (size_t)D.nP * 36));
+        AMG_TRY(amg_alloc(A, &D.d_Pf, (size_t)D.nP * 36));
         AMG_TRY(amg_alloc(A, &D.d_P16, (size_t)D.nP * 36));
         AMG_TRY(amg_alloc(A, &D.d_PT16, (size_t)D.nP * 36));
-        AMG_TRY(amg_alloc(A, &D.d_W64, (size_t)D.nW * 36));
+        AMG_TRY(amg_alloc(A, &D.d_Wf, (size_t)D.nW * 36));
         AMG_TRY(amg_alloc(A, &D.d_rho, (size_t)ns));
         AMG_TRY(amg_alloc(A, &D.d_omega, (size_t)ns));
         AMG_TRY(amg_alloc(A, &D.d_theta, (size_t)ns));
@@ -2298,17 +2315,17 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
         if (D.nP > 0) {
             const int64_t npos = l == 0 ? pl->ell_slots : D.ell_slots;
             hipLaunchKernelGGL(k_amg_unpack, dim3(nblk(npos * 18, 256)), dim3(256), 0, s, npos, (const double2*)A64,
-                               (double2*)A.d_Anat);
+                               (float2*)A.d_Anat);
             hipLaunchKernelGGL(k_amg_psmooth, xcd_grid(nblk(D.nP, 256)), dim3(256), 0, s, D.nP, D.d_p_row,
                                D.d_p_cptr, D.d_p_cj, D.d_p_caref, D.d_node_sys, (const double*)D.d_omega,
-                               (const double*)A.d_Anat, (const double*)D.d_Q, D.d_P64);
+                               (const float*)A.d_Anat, (const double*)D.d_Q, D.d_Pf);
             hipLaunchKernelGGL(k_amg_w, xcd_grid(nblk(D.nW, 256)), dim3(256), 0, s, D.nW, D.d_w_cptr, D.d_w_caref,
-                               D.d_w_cp, (const double*)A.d_Anat, (const double*)D.d_P64, D.d_W64);
+                               D.d_w_cp, (const float*)A.d_Anat, (const float*)D.d_Pf, D.d_Wf);
         }
         const int64_t nnzc = A.host[l + 1].colind.size();
         if (nnzc > 0) {
             hipLaunchKernelGGL(k_amg_galerkin, xcd_grid(nblk(nnzc, 256)), dim3(256), 0, s, nnzc, D.d_c_cptr, D.d_c_p,
-                               D.d_c_w, (const double*)D.d_P64, (const double*)D.d_W64, A.d_Ac[l + 1]);
+                               D.d_c_w, (const float*)D.d_Pf, (const float*)D.d_Wf, A.d_Ac[l + 1]);
             hipLaunchKernelGGL(k_amg_cfactor, dim3(nblk(C.n, 128)), dim3(128), 0, s, C.n, A.d_cdiag[l + 1],
                                (const double*)A.d_Ac[l + 1], A.d_Lc[l + 1], A.d_Lci[l + 1]);
             const int64_t nval = C.ell_slots - ROWS_PER_WG;
@@ -2319,7 +2336,7 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
         }
         if (D.nP > 0)
             hipLaunchKernelGGL(k_amg_pscale, xcd_grid(nblk(D.nP * 36, 256)), dim3(256), 0, s, D.nP, D.d_p_col,
-                               D.d_p_tpos, (const double*)D.d_P64, (const double*)A.d_Lci[l + 1], D.d_P16,
+                               D.d_p_tpos, (const float*)D.d_Pf, (const double*)A.d_Lci[l + 1], D.d_P16,
                                D.d_PT16);
         if (C.d_B && C.n > 0)
             hipLaunchKernelGGL(k_amg_nullc, xcd_grid(nblk((int64_t)C.n * 36, 256)), dim3(256), 0, s, C.n,
