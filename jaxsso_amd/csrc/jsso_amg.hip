@@ -63,6 +63,7 @@ constexpr double AMG_FLOOR_FACTOR = 16.0;
 constexpr int AMG_FLOOR_WINDOW = 24;
 constexpr double AMG_RHO_SAFETY = 1.12;   // the power iteration approaches rho from below
 constexpr int AMG_MAXLEVELS = 12;
+constexpr int AMG_CUT_GRAPHS = 7;         // captured chunk lengths below the replacement interval: 2 .. 14
 
 // ---------------------------------------------------------------------------
 // host: symbolic hierarchy
@@ -156,6 +157,8 @@ struct AmgState {
     uint64_t gen = 1;                 // bumped whenever a buffer the graphs point at changes
     std::vector<void*> allocs;
     CgGraph graph[3][9][2];       // per precision slot (prec_slot) x load cases x [iterations | replacement]
+    // chunks cut short at a predicted convergence: one pair per even length 2, 4, ... 2 * AMG_CUT_GRAPHS
+    CgGraph graph_cut[3][9][AMG_CUT_GRAPHS][2];
     bool built = false;           // a full numeric set-up has run (a kept hierarchy can be reused)
     int power_next = AMG_POWER_STEPS;   // level 0: index of the next power-iteration step (parities of
                                         // the vectors and partials continue across refreshes)
@@ -2727,20 +2730,29 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
     // slices active, so the launch processes the whole plan)
     std::vector<std::vector<std::pair<hipEvent_t, hipEvent_t>>> samples;
     int chunk_no = 0;
-    auto enqueue_chunk = [&](int it0, int steps) -> int {
+    // `may_sample`: every system was still running at the last chunk end read
+    // (only such chunks can count as samples, so no other one is launched eagerly
+    // for timing)
+    auto enqueue_chunk = [&](int it0, int steps, bool may_sample) -> int {
         hipGraphExec_t ga = nullptr, gb = nullptr;
         // a chunk sampled for timing runs eagerly with its k_pcg_dir launches bracketed
-        const bool sampled = timing_sample_chunk();
+        const bool sampled = may_sample && timing_sample_chunk();
         samples.emplace_back();
         auto* tv = sampled ? &samples[chunk_no] : nullptr;
         ++chunk_no;
-        if (!sampled && steps == E && E % 2 == 0 && it0 % 2 == 0 && pl->graphs_ok && graphs_enabled()) {
-            ga = capture_graph(pl, gc[0], key, A.wps0, E, tol, [&](hipStream_t cs) {
-                for (int it = 1; it < E; ++it) amg_step<MT, VT, NR>(pl, A, w, bf, A0, it, cs);
+        // the full-length chunk, or one cut short (even lengths; k only matters
+        // through its parity, so a graph serves every even start)
+        CgGraph* gp = steps == E ? gc
+                      : (steps < E && steps / 2 - 1 < AMG_CUT_GRAPHS)
+                          ? A.graph_cut[prec_slot<MT, VT>()][NR][steps / 2 - 1]
+                          : nullptr;
+        if (!sampled && gp && steps % 2 == 0 && it0 % 2 == 0 && pl->graphs_ok && graphs_enabled()) {
+            ga = capture_graph(pl, gp[0], key, A.wps0, steps, tol, [&](hipStream_t cs) {
+                for (int it = 1; it < steps; ++it) amg_step<MT, VT, NR>(pl, A, w, bf, A0, it, cs);
             });
             if (ga)
-                gb = capture_graph(pl, gc[1], key, A.wps0, E, tol, [&](hipStream_t cs) {
-                    amg_replace<MT, VT, NR>(pl, A, w, bf, A0, E, E, tol, cs);
+                gb = capture_graph(pl, gp[1], key, A.wps0, steps, tol, [&](hipStream_t cs) {
+                    amg_replace<MT, VT, NR>(pl, A, w, bf, A0, steps, steps, tol, cs);
                 });
         }
         if (ga && gb) {
@@ -2756,35 +2768,83 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
 
     // Convergence flags (and the FP64 relative residuals) are read one chunk
     // behind the GPU, as in the block-Jacobi solver.  The residual is only tested
-    // at a chunk's end, so a chunk that would straddle the predicted crossing of
-    // tol is cut to end just after it: the slowest running system's residuals at
-    // the last two known chunk ends give its convergence factor per iteration,
-    // extrapolated over the chunk in flight.  A cut chunk has an even length
-    // (the chunk graphs need even starts) and runs with eager launches; a wrong
-    // prediction costs one more replacement.
+    // at a chunk's end; the slowest running system's residuals at the last two
+    // known chunk ends give its convergence factor per iteration, and so the
+    // iteration at which it is predicted to reach tol.  A chunk that would
+    // straddle that iteration is cut to end just after it (an even length: its
+    // graphs are captured once per length); a wrong prediction costs one more
+    // replacement.  A chunk in flight that is predicted to end the solve is read
+    // before the next one is enqueued: the GPU then waits for the host's
+    // turnaround instead of running a whole chunk after convergence.  After two
+    // such predictions that failed, the solve keeps reading one chunk behind.
     std::vector<int> chunk_end;
+    std::vector<char> ends_solve;                // per chunk: read early as predicted to end the solve
     std::vector<std::pair<int, double>> known;   // (iteration, slowest running system's residual)
     std::vector<char> all_running_after;         // per chunk read: every system still running after it
-    int it = 0, it_done = 0, chunks_done = 0, c = 0;
+    int it = 0, it_done = 0, chunks_done = 0, c = 0, next_read = 0, misses = 0;
+    // flags of chunks next_read..upto in order: 1 while a system is still running
+    // (0 once none is, with it_done / chunks_done set), or an error code
+    auto read_upto = [&](int upto) -> int {
+        for (; next_read <= upto; ++next_read) {
+            const int chk = next_read;
+            JSSO_HIP(hipEventSynchronize(pl->flag_ev[chk & 1]));
+            timing_collect();
+            const int32_t* fl = pl->h_flags + (size_t)(chk & 1) * nsys;
+            const double* rl = pl->h_rel + (size_t)(chk & 1) * nsys;
+            int running = 0;
+            double worst = 0.0;
+            for (int k = 0; k < nsys; ++k)
+                if (fl[k] != 0) {
+                    ++running;
+                    worst = std::max(worst, rl[k]);
+                }
+            all_running_after.push_back(running == nsys);
+            if (running == 0) {
+                it_done = chunk_end[chk];
+                chunks_done = chk + 1;
+                return 0;
+            }
+            if (ends_solve[chk]) ++misses;
+            known.push_back({chunk_end[chk], worst});
+        }
+        return 1;
+    };
+    // predicted iteration at which the slowest running system reaches tol (0: no prediction)
+    auto predict = [&]() -> double {
+        const size_t nk = known.size();
+        if (nk < 2 || !(known[nk - 1].first > known[nk - 2].first && known[nk - 1].second > tol &&
+                        known[nk - 1].second < known[nk - 2].second))
+            return 0.0;
+        const double per_it = std::log(known[nk - 1].second / known[nk - 2].second) /
+                              (known[nk - 1].first - known[nk - 2].first);
+        return known[nk - 1].first + std::log(tol / known[nk - 1].second) / per_it;
+    };
     for (;;) {
         const bool more = it < maxit;
+        if (more && next_read == c - 1 && misses < 2) {
+            const double kstar = predict();
+            if (kstar > 0.0 && kstar <= chunk_end[c - 1]) {
+                ends_solve[c - 1] = 1;
+                const int r = read_upto(c - 1);
+                if (r < 0) return r;
+                if (r == 0) break;
+            }
+        }
         if (more) {
             int steps = std::min(E, maxit - it);
-            const size_t nk = known.size();
-            if (nk >= 2 && known[nk - 1].first > known[nk - 2].first && known[nk - 1].second > tol &&
-                known[nk - 1].second < known[nk - 2].second) {
-                const double per_it = std::log(known[nk - 1].second / known[nk - 2].second) /
-                                      (known[nk - 1].first - known[nk - 2].first);
-                const double kstar = known[nk - 1].first + std::log(tol / known[nk - 1].second) / per_it;
-                if (kstar > it && kstar < it + steps - 1) {
-                    int cut = (int)std::ceil(kstar - it) + 1;
-                    cut += cut & 1;
-                    steps = std::max(2, std::min(steps, cut));
-                }
+            const double kstar = predict();
+            if (kstar > it && kstar < it + steps - 1) {
+                int cut = (int)std::ceil(kstar - it) + 1;
+                cut += cut & 1;
+                steps = std::max(2, std::min(steps, cut));
             }
-            if ((rc = enqueue_chunk(it, steps))) return rc;
+            // a chunk can only count as a timing sample if every system was still
+            // running at the last chunk end read
+            const bool may_sample = all_running_after.empty() || all_running_after.back();
+            if ((rc = enqueue_chunk(it, steps, may_sample))) return rc;
             it += steps;
             chunk_end.push_back(it);
+            ends_solve.push_back(0);
             JSSO_HIP(hipMemcpyAsync(pl->h_flags + (size_t)(c & 1) * nsys, w.st.active, nsys * sizeof(int32_t),
                                     hipMemcpyDeviceToHost, s));
             JSSO_HIP(hipMemcpyAsync(pl->h_rel + (size_t)(c & 1) * nsys, w.st.relres, nsys * sizeof(double),
@@ -2792,28 +2852,12 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
             JSSO_HIP(hipEventRecord(pl->flag_ev[c & 1], s));
             ++c;
         }
-        const int chk = more ? c - 2 : c - 1;
-        if (chk >= 0) {
-            JSSO_HIP(hipEventSynchronize(pl->flag_ev[chk & 1]));
-            timing_collect();
-            const int32_t* fl = pl->h_flags + (size_t)(chk & 1) * nsys;
-            const double* rl = pl->h_rel + (size_t)(chk & 1) * nsys;
-            bool any = false;
-            int running = 0;
-            double worst = 0.0;
-            for (int k = 0; k < nsys; ++k)
-                if (fl[k] != 0) {
-                    any = true;
-                    ++running;
-                    worst = std::max(worst, rl[k]);
-                }
-            if ((int)all_running_after.size() == chk) all_running_after.push_back(running == nsys);
-            if (!any) {
-                it_done = chunk_end[chk];
-                chunks_done = chk + 1;
-                break;
-            }
-            known.push_back({chunk_end[chk], worst});
+        // one chunk behind (the host buffers hold two chunks), all of them at the end
+        const int upto = more ? c - 2 : c - 1;
+        if (upto >= 0) {
+            const int r = read_upto(upto);
+            if (r < 0) return r;
+            if (r == 0) break;
         }
         if (!more) {
             it_done = it;
@@ -2911,6 +2955,11 @@ void amg_destroy(Plan* pl) {
         for (auto& pair : row)
             for (auto& g : pair)
                 if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    for (auto& row : A->graph_cut)
+        for (auto& lens : row)
+            for (auto& pair : lens)
+                for (auto& g : pair)
+                    if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (void* a : A->allocs) (void)hipFree(a);
     for (void* v : {A->d_z, A->d_t0, A->d_p1})
         if (v) (void)hipFree(v);
