@@ -1867,6 +1867,36 @@ __device__ __forceinline__ void tail_smooth(const Slices& sl, const float* __res
     using S = TailShape<NR>;
     const int lane = threadIdx.x & 63, grp = (threadIdx.x >> 6) / S::TW, sub = (threadIdx.x >> 6) % S::TW;
     auto xs = [&](int64_t node, int c, int h) { return src[(6 * (node - nb) + c) * NR + h]; };
+    if (q1 - q0 == 1) {
+        // a single slice: every wavefront of the workgroup takes a share of its
+        // ELL slots (a coarse row has many), rather than one group of TW
+        constexpr int W = S::TW * S::TG;
+        const int wave = threadIdx.x >> 6;
+        const int4 ma = sl.meta[3 * q0], mb = sl.meta[3 * q0 + 1];
+        const int64_t i = (int64_t)ma.y + lane;
+        const bool on = i < ma.z;
+        const int64_t ii = on ? i : (int64_t)ma.y;
+        float acc[6][NR];
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+#pragma unroll
+            for (int h = 0; h < NR; ++h) acc[c][h] = wave == 0 ? xs(ii, c, h) : 0.f;
+        ell_apply<float, float, NR, W, 1>(sl, A, q0, ma, mb, acc, xs);
+        if (wave == 0 && on) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+#pragma unroll
+                for (int h = 0; h < NR; ++h) {
+                    const int64_t o = (6 * (i - nb) + c) * NR + h;
+                    float y = k.cv * src[o] + k.ca * acc[c][h];
+                    if (k.use_b) y += k.cb * b[o];
+                    if (k.use_w) y += w[o];
+                    dst[o] = y;
+                }
+        }
+        __syncthreads();
+        return;
+    }
     for (int base = q0; base < q1; base += S::TG) {
         // every group runs a slice (groups past the end repeat the last one) so that
         // all waves meet the same barriers; only owners store
