@@ -762,10 +762,18 @@ __global__ __launch_bounds__(256) void k_amg_psmooth(int64_t nP, const int32_t* 
     double out[36];
 #pragma unroll
     for (int e = 0; e < 36; ++e) out[e] = 0.0;
-    for (int q = p_cptr[b]; q < p_cptr[b + 1]; ++q) {
+    // (the next product's indices are requested while this one is formed)
+    const int q1 = p_cptr[b + 1];
+    int q = p_cptr[b];
+    int32_t cjn = q < q1 ? p_cj[q] : 0, arn = q < q1 ? p_caref[q] : -1;
+    for (; q < q1; ++q) {
+        const int32_t cj = cjn, aref = arn;
+        if (q + 1 < q1) {
+            cjn = p_cj[q + 1];
+            arn = p_caref[q + 1];
+        }
         double qv[36];
-        load_blk(Q + (int64_t)p_cj[q] * 36, qv);
-        const int32_t aref = p_caref[q];
+        load_blk(Q + (int64_t)cj * 36, qv);
         if (aref < 0) {
 #pragma unroll
             for (int e = 0; e < 36; ++e) out[e] += (1.0 - om) * qv[e];
@@ -796,10 +804,18 @@ __global__ __launch_bounds__(256) void k_amg_w(int64_t nW, const int32_t* __rest
     double out[36];
 #pragma unroll
     for (int e = 0; e < 36; ++e) out[e] = 0.0;
-    for (int q = w_cptr[b]; q < w_cptr[b + 1]; ++q) {
+    // (the next product's indices are requested while this one is formed)
+    const int q1 = w_cptr[b + 1];
+    int q = w_cptr[b];
+    int32_t cpn = q < q1 ? w_cp[q] : 0, arn = q < q1 ? w_caref[q] : -1;
+    for (; q < q1; ++q) {
+        const int32_t cp = cpn, aref = arn;
+        if (q + 1 < q1) {
+            cpn = w_cp[q + 1];
+            arn = w_caref[q + 1];
+        }
         double pv[36];
-        load_blk(P + (int64_t)w_cp[q] * 36, pv);
-        const int32_t aref = w_caref[q];
+        load_blk(P + (int64_t)cp * 36, pv);
         if (aref < 0) {
 #pragma unroll
             for (int e = 0; e < 36; ++e) out[e] += pv[e];
@@ -831,10 +847,19 @@ __global__ __launch_bounds__(256) void k_amg_galerkin(int64_t nnzc, const int32_
     double out[36];
 #pragma unroll
     for (int e = 0; e < 36; ++e) out[e] = 0.0;
-    for (int q = c_cptr[k]; q < c_cptr[k + 1]; ++q) {
+    // (the next product's indices are requested while this one is formed)
+    const int q1 = c_cptr[k + 1];
+    int q = c_cptr[k];
+    int32_t cpn = q < q1 ? c_p[q] : 0, cwn = q < q1 ? c_w[q] : 0;
+    for (; q < q1; ++q) {
+        const int32_t cp = cpn, cw = cwn;
+        if (q + 1 < q1) {
+            cpn = c_p[q + 1];
+            cwn = c_w[q + 1];
+        }
         double pv[36], wv[36];
-        load_blk(P + (int64_t)c_p[q] * 36, pv);
-        load_blk(W + (int64_t)c_w[q] * 36, wv);
+        load_blk(P + (int64_t)cp * 36, pv);
+        load_blk(W + (int64_t)cw * 36, wv);
 #pragma unroll
         for (int r = 0; r < 6; ++r)
 #pragma unroll
