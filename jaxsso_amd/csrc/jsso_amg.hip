@@ -692,14 +692,30 @@ __global__ __launch_bounds__(64 * WPS) void k_amg_power(Slices sl, const MT* __r
 //   c = 2 sigma^2 / ((2 sigma^2 - 1) theta),
 // so x2 = x0 + c (r0 + r1), r1 = r0 - A r0 / theta (Saad, Iterative Methods for
 // Sparse Linear Systems, 2nd ed., ch. 12: the three-term Chebyshev recurrence).
-__global__ void k_amg_theta(int32_t ns, const int32_t* __restrict__ sys_wg_off, const double* __restrict__ part,
-                            int last, const uint8_t* __restrict__ cont, int32_t n_wg, double* __restrict__ rho,
-                            double* __restrict__ omega, float* __restrict__ theta, float* __restrict__ cheb) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+// A wavefront per structure: the lanes bring the per-slice partials into LDS
+// (a structure of a single plan can have hundreds of slices, whose loads one
+// lane would wait for in turn), lane 0 adds them in slice order.
+constexpr int AMG_THETA_CHUNK = 2048;
+__global__ __launch_bounds__(64) void k_amg_theta(int32_t ns, const int32_t* __restrict__ sys_wg_off,
+                                                  const double* __restrict__ part, int last,
+                                                  const uint8_t* __restrict__ cont, int32_t n_wg,
+                                                  double* __restrict__ rho, double* __restrict__ omega,
+                                                  float* __restrict__ theta, float* __restrict__ cheb) {
+    __shared__ double buf[AMG_THETA_CHUNK];
+    const int s = blockIdx.x;
     if (s >= ns || !cont[s]) return;
     const double* pp = part + (int64_t)(last & 1) * n_wg;
+    const int q0 = sys_wg_off[s], q1 = sys_wg_off[s + 1];
     double t = 0.0;
-    for (int q = sys_wg_off[s]; q < sys_wg_off[s + 1]; ++q) t += pp[q];
+    for (int c0 = q0; c0 < q1; c0 += AMG_THETA_CHUNK) {
+        const int n = min(AMG_THETA_CHUNK, q1 - c0);
+        for (int u = threadIdx.x; u < n; u += 64) buf[u] = pp[c0 + u];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int u = 0; u < n; ++u) t += buf[u];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
     double r = AMG_RHO_SAFETY * sqrt(t);
     if (!(r > 0.0) || !isfinite(r)) r = 1.0;
     rho[s] = r;
@@ -2367,7 +2383,7 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
                                    (const float*)D.d_A32, step, (const double*)D.d_pv[(step + 1) & 1],
                                    D.d_pv[step & 1], D.d_ppart, (const uint8_t*)D.d_cont, D.n_wg);
         }
-        hipLaunchKernelGGL(k_amg_theta, dim3(nblk(ns, 64)), dim3(64), 0, s, ns, sys_wg_off,
+        hipLaunchKernelGGL(k_amg_theta, dim3(ns), dim3(64), 0, s, ns, sys_wg_off,
                            (const double*)D.d_ppart, AMG_POWER_STEPS - 1, (const uint8_t*)D.d_cont, D.n_wg, D.d_rho,
                            D.d_omega, D.d_theta, l == 0 ? nullptr : D.d_cheb);
         if (D.nP > 0) {
@@ -2429,7 +2445,7 @@ static int amg_setup_light(Plan* pl, AmgState& A, hipStream_t s) {
                            (const _Float16*)pl->d_Ke16, step, (const double*)D.d_pv[(step + 1) & 1],
                            D.d_pv[step & 1], D.d_ppart, (const uint8_t*)D.d_cont, D.n_wg);
     const int last = s0 + AMG_POWER_STEPS_WARM - 1;
-    hipLaunchKernelGGL(k_amg_theta, dim3(nblk(ns, 64)), dim3(64), 0, s, ns, (const int32_t*)pl->d_sys_wg_off,
+    hipLaunchKernelGGL(k_amg_theta, dim3(ns), dim3(64), 0, s, ns, (const int32_t*)pl->d_sys_wg_off,
                        (const double*)D.d_ppart, last, (const uint8_t*)D.d_cont, D.n_wg, D.d_rho, D.d_omega,
                        D.d_theta, (float*)nullptr);
     JSSO_HIP(hipGetLastError());
