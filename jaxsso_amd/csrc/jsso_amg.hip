@@ -63,6 +63,10 @@ constexpr double AMG_FLOOR_FACTOR = 16.0;
 constexpr int AMG_FLOOR_WINDOW = 24;
 constexpr double AMG_RHO_SAFETY = 1.12;   // the power iteration approaches rho from below
 constexpr int AMG_MAXLEVELS = 12;
+// output blocks up to which the set-up products run one lane per block entry
+// (measured crossovers: P and W between 4 k and 25 k blocks, A_c above 75 k)
+constexpr int64_t AMG_ENTRY_PW_MAX = 8192;
+constexpr int64_t AMG_ENTRY_C_MAX = 131072;
 constexpr int AMG_CUT_GRAPHS = 7;         // captured chunk lengths below the replacement interval: 2 .. 14
 
 // ---------------------------------------------------------------------------
@@ -886,6 +890,200 @@ __global__ __launch_bounds__(256) void k_amg_galerkin(int64_t nnzc, const int32_
     double2* o = reinterpret_cast<double2*>(Ac + k * 36);
 #pragma unroll
     for (int e = 0; e < 18; ++e) o[e] = make_double2(out[2 * e], out[2 * e + 1]);
+}
+
+// The same three products for the small levels of a hierarchy: one lane per
+// entry (r, c) of an output block (36 lanes per block, so a workgroup of 256
+// lanes spans about seven blocks):
+// each lane adds, product by product, the six terms of its entry in the order in
+// which a thread holding the whole 6x6 block adds them (the sums are those of
+// that formulation, bit for bit), with the operands' block indices requested two
+// products ahead and the lane's twelve operand values one product ahead.  On a
+// level of few blocks the thread-per-block kernels wait on one chain of
+// dependent block loads per lane (~100 us for the Galerkin product at every
+// coarse level of the 200x200 shell, 15 us this way); on a large level the
+// lane-per-entry form issues 36 times the load instructions and loses (the W
+// product at level 0 of the SIMP batch: 0.65 ms per block, 3.4 ms per entry).
+//
+// P(i, J) = sum over its products of ((1 - omega) delta_ij I - omega A_off(i, j)) Q_j
+__global__ __launch_bounds__(256) void k_amg_psmooth_e(int64_t nP, const int32_t* __restrict__ p_row,
+                                                     const int32_t* __restrict__ p_cptr,
+                                                     const int32_t* __restrict__ p_cj,
+                                                     const int32_t* __restrict__ p_caref,
+                                                     const int32_t* __restrict__ node_sys,
+                                                     const double* __restrict__ omega, const float* __restrict__ An,
+                                                     const double* __restrict__ Q, float* __restrict__ P) {
+    const int64_t t = xcd_wg() * blockDim.x + threadIdx.x;
+    if (t >= nP * 36) return;
+    const int64_t b = t / 36;
+    const int e = (int)(t - b * 36), r = e / 6, c = e - 6 * r;
+    const double om = omega[node_sys[p_row[b]]];
+    const int q0 = p_cptr[b], q1 = p_cptr[b + 1];
+    // operands of one product: Q_j column c (and entry e), A row r (column r transposed)
+    double qn[6], qd = 0.0;
+    float an[6];
+    int32_t arn = -1;
+    auto fetch = [&](int32_t cj, int32_t aref) {
+        const double* qb = Q + (int64_t)cj * 36;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) qn[m] = qb[m * 6 + c];
+        qd = qb[e];
+        arn = aref;
+        if (aref >= 0) {
+            const float* ab = An + (int64_t)(aref >> 1) * 36;
+            const bool tr = aref & 1;
+#pragma unroll
+            for (int m = 0; m < 6; ++m) an[m] = tr ? ab[m * 6 + r] : ab[r * 6 + m];
+        }
+    };
+    int32_t cj2 = 0, ar2 = -1;
+    if (q0 < q1) {
+        fetch(p_cj[q0], p_caref[q0]);
+        if (q0 + 1 < q1) {
+            cj2 = p_cj[q0 + 1];
+            ar2 = p_caref[q0 + 1];
+        }
+    }
+    double acc = 0.0;
+    for (int q = q0; q < q1; ++q) {
+        double qc[6];
+        float ac[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            qc[m] = qn[m];
+            ac[m] = an[m];
+        }
+        const double qdc = qd;
+        const int32_t aref = arn;
+        if (q + 1 < q1) {
+            fetch(cj2, ar2);
+            if (q + 2 < q1) {
+                cj2 = p_cj[q + 2];
+                ar2 = p_caref[q + 2];
+            }
+        }
+        if (aref < 0) {
+            acc += (1.0 - om) * qdc;
+        } else {
+#pragma unroll
+            for (int m = 0; m < 6; ++m) {
+                const double a = -om * (double)ac[m];
+                acc += a * qc[m];
+            }
+        }
+    }
+    P[t] = (float)acc;
+}
+
+// W(i, J) = sum over its products of A(i, j) P(j, J)
+__global__ __launch_bounds__(256) void k_amg_w_e(int64_t nW, const int32_t* __restrict__ w_cptr,
+                                               const int32_t* __restrict__ w_caref, const int32_t* __restrict__ w_cp,
+                                               const float* __restrict__ An, const float* __restrict__ P,
+                                               float* __restrict__ W) {
+    const int64_t t = xcd_wg() * blockDim.x + threadIdx.x;
+    if (t >= nW * 36) return;
+    const int64_t b = t / 36;
+    const int e = (int)(t - b * 36), r = e / 6, c = e - 6 * r;
+    const int q0 = w_cptr[b], q1 = w_cptr[b + 1];
+    // operands of one product: P column c (and entry e), A row r (column r transposed)
+    float pn[6], pd = 0.f, an[6];
+    int32_t arn = -1;
+    auto fetch = [&](int32_t cp, int32_t aref) {
+        const float* pb = P + (int64_t)cp * 36;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) pn[m] = pb[m * 6 + c];
+        pd = pb[e];
+        arn = aref;
+        if (aref >= 0) {
+            const float* ab = An + (int64_t)(aref >> 1) * 36;
+            const bool tr = aref & 1;
+#pragma unroll
+            for (int m = 0; m < 6; ++m) an[m] = tr ? ab[m * 6 + r] : ab[r * 6 + m];
+        }
+    };
+    int32_t cp2 = 0, ar2 = -1;
+    if (q0 < q1) {
+        fetch(w_cp[q0], w_caref[q0]);
+        if (q0 + 1 < q1) {
+            cp2 = w_cp[q0 + 1];
+            ar2 = w_caref[q0 + 1];
+        }
+    }
+    double acc = 0.0;
+    for (int q = q0; q < q1; ++q) {
+        float pc[6], ac[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            pc[m] = pn[m];
+            ac[m] = an[m];
+        }
+        const float pdc = pd;
+        const int32_t aref = arn;
+        if (q + 1 < q1) {
+            fetch(cp2, ar2);
+            if (q + 2 < q1) {
+                cp2 = w_cp[q + 2];
+                ar2 = w_caref[q + 2];
+            }
+        }
+        if (aref < 0) {
+            acc += (double)pdc;
+        } else {
+#pragma unroll
+            for (int m = 0; m < 6; ++m) acc += (double)ac[m] * (double)pc[m];
+        }
+    }
+    W[t] = (float)acc;
+}
+
+// A_c(I, J) = sum over its products of P(i, I)^T W(i, J) (blocks with I <= J)
+__global__ __launch_bounds__(256) void k_amg_galerkin_e(int64_t nnzc, const int32_t* __restrict__ c_cptr,
+                                                      const int32_t* __restrict__ c_p, const int32_t* __restrict__ c_w,
+                                                      const float* __restrict__ P, const float* __restrict__ W,
+                                                      double* __restrict__ Ac) {
+    const int64_t t = xcd_wg() * blockDim.x + threadIdx.x;
+    if (t >= nnzc * 36) return;
+    const int64_t k = t / 36;
+    const int e = (int)(t - k * 36), r = e / 6, c = e - 6 * r;
+    const int q0 = c_cptr[k], q1 = c_cptr[k + 1];
+    // operands of one product: column r of P(i, I), column c of W(i, J)
+    float pn[6], wn[6];
+    auto fetch = [&](int32_t cp, int32_t cw) {
+        const float* pb = P + (int64_t)cp * 36;
+        const float* wb = W + (int64_t)cw * 36;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            pn[m] = pb[m * 6 + r];
+            wn[m] = wb[m * 6 + c];
+        }
+    };
+    int32_t cp2 = 0, cw2 = 0;
+    if (q0 < q1) {
+        fetch(c_p[q0], c_w[q0]);
+        if (q0 + 1 < q1) {
+            cp2 = c_p[q0 + 1];
+            cw2 = c_w[q0 + 1];
+        }
+    }
+    double acc = 0.0;
+    for (int q = q0; q < q1; ++q) {
+        float pc[6], wc[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            pc[m] = pn[m];
+            wc[m] = wn[m];
+        }
+        if (q + 1 < q1) {
+            fetch(cp2, cw2);
+            if (q + 2 < q1) {
+                cp2 = c_p[q + 2];
+                cw2 = c_w[q + 2];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 6; ++m) acc += (double)pc[m] * (double)wc[m];
+    }
+    Ac[t] = acc;
 }
 
 // Cholesky factor of each coarse diagonal block and its inverse.  A pivot that
@@ -2390,16 +2588,21 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
             const int64_t npos = l == 0 ? pl->ell_slots : D.ell_slots;
             hipLaunchKernelGGL(k_amg_unpack, dim3(nblk(npos * 18, 256)), dim3(256), 0, s, npos, (const double2*)A64,
                                (float2*)A.d_Anat);
-            hipLaunchKernelGGL(k_amg_psmooth, xcd_grid(nblk(D.nP, 256)), dim3(256), 0, s, D.nP, D.d_p_row,
-                               D.d_p_cptr, D.d_p_cj, D.d_p_caref, D.d_node_sys, (const double*)D.d_omega,
-                               (const float*)A.d_Anat, (const double*)D.d_Q, D.d_Pf);
-            hipLaunchKernelGGL(k_amg_w, xcd_grid(nblk(D.nW, 256)), dim3(256), 0, s, D.nW, D.d_w_cptr, D.d_w_caref,
-                               D.d_w_cp, (const float*)A.d_Anat, (const float*)D.d_Pf, D.d_Wf);
+            // (the lane-per-entry forms on levels of few blocks: the same sums, bit for bit)
+            const bool pe = D.nP <= AMG_ENTRY_PW_MAX, we = D.nW <= AMG_ENTRY_PW_MAX;
+            hipLaunchKernelGGL(pe ? k_amg_psmooth_e : k_amg_psmooth, xcd_grid(nblk(D.nP * (pe ? 36 : 1), 256)),
+                               dim3(256), 0, s, D.nP, D.d_p_row, D.d_p_cptr, D.d_p_cj, D.d_p_caref, D.d_node_sys,
+                               (const double*)D.d_omega, (const float*)A.d_Anat, (const double*)D.d_Q, D.d_Pf);
+            hipLaunchKernelGGL(we ? k_amg_w_e : k_amg_w, xcd_grid(nblk(D.nW * (we ? 36 : 1), 256)), dim3(256), 0, s,
+                               D.nW, D.d_w_cptr, D.d_w_caref, D.d_w_cp, (const float*)A.d_Anat, (const float*)D.d_Pf,
+                               D.d_Wf);
         }
         const int64_t nnzc = A.host[l + 1].colind.size();
         if (nnzc > 0) {
-            hipLaunchKernelGGL(k_amg_galerkin, xcd_grid(nblk(nnzc, 256)), dim3(256), 0, s, nnzc, D.d_c_cptr, D.d_c_p,
-                               D.d_c_w, (const float*)D.d_Pf, (const float*)D.d_Wf, A.d_Ac[l + 1]);
+            const bool ge = nnzc <= AMG_ENTRY_C_MAX;
+            hipLaunchKernelGGL(ge ? k_amg_galerkin_e : k_amg_galerkin, xcd_grid(nblk(nnzc * (ge ? 36 : 1), 256)),
+                               dim3(256), 0, s, nnzc, D.d_c_cptr, D.d_c_p, D.d_c_w, (const float*)D.d_Pf,
+                               (const float*)D.d_Wf, A.d_Ac[l + 1]);
             hipLaunchKernelGGL(k_amg_cfactor, dim3(nblk(C.n, 128)), dim3(128), 0, s, C.n, A.d_cdiag[l + 1],
                                (const double*)A.d_Ac[l + 1], A.d_Lc[l + 1], A.d_Lci[l + 1]);
             const int64_t nval = C.ell_slots - ROWS_PER_WG;
