@@ -67,6 +67,14 @@ constexpr int AMG_MAXLEVELS = 12;
 // (measured crossovers: P and W between 4 k and 25 k blocks, A_c above 75 k)
 constexpr int64_t AMG_ENTRY_PW_MAX = 8192;
 constexpr int64_t AMG_ENTRY_C_MAX = 131072;
+// JSSO_AMG_ENTRY_PRODUCTS=0 / 1 takes the thread-per-block / lane-per-entry form
+// on every level (the two give the same bits; the tests compare them)
+static bool amg_entry_form(int64_t blocks, int64_t limit) {
+    const char* e = getenv("JSSO_AMG_ENTRY_PRODUCTS");
+    if (e && e[0] == '0') return false;
+    if (e && e[0] == '1') return true;
+    return blocks <= limit;
+}
 constexpr int AMG_CUT_GRAPHS = 7;         // captured chunk lengths below the replacement interval: 2 .. 14
 
 // ---------------------------------------------------------------------------
@@ -2589,7 +2597,7 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
             hipLaunchKernelGGL(k_amg_unpack, dim3(nblk(npos * 18, 256)), dim3(256), 0, s, npos, (const double2*)A64,
                                (float2*)A.d_Anat);
             // (the lane-per-entry forms on levels of few blocks: the same sums, bit for bit)
-            const bool pe = D.nP <= AMG_ENTRY_PW_MAX, we = D.nW <= AMG_ENTRY_PW_MAX;
+            const bool pe = amg_entry_form(D.nP, AMG_ENTRY_PW_MAX), we = amg_entry_form(D.nW, AMG_ENTRY_PW_MAX);
             hipLaunchKernelGGL(pe ? k_amg_psmooth_e : k_amg_psmooth, xcd_grid(nblk(D.nP * (pe ? 36 : 1), 256)),
                                dim3(256), 0, s, D.nP, D.d_p_row, D.d_p_cptr, D.d_p_cj, D.d_p_caref, D.d_node_sys,
                                (const double*)D.d_omega, (const float*)A.d_Anat, (const double*)D.d_Q, D.d_Pf);
@@ -2599,7 +2607,7 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
         }
         const int64_t nnzc = A.host[l + 1].colind.size();
         if (nnzc > 0) {
-            const bool ge = nnzc <= AMG_ENTRY_C_MAX;
+            const bool ge = amg_entry_form(nnzc, AMG_ENTRY_C_MAX);
             hipLaunchKernelGGL(ge ? k_amg_galerkin_e : k_amg_galerkin, xcd_grid(nblk(nnzc * (ge ? 36 : 1), 256)),
                                dim3(256), 0, s, nnzc, D.d_c_cptr, D.d_c_p, D.d_c_w, (const float*)D.d_Pf,
                                (const float*)D.d_Wf, A.d_Ac[l + 1]);

@@ -79,6 +79,26 @@ def test_amg_graph_and_eager_agree_bitwise(monkeypatch):
     assert np.array_equal(ug, ue)
 
 
+@pytest.mark.parametrize("nrhs", [1, 4])
+def test_amg_set_up_product_forms_agree_bitwise(monkeypatch, nrhs):
+    """The AMG set-up products (P, W = A P, A_c = P^T W) come in a
+    thread-per-block and a lane-per-entry form (chosen by the level's size); both
+    add every entry's terms in the same order, so a hierarchy built with either
+    form on every level (JSSO_AMG_ENTRY_PRODUCTS=0 / 1, read at each set-up)
+    gives the same solution bit for bit.  Uneven shells with one and four load
+    cases."""
+    designs = [wl.cfg1(0), wl.cfg1(1), wl.shell_pinned_edges(60, 6.0, 1.0)]
+    print(f"amg product forms nrhs {nrhs}", flush=True)
+    out = {}
+    for form in ("0", "1"):
+        monkeypatch.setenv("JSSO_AMG_ENTRY_PRODUCTS", form)
+        b = StructureBatch(designs, n_cases=nrhs)
+        u = b.solve(precision="fp64", tol=1e-10, preconditioner="amg").cpu().numpy()
+        out[form] = (u, dict(b.plan.last_info))
+    assert out["0"][1]["converged"] == 3 * nrhs and out["0"][1]["iterations"] == out["1"][1]["iterations"]
+    assert np.array_equal(out["0"][0], out["1"][0])
+
+
 def test_amg_set_up_follows_the_assembly():
     """A new assembly on the same plan (other coordinates and thicknesses)
     rebuilds the numeric hierarchy before the next solve."""
