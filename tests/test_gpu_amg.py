@@ -99,6 +99,26 @@ def test_amg_set_up_product_forms_agree_bitwise(monkeypatch, nrhs):
     assert np.array_equal(out["0"][0], out["1"][0])
 
 
+@pytest.mark.parametrize("maxit", [13, 24])
+def test_amg_stops_at_maxit_with_the_best_iterate(maxit):
+    """An AMG solve cut off by maxit (inside a replacement chunk, or at its end)
+    warns, reports exactly maxit iterations and no converged system, and returns
+    its best iterate: a finite u whose relative error against the oracle lies
+    between the converged solve's and the zero guess's."""
+    designs = [wl.cfg1(0), wl.cfg1(1)]
+    b = StructureBatch(designs)
+    print(f"amg maxit {maxit}", flush=True)
+    with pytest.warns(RuntimeWarning, match="did not converge"):
+        u = b.solve(precision="fp64", tol=1e-14, maxit=maxit, preconditioner="amg").cpu().numpy()
+    info = b.plan.last_info
+    assert info["iterations"] == maxit and info["converged"] == 0, info
+    assert np.all(np.isfinite(u))
+    for i, st in enumerate(designs):
+        o = 6 * int(b.node_off[i])
+        e = rel(u[o:o + st.ndof], oracle_u(st))
+        assert 1e-12 < e < 0.5, (i, e, info)
+
+
 def test_amg_set_up_follows_the_assembly():
     """A new assembly on the same plan (other coordinates and thicknesses)
     rebuilds the numeric hierarchy before the next solve."""
