@@ -3024,10 +3024,11 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
         ++chunk_no;
         // the full-length chunk, or one cut short (even lengths; k only matters
         // through its parity, so a graph serves every even start)
-        CgGraph* gp = steps == E ? gc
-                      : (steps < E && steps / 2 - 1 < AMG_CUT_GRAPHS)
-                          ? A.graph_cut[prec_slot<MT, VT>()][NR][steps / 2 - 1]
-                          : nullptr;
+        CgGraph* gp = nullptr;
+        if (steps == E)
+            gp = gc;
+        else if (steps < E && steps >= 2 && steps % 2 == 0 && steps / 2 - 1 < AMG_CUT_GRAPHS)
+            gp = A.graph_cut[prec_slot<MT, VT>()][NR][steps / 2 - 1];
         if (!sampled && gp && steps % 2 == 0 && it0 % 2 == 0 && pl->graphs_ok && graphs_enabled()) {
             ga = capture_graph(pl, gp[0], key, A.wps0, steps, tol, [&](hipStream_t cs) {
                 for (int it = 1; it < steps; ++it) amg_step<MT, VT, NR>(pl, A, w, bf, A0, it, cs);
