@@ -468,17 +468,22 @@ __device__ __forceinline__ double ablk_nat(const double* __restrict__ An, int32_
     return (aref & 1) ? An[vp * 36 + c * 6 + r] : An[vp * 36 + r * 6 + c];
 }
 
-// FP64 ELL [slot][18][64 positions][2] -> [position][36], thread per (slot, pair
-// group, position): reads cover consecutive addresses, each position's 18 writes
-// fill its 288 bytes.
-__global__ __launch_bounds__(256) void k_amg_unpack(int64_t npos, const double2* __restrict__ ell,
+// FP64 ELL [slot][18][64 positions][2] -> [position][36] (FP32), a workgroup
+// per slot of 64 positions transposed through LDS: the reads and the writes
+// both cover consecutive addresses.
+__global__ __launch_bounds__(256) void k_amg_unpack(int64_t nslot, const double2* __restrict__ ell,
                                                     float2* __restrict__ nat) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npos * 18) return;
-    const int64_t lane = t & (ROWS_PER_WG - 1), sg = t / ROWS_PER_WG;
-    const int64_t slot = sg / 18, g = sg % 18;
-    const double2 v = ell[t];
-    nat[(slot * ROWS_PER_WG + lane) * 18 + g] = make_float2((float)v.x, (float)v.y);
+    __shared__ float2 tile[18 * ROWS_PER_WG];
+    const int64_t slot = xcd_wg();
+    if (slot >= nslot) return;
+    const double2* src = ell + slot * 18 * ROWS_PER_WG;
+    for (int u = threadIdx.x; u < 18 * ROWS_PER_WG; u += blockDim.x) {
+        const double2 v = src[u];   // u = g * 64 + position
+        tile[(u & (ROWS_PER_WG - 1)) * 18 + (u >> 6)] = make_float2((float)v.x, (float)v.y);
+    }
+    __syncthreads();
+    float2* dst = nat + slot * ROWS_PER_WG * 18;
+    for (int u = threadIdx.x; u < 18 * ROWS_PER_WG; u += blockDim.x) dst[u] = tile[u];
 }
 
 // Rigid-body modes of every aggregated fine node relative to its aggregate's
@@ -1187,17 +1192,25 @@ __global__ __launch_bounds__(256) void k_amg_pscale(int64_t nP, const int32_t* _
                                                     const int32_t* __restrict__ p_tpos,
                                                     const float* __restrict__ P, const double* __restrict__ Lci,
                                                     _Float16* __restrict__ P16, _Float16* __restrict__ PT16) {
+    // thread per block row r: the row of P once, the six entries of the row
     const int64_t t = xcd_wg() * blockDim.x + threadIdx.x;
-    if (t >= nP * 36) return;
-    const int64_t b = t / 36;
-    const int e = (int)(t % 36), r = e / 6, c = e % 6;
-    const float* p = P + b * 36;
+    if (t >= nP * 6) return;
+    const int64_t b = t / 6;
+    const int r = (int)(t % 6);
+    const float* p = P + b * 36 + r * 6;
     const double* l = Lci + (int64_t)p_col[b] * 36;
-    double s = 0.0;
+    const int64_t tp = (int64_t)p_tpos[b] * 36;
+    float pr[6];
 #pragma unroll
-    for (int m = 0; m < 6; ++m) s += p[r * 6 + m] * l[c * 6 + m];
-    P16[t] = (_Float16)s;
-    PT16[(int64_t)p_tpos[b] * 36 + c * 6 + r] = (_Float16)s;   // per coarse node, block transposed
+    for (int m = 0; m < 6; ++m) pr[m] = p[m];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) s += pr[m] * l[c * 6 + m];
+        P16[b * 36 + r * 6 + c] = (_Float16)s;
+        PT16[tp + c * 6 + r] = (_Float16)s;   // per coarse node, block transposed
+    }
 }
 
 // coarse null space in the scaled coarse variables: B_J = Lc_J^T R_J
@@ -2594,8 +2607,8 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
                            D.d_omega, D.d_theta, l == 0 ? nullptr : D.d_cheb);
         if (D.nP > 0) {
             const int64_t npos = l == 0 ? pl->ell_slots : D.ell_slots;
-            hipLaunchKernelGGL(k_amg_unpack, dim3(nblk(npos * 18, 256)), dim3(256), 0, s, npos, (const double2*)A64,
-                               (float2*)A.d_Anat);
+            hipLaunchKernelGGL(k_amg_unpack, xcd_grid(npos / ROWS_PER_WG), dim3(256), 0, s, npos / ROWS_PER_WG,
+                               (const double2*)A64, (float2*)A.d_Anat);
             // (the lane-per-entry forms on levels of few blocks: the same sums, bit for bit)
             const bool pe = amg_entry_form(D.nP, AMG_ENTRY_PW_MAX), we = amg_entry_form(D.nW, AMG_ENTRY_PW_MAX);
             hipLaunchKernelGGL(pe ? k_amg_psmooth_e : k_amg_psmooth, xcd_grid(nblk(D.nP * (pe ? 36 : 1), 256)),
@@ -2620,7 +2633,7 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
                                    (const double*)A.d_Lci[l + 1], C.d_A64, C.d_A32);
         }
         if (D.nP > 0)
-            hipLaunchKernelGGL(k_amg_pscale, xcd_grid(nblk(D.nP * 36, 256)), dim3(256), 0, s, D.nP, D.d_p_col,
+            hipLaunchKernelGGL(k_amg_pscale, xcd_grid(nblk(D.nP * 6, 256)), dim3(256), 0, s, D.nP, D.d_p_col,
                                D.d_p_tpos, (const float*)D.d_Pf, (const double*)A.d_Lci[l + 1], D.d_P16,
                                D.d_PT16);
         if (C.d_B && C.n > 0)
