@@ -2395,7 +2395,7 @@ static Slices level_slices(Plan* pl, AmgState& A, int l) {
     if (l == 0) return make_slices(pl, 0, pl->n_wg);
     const AmgLevelDev& D = A.dev[l];
     return Slices{D.d_wg_sys, D.d_wg_row0, D.d_wg_row1, D.d_sys_wg_off, D.d_ell_vpos, D.d_ell_col,
-                  (const int4*)D.d_meta, 0, D.n_wg, (D.n_wg + 7) / 8};
+                  (const int4*)D.d_meta, 0, D.n_wg, slice_run(D.n_wg, pl->slice_spread)};
 }
 
 static int amg_build_device(Plan* pl, AmgState& A) {
@@ -2594,11 +2594,11 @@ static int amg_setup_numeric(Plan* pl, AmgState& A, hipStream_t s) {
                                D.d_node_sys, A.d_node_off0, D.d_Q, D.d_R);
         for (int step = 0; step < AMG_POWER_STEPS; ++step) {
             if (l == 0)
-                hipLaunchKernelGGL((k_amg_power<_Float16, 4>), slice_grid(D.n_wg), dim3(256), 0, s, sl,
+                hipLaunchKernelGGL((k_amg_power<_Float16, 4>), slice_grid(sl), dim3(256), 0, s, sl,
                                    (const _Float16*)pl->d_Ke16, step, (const double*)D.d_pv[(step + 1) & 1],
                                    D.d_pv[step & 1], D.d_ppart, (const uint8_t*)D.d_cont, D.n_wg);
             else
-                hipLaunchKernelGGL((k_amg_power<float, 4>), slice_grid(D.n_wg), dim3(256), 0, s, sl,
+                hipLaunchKernelGGL((k_amg_power<float, 4>), slice_grid(sl), dim3(256), 0, s, sl,
                                    (const float*)D.d_A32, step, (const double*)D.d_pv[(step + 1) & 1],
                                    D.d_pv[step & 1], D.d_ppart, (const uint8_t*)D.d_cont, D.n_wg);
         }
@@ -2665,7 +2665,7 @@ static int amg_setup_light(Plan* pl, AmgState& A, hipStream_t s) {
     const int ns = pl->n_sys;
     const int s0 = A.power_next;
     for (int step = s0; step < s0 + AMG_POWER_STEPS_WARM; ++step)
-        hipLaunchKernelGGL((k_amg_power<_Float16, 4>), slice_grid(D.n_wg), dim3(256), 0, s, sl,
+        hipLaunchKernelGGL((k_amg_power<_Float16, 4>), slice_grid(sl), dim3(256), 0, s, sl,
                            (const _Float16*)pl->d_Ke16, step, (const double*)D.d_pv[(step + 1) & 1],
                            D.d_pv[step & 1], D.d_ppart, (const uint8_t*)D.d_cont, D.n_wg);
     const int last = s0 + AMG_POWER_STEPS_WARM - 1;
@@ -2749,11 +2749,11 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
                 constexpr int WP = decltype(W)::value;
                 const Slices cs = level_slices(pl, A, l + 1);
                 // t = b - A b / theta, then r = b - c (theta (b - t) + A t)
-                hipLaunchKernelGGL((k_mg_smooth<NR, WP>), slice_grid(C.n_wg), dim3(64 * WP), 0, s, cs,
+                hipLaunchKernelGGL((k_mg_smooth<NR, WP>), slice_grid(cs), dim3(64 * WP), 0, s, cs,
                                    (const float*)C.d_A32, (const float*)C.d_theta, (const float*)C.d_cheb,
                                    (const uint8_t*)A.d_smooth[l + 1], w.st, (int)SM_PRE1, (const float*)C.d_b,
                                    (const float*)C.d_b, (const float*)nullptr, C.d_t);
-                hipLaunchKernelGGL((k_mg_smooth<NR, WP>), slice_grid(C.n_wg), dim3(64 * WP), 0, s, cs,
+                hipLaunchKernelGGL((k_mg_smooth<NR, WP>), slice_grid(cs), dim3(64 * WP), 0, s, cs,
                                    (const float*)C.d_A32, (const float*)C.d_theta, (const float*)C.d_cheb,
                                    (const uint8_t*)A.d_smooth[l + 1], w.st, (int)SM_PRE2, (const float*)C.d_b,
                                    (const float*)C.d_t, (const float*)nullptr, C.d_r);
@@ -2794,9 +2794,9 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
                                (const float*)nullptr, (const float*)C.d_x, bf.t0);
             wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
                 constexpr int WP = decltype(W)::value;
-                hipLaunchKernelGGL((k_pcg_post<VT, NR, WP>), slice_grid(pl->n_wg), dim3(64 * WP), 0, s,
-                                   level_slices(pl, A, 0), w.P, w.st, k, (const _Float16*)pl->d_Ke16,
-                                   (const float*)D.d_theta,
+                const Slices s0 = level_slices(pl, A, 0);
+                hipLaunchKernelGGL((k_pcg_post<VT, NR, WP>), slice_grid(s0), dim3(64 * WP), 0, s, s0, w.P, w.st, k,
+                                   (const _Float16*)pl->d_Ke16, (const float*)D.d_theta,
                                    (const VT*)bf.r[k & 1], (const VT*)bf.r[(k - 1) & 1], (const float*)bf.t0, bf.z);
             });
         } else if (A.tail_list[l].size() < (size_t)std::count(A.host[l].cont.begin(), A.host[l].cont.end(), 1)) {
@@ -2808,11 +2808,11 @@ static void amg_vcycle_rest(Plan* pl, AmgState& A, const Workspace& w, const Amg
             wps_dispatch<float, NR>(4, [&](auto W) {
                 constexpr int WP = decltype(W)::value;
                 const Slices fs = level_slices(pl, A, l);
-                hipLaunchKernelGGL((k_mg_smooth<NR, WP>), slice_grid(D.n_wg), dim3(64 * WP), 0, s, fs,
+                hipLaunchKernelGGL((k_mg_smooth<NR, WP>), slice_grid(fs), dim3(64 * WP), 0, s, fs,
                                    (const float*)D.d_A32, (const float*)D.d_theta, (const float*)D.d_cheb,
                                    (const uint8_t*)A.d_smooth[l], w.st, (int)SM_POST1, (const float*)D.d_b,
                                    (const float*)D.d_r, (const float*)nullptr, D.d_t);
-                hipLaunchKernelGGL((k_mg_smooth<NR, WP>), slice_grid(D.n_wg), dim3(64 * WP), 0, s, fs,
+                hipLaunchKernelGGL((k_mg_smooth<NR, WP>), slice_grid(fs), dim3(64 * WP), 0, s, fs,
                                    (const float*)D.d_A32, (const float*)D.d_theta, (const float*)D.d_cheb,
                                    (const uint8_t*)A.d_smooth[l], w.st, (int)SM_POST2, (const float*)nullptr,
                                    (const float*)D.d_t, (const float*)D.d_r, D.d_x);
@@ -2828,7 +2828,7 @@ template <typename MT, typename VT, int NR>
 static void amg_step(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs<VT>& bf, const MT* A0, int k,
                      hipStream_t s, std::vector<std::pair<hipEvent_t, hipEvent_t>>* timed = nullptr) {
     const Slices sl = level_slices(pl, A, 0);
-    const dim3 g = slice_grid(pl->n_wg);
+    const dim3 g = slice_grid(sl);
     const _Float16* A16 = (const _Float16*)pl->d_Ke16;
     wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
         constexpr int WP = decltype(W)::value;
@@ -2859,7 +2859,7 @@ template <typename MT, typename VT, int NR>
 static void amg_replace(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs<VT>& bf, const MT* A0, int k,
                         int steps, double tol, hipStream_t s) {
     const Slices sl = level_slices(pl, A, 0);
-    const dim3 g = slice_grid(pl->n_wg);
+    const dim3 g = slice_grid(sl);
     const _Float16* A16 = (const _Float16*)pl->d_Ke16;
     wps_dispatch<VT, NR>(A.wps0, [&](auto W) {
         constexpr int WP = decltype(W)::value;
@@ -2938,7 +2938,7 @@ template <typename MT, typename VT, int NR>
 static int tune_amg_wps(Plan* pl, AmgState& A, const Workspace& w, const AmgBufs<VT>& bf, const MT* A0,
                         hipStream_t s, int32_t& out) {
     const Slices sl = level_slices(pl, A, 0);
-    const dim3 g = slice_grid(pl->n_wg);
+    const dim3 g = slice_grid(sl);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     JSSO_HIP(hipEventCreate(&e0));
     JSSO_HIP(hipEventCreate(&e1));
@@ -2994,7 +2994,7 @@ static int run_amg(Plan* pl, const double* f, double* u, double tol, int maxit, 
     bf.z = (VT*)A.d_z;
     bf.t0 = (float*)A.d_t0;
     bf.xlo = (VT*)w.xlo;
-    const dim3 gridS = slice_grid(pl->n_wg);
+    const dim3 gridS = slice_grid(sl);
     // start: b, x = 0, r_0 = b, p = 0; both partial buffers cleared (so the
     // first direction has beta = 0); z_0 = M r_0
     JSSO_HIP(hipMemsetAsync(w.P.cg[0], 0, 2 * 4 * (size_t)pl->n_wg * NR * sizeof(double), s));

@@ -123,6 +123,7 @@ struct Plan {
     int32_t *d_wg_sys = nullptr, *d_wg_row0 = nullptr, *d_wg_row1 = nullptr;
     int32_t* d_sys_wg_off = nullptr;
     int32_t n_wg = 0;
+    bool slice_spread = false;   // several structures, one of >= 64 slices: slice runs spread over XCDs
     // sliced ELL: slice = workgroup above; values [value slot][36][64 lanes]
     int32_t *d_ell_vpos = nullptr, *d_ell_col = nullptr, *d_slice_meta = nullptr;
     int32_t* d_ell_src = nullptr;    // per value position: source block or -1

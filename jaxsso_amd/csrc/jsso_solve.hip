@@ -622,7 +622,7 @@ int check_status(Plan* p, hipStream_t s) {
 Slices make_slices(Plan* pl, int base, int count) {
     return Slices{pl->d_wg_sys,  pl->d_wg_row0, pl->d_wg_row1, pl->d_sys_wg_off,
                   pl->d_ell_vpos, pl->d_ell_col, (const int4*)pl->d_slice_meta, base,
-                  count,          (count + 7) / 8};
+                  count,          slice_run(count, pl->slice_spread && base == 0 && count == pl->n_wg)};
 }
 
 // Launch one ordinary CG iteration with `wps` waves per slice (falls back to
@@ -630,7 +630,7 @@ Slices make_slices(Plan* pl, int base, int count) {
 template <typename MT, typename VT, int NR>
 static void launch_step(int wps, const Slices& sl, const Workspace& w, int it, const MT* A, VT* const (&rb)[2],
                         VT* const (&qb)[2], VT* p, VT* xlo, hipStream_t s) {
-    const dim3 grid = slice_grid(sl.count);
+    const dim3 grid = slice_grid(sl);
     const VT* rp = rb[(it - 1) & 1];
     const VT* qp = qb[(it - 1) & 1];
     if constexpr (WpsFits<VT, NR, 4>::value) {
@@ -654,7 +654,7 @@ static void launch_step(int wps, const Slices& sl, const Workspace& w, int it, c
 template <typename MT, typename VT, int NR>
 static void launch_init(Plan* pl, const Slices& sl, const Workspace& w, const double* f, const MT* A,
                         VT* const (&rb)[2], VT* const (&qb)[2], VT* p, VT* xlo, double tol, hipStream_t s) {
-    const dim3 gridS = slice_grid(sl.count);
+    const dim3 gridS = slice_grid(sl);
     hipLaunchKernelGGL((k_init<VT, NR>), gridS, dim3(SOLVER_WG), 0, s, sl, w.P, pl->d_fixed, pl->d_Linv, f, w.b64,
                        w.x64, w.xbest, xlo, rb[0], p);
     hipLaunchKernelGGL((k_decide<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, w.P, w.st, pl->n_sys, 0, tol, 0, 0,
@@ -710,7 +710,7 @@ static int tune_wps(Plan* pl, const Slices& sl, const Workspace& w, const MT* A,
 template <typename MT, typename VT, int NR>
 static void launch_replace(Plan* pl, int wps, const Slices& sl, const Workspace& w, int it, int steps, double tol,
                            const MT* A, VT* const (&rb)[2], VT* const (&qb)[2], VT* p, VT* xlo, hipStream_t s) {
-    const dim3 gridS = slice_grid(sl.count);
+    const dim3 gridS = slice_grid(sl);
     hipLaunchKernelGGL((k_upd<VT, NR>), gridS, dim3(SOLVER_WG), 0, s, sl, w.P, w.st, it, (const VT*)rb[(it - 1) & 1],
                        (const VT*)qb[(it - 1) & 1], rb[it & 1], (const VT*)p, xlo, w.x64);
     const double* A64 = pl->d_Ke64;
@@ -952,7 +952,7 @@ static int run_compliance(Plan* pl, const double* f, const double* u, double sca
         g.bytes = need;
     }
     const Slices sl = make_slices(pl, 0, pl->n_wg);
-    hipLaunchKernelGGL((k_fu_partial<NR>), slice_grid(pl->n_wg), dim3(SOLVER_WG), 0, s, sl, f, u, (double*)g.p);
+    hipLaunchKernelGGL((k_fu_partial<NR>), slice_grid(sl), dim3(SOLVER_WG), 0, s, sl, f, u, (double*)g.p);
     hipLaunchKernelGGL((k_fu_total<NR>), dim3(pl->n_sys), dim3(64), 0, s, sl, pl->n_sys, (const double*)g.p, scale,
                        C);
     JSSO_HIP(hipGetLastError());
@@ -1148,7 +1148,7 @@ int solve_plain(Plan* pl, const double* f, double* u, int nrhs, double tol, int 
     for (int sidx = 0; sidx < pl->n_sys; ++sidx) {
         const int w0 = pl->h_sys_wg_off[sidx], w1 = pl->h_sys_wg_off[sidx + 1];
         const Slices sl = make_slices(pl, w0, w1 - w0);
-        const dim3 grid = slice_grid(w1 - w0), block(SOLVER_WG);
+        const dim3 grid = slice_grid(sl), block(SOLVER_WG);
         for (int h = 0; h < nrhs; ++h) {
             hipLaunchKernelGGL(kp_init, grid, block, 0, s, sl, w.P, h, nrhs, pl->d_fixed, pl->d_Minv, f, x, r, z, pp);
             hipLaunchKernelGGL(kp_start, dim3(1), dim3(1), 0, s, sl, w.P, active, relres);

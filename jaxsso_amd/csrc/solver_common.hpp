@@ -26,20 +26,38 @@ struct Slices {
     const int4* meta;
     int base;           // first slice of the launch (plain path)
     int count;          // slices in the launch
-    int chunk;          // ceil(count / 8): slices per XCD
+    int run;            // slices per run of the workgroup -> slice map (slice_run)
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs; map them so that each
-// XCD processes a contiguous run of slices, identically in every kernel: the
-// vectors a slice touches then stay in one XCD's L2 from kernel to kernel.
-// Returns -1 for the (at most 7) padding workgroups of the grid.
+// Workgroups are dispatched round-robin over the 8 XCDs.  They are mapped to
+// slices in runs of `run` consecutive slices, run r going to XCD r mod 8,
+// identically in every kernel of a level, so that neighbouring slices (which
+// share vector rows, and the stored blocks read through the transpose) stay in
+// one XCD's L2 within a run.  One structure, or only small ones (below 64
+// slices): one run per XCD (a contiguous eighth of the slices each).  Several
+// structures of which one or more is large: runs of 16 slices, shorter
+// on small levels so that every XCD gets ~8 or more, which spreads every
+// structure over all eight XCDs: as structures converge and their slices go
+// idle, the ones still iterating keep every XCD equally busy (with an eighth of
+// the plan per XCD a straggler structure ran on a single XCD).  The map is a
+// bijection of [0, grid); the workgroups it sends past `count` return -1.
+static inline int slice_run(int count, bool spread) {
+    if (!spread) return count > 8 ? (count + 7) / 8 : 1;
+    int run = 1;
+    while (run < 16 && (count / (2 * run)) >= 64) run *= 2;   // run <= count / 64, at most 16
+    return run;
+}
+
 __device__ __forceinline__ int slice_of_block(const Slices& sl) {
-    const int b = blockIdx.x;
-    const int l = (b & 7) * sl.chunk + (b >> 3);
+    const int b = blockIdx.x, bx = b >> 3, run = sl.run;
+    const int l = ((bx / run) * 8 + (b & 7)) * run + bx % run;
     return l < sl.count ? sl.base + l : -1;
 }
 
-static inline dim3 slice_grid(int count) { return dim3(8 * ((count + 7) / 8)); }
+static inline dim3 slice_grid(const Slices& sl) {
+    const int u = 8 * sl.run;
+    return dim3(u * ((sl.count + u - 1) / u));
+}
 
 // per-(structure, rhs) state, written only by k_decide
 struct SysState {
