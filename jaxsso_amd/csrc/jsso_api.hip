@@ -323,10 +323,13 @@ static int build_plan(Plan* p, int32_t n_sys, const int32_t* node_off, int32_t n
     if (rc) return rc;
     p->n_wg = ell.n_wg;
     p->h_sys_wg_off = ell.sys_wg_off;
-    // (structures below 64 slices are latency-bound per launch wherever they run)
+    // (structures below 64 slices are latency-bound per launch wherever they run;
+    // JSSO_SLICE_SPREAD=0 / 1 forces the contiguous map / the runs for any batch:
+    // the results are the same bits either way, the tests compare them)
     int max_wg = 0;
     for (int s = 0; s < n_sys; ++s) max_wg = std::max(max_wg, ell.sys_wg_off[s + 1] - ell.sys_wg_off[s]);
     p->slice_spread = n_sys > 1 && max_wg >= 64;
+    if (const char* e = getenv("JSSO_SLICE_SPREAD")) p->slice_spread = e[0] == '1' && n_sys > 1;
     p->ell_slots = ell.ell_slots;
     p->h_kref.swap(ell.kref);
     std::vector<int32_t>& meta = ell.meta;
