@@ -39,8 +39,8 @@ struct Slices {
 // on small levels so that every XCD gets ~8 or more, which spreads every
 // structure over all eight XCDs: as structures converge and their slices go
 // idle, the ones still iterating keep every XCD equally busy (with an eighth of
-// the plan per XCD a straggler structure ran on a single XCD).  The map is a
-// bijection of [0, grid); the workgroups it sends past `count` return -1.
+// the plan per XCD, all the slices of a straggler would sit on one XCD).  The
+// map is a bijection of [0, grid); the workgroups it sends past `count` return -1.
 static inline int slice_run(int count, bool spread) {
     if (!spread) return count > 8 ? (count + 7) / 8 : 1;
     int run = 1;
