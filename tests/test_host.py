@@ -152,3 +152,15 @@ def test_amg_hierarchy_is_per_structure_and_coarsens():
         p.set_preconditioner("ilu")
     p.set_preconditioner("amg")
     assert p.preconditioner == "amg"
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and says so,
+    before anything touches a device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300, cwd=root)
+    assert r.returncode == 2
+    assert "one GPU" in (r.stdout + r.stderr)
