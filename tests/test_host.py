@@ -164,3 +164,47 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=300, cwd=root)
     assert r.returncode == 2
     assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_trace_summary_averages_only_full_activity_launches(tmp_path, capsys):
+    """bench.py --trace-summary: of the roofline kernel's launches in a kernel
+    trace it drops the live tuner's (before the first solve) and averages, per
+    solve, only the first full_iterations of its dir_launches.  Synthetic trace:
+    3 tuner launches of 999 us, two solves of 6 launches each, the first 4 of
+    each (every design running) 200 and 210 us, the rest 50 us, other kernels in
+    between."""
+    import json
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import bench
+    kname = "k_pcg_dir<double, double, 1, 4>"
+    rows, t = [], 1_000_000
+    def add(name, us):
+        nonlocal t
+        rows.append((t, t + int(us * 1000), name))
+        t += int(us * 1000) + 5000
+    for _ in range(3):
+        add(f"void jsso::{kname}(jsso::Slices)", 999.0)
+    for solve in range(2):
+        for k in range(6):
+            add(f"void jsso::{kname}(jsso::Slices)", (200.0 if solve == 0 else 210.0) if k < 4 else 50.0)
+            add("_ZN4jsso9k_pcg_preIdLi1ELb0ELi4EE", 120.0)
+    d = tmp_path / "trace" / "host"
+    d.mkdir(parents=True)
+    with open(d / "run_kernel_trace.csv", "w") as fh:
+        fh.write("Kernel_Name,Start_Timestamp,End_Timestamp\n")
+        for s0, s1, name in rows:
+            fh.write(f"\"{name}\",{s0},{s1}\n")
+    log = tmp_path / "bench.log"
+    alg = 897189888
+    log.write_text("# header\n" + json.dumps({"roofline": {"kernel": kname, "alg_bytes_per_launch": alg,
+                                                           "avg_launch_us": 205.0},
+                                              "solve_log": {"solves": [[6, 6, 4], [6, 6, 4]]}}) + "\n")
+    bench.trace_summary(str(tmp_path / "trace"), str(log))
+    out = capsys.readouterr().out
+    line = [l for l in out.splitlines() if l.startswith("roofline_launches")][0]
+    assert f"{kname}: 8 launches with every design running" in line
+    assert "avg 205.00 us, min 200.00, max 210.00" in line
+    assert f"-> {alg / 205e-6 / 1e9:.1f} GB/s" in line
+    assert "the first 3: the tuner" in out
