@@ -208,3 +208,46 @@ def test_trace_summary_averages_only_full_activity_launches(tmp_path, capsys):
     assert "avg 205.00 us, min 200.00, max 210.00" in line
     assert f"-> {alg / 205e-6 / 1e9:.1f} GB/s" in line
     assert "the first 3: the tuner" in out
+
+
+def test_traffic_summary_scales_by_the_copy_and_keeps_full_activity(tmp_path, capsys):
+    """bench.py --traffic-summary: each counter (KiB per dispatch) is scaled by
+    what it reads for the run's 1 GiB device copy, and only the roofline kernel's
+    full-activity launches are averaged (tuner launches dropped).  Synthetic PMC
+    output: FETCH_SIZE reads the copy as 512 MiB (factor 2), WRITE_SIZE as 1 GiB
+    (factor 1); 2 tuner launches, then one solve of 4 launches, 2 of them with
+    every design running."""
+    import json
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import bench
+    kname = "k_pcg_dir<double, double, 1, 4>"
+    full = {"FETCH_SIZE": [400000.0, 600000.0], "WRITE_SIZE": [30000.0, 34000.0]}
+    copy = {"FETCH_SIZE": 524288.0, "WRITE_SIZE": 1048576.0}
+    for counter, sub in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write")):
+        d = tmp_path / sub / "host"
+        d.mkdir(parents=True)
+        lines = ["Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value"]
+        did = 0
+        def row(name, v):
+            nonlocal did
+            did += 1
+            lines.append(f"{did},\"{name}\",{counter},{v}")
+        row("__amd_rocclr_copyBuffer", copy[counter])
+        for v in (9e6, 9e6) + tuple(full[counter]) + (10.0, 10.0):
+            row(f"void jsso::{kname}(jsso::Slices)", v)
+            row("void jsso::ell_assemble_kernel(long)", 1000.0)
+        (d / "run_counter_collection.csv").write_text("\n".join(lines) + "\n")
+    log = tmp_path / "bench.log"
+    alg = 897189888
+    log.write_text(json.dumps({"roofline": {"kernel": kname, "alg_bytes_per_launch": alg},
+                               "solve_log": {"solves": [[4, 4, 2]]}}) + "\n")
+    bench.traffic_summary(str(tmp_path / "fetch"), str(tmp_path / "write"), str(log))
+    out = capsys.readouterr().out
+    fetch = 500000.0 * 1024 * 2.0
+    write = 32000.0 * 1024 * 1.0
+    assert f"fetch_bytes_per_launch {kname}: {int(fetch)}" in out
+    assert f"write_bytes_per_launch {kname}: {int(write)}" in out
+    assert f"traffic / algorithmic bytes = {(fetch + write) / alg:.3f} over 2 launches" in out
+    assert f"assembly_traffic ell_assemble_kernel: fetch {int(1000.0 * 1024 * 2)} write {int(1000.0 * 1024)}" in out
